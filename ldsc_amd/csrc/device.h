@@ -1,0 +1,79 @@
+// device.h -- device-side layout constants, the panel object and the
+// per-call workspace shared by the engine and the request server.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <memory>
+#include <mutex>
+#include <vector>
+
+#include "internal.h"
+
+namespace ldsc {
+
+// Tiled operand layout (built per call from the resident .bed slabs):
+//  * SNPs in groups of 32 (one MFMA row/column block); band tiles are 64x64.
+//  * Individuals in K-blocks of 128.  Tile (group g, K-block kb) is 1 KiB:
+//    64 lanes x 16 bytes.  Lane l holds SNP 32g + l%32; its dword t (0..3)
+//    holds the 16 individuals kb*128 + t*32 + (l/32)*16 + e, e = 4f + k,
+//    with the 2-bit PLINK code of individual e at bits 8k + 2f.  A wave
+//    loads a whole tile with one coalesced 16-byte load per lane.
+//  * Padding individuals and SNPs carry code 01 (missing): they add nothing
+//    to any integer product and are masked out of every pair.
+constexpr int kTile = 64;
+constexpr int kGroup = 32;
+constexpr int kKBlock = 128;
+
+inline uint64_t round_up(uint64_t x, uint64_t a) { return (x + a - 1) / a * a; }
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t cap = 0;
+    int device = -1;
+    hipError_t ensure(size_t bytes);
+    void release();
+    template <class T>
+    T* as() const { return static_cast<T*>(p); }
+};
+
+// One in-flight computation: its own stream, events and grow-only buffers.
+struct Workspace {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[8] = {};
+    DevBuf tiled, stats, bed_idx, lo, info, aeff, annot, maf, tiles, j0, tstart, rowp, colp,
+        l2, sk_bucket, sk_sign, sk_cols, sk_perm, sk_off, scratch;
+    size_t bytes() const;
+    hipError_t init(int dev);
+    void destroy();
+};
+
+Workspace* acquire_workspace(int device);
+void release_workspace(Workspace* ws);
+
+}  // namespace ldsc
+
+struct ldsc_panel {
+    int device = 0;
+    uint64_t n_fam = 0, m_bim = 0, bps = 0, n = 0;
+    uint8_t* d_slabs = nullptr;  // m_bim x bps bytes, the .bed body
+    int64_t* d_keep = nullptr;   // n FAM rows, or nullptr (all individuals)
+    std::vector<int64_t> keep;
+};
+
+namespace ldsc {
+
+// The exact / sketch engine.  `ws` must belong to panel->device.  When
+// `sync` is false the call returns after enqueueing and copying back happens
+// on ws->stream (the caller synchronizes the stream before reading outputs).
+int run_l2(const ldsc_panel* panel, const ldsc_snp* snps, uint64_t m, const ldsc_l2_config* cfg,
+           const double* annot, uint64_t K, double* l2_out, double* maf_out,
+           ldsc_l2_perf* perf, ldsc_progress_cb cb, void* cb_ctx, Workspace* ws, char* err,
+           uint64_t err_len);
+
+// Device bytes a call of this shape needs (admission control).
+size_t l2_workspace_bytes(const ldsc_panel* panel, uint64_t m, const ldsc_l2_config* cfg,
+                          uint64_t K);
+
+}  // namespace ldsc

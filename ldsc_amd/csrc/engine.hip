@@ -1,0 +1,985 @@
+// engine.hip -- resident panels and the exact LD-score engine for gfx950.
+//
+// Pipeline of one ldsc_l2_compute call (exact mode):
+//   1. k_tile_stats   gather the call's SNP slabs (bed_idx order, --keep
+//                     subset) from the resident .bed copy, write them in the
+//                     MFMA tile layout (device.h) and count per-SNP integer
+//                     stats (sum, count, sum_sq, het) -- HBM-bound.
+//   2. k_snp_info     per-SNP constants of the finish: exact-integer (C, S,
+//                     1/sqrt(C*Q)) or the reference's f32 standardization LUT.
+//   3. k_band_i8 / k_band_f32
+//                     one wave per 64x64 tile of the SNP band.  int8 form:
+//                     four exact integer products G.G, G.M, M.G, M.M on
+//                     v_mfma_i32_32x32x32_i8 with int32 accumulation, FP64
+//                     finish of the mean-imputed covariance and r^2.  f32
+//                     form: standardized columns on v_mfma_f32_32x32x2f32.
+//                     Then the unbiased r^2, the pair bound mask and the
+//                     r2u . A_eff contractions for both members of each pair,
+//                     written as per-tile partial sums (no atomics).
+//   4. k_reduce       per-SNP sum of its tile partials in a fixed order plus
+//                     the self term -> deterministic l2.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+
+#include "device.h"
+
+namespace ldsc {
+
+#define HIPC(expr)                                                                        \
+    do {                                                                                  \
+        hipError_t e_ = (expr);                                                           \
+        if (e_ != hipSuccess)                                                             \
+            return fail(err, err_len, e_ == hipErrorOutOfMemory ? LDSC_ERR_OOM            \
+                                                                 : LDSC_ERR_DEVICE,       \
+                        "%s failed: %s", #expr, hipGetErrorString(e_));                    \
+    } while (0)
+
+typedef int v4i __attribute__((ext_vector_type(4)));
+typedef int v16i __attribute__((ext_vector_type(16)));
+typedef float v16f __attribute__((ext_vector_type(16)));
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+
+// ------------------------------------------------------------------------
+// Buffers and workspaces
+// ------------------------------------------------------------------------
+hipError_t DevBuf::ensure(size_t bytes) {
+    if (bytes <= cap && p) return hipSuccess;
+    release();
+    size_t want = std::max<size_t>(bytes, 256);
+    hipError_t e = hipMalloc(&p, want);
+    if (e != hipSuccess) {
+        p = nullptr;
+        cap = 0;
+        return e;
+    }
+    cap = want;
+    return hipSuccess;
+}
+void DevBuf::release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+}
+
+size_t Workspace::bytes() const {
+    const DevBuf* all[] = {&tiled, &stats, &bed_idx, &lo, &info, &aeff, &annot, &maf, &tiles,
+                           &j0, &tstart, &rowp, &colp, &l2, &sk_bucket, &sk_sign, &sk_cols,
+                           &sk_perm, &sk_off, &scratch};
+    size_t s = 0;
+    for (auto* b : all) s += b->cap;
+    return s;
+}
+
+hipError_t Workspace::init(int dev) {
+    device = dev;
+    hipError_t e = hipSetDevice(dev);
+    if (e != hipSuccess) return e;
+    e = hipStreamCreateWithFlags(&stream, hipStreamNonBlocking);
+    if (e != hipSuccess) return e;
+    for (auto& x : ev) {
+        e = hipEventCreate(&x);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+void Workspace::destroy() {
+    DevBuf* all[] = {&tiled, &stats, &bed_idx, &lo, &info, &aeff, &annot, &maf, &tiles,
+                     &j0, &tstart, &rowp, &colp, &l2, &sk_bucket, &sk_sign, &sk_cols,
+                     &sk_perm, &sk_off, &scratch};
+    for (auto* b : all) b->release();
+    for (auto& x : ev)
+        if (x) (void)hipEventDestroy(x);
+    if (stream) (void)hipStreamDestroy(stream);
+    stream = nullptr;
+}
+
+static std::mutex g_ws_mu;
+static std::vector<Workspace*> g_ws_free;
+
+Workspace* acquire_workspace(int device) {
+    {
+        std::lock_guard<std::mutex> lk(g_ws_mu);
+        for (size_t i = 0; i < g_ws_free.size(); ++i) {
+            if (g_ws_free[i]->device == device) {
+                Workspace* w = g_ws_free[i];
+                g_ws_free.erase(g_ws_free.begin() + i);
+                return w;
+            }
+        }
+    }
+    Workspace* w = new Workspace();
+    if (w->init(device) != hipSuccess) {
+        w->destroy();
+        delete w;
+        return nullptr;
+    }
+    return w;
+}
+
+void release_workspace(Workspace* w) {
+    if (!w) return;
+    std::lock_guard<std::mutex> lk(g_ws_mu);
+    g_ws_free.push_back(w);
+}
+
+// ------------------------------------------------------------------------
+// Kernel 1: gather + tile + integer stats
+// ------------------------------------------------------------------------
+// Each workgroup owns one group of 32 SNPs and walks all K-blocks.  Thread
+// tau writes dword (lane = tau/4, t = tau%4) of each 1 KiB tile, so a
+// workgroup writes whole tiles with coalesced stores.
+__global__ __launch_bounds__(256) void k_tile_stats(const uint8_t* __restrict__ slabs,
+                                                     uint64_t bps,
+                                                     const int64_t* __restrict__ keep,
+                                                     uint32_t n, const uint64_t* __restrict__ bed_idx,
+                                                     uint32_t m, uint32_t nkb,
+                                                     uint32_t* __restrict__ tiled,
+                                                     uint32_t* __restrict__ stats) {
+    __shared__ uint32_t sst[kGroup * 4];
+    const int tid = threadIdx.x;
+    if (tid < kGroup * 4) sst[tid] = 0;
+    __syncthreads();
+    const uint32_t g = blockIdx.x;
+    const int lane = tid >> 2, t = tid & 3;
+    const uint32_t snp = g * kGroup + (lane & 31);
+    const uint32_t h = (uint32_t)lane >> 5;
+    const bool valid = snp < m;
+    const uint8_t* slab = valid ? slabs + bed_idx[snp] * bps : nullptr;
+    const bool aligned = (bps & 3) == 0;
+    uint32_t n2 = 0, n1 = 0, n0 = 0;
+    uint32_t* out = tiled + ((size_t)g * nkb * 64 + lane) * 4 + t;
+    for (uint32_t kb = 0; kb < nkb; ++kb) {
+        const uint32_t base = kb * kKBlock + t * 32 + h * 16;
+        uint32_t w = 0x55555555u;  // 16 x code 01 (missing)
+        if (valid && base < n) {
+            const uint32_t nval = n - base < 16 ? n - base : 16;
+            uint32_t raw = 0;
+            if (!keep) {
+                const uint64_t b0 = base >> 2;
+                if (aligned && b0 + 4 <= bps) {
+                    raw = *reinterpret_cast<const uint32_t*>(slab + b0);
+                } else {
+                    for (int q = 0; q < 4; ++q)
+                        if (b0 + q < bps) raw |= (uint32_t)slab[b0 + q] << (8 * q);
+                }
+            } else {
+                for (uint32_t e = 0; e < nval; ++e) {
+                    const uint64_t row = (uint64_t)keep[base + e];
+                    raw |= (uint32_t)((slab[row >> 2] >> (2 * (row & 3))) & 3u) << (2 * e);
+                }
+            }
+            if (nval < 16) {
+                const uint32_t keepmask = (1u << (2 * nval)) - 1u;
+                raw = (raw & keepmask) | (0x55555555u & ~keepmask);
+            }
+            const uint32_t L = raw & 0x55555555u, H = (raw >> 1) & 0x55555555u;
+            n2 += __popc(~(L | H) & 0x55555555u);  // 00: dosage 2
+            n1 += __popc(~L & H);                   // 10: dosage 1 (het)
+            n0 += __popc(L & H);                    // 11: dosage 0
+            // 4x4 transpose of 2-bit cells: individual 4f+k moves from bit
+            // 8f+2k to bit 8k+2f (two delta swaps).
+            uint32_t x = raw, s;
+            s = ((x >> 6) ^ x) & 0x00CC00CCu;
+            x ^= s ^ (s << 6);
+            s = ((x >> 12) ^ x) & 0x0000F0F0u;
+            x ^= s ^ (s << 12);
+            w = x;
+        }
+        out[(size_t)kb * 256] = w;
+    }
+    atomicAdd(&sst[(lane & 31) * 4 + 0], 2 * n2 + n1);
+    atomicAdd(&sst[(lane & 31) * 4 + 1], n2 + n1 + n0);
+    atomicAdd(&sst[(lane & 31) * 4 + 2], 4 * n2 + n1);
+    atomicAdd(&sst[(lane & 31) * 4 + 3], n1);
+    __syncthreads();
+    if (tid < kGroup * 4) stats[(size_t)g * kGroup * 4 + tid] = sst[tid];
+}
+
+// ------------------------------------------------------------------------
+// Kernel 2: per-SNP constants
+// ------------------------------------------------------------------------
+// int8 form: cs = (C, S), rs = 1/sqrt(C*Q), Q = C*SS - S^2 (exact integers;
+// C*Q = count^2 * centered_sum_sq).  f32 form: the 4-entry LUT of
+// normalize_col_f32_with_stats (normalize.rs:115-154) indexed by the PLINK
+// code: [(2-mean)inv, 0, (1-mean)inv, -mean*inv] in f32.
+__global__ void k_snp_info(const uint32_t* __restrict__ stats, uint32_t m_pad, uint32_t n,
+                           int2* __restrict__ cs, double* __restrict__ rs,
+                           float4* __restrict__ lut) {
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= m_pad) return;
+    const uint32_t S = stats[4 * j + 0], C = stats[4 * j + 1], SS = stats[4 * j + 2];
+    cs[j] = make_int2((int)C, (int)S);
+    const int64_t Q = (int64_t)C * (int64_t)SS - (int64_t)S * (int64_t)S;
+    const int64_t V = (int64_t)C * Q;
+    rs[j] = V > 0 ? 1.0 / sqrt((double)V) : 0.0;
+    const double avg = C > 0 ? (double)S / (double)C : 0.0;
+    const double css = (double)SS - (double)C * avg * avg;
+    const double sd = sqrt(css / (double)n);
+    const float a32 = (float)avg;
+    const float i32 = sd > 0.0 ? (float)(1.0 / sd) : 0.0f;
+    lut[j] = make_float4((2.0f - a32) * i32, 0.0f, (1.0f - a32) * i32, -a32 * i32);
+}
+
+// Fill A_eff with ones (no --annot, no pq) for rows < m, zero padding rows.
+__global__ void k_fill_ones(double* __restrict__ a, uint32_t m, uint32_t m_pad) {
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < m_pad) a[j] = j < m ? 1.0 : 0.0;
+}
+
+// ------------------------------------------------------------------------
+// Kernel 3: banded contraction
+// ------------------------------------------------------------------------
+struct BandArgs {
+    const u32x4* tiled;
+    const int2* tiles;
+    const uint32_t* lo;
+    const int2* cs;
+    const double* rs;
+    const float4* lut;
+    const double* aeff;
+    double* rowp;
+    double* colp;
+    uint32_t nkb, ntiles, per_xcd, m, m_pad, K;
+    double r2u_a, r2u_b, n_inv_sq_r2u_a;
+};
+
+// Tile -> XCD mapping: workgroups are dealt to the 8 XCDs round-robin, so
+// workgroup b runs on XCD b%8.  Give each XCD one contiguous stretch of the
+// (I, J)-sorted tile list: neighbouring tiles share SNP groups, which then
+// hit in that XCD's L2.
+__device__ inline int tile_of_block(const BandArgs& a) {
+    const uint32_t xcd = blockIdx.x & 7u, slot = blockIdx.x >> 3;
+    const uint32_t t = xcd * a.per_xcd + slot;
+    return t < a.ntiles ? (int)t : -1;
+}
+
+// Expand one dword of 16 PLINK codes (device.h bit order) into int8 dosage
+// G (00->2, 01->0, 10->1, 11->0) and presence M (01->0, else 1).  Each
+// byte of D holds one code 0..3; v_perm_b32 uses it as a byte selector into
+// the 4-byte table.  Both perm sources carry the table, so selectors 0..3
+// and 4..7 read the same entry.
+__device__ __forceinline__ void expand_codes(uint32_t w, v4i& G, v4i& M) {
+#pragma unroll
+    for (int f = 0; f < 4; ++f) {
+        const uint32_t D = (w >> (2 * f)) & 0x03030303u;
+        G[f] = (int)__builtin_amdgcn_perm(0x00010002u, 0x00010002u, D);
+        M[f] = (int)__builtin_amdgcn_perm(0x01010001u, 0x01010001u, D);
+    }
+}
+
+struct EpiShared {
+    double R[32][65];     // one 32-row half of the tile's masked r2u
+    double ar[64][8];     // A_eff of the tile's row SNPs (chunk of 8 annotations)
+    double ac[64][8];     // A_eff of the tile's column SNPs
+    int2 csr[64], csc[64];
+    double rsr[64], rsc[64];
+    uint32_t lor[64];
+};
+
+__device__ __forceinline__ int row_of(int bi, int r, int lane) {
+    return 32 * bi + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+}
+
+// r2u (masked, 64 per lane in MFMA output order) -> per-tile partial sums.
+//   rowp[tile][k][a] = sum_b R[a][b] * A_eff[col b][k]
+//   colp[tile][k][b] = sum_a R[a][b] * A_eff[row a][k]
+__device__ void band_contract(const BandArgs& a, EpiShared& sh, int tile, uint32_t I, uint32_t J,
+                              const double (&r2u)[2][2][16]) {
+    const int lane = threadIdx.x;
+    for (uint32_t kc0 = 0; kc0 < a.K; kc0 += 8) {
+        const uint32_t kcn = a.K - kc0 < 8 ? a.K - kc0 : 8;
+        __syncthreads();
+        for (uint32_t kk = 0; kk < kcn; ++kk) {
+            sh.ar[lane][kk] = a.aeff[(size_t)(kc0 + kk) * a.m_pad + (size_t)I * kTile + lane];
+            sh.ac[lane][kk] = a.aeff[(size_t)(kc0 + kk) * a.m_pad + (size_t)J * kTile + lane];
+        }
+        double cacc[8];
+#pragma unroll
+        for (int kk = 0; kk < 8; ++kk) cacc[kk] = 0.0;
+#pragma unroll
+        for (int bi = 0; bi < 2; ++bi) {
+            __syncthreads();
+#pragma unroll
+            for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+                for (int r = 0; r < 16; ++r)
+                    sh.R[row_of(0, r, lane)][32 * bj + (lane & 31)] = r2u[bi][bj][r];
+            __syncthreads();
+            // row role: lane covers row lane%32 over columns of its half
+            double racc[8];
+#pragma unroll
+            for (int kk = 0; kk < 8; ++kk) racc[kk] = 0.0;
+            const int rr = lane & 31, b0 = 32 * (lane >> 5);
+            for (int b = 0; b < 32; ++b) {
+                const double rv = sh.R[rr][b0 + b];
+#pragma unroll
+                for (int kk = 0; kk < 8; ++kk)
+                    if ((uint32_t)kk < kcn) racc[kk] += rv * sh.ac[b0 + b][kk];
+            }
+#pragma unroll
+            for (int kk = 0; kk < 8; ++kk)
+                if ((uint32_t)kk < kcn) racc[kk] += __shfl_xor(racc[kk], 32);
+            if (lane < 32)
+                for (uint32_t kk = 0; kk < kcn; ++kk)
+                    a.rowp[((size_t)tile * a.K + kc0 + kk) * kTile + 32 * bi + lane] = racc[kk];
+            // column role: lane = column, rows of this half
+            for (int r = 0; r < 32; ++r) {
+                const double rv = sh.R[r][lane];
+#pragma unroll
+                for (int kk = 0; kk < 8; ++kk)
+                    if ((uint32_t)kk < kcn) cacc[kk] += rv * sh.ar[32 * bi + r][kk];
+            }
+        }
+        for (uint32_t kk = 0; kk < kcn; ++kk)
+            a.colp[((size_t)tile * a.K + kc0 + kk) * kTile + lane] = cacc[kk];
+    }
+}
+
+__device__ __forceinline__ void stage_info(const BandArgs& a, EpiShared& sh, uint32_t I,
+                                           uint32_t J) {
+    const int lane = threadIdx.x;
+    sh.csr[lane] = a.cs[(size_t)I * kTile + lane];
+    sh.rsr[lane] = a.rs[(size_t)I * kTile + lane];
+    sh.lor[lane] = a.lo[(size_t)I * kTile + lane];
+    sh.csc[lane] = a.cs[(size_t)J * kTile + lane];
+    sh.rsc[lane] = a.rs[(size_t)J * kTile + lane];
+}
+
+__device__ __forceinline__ bool pair_included(const BandArgs& a, const EpiShared& sh,
+                                              uint32_t I, uint32_t J, int row, int col) {
+    const uint32_t jj = I * kTile + row, ii = J * kTile + col;
+    return ii < jj && jj < a.m && ii >= sh.lor[row];
+}
+
+__global__ __launch_bounds__(64) void k_band_i8(BandArgs a) {
+    __shared__ EpiShared sh;
+    const int tile = tile_of_block(a);
+    if (tile < 0) return;
+    const int2 tl = a.tiles[tile];
+    const uint32_t I = (uint32_t)tl.x, J = (uint32_t)tl.y;
+    const int lane = threadIdx.x;
+    const size_t gstride = (size_t)a.nkb * 64;
+    const u32x4* pa0 = a.tiled + (size_t)(2 * I) * gstride + lane;
+    const u32x4* pa1 = pa0 + gstride;
+    const u32x4* pb0 = a.tiled + (size_t)(2 * J) * gstride + lane;
+    const u32x4* pb1 = pb0 + gstride;
+
+    v16i acc[4][2][2];
+#pragma unroll
+    for (int p = 0; p < 4; ++p)
+#pragma unroll
+        for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+            for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+                for (int r = 0; r < 16; ++r) acc[p][bi][bj][r] = 0;
+
+    u32x4 ca0 = pa0[0], ca1 = pa1[0], cb0 = pb0[0], cb1 = pb1[0];
+    for (uint32_t kb = 0; kb < a.nkb; ++kb) {
+        const size_t nk = (size_t)(kb + 1 < a.nkb ? kb + 1 : kb) * 64;
+        const u32x4 na0 = pa0[nk], na1 = pa1[nk], nb0 = pb0[nk], nb1 = pb1[nk];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            v4i ga[2], ma[2], gb[2], mb[2];
+            expand_codes(ca0[t], ga[0], ma[0]);
+            expand_codes(ca1[t], ga[1], ma[1]);
+            expand_codes(cb0[t], gb[0], mb[0]);
+            expand_codes(cb1[t], gb[1], mb[1]);
+#pragma unroll
+            for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+                for (int bj = 0; bj < 2; ++bj) {
+                    acc[0][bi][bj] = __builtin_amdgcn_mfma_i32_32x32x32_i8(ga[bi], gb[bj], acc[0][bi][bj], 0, 0, 0);
+                    acc[1][bi][bj] = __builtin_amdgcn_mfma_i32_32x32x32_i8(ga[bi], mb[bj], acc[1][bi][bj], 0, 0, 0);
+                    acc[2][bi][bj] = __builtin_amdgcn_mfma_i32_32x32x32_i8(ma[bi], gb[bj], acc[2][bi][bj], 0, 0, 0);
+                    acc[3][bi][bj] = __builtin_amdgcn_mfma_i32_32x32x32_i8(ma[bi], mb[bj], acc[3][bi][bj], 0, 0, 0);
+                }
+        }
+        ca0 = na0;
+        ca1 = na1;
+        cb0 = nb0;
+        cb1 = nb1;
+    }
+
+    stage_info(a, sh, I, J);
+    __syncthreads();
+    // FP64 finish from exact integers:
+    //   T  = C_r (C_c GG - S_c GM) - S_r (C_c MG - S_c MM)  = C_r C_c * cov
+    //   r  = T / sqrt(C_r Q_r C_c Q_c),  r2u = r^2 (1 + 1/D) - 1/D
+    double r2u[2][2][16];
+#pragma unroll
+    for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+        for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int row = row_of(bi, r, lane), col = 32 * bj + (lane & 31);
+                const int2 cr = sh.csr[row], cc = sh.csc[col];
+                const double u = (double)cc.x * (double)acc[0][bi][bj][r] -
+                                 (double)cc.y * (double)acc[1][bi][bj][r];
+                const double v = (double)cc.x * (double)acc[2][bi][bj][r] -
+                                 (double)cc.y * (double)acc[3][bi][bj][r];
+                const double T = (double)cr.x * u - (double)cr.y * v;
+                const double rr = T * sh.rsr[row] * sh.rsc[col];
+                const double val = rr * rr * a.r2u_a + a.r2u_b;
+                r2u[bi][bj][r] = pair_included(a, sh, I, J, row, col) ? val : 0.0;
+            }
+    band_contract(a, sh, tile, I, J, r2u);
+}
+
+__device__ __forceinline__ float pick_code(const float4& L, uint32_t code) {
+    const float lo = (code & 1u) ? L.y : L.x;
+    const float hi = (code & 1u) ? L.w : L.z;
+    return (code & 2u) ? hi : lo;
+}
+
+__global__ __launch_bounds__(64) void k_band_f32(BandArgs a) {
+    __shared__ EpiShared sh;
+    const int tile = tile_of_block(a);
+    if (tile < 0) return;
+    const int2 tl = a.tiles[tile];
+    const uint32_t I = (uint32_t)tl.x, J = (uint32_t)tl.y;
+    const int lane = threadIdx.x;
+    const size_t gstride = (size_t)a.nkb * 64;
+    const u32x4* pa0 = a.tiled + (size_t)(2 * I) * gstride + lane;
+    const u32x4* pa1 = pa0 + gstride;
+    const u32x4* pb0 = a.tiled + (size_t)(2 * J) * gstride + lane;
+    const u32x4* pb1 = pb0 + gstride;
+    const float4 la0 = a.lut[(size_t)I * kTile + (lane & 31)];
+    const float4 la1 = a.lut[(size_t)I * kTile + 32 + (lane & 31)];
+    const float4 lb0 = a.lut[(size_t)J * kTile + (lane & 31)];
+    const float4 lb1 = a.lut[(size_t)J * kTile + 32 + (lane & 31)];
+
+    v16f acc[2][2];
+#pragma unroll
+    for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+        for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[bi][bj][r] = 0.0f;
+
+    u32x4 ca0 = pa0[0], ca1 = pa1[0], cb0 = pb0[0], cb1 = pb1[0];
+    for (uint32_t kb = 0; kb < a.nkb; ++kb) {
+        const size_t nk = (size_t)(kb + 1 < a.nkb ? kb + 1 : kb) * 64;
+        const u32x4 na0 = pa0[nk], na1 = pa1[nk], nb0 = pb0[nk], nb1 = pb1[nk];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const uint32_t wa0 = ca0[t], wa1 = ca1[t], wb0 = cb0[t], wb1 = cb1[t];
+#pragma unroll
+            for (int s = 0; s < 16; ++s) {
+                const int sh_bits = 8 * (s & 3) + 2 * (s >> 2);
+                const float fa0 = pick_code(la0, (wa0 >> sh_bits) & 3u);
+                const float fa1 = pick_code(la1, (wa1 >> sh_bits) & 3u);
+                const float fb0 = pick_code(lb0, (wb0 >> sh_bits) & 3u);
+                const float fb1 = pick_code(lb1, (wb1 >> sh_bits) & 3u);
+                acc[0][0] = __builtin_amdgcn_mfma_f32_32x32x2f32(fa0, fb0, acc[0][0], 0, 0, 0);
+                acc[0][1] = __builtin_amdgcn_mfma_f32_32x32x2f32(fa0, fb1, acc[0][1], 0, 0, 0);
+                acc[1][0] = __builtin_amdgcn_mfma_f32_32x32x2f32(fa1, fb0, acc[1][0], 0, 0, 0);
+                acc[1][1] = __builtin_amdgcn_mfma_f32_32x32x2f32(fa1, fb1, acc[1][1], 0, 0, 0);
+            }
+        }
+        ca0 = na0;
+        ca1 = na1;
+        cb0 = nb0;
+        cb1 = nb1;
+    }
+
+    stage_info(a, sh, I, J);
+    __syncthreads();
+    // compute.rs:2064-2077: val (f32 GEMM output) -> f64, val^2 * n_inv^2 * r2u_a + r2u_b
+    double r2u[2][2][16];
+#pragma unroll
+    for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+        for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int row = row_of(bi, r, lane), col = 32 * bj + (lane & 31);
+                const double v = (double)acc[bi][bj][r];
+                const double val = __dadd_rn(__dmul_rn(__dmul_rn(v, v), a.n_inv_sq_r2u_a), a.r2u_b);
+                r2u[bi][bj][r] = pair_included(a, sh, I, J, row, col) ? val : 0.0;
+            }
+    band_contract(a, sh, tile, I, J, r2u);
+}
+
+// ------------------------------------------------------------------------
+// Kernel 4: deterministic per-SNP reduction
+// ------------------------------------------------------------------------
+__global__ void k_reduce(const double* __restrict__ rowp, const double* __restrict__ colp,
+                         const double* __restrict__ aeff, const int* __restrict__ j0,
+                         const int* __restrict__ tstart, uint32_t nb, uint32_t m,
+                         uint32_t m_pad, uint32_t K, double* __restrict__ l2) {
+    const uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= (uint64_t)m * K) return;
+    const uint32_t k = (uint32_t)(gid / m), j = (uint32_t)(gid % m);
+    const uint32_t I = j / kTile, r = j % kTile;
+    double s = aeff[(size_t)k * m_pad + j];  // self pair, r2u(j, j) = 1 (compute.rs:1658)
+    for (int J = j0[I]; J <= (int)I; ++J) {
+        const size_t t = (size_t)tstart[I] + (J - j0[I]);
+        s += rowp[(t * K + k) * kTile + r];
+    }
+    for (uint32_t I2 = I; I2 < nb && (uint32_t)j0[I2] <= I; ++I2) {
+        const size_t t = (size_t)tstart[I2] + (I - j0[I2]);
+        s += colp[(t * K + k) * kTile + r];
+    }
+    l2[(size_t)k * m + j] = s;
+}
+
+// ------------------------------------------------------------------------
+// Decode entry point kernels
+// ------------------------------------------------------------------------
+__global__ void k_decode(const uint8_t* __restrict__ slabs, uint64_t bps,
+                         const int64_t* __restrict__ keep, uint32_t n,
+                         const uint64_t* __restrict__ bed_idx, uint32_t m,
+                         const float4* __restrict__ lut, int8_t* __restrict__ dosage,
+                         uint32_t* __restrict__ miss, float* __restrict__ stdv) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t j = blockIdx.y;
+    if (j >= m) return;
+    const uint8_t* slab = slabs + bed_idx[j] * bps;
+    uint32_t code = 1;
+    if (i < n) {
+        const uint64_t row = keep ? (uint64_t)keep[i] : i;
+        code = (slab[row >> 2] >> (2 * (row & 3))) & 3u;
+        if (dosage) dosage[(size_t)j * n + i] = (int8_t)(code == 0 ? 2 : code == 1 ? -1 : code == 2 ? 1 : 0);
+        if (stdv) stdv[(size_t)j * n + i] = pick_code(lut[j], code);
+    }
+    if (miss) {
+        const uint64_t ball = __ballot(i < n && code == 1);
+        const uint32_t words = (n + 31) / 32;
+        const uint32_t lane = threadIdx.x & 63;
+        const uint32_t w0 = (i - lane) / 32;  // first 32-bit word of this wave
+        if (lane == 0 && w0 < words) miss[(size_t)j * words + w0] = (uint32_t)ball;
+        if (lane == 32 && w0 + 1 < words) miss[(size_t)j * words + w0 + 1] = (uint32_t)(ball >> 32);
+    }
+}
+
+// Per-SNP stats for an arbitrary SNP list without tiling (decode entry).
+__global__ __launch_bounds__(256) void k_stats_only(const uint8_t* __restrict__ slabs, uint64_t bps,
+                                                    const int64_t* __restrict__ keep, uint32_t n,
+                                                    const uint64_t* __restrict__ bed_idx,
+                                                    uint32_t* __restrict__ stats) {
+    __shared__ uint32_t red[4][256];
+    const uint32_t j = blockIdx.x;
+    const uint8_t* slab = slabs + bed_idx[j] * bps;
+    uint32_t c2 = 0, c1 = 0, c0 = 0;
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+        const uint64_t row = keep ? (uint64_t)keep[i] : i;
+        const uint32_t code = (slab[row >> 2] >> (2 * (row & 3))) & 3u;
+        c2 += code == 0;
+        c1 += code == 2;
+        c0 += code == 3;
+    }
+    red[0][threadIdx.x] = 2 * c2 + c1;
+    red[1][threadIdx.x] = c2 + c1 + c0;
+    red[2][threadIdx.x] = 4 * c2 + c1;
+    red[3][threadIdx.x] = c1;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s)
+            for (int q = 0; q < 4; ++q) red[q][threadIdx.x] += red[q][threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x < 4) stats[4 * j + threadIdx.x] = red[threadIdx.x][0];
+}
+
+// ------------------------------------------------------------------------
+// Host orchestration
+// ------------------------------------------------------------------------
+struct BandPlan {
+    uint32_t m_pad = 0, nb = 0, ntiles = 0;
+    std::vector<int2> tiles;
+    std::vector<int> j0, tstart;
+    uint64_t pairs_included = 0;
+};
+
+static void plan_band(const std::vector<uint64_t>& lo, uint64_t m, BandPlan& p) {
+    p.m_pad = (uint32_t)round_up(m, kTile);
+    p.nb = p.m_pad / kTile;
+    p.j0.assign(p.nb, 0);
+    p.tstart.assign(p.nb + 1, 0);
+    p.tiles.clear();
+    for (uint32_t I = 0; I < p.nb; ++I) {
+        const uint64_t first = (uint64_t)I * kTile;
+        const int J0 = (int)(lo[first] / kTile);
+        p.j0[I] = J0;
+        p.tstart[I + 1] = p.tstart[I] + ((int)I - J0 + 1);
+        for (int J = J0; J <= (int)I; ++J) p.tiles.push_back(make_int2((int)I, J));
+    }
+    p.ntiles = (uint32_t)p.tiles.size();
+    p.pairs_included = 0;
+    for (uint64_t j = 0; j < m; ++j) p.pairs_included += j - lo[j];
+}
+
+static double now_s() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch())
+        .count();
+}
+
+size_t l2_workspace_bytes(const ldsc_panel* panel, uint64_t m, const ldsc_l2_config* cfg,
+                          uint64_t K) {
+    const uint64_t m_pad = round_up(m ? m : 1, kTile);
+    const uint64_t n_pad = round_up(panel->n, kKBlock);
+    // band width in tiles from the window, conservatively from the SNP count
+    size_t bytes = m_pad * n_pad / 4;                    // tiled panel slice
+    bytes += m_pad * (16 + 8 + 8 + 16 + 8 + 8) + m_pad * K * 8 * 2;  // per-SNP + aeff + l2
+    const uint64_t tiles_est = (m_pad / kTile) * 16;     // refined by the caller's plan
+    bytes += tiles_est * kTile * K * 8 * 2 + tiles_est * 8;
+    if (cfg && cfg->sketch_d) bytes += m_pad * round_up(cfg->sketch_d, 64) * 4 + panel->n * 16;
+    return bytes;
+}
+
+int run_l2_sketch(const ldsc_panel* panel, const ldsc_snp* snps, uint64_t m,
+                  const ldsc_l2_config* cfg, const std::vector<uint64_t>& lo, const BandPlan& plan,
+                  const double* annot, uint64_t K, double* l2_out, double* maf_out,
+                  ldsc_l2_perf* perf, Workspace* ws, char* err, uint64_t err_len);
+
+// MAF of one SNP from exact integer stats (normalize.rs:71-81).
+static double maf_from_stats(uint32_t S, uint32_t C) {
+    const double avg = C > 0 ? (double)S / (double)C : 0.0;
+    double freq = avg / 2.0;
+    freq = freq < 0.0 ? 0.0 : (freq > 1.0 ? 1.0 : freq);
+    return freq < 1.0 - freq ? freq : 1.0 - freq;
+}
+
+int run_l2(const ldsc_panel* panel, const ldsc_snp* snps, uint64_t m, const ldsc_l2_config* cfg,
+           const double* annot, uint64_t K, double* l2_out, double* maf_out, ldsc_l2_perf* perf,
+           ldsc_progress_cb cb, void* cb_ctx, Workspace* ws, char* err, uint64_t err_len) {
+    const double t_start = now_s();
+    int rc = check_config(cfg, err, err_len);
+    if (rc) return rc;
+    if (!panel) return fail(err, err_len, LDSC_ERR_INVALID_ARG, "panel is NULL");
+    if (K < 1) return fail(err, err_len, LDSC_ERR_ANNOT, "annotation column count must be >= 1");
+    if (!annot && K != 1)
+        return fail(err, err_len, LDSC_ERR_ANNOT, "K must be 1 without an annotation matrix");
+    if (m == 0) {
+        if (perf) {
+            std::memset(perf, 0, sizeof(*perf));
+            perf->n_indiv = panel->n;
+        }
+        return LDSC_OK;
+    }
+    if (!snps || !l2_out) return fail(err, err_len, LDSC_ERR_INVALID_ARG, "NULL SNP or output array");
+    if (m >= (1ull << 31)) return fail(err, err_len, LDSC_ERR_INVALID_ARG, "too many SNPs");
+    for (uint64_t j = 0; j < m; ++j)
+        if (snps[j].bed_idx >= panel->m_bim)
+            return fail(err, err_len, LDSC_ERR_INVALID_ARG,
+                        "SNP %llu has bed_idx %llu beyond the panel's %llu SNPs",
+                        (unsigned long long)j, (unsigned long long)snps[j].bed_idx,
+                        (unsigned long long)panel->m_bim);
+    const uint64_t n = panel->n;
+    if (cfg->sketch_d && !(cfg->sketch_d >= 3 && cfg->sketch_d < n))
+        return fail(err, err_len, LDSC_ERR_SKETCH_RANGE,
+                    "--sketch %llu is out of bounds (must be 3 <= d < n_indiv=%llu)",
+                    (unsigned long long)cfg->sketch_d, (unsigned long long)n);
+
+    std::vector<uint64_t> lo;
+    bool any_full = false;
+    pair_bounds(snps, m, cfg, lo, &any_full);
+    if (!cfg->yes_really && any_full)
+        printf("WARNING: LD window spans the entire chromosome. "
+               "Use --yes-really to silence this warning.\n");
+    BandPlan plan;
+    plan_band(lo, m, plan);
+    const uint32_t m_pad = plan.m_pad;
+
+    HIPC(hipSetDevice(panel->device));
+    hipStream_t st = ws->stream;
+    const double t_prep = now_s();
+
+    // per-SNP host inputs
+    std::vector<uint64_t> bidx(m);
+    for (uint64_t j = 0; j < m; ++j) bidx[j] = snps[j].bed_idx;
+    std::vector<uint32_t> lo32(m_pad);
+    for (uint64_t j = 0; j < m_pad; ++j) lo32[j] = j < m ? (uint32_t)lo[j] : (uint32_t)j;
+
+    if (cfg->sketch_d) {
+        rc = run_l2_sketch(panel, snps, m, cfg, lo, plan, annot, K, l2_out, maf_out, perf, ws,
+                           err, err_len);
+        if (rc == LDSC_OK && perf) perf->total_secs = now_s() - t_start;
+        if (rc == LDSC_OK && cb) {
+            ldsc_l2_progress pr{(m + cfg->chunk_size - 1) / cfg->chunk_size,
+                                (m + cfg->chunk_size - 1) / cfg->chunk_size, m, m,
+                                (now_s() - t_start) * 1e3};
+            cb(&pr, cb_ctx);
+        }
+        return rc;
+    }
+
+    const uint64_t n_pad = round_up(n, kKBlock);
+    const uint32_t nkb = (uint32_t)(n_pad / kKBlock);
+    const uint32_t ngroups = m_pad / kGroup;
+    HIPC(ws->tiled.ensure((size_t)m_pad * n_pad / 4));
+    HIPC(ws->stats.ensure((size_t)m_pad * 16));
+    HIPC(ws->bed_idx.ensure((size_t)m * 8));
+    HIPC(ws->lo.ensure((size_t)m_pad * 4));
+    HIPC(ws->info.ensure((size_t)m_pad * (8 + 8 + 16)));
+    HIPC(ws->aeff.ensure((size_t)m_pad * K * 8));
+    HIPC(ws->tiles.ensure((size_t)plan.ntiles * 8));
+    HIPC(ws->j0.ensure((size_t)plan.nb * 4));
+    HIPC(ws->tstart.ensure((size_t)(plan.nb + 1) * 4));
+    HIPC(ws->rowp.ensure((size_t)plan.ntiles * K * kTile * 8));
+    HIPC(ws->colp.ensure((size_t)plan.ntiles * K * kTile * 8));
+    HIPC(ws->l2.ensure((size_t)m * K * 8));
+
+    int2* d_cs = ws->info.as<int2>();
+    double* d_rs = reinterpret_cast<double*>(ws->info.as<char>() + (size_t)m_pad * 8);
+    float4* d_lut = reinterpret_cast<float4*>(ws->info.as<char>() + (size_t)m_pad * 16);
+
+    HIPC(hipMemcpyAsync(ws->bed_idx.p, bidx.data(), m * 8, hipMemcpyHostToDevice, st));
+    HIPC(hipMemcpyAsync(ws->lo.p, lo32.data(), (size_t)m_pad * 4, hipMemcpyHostToDevice, st));
+    HIPC(hipMemcpyAsync(ws->tiles.p, plan.tiles.data(), (size_t)plan.ntiles * 8,
+                        hipMemcpyHostToDevice, st));
+    HIPC(hipMemcpyAsync(ws->j0.p, plan.j0.data(), (size_t)plan.nb * 4, hipMemcpyHostToDevice, st));
+    HIPC(hipMemcpyAsync(ws->tstart.p, plan.tstart.data(), (size_t)(plan.nb + 1) * 4,
+                        hipMemcpyHostToDevice, st));
+
+    HIPC(hipEventRecord(ws->ev[0], st));
+    hipLaunchKernelGGL(k_tile_stats, dim3(ngroups), dim3(256), 0, st, panel->d_slabs, panel->bps,
+                       panel->d_keep, (uint32_t)n, ws->bed_idx.as<uint64_t>(), (uint32_t)m, nkb,
+                       ws->tiled.as<uint32_t>(), ws->stats.as<uint32_t>());
+    HIPC(hipGetLastError());
+    hipLaunchKernelGGL(k_snp_info, dim3((m_pad + 255) / 256), dim3(256), 0, st,
+                       ws->stats.as<uint32_t>(), m_pad, (uint32_t)n, d_cs, d_rs, d_lut);
+    HIPC(hipGetLastError());
+    HIPC(hipEventRecord(ws->ev[1], st));
+
+    // MAF (and pq-scaled annotations) on the host from the exact integer stats.
+    const bool use_f32 = cfg->precision == LDSC_PREC_FP32;
+    const bool has_pq = !std::isnan(cfg->pq_exp);
+    std::vector<uint32_t> hstats;
+    if (maf_out || has_pq) {
+        hstats.resize((size_t)m_pad * 4);
+        HIPC(hipMemcpyAsync(hstats.data(), ws->stats.p, (size_t)m_pad * 16, hipMemcpyDeviceToHost, st));
+        HIPC(hipStreamSynchronize(st));
+    }
+    std::vector<double> maf(hstats.empty() ? 0 : m);
+    for (uint64_t j = 0; j < maf.size(); ++j) {
+        double v = maf_from_stats(hstats[4 * j], hstats[4 * j + 1]);
+        maf[j] = use_f32 ? (double)(float)v : v;  // normalize.rs:153 returns (f32)maf
+    }
+    if (annot || has_pq) {
+        std::vector<double> ae((size_t)m_pad * K, 0.0);
+        for (uint64_t k = 0; k < K; ++k)
+            for (uint64_t j = 0; j < m; ++j) {
+                double a = annot ? annot[k * m + j] : 1.0;
+                if (std::isnan(a)) a = 0.0;  // read_annot maps NaN to 0 (parse.rs:224)
+                if (has_pq) a *= std::pow(maf[j] * (1.0 - maf[j]), cfg->pq_exp);
+                ae[k * m_pad + j] = a;
+            }
+        HIPC(hipMemcpyAsync(ws->aeff.p, ae.data(), ae.size() * 8, hipMemcpyHostToDevice, st));
+        HIPC(hipStreamSynchronize(st));  // `ae` is pageable and local
+    } else {
+        hipLaunchKernelGGL(k_fill_ones, dim3((m_pad + 255) / 256), dim3(256), 0, st,
+                           ws->aeff.as<double>(), (uint32_t)m, m_pad);
+        HIPC(hipGetLastError());
+    }
+
+    BandArgs ba{};
+    ba.tiled = ws->tiled.as<u32x4>();
+    ba.tiles = ws->tiles.as<int2>();
+    ba.lo = ws->lo.as<uint32_t>();
+    ba.cs = d_cs;
+    ba.rs = d_rs;
+    ba.lut = d_lut;
+    ba.aeff = ws->aeff.as<double>();
+    ba.rowp = ws->rowp.as<double>();
+    ba.colp = ws->colp.as<double>();
+    ba.nkb = nkb;
+    ba.ntiles = plan.ntiles;
+    ba.per_xcd = (plan.ntiles + 7) / 8;
+    ba.m = (uint32_t)m;
+    ba.m_pad = m_pad;
+    ba.K = (uint32_t)K;
+    const double nd = (double)n, n_inv = 1.0 / nd;
+    const double denom = n > 2 ? nd - 2.0 : nd;
+    ba.r2u_a = 1.0 + 1.0 / denom;
+    ba.r2u_b = -1.0 / denom;
+    ba.n_inv_sq_r2u_a = n_inv * n_inv * ba.r2u_a;
+
+    HIPC(hipEventRecord(ws->ev[2], st));
+    if (use_f32)
+        hipLaunchKernelGGL(k_band_f32, dim3(8 * ba.per_xcd), dim3(64), 0, st, ba);
+    else
+        hipLaunchKernelGGL(k_band_i8, dim3(8 * ba.per_xcd), dim3(64), 0, st, ba);
+    HIPC(hipGetLastError());
+    HIPC(hipEventRecord(ws->ev[3], st));
+    const uint64_t nred = m * K;
+    hipLaunchKernelGGL(k_reduce, dim3((unsigned)((nred + 255) / 256)), dim3(256), 0, st,
+                       ws->rowp.as<double>(), ws->colp.as<double>(), ws->aeff.as<double>(),
+                       ws->j0.as<int>(), ws->tstart.as<int>(), plan.nb, (uint32_t)m, m_pad,
+                       (uint32_t)K, ws->l2.as<double>());
+    HIPC(hipGetLastError());
+    HIPC(hipEventRecord(ws->ev[4], st));
+    HIPC(hipMemcpyAsync(l2_out, ws->l2.p, (size_t)m * K * 8, hipMemcpyDeviceToHost, st));
+    HIPC(hipStreamSynchronize(st));
+    if (maf_out) std::memcpy(maf_out, maf.data(), m * 8);
+
+    if (perf) {
+        float t01 = 0, t23 = 0, t34 = 0;
+        (void)hipEventElapsedTime(&t01, ws->ev[0], ws->ev[1]);
+        (void)hipEventElapsedTime(&t23, ws->ev[2], ws->ev[3]);
+        (void)hipEventElapsedTime(&t34, ws->ev[3], ws->ev[4]);
+        std::memset(perf, 0, sizeof(*perf));
+        perf->m = m;
+        perf->n_indiv = n;
+        perf->bed_read_secs = t_prep - t_start;
+        perf->norm_secs = t01 * 1e-3;
+        perf->bb_dot_secs = t23 * 1e-3;
+        perf->ring_store_secs = t34 * 1e-3;
+        perf->total_secs = now_s() - t_start;
+        perf->pairs_included = plan.pairs_included;
+        perf->pairs_evaluated = (uint64_t)plan.ntiles * kTile * kTile;
+        perf->tiles = plan.ntiles;
+        perf->any_full_chr_window = any_full ? 1 : 0;
+    }
+    if (cb) {
+        const uint64_t chunks = (m + cfg->chunk_size - 1) / cfg->chunk_size;
+        ldsc_l2_progress pr{chunks, chunks, m, m, (now_s() - t_start) * 1e3};
+        cb(&pr, cb_ctx);
+    }
+    return LDSC_OK;
+}
+
+}  // namespace ldsc
+
+using namespace ldsc;
+
+// ------------------------------------------------------------------------
+// C ABI: panels
+// ------------------------------------------------------------------------
+extern "C" int ldsc_l2_open_panel(const uint8_t* bed, uint64_t bed_len, uint64_t n_fam,
+                                  uint64_t m_bim, const int64_t* keep_idx, uint64_t n_keep,
+                                  ldsc_panel** out, char* err, uint64_t err_len) {
+    if (!out) return fail(err, err_len, LDSC_ERR_INVALID_ARG, "out is NULL");
+    *out = nullptr;
+    if (!bed && bed_len) return fail(err, err_len, LDSC_ERR_INVALID_ARG, "bed bytes are NULL");
+    // Bed::from_source validation (src/bed.rs:76-109)
+    if (bed_len < 3) return fail(err, err_len, LDSC_ERR_BAD_BED, "BED too short (%lluB)", (unsigned long long)bed_len);
+    if (bed[0] != 0x6c || bed[1] != 0x1b) return fail(err, err_len, LDSC_ERR_BAD_BED, "BED has invalid magic bytes");
+    if (bed[2] != 0x01) return fail(err, err_len, LDSC_ERR_BAD_BED, "BED is not SNP-major (mode=1 required)");
+    const uint64_t bps = (n_fam + 3) / 4;
+    const uint64_t expect = bps * m_bim + 3;
+    if (bed_len != expect)
+        return fail(err, err_len, LDSC_ERR_BAD_BED, "BED has invalid length (expected %llu, got %llu)",
+                    (unsigned long long)expect, (unsigned long long)bed_len);
+    // compute_l2_from_bytes (src/l2/mod.rs:1062-1069)
+    if (m_bim == 0) return fail(err, err_len, LDSC_ERR_EMPTY, "BIM has zero SNPs");
+    if (n_fam == 0) return fail(err, err_len, LDSC_ERR_EMPTY, "FAM has zero individuals");
+    if (keep_idx && n_keep == 0) return fail(err, err_len, LDSC_ERR_EMPTY, "--keep selects zero individuals");
+    const uint64_t n = keep_idx ? n_keep : n_fam;
+    if (n >= (1ull << 31)) return fail(err, err_len, LDSC_ERR_INVALID_ARG, "too many individuals");
+    if (keep_idx)
+        for (uint64_t i = 0; i < n_keep; ++i)
+            if (keep_idx[i] < 0 || (uint64_t)keep_idx[i] >= n_fam)
+                return fail(err, err_len, LDSC_ERR_INVALID_ARG, "--keep index %lld out of range",
+                            (long long)keep_idx[i]);
+    auto* p = new ldsc_panel();
+    p->n_fam = n_fam;
+    p->m_bim = m_bim;
+    p->bps = bps;
+    p->n = n;
+    hipError_t e = hipGetDevice(&p->device);
+    if (e == hipSuccess) e = hipMalloc(&p->d_slabs, bps * m_bim);
+    if (e == hipSuccess) e = hipMemcpy(p->d_slabs, bed + 3, bps * m_bim, hipMemcpyHostToDevice);
+    if (e == hipSuccess && keep_idx) {
+        p->keep.assign(keep_idx, keep_idx + n_keep);
+        e = hipMalloc(&p->d_keep, n_keep * 8);
+        if (e == hipSuccess) e = hipMemcpy(p->d_keep, keep_idx, n_keep * 8, hipMemcpyHostToDevice);
+    }
+    if (e != hipSuccess) {
+        ldsc_l2_close_panel(p);
+        return fail(err, err_len, e == hipErrorOutOfMemory ? LDSC_ERR_OOM : LDSC_ERR_DEVICE,
+                    "uploading the panel failed: %s", hipGetErrorString(e));
+    }
+    *out = p;
+    return LDSC_OK;
+}
+
+extern "C" int ldsc_l2_close_panel(ldsc_panel* p) {
+    if (!p) return LDSC_OK;
+    if (p->d_slabs) (void)hipFree(p->d_slabs);
+    if (p->d_keep) (void)hipFree(p->d_keep);
+    delete p;
+    return LDSC_OK;
+}
+
+extern "C" uint64_t ldsc_l2_panel_n_indiv(const ldsc_panel* p) { return p ? p->n : 0; }
+
+extern "C" int ldsc_l2_compute(const ldsc_panel* panel, const ldsc_snp* snps, uint64_t m,
+                               const ldsc_l2_config* cfg, const double* annot, uint64_t K,
+                               double* l2_out, double* maf_out, ldsc_l2_perf* perf,
+                               ldsc_progress_cb cb, void* cb_ctx, char* err, uint64_t err_len) {
+    if (!panel) return fail(err, err_len, LDSC_ERR_INVALID_ARG, "panel is NULL");
+    Workspace* ws = acquire_workspace(panel->device);
+    if (!ws) return fail(err, err_len, LDSC_ERR_DEVICE, "could not create a device workspace");
+    int rc = run_l2(panel, snps, m, cfg, annot, K, l2_out, maf_out, perf, cb, cb_ctx, ws, err, err_len);
+    release_workspace(ws);
+    return rc;
+}
+
+extern "C" int ldsc_l2_decode(const ldsc_panel* panel, const uint64_t* bed_idx, uint64_t m,
+                              int8_t* dosage, uint32_t* miss, uint32_t* stats, float* std_f32,
+                              char* err, uint64_t err_len) {
+    if (!panel) return fail(err, err_len, LDSC_ERR_INVALID_ARG, "panel is NULL");
+    if (m == 0) return LDSC_OK;
+    if (!bed_idx) return fail(err, err_len, LDSC_ERR_INVALID_ARG, "bed_idx is NULL");
+    for (uint64_t j = 0; j < m; ++j)
+        if (bed_idx[j] >= panel->m_bim)
+            return fail(err, err_len, LDSC_ERR_INVALID_ARG, "bed_idx %llu out of range",
+                        (unsigned long long)bed_idx[j]);
+    if (m > 65535) return fail(err, err_len, LDSC_ERR_INVALID_ARG, "decode at most 65535 SNPs per call");
+    Workspace* ws = acquire_workspace(panel->device);
+    if (!ws) return fail(err, err_len, LDSC_ERR_DEVICE, "could not create a device workspace");
+    const uint64_t n = panel->n;
+    const uint32_t words = (uint32_t)((n + 31) / 32);
+    int rc = LDSC_OK;
+    do {
+        hipStream_t st = ws->stream;
+        hipError_t e;
+#define DCHK(x) if ((e = (x)) != hipSuccess) { rc = fail(err, err_len, LDSC_ERR_DEVICE, "%s: %s", #x, hipGetErrorString(e)); break; }
+        DCHK(hipSetDevice(panel->device));
+        DCHK(ws->bed_idx.ensure(m * 8));
+        DCHK(ws->stats.ensure(m * 16));
+        DCHK(ws->info.ensure(m * 32));
+        DCHK(ws->scratch.ensure((dosage ? n * m : 0) + (miss ? (size_t)words * 4 * m : 0) +
+                                (std_f32 ? n * m * 4 : 0) + 256));
+        DCHK(hipMemcpyAsync(ws->bed_idx.p, bed_idx, m * 8, hipMemcpyHostToDevice, st));
+        hipLaunchKernelGGL(k_stats_only, dim3((unsigned)m), dim3(256), 0, st, panel->d_slabs,
+                           panel->bps, panel->d_keep, (uint32_t)n, ws->bed_idx.as<uint64_t>(),
+                           ws->stats.as<uint32_t>());
+        DCHK(hipGetLastError());
+        int2* d_cs = ws->info.as<int2>();
+        double* d_rs = reinterpret_cast<double*>(ws->info.as<char>() + m * 8);
+        float4* d_lut = reinterpret_cast<float4*>(ws->info.as<char>() + m * 16);
+        hipLaunchKernelGGL(k_snp_info, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, st,
+                           ws->stats.as<uint32_t>(), (uint32_t)m, (uint32_t)n, d_cs, d_rs, d_lut);
+        DCHK(hipGetLastError());
+        char* sp = ws->scratch.as<char>();
+        int8_t* d_dos = dosage ? reinterpret_cast<int8_t*>(sp) : nullptr;
+        sp += dosage ? n * m : 0;
+        sp = reinterpret_cast<char*>(round_up((uintptr_t)sp, 16));
+        uint32_t* d_miss = miss ? reinterpret_cast<uint32_t*>(sp) : nullptr;
+        sp += miss ? (size_t)words * 4 * m : 0;
+        sp = reinterpret_cast<char*>(round_up((uintptr_t)sp, 16));
+        float* d_std = std_f32 ? reinterpret_cast<float*>(sp) : nullptr;
+        if (dosage || miss || std_f32) {
+            const uint32_t gx = (uint32_t)((round_up(n, 64)) / 64);
+            hipLaunchKernelGGL(k_decode, dim3(gx, (unsigned)m), dim3(64), 0, st, panel->d_slabs,
+                               panel->bps, panel->d_keep, (uint32_t)n, ws->bed_idx.as<uint64_t>(),
+                               (uint32_t)m, d_lut, d_dos, d_miss, d_std);
+            DCHK(hipGetLastError());
+        }
+        if (stats) DCHK(hipMemcpyAsync(stats, ws->stats.p, m * 16, hipMemcpyDeviceToHost, st));
+        if (dosage) DCHK(hipMemcpyAsync(dosage, d_dos, n * m, hipMemcpyDeviceToHost, st));
+        if (miss) DCHK(hipMemcpyAsync(miss, d_miss, (size_t)words * 4 * m, hipMemcpyDeviceToHost, st));
+        if (std_f32) DCHK(hipMemcpyAsync(std_f32, d_std, n * m * 4, hipMemcpyDeviceToHost, st));
+        DCHK(hipStreamSynchronize(st));
+#undef DCHK
+    } while (0);
+    release_workspace(ws);
+    return rc;
+}

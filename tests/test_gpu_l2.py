@@ -1,0 +1,51 @@
+"""Device LD scores against the CPU restatement (both arithmetic forms)."""
+import numpy as np
+import pytest
+
+import ldsc_amd as L
+from ldsc_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+# int8 products are exact and finished in FP64: agreement with the FP64
+# restatement is to rounding.  The FP32 form is compared with the f32
+# restatement of --fast-f32 (different summation order).
+TOL_EXACT = 1e-9
+TOL_F32 = 2e-3
+
+
+def _rel(a, b):
+    return float(np.max(np.abs(a - b) / np.maximum(1.0, np.abs(b))))
+
+
+def test_reference_smoke_case():
+    """compute_l2_from_bytes_smoke (src/l2/mod.rs:1191-1225): l2 = [2, 2]."""
+    bed = bytes([0x6C, 0x1B, 0x01, 0xF0, 0xF0])
+    bim = "1\trs1\t0\t100\tA\tG\n1\trs2\t0\t200\tA\tG\n"
+    fam = "".join(f"F{i}\tI{i}\t0\t0\t1\t-9\n" for i in range(1, 5))
+    for use_f32 in (False, True):
+        cfg = L.L2Config(mode=L.WindowMode.Kb(100.0), chunk_size=2, yes_really=True,
+                         use_f32=use_f32)
+        out = L.compute_l2_from_bytes(bed, bim, fam, cfg)
+        assert np.allclose(out.l2, [2.0, 2.0], atol=1e-9)
+        assert np.allclose(out.maf, [0.5, 0.5], atol=1e-9)
+
+
+@pytest.mark.parametrize("mode,window,masking,chunk", [
+    ("cm", 0.3, True, 50), ("cm", 0.3, False, 50), ("kb", 250.0, False, 37),
+    ("snp", 90, True, 64), ("snp", 90, False, 200),
+])
+@pytest.mark.parametrize("use_f32", [False, True])
+def test_l2_matches_oracle(oracle, mode, window, masking, chunk, use_f32):
+    p = synth.make_panel(n=517, m=1500, seed=11, miss_rate=0.02)
+    md = {"cm": L.WindowMode.Cm, "kb": L.WindowMode.Kb, "snp": L.WindowMode.Snp}[mode](window)
+    om = {"cm": oracle.WINDOW_CM, "kb": oracle.WINDOW_KB, "snp": oracle.WINDOW_SNP}[mode]
+    snps = L.make_snp_array(p.chr, p.cm, p.bp)
+    cfg = L.L2Config(mode=md, chunk_size=chunk, use_f32=use_f32, snp_level_masking=masking)
+    print(f"case mode={mode} masking={masking} chunk={chunk} f32={use_f32}", flush=True)
+    with L.Panel(p.bed_bytes(), p.n, p.m) as panel:
+        got, maf, perf = L.compute_ldscore(panel, snps, cfg)
+    exp, emaf = oracle.l2(p.bed_bytes(), p.n, p.chr, p.cm, p.bp, mode=om, window=window,
+                          chunk=chunk, use_f32=use_f32, masking=masking)
+    assert _rel(got, exp) < (TOL_F32 if use_f32 else TOL_EXACT)
+    assert np.array_equal(maf, emaf)
