@@ -27,6 +27,20 @@ INT8_PEAK_TOPS = 5000.0    # MI355X dense int8 MFMA, TOP/s
 FP32_PEAK_TFLOPS = 157.3   # MI355X FP32 matrix, TFLOP/s
 
 
+def profiled_traffic(kernel: str):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 summary
+    (FETCH_SIZE/WRITE_SIZE scaled by a copy-kernel calibration), or None."""
+    path = os.path.join(ROOT, "profiles", "k_band_i8_c2.md")
+    try:
+        for line in open(path):
+            parts = line.split()
+            if len(parts) == 3 and parts[0] == "traffic_bytes_per_launch" and parts[1] == kernel:
+                return int(parts[2])
+    except OSError:
+        pass
+    return None
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -42,7 +56,21 @@ def main() -> int:
     ap.add_argument("--cpu-sample-snps", type=int, default=6000)
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--calib-copy-mib", type=int, default=0,
+                    help="only run a device-to-device copy of this many MiB (torch) to "
+                         "calibrate HBM byte counters, then exit")
     args = ap.parse_args()
+    if args.calib_copy_mib:
+        import torch
+
+        n = args.calib_copy_mib * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"calib_copy_bytes_each_way": n * 4}), flush=True)
+        return 0
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the LD-score "
               "engine has no multi-GPU path in this build.", file=sys.stderr)
@@ -149,7 +177,7 @@ def main() -> int:
         "ld_scored_snps_per_s": p.m / t_step,
         "roofline": {"bound": "mfma", "kernel": "k_band_i8", "achieved": achieved_tops,
                      "peak": INT8_PEAK_TOPS, "unit": "TFLOP/s", "frac": achieved_tops / INT8_PEAK_TOPS,
-                     "traffic": None, "band_kernel_ms": k8m * 1e3,
+                     "traffic": profiled_traffic("k_band_i8"), "band_kernel_ms": k8m * 1e3,
                      "ops_per_launch": algo_ops},
         "cpu_baseline": cpu,
         "sample_accuracy": sample_err,
