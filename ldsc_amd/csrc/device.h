@@ -54,19 +54,40 @@ void release_workspace(Workspace* ws);
 
 }  // namespace ldsc
 
+namespace ldsc {
+// Device-side CountSketch plan for one sketch dimension d (see sketch.hip).
+struct SketchPlan {
+    uint64_t d = 0, d_pad = 0, nblocks = 0, ntiles = 0;
+    int32_t* d_fam = nullptr;   // FAM row of member k (members sorted by bucket)
+    void* d_ptiles = nullptr;   // ntiles x 1 KiB signed int8 MFMA A-operand tiles
+    int4* d_blocks = nullptr;   // per bucket block: first tile, first K-step, K-steps
+    void release();
+};
+}  // namespace ldsc
+
 struct ldsc_panel {
     int device = 0;
     uint64_t n_fam = 0, m_bim = 0, bps = 0, n = 0;
     uint8_t* d_slabs = nullptr;  // m_bim x bps bytes, the .bed body
     int64_t* d_keep = nullptr;   // n FAM rows, or nullptr (all individuals)
+    int32_t* d_keep32 = nullptr; // the same rows as int32 (gather map)
     std::vector<int64_t> keep;
+    std::mutex sketch_mu;
+    std::vector<ldsc::SketchPlan*> sketch_plans;
 };
 
 namespace ldsc {
 
-// The exact / sketch engine.  `ws` must belong to panel->device.  When
-// `sync` is false the call returns after enqueueing and copying back happens
-// on ws->stream (the caller synchronizes the stream before reading outputs).
+// Tiles the call's SNP rows into ws->tiled and counts their integer stats
+// into ws->stats.  `d_map` (nullable) lists, for each of the panel's N
+// individuals in the order wanted, its FAM row (--keep order, or the
+// CountSketch bucket order); it is applied by an LDS-staged row gather.
+int gather_and_tile(const ldsc_panel* panel, const uint64_t* d_bed_idx, uint64_t m,
+                    const int32_t* d_map, uint64_t m_pad, Workspace* ws, char* err,
+                    uint64_t err_len);
+
+// The exact / sketch engine on workspace `ws` (which must belong to
+// panel->device).  Synchronous: outputs are written when it returns.
 int run_l2(const ldsc_panel* panel, const ldsc_snp* snps, uint64_t m, const ldsc_l2_config* cfg,
            const double* annot, uint64_t K, double* l2_out, double* maf_out,
            ldsc_l2_perf* perf, ldsc_progress_cb cb, void* cb_ctx, Workspace* ws, char* err,

@@ -23,7 +23,7 @@
 #include <cmath>
 #include <cstring>
 
-#include "device.h"
+#include "band_common.h"
 
 namespace ldsc {
 
@@ -36,10 +36,6 @@ namespace ldsc {
                         "%s failed: %s", #expr, hipGetErrorString(e_));                    \
     } while (0)
 
-typedef int v4i __attribute__((ext_vector_type(4)));
-typedef int v16i __attribute__((ext_vector_type(16)));
-typedef float v16f __attribute__((ext_vector_type(16)));
-typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
 
 // ------------------------------------------------------------------------
 // Buffers and workspaces
@@ -126,15 +122,60 @@ void release_workspace(Workspace* w) {
 }
 
 // ------------------------------------------------------------------------
-// Kernel 1: gather + tile + integer stats
+// Kernel 1a: row gather in a chosen individual order (--keep, bucket order)
+// ------------------------------------------------------------------------
+// One SNP row at a time is staged in LDS (coalesced loads), then each thread
+// assembles 16 consecutive members of the output row from random LDS reads
+// and writes one dword.  Output rows are `bps_out` bytes (a multiple of 4);
+// members at or beyond `n` get code 01 (missing).
+__global__ __launch_bounds__(256) void k_permute_rows(const uint8_t* __restrict__ slabs,
+                                                       uint64_t bps_in,
+                                                       const uint64_t* __restrict__ bed_idx,
+                                                       uint32_t m, const int32_t* __restrict__ map,
+                                                       uint32_t n, uint64_t bps_out,
+                                                       uint8_t* __restrict__ out) {
+    extern __shared__ uint8_t row[];
+    const int tid = threadIdx.x;
+    const bool aligned = (bps_in & 3) == 0;
+    for (uint32_t j = blockIdx.x; j < m; j += gridDim.x) {
+        const uint8_t* src = slabs + bed_idx[j] * bps_in;
+        __syncthreads();
+        if (aligned) {
+            const uint32_t* s4 = reinterpret_cast<const uint32_t*>(src);
+            uint32_t* r4 = reinterpret_cast<uint32_t*>(row);
+            for (uint64_t q = tid; q < bps_in / 4; q += blockDim.x) r4[q] = s4[q];
+        } else {
+            for (uint64_t q = tid; q < bps_in; q += blockDim.x) row[q] = src[q];
+        }
+        __syncthreads();
+        uint32_t* dst = reinterpret_cast<uint32_t*>(out + (uint64_t)j * bps_out);
+        for (uint64_t q = tid; q < bps_out / 4; q += blockDim.x) {
+            uint32_t w = 0;
+            const uint64_t k0 = q * 16;
+#pragma unroll 4
+            for (uint32_t e = 0; e < 16; ++e) {
+                uint32_t code = 1u;
+                if (k0 + e < n) {
+                    const uint32_t r = (uint32_t)map[k0 + e];
+                    code = (row[r >> 2] >> (2 * (r & 3))) & 3u;
+                }
+                w |= code << (2 * e);
+            }
+            dst[q] = w;
+        }
+    }
+}
+
+// ------------------------------------------------------------------------
+// Kernel 1b: tile + integer stats
 // ------------------------------------------------------------------------
 // Each workgroup owns one group of 32 SNPs and walks all K-blocks.  Thread
 // tau writes dword (lane = tau/4, t = tau%4) of each 1 KiB tile, so a
-// workgroup writes whole tiles with coalesced stores.
+// workgroup writes whole tiles with coalesced stores.  Row j of the source
+// is slabs + (bed_idx ? bed_idx[j] : j) * bps.
 __global__ __launch_bounds__(256) void k_tile_stats(const uint8_t* __restrict__ slabs,
-                                                     uint64_t bps,
-                                                     const int64_t* __restrict__ keep,
-                                                     uint32_t n, const uint64_t* __restrict__ bed_idx,
+                                                     uint64_t bps, uint32_t n,
+                                                     const uint64_t* __restrict__ bed_idx,
                                                      uint32_t m, uint32_t nkb,
                                                      uint32_t* __restrict__ tiled,
                                                      uint32_t* __restrict__ stats) {
@@ -147,7 +188,7 @@ __global__ __launch_bounds__(256) void k_tile_stats(const uint8_t* __restrict__ 
     const uint32_t snp = g * kGroup + (lane & 31);
     const uint32_t h = (uint32_t)lane >> 5;
     const bool valid = snp < m;
-    const uint8_t* slab = valid ? slabs + bed_idx[snp] * bps : nullptr;
+    const uint8_t* slab = valid ? slabs + (bed_idx ? bed_idx[snp] : snp) * bps : nullptr;
     const bool aligned = (bps & 3) == 0;
     uint32_t n2 = 0, n1 = 0, n0 = 0;
     uint32_t* out = tiled + ((size_t)g * nkb * 64 + lane) * 4 + t;
@@ -157,19 +198,12 @@ __global__ __launch_bounds__(256) void k_tile_stats(const uint8_t* __restrict__ 
         if (valid && base < n) {
             const uint32_t nval = n - base < 16 ? n - base : 16;
             uint32_t raw = 0;
-            if (!keep) {
-                const uint64_t b0 = base >> 2;
-                if (aligned && b0 + 4 <= bps) {
-                    raw = *reinterpret_cast<const uint32_t*>(slab + b0);
-                } else {
-                    for (int q = 0; q < 4; ++q)
-                        if (b0 + q < bps) raw |= (uint32_t)slab[b0 + q] << (8 * q);
-                }
+            const uint64_t b0 = base >> 2;
+            if (aligned && b0 + 4 <= bps) {
+                raw = *reinterpret_cast<const uint32_t*>(slab + b0);
             } else {
-                for (uint32_t e = 0; e < nval; ++e) {
-                    const uint64_t row = (uint64_t)keep[base + e];
-                    raw |= (uint32_t)((slab[row >> 2] >> (2 * (row & 3))) & 3u) << (2 * e);
-                }
+                for (int q = 0; q < 4; ++q)
+                    if (b0 + q < bps) raw |= (uint32_t)slab[b0 + q] << (8 * q);
             }
             if (nval < 16) {
                 const uint32_t keepmask = (1u << (2 * nval)) - 1u;
@@ -204,10 +238,10 @@ __global__ __launch_bounds__(256) void k_tile_stats(const uint8_t* __restrict__ 
 // int8 form: cs = (C, S), rs = 1/sqrt(C*Q), Q = C*SS - S^2 (exact integers;
 // C*Q = count^2 * centered_sum_sq).  f32 form: the 4-entry LUT of
 // normalize_col_f32_with_stats (normalize.rs:115-154) indexed by the PLINK
-// code: [(2-mean)inv, 0, (1-mean)inv, -mean*inv] in f32.
+// code: [(2-mean)inv, 0, (1-mean)inv, (0-mean)inv] in f32.
 __global__ void k_snp_info(const uint32_t* __restrict__ stats, uint32_t m_pad, uint32_t n,
                            int2* __restrict__ cs, double* __restrict__ rs,
-                           float4* __restrict__ lut) {
+                           float4* __restrict__ lut, float2* __restrict__ musc) {
     const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= m_pad) return;
     const uint32_t S = stats[4 * j + 0], C = stats[4 * j + 1], SS = stats[4 * j + 2];
@@ -215,143 +249,23 @@ __global__ void k_snp_info(const uint32_t* __restrict__ stats, uint32_t m_pad, u
     const int64_t Q = (int64_t)C * (int64_t)SS - (int64_t)S * (int64_t)S;
     const int64_t V = (int64_t)C * Q;
     rs[j] = V > 0 ? 1.0 / sqrt((double)V) : 0.0;
+    // same operation order as the reference, no contraction into FMA
     const double avg = C > 0 ? (double)S / (double)C : 0.0;
-    const double css = (double)SS - (double)C * avg * avg;
+    const double css = __dsub_rn((double)SS, __dmul_rn(__dmul_rn((double)C, avg), avg));
     const double sd = sqrt(css / (double)n);
     const float a32 = (float)avg;
     const float i32 = sd > 0.0 ? (float)(1.0 / sd) : 0.0f;
-    lut[j] = make_float4((2.0f - a32) * i32, 0.0f, (1.0f - a32) * i32, -a32 * i32);
+    // (0 - mean) * inv: written so that no backend fold of 0 - x into -x can
+    // turn the reference's +0 (mean 0) into -0; otherwise negation is exact.
+    const float w0 = a32 == 0.0f ? 0.0f : -(a32 * i32);
+    lut[j] = make_float4((2.0f - a32) * i32, 0.0f, (1.0f - a32) * i32, w0);
+    if (musc) musc[j] = make_float2(a32, i32);
 }
 
 // Fill A_eff with ones (no --annot, no pq) for rows < m, zero padding rows.
 __global__ void k_fill_ones(double* __restrict__ a, uint32_t m, uint32_t m_pad) {
     const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j < m_pad) a[j] = j < m ? 1.0 : 0.0;
-}
-
-// ------------------------------------------------------------------------
-// Kernel 3: banded contraction
-// ------------------------------------------------------------------------
-struct BandArgs {
-    const u32x4* tiled;
-    const int2* tiles;
-    const uint32_t* lo;
-    const int2* cs;
-    const double* rs;
-    const float4* lut;
-    const double* aeff;
-    double* rowp;
-    double* colp;
-    uint32_t nkb, ntiles, per_xcd, m, m_pad, K;
-    double r2u_a, r2u_b, n_inv_sq_r2u_a;
-};
-
-// Tile -> XCD mapping: workgroups are dealt to the 8 XCDs round-robin, so
-// workgroup b runs on XCD b%8.  Give each XCD one contiguous stretch of the
-// (I, J)-sorted tile list: neighbouring tiles share SNP groups, which then
-// hit in that XCD's L2.
-__device__ inline int tile_of_block(const BandArgs& a) {
-    const uint32_t xcd = blockIdx.x & 7u, slot = blockIdx.x >> 3;
-    const uint32_t t = xcd * a.per_xcd + slot;
-    return t < a.ntiles ? (int)t : -1;
-}
-
-// Expand one dword of 16 PLINK codes (device.h bit order) into int8 dosage
-// G (00->2, 01->0, 10->1, 11->0) and presence M (01->0, else 1).  Each
-// byte of D holds one code 0..3; v_perm_b32 uses it as a byte selector into
-// the 4-byte table.  Both perm sources carry the table, so selectors 0..3
-// and 4..7 read the same entry.
-__device__ __forceinline__ void expand_codes(uint32_t w, v4i& G, v4i& M) {
-#pragma unroll
-    for (int f = 0; f < 4; ++f) {
-        const uint32_t D = (w >> (2 * f)) & 0x03030303u;
-        G[f] = (int)__builtin_amdgcn_perm(0x00010002u, 0x00010002u, D);
-        M[f] = (int)__builtin_amdgcn_perm(0x01010001u, 0x01010001u, D);
-    }
-}
-
-struct EpiShared {
-    double R[32][65];     // one 32-row half of the tile's masked r2u
-    double ar[64][8];     // A_eff of the tile's row SNPs (chunk of 8 annotations)
-    double ac[64][8];     // A_eff of the tile's column SNPs
-    int2 csr[64], csc[64];
-    double rsr[64], rsc[64];
-    uint32_t lor[64];
-};
-
-__device__ __forceinline__ int row_of(int bi, int r, int lane) {
-    return 32 * bi + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-}
-
-// r2u (masked, 64 per lane in MFMA output order) -> per-tile partial sums.
-//   rowp[tile][k][a] = sum_b R[a][b] * A_eff[col b][k]
-//   colp[tile][k][b] = sum_a R[a][b] * A_eff[row a][k]
-__device__ void band_contract(const BandArgs& a, EpiShared& sh, int tile, uint32_t I, uint32_t J,
-                              const double (&r2u)[2][2][16]) {
-    const int lane = threadIdx.x;
-    for (uint32_t kc0 = 0; kc0 < a.K; kc0 += 8) {
-        const uint32_t kcn = a.K - kc0 < 8 ? a.K - kc0 : 8;
-        __syncthreads();
-        for (uint32_t kk = 0; kk < kcn; ++kk) {
-            sh.ar[lane][kk] = a.aeff[(size_t)(kc0 + kk) * a.m_pad + (size_t)I * kTile + lane];
-            sh.ac[lane][kk] = a.aeff[(size_t)(kc0 + kk) * a.m_pad + (size_t)J * kTile + lane];
-        }
-        double cacc[8];
-#pragma unroll
-        for (int kk = 0; kk < 8; ++kk) cacc[kk] = 0.0;
-#pragma unroll
-        for (int bi = 0; bi < 2; ++bi) {
-            __syncthreads();
-#pragma unroll
-            for (int bj = 0; bj < 2; ++bj)
-#pragma unroll
-                for (int r = 0; r < 16; ++r)
-                    sh.R[row_of(0, r, lane)][32 * bj + (lane & 31)] = r2u[bi][bj][r];
-            __syncthreads();
-            // row role: lane covers row lane%32 over columns of its half
-            double racc[8];
-#pragma unroll
-            for (int kk = 0; kk < 8; ++kk) racc[kk] = 0.0;
-            const int rr = lane & 31, b0 = 32 * (lane >> 5);
-            for (int b = 0; b < 32; ++b) {
-                const double rv = sh.R[rr][b0 + b];
-#pragma unroll
-                for (int kk = 0; kk < 8; ++kk)
-                    if ((uint32_t)kk < kcn) racc[kk] += rv * sh.ac[b0 + b][kk];
-            }
-#pragma unroll
-            for (int kk = 0; kk < 8; ++kk)
-                if ((uint32_t)kk < kcn) racc[kk] += __shfl_xor(racc[kk], 32);
-            if (lane < 32)
-                for (uint32_t kk = 0; kk < kcn; ++kk)
-                    a.rowp[((size_t)tile * a.K + kc0 + kk) * kTile + 32 * bi + lane] = racc[kk];
-            // column role: lane = column, rows of this half
-            for (int r = 0; r < 32; ++r) {
-                const double rv = sh.R[r][lane];
-#pragma unroll
-                for (int kk = 0; kk < 8; ++kk)
-                    if ((uint32_t)kk < kcn) cacc[kk] += rv * sh.ar[32 * bi + r][kk];
-            }
-        }
-        for (uint32_t kk = 0; kk < kcn; ++kk)
-            a.colp[((size_t)tile * a.K + kc0 + kk) * kTile + lane] = cacc[kk];
-    }
-}
-
-__device__ __forceinline__ void stage_info(const BandArgs& a, EpiShared& sh, uint32_t I,
-                                           uint32_t J) {
-    const int lane = threadIdx.x;
-    sh.csr[lane] = a.cs[(size_t)I * kTile + lane];
-    sh.rsr[lane] = a.rs[(size_t)I * kTile + lane];
-    sh.lor[lane] = a.lo[(size_t)I * kTile + lane];
-    sh.csc[lane] = a.cs[(size_t)J * kTile + lane];
-    sh.rsc[lane] = a.rs[(size_t)J * kTile + lane];
-}
-
-__device__ __forceinline__ bool pair_included(const BandArgs& a, const EpiShared& sh,
-                                              uint32_t I, uint32_t J, int row, int col) {
-    const uint32_t jj = I * kTile + row, ii = J * kTile + col;
-    return ii < jj && jj < a.m && ii >= sh.lor[row];
 }
 
 __global__ __launch_bounds__(64) void k_band_i8(BandArgs a) {
@@ -428,12 +342,6 @@ __global__ __launch_bounds__(64) void k_band_i8(BandArgs a) {
                 r2u[bi][bj][r] = pair_included(a, sh, I, J, row, col) ? val : 0.0;
             }
     band_contract(a, sh, tile, I, J, r2u);
-}
-
-__device__ __forceinline__ float pick_code(const float4& L, uint32_t code) {
-    const float lo = (code & 1u) ? L.y : L.x;
-    const float hi = (code & 1u) ? L.w : L.z;
-    return (code & 2u) ? hi : lo;
 }
 
 __global__ __launch_bounds__(64) void k_band_f32(BandArgs a) {
@@ -589,31 +497,6 @@ __global__ __launch_bounds__(256) void k_stats_only(const uint8_t* __restrict__ 
 // ------------------------------------------------------------------------
 // Host orchestration
 // ------------------------------------------------------------------------
-struct BandPlan {
-    uint32_t m_pad = 0, nb = 0, ntiles = 0;
-    std::vector<int2> tiles;
-    std::vector<int> j0, tstart;
-    uint64_t pairs_included = 0;
-};
-
-static void plan_band(const std::vector<uint64_t>& lo, uint64_t m, BandPlan& p) {
-    p.m_pad = (uint32_t)round_up(m, kTile);
-    p.nb = p.m_pad / kTile;
-    p.j0.assign(p.nb, 0);
-    p.tstart.assign(p.nb + 1, 0);
-    p.tiles.clear();
-    for (uint32_t I = 0; I < p.nb; ++I) {
-        const uint64_t first = (uint64_t)I * kTile;
-        const int J0 = (int)(lo[first] / kTile);
-        p.j0[I] = J0;
-        p.tstart[I + 1] = p.tstart[I] + ((int)I - J0 + 1);
-        for (int J = J0; J <= (int)I; ++J) p.tiles.push_back(make_int2((int)I, J));
-    }
-    p.ntiles = (uint32_t)p.tiles.size();
-    p.pairs_included = 0;
-    for (uint64_t j = 0; j < m; ++j) p.pairs_included += j - lo[j];
-}
-
 static double now_s() {
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch())
         .count();
@@ -621,21 +504,24 @@ static double now_s() {
 
 size_t l2_workspace_bytes(const ldsc_panel* panel, uint64_t m, const ldsc_l2_config* cfg,
                           uint64_t K) {
-    const uint64_t m_pad = round_up(m ? m : 1, kTile);
+    if (m == 0) return 0;
+    const uint64_t m_pad = round_up(m, kTile);
     const uint64_t n_pad = round_up(panel->n, kKBlock);
-    // band width in tiles from the window, conservatively from the SNP count
-    size_t bytes = m_pad * n_pad / 4;                    // tiled panel slice
-    bytes += m_pad * (16 + 8 + 8 + 16 + 8 + 8) + m_pad * K * 8 * 2;  // per-SNP + aeff + l2
-    const uint64_t tiles_est = (m_pad / kTile) * 16;     // refined by the caller's plan
-    bytes += tiles_est * kTile * K * 8 * 2 + tiles_est * 8;
-    if (cfg && cfg->sketch_d) bytes += m_pad * round_up(cfg->sketch_d, 64) * 4 + panel->n * 16;
+    const uint64_t nb = m_pad / kTile;
+    // band width of the call, from its own pair bounds would need the SNP
+    // list; bound it by the window in SNPs when the mode gives one, else by a
+    // generous default of 16 tiles per block row
+    uint64_t tiles = nb * 16;
+    if (cfg && cfg->window_mode == LDSC_WINDOW_SNP)
+        tiles = nb * (2 + (uint64_t)(cfg->window_value + cfg->chunk_size) / kTile);
+    size_t bytes = m_pad * n_pad / 4;                           // tiled SNP slice
+    bytes += m_pad * (16 + 8 + 4 + 48) + m_pad * K * 8 + m * K * 8;  // stats, idx, lo, info, aeff, l2
+    bytes += tiles * (kTile * K * 8 * 2 + 8);                   // tile partials + list
+    const bool gather = cfg && (cfg->sketch_d || panel->d_keep32);
+    if (gather) bytes += m * round_up((panel->n + 3) / 4, 4);   // permuted rows
+    if (cfg && cfg->sketch_d) bytes += m_pad * round_up(cfg->sketch_d, 32) * 4 + m_pad * 8;
     return bytes;
 }
-
-int run_l2_sketch(const ldsc_panel* panel, const ldsc_snp* snps, uint64_t m,
-                  const ldsc_l2_config* cfg, const std::vector<uint64_t>& lo, const BandPlan& plan,
-                  const double* annot, uint64_t K, double* l2_out, double* maf_out,
-                  ldsc_l2_perf* perf, Workspace* ws, char* err, uint64_t err_len);
 
 // MAF of one SNP from exact integer stats (normalize.rs:71-81).
 static double maf_from_stats(uint32_t S, uint32_t C) {
@@ -643,6 +529,45 @@ static double maf_from_stats(uint32_t S, uint32_t C) {
     double freq = avg / 2.0;
     freq = freq < 0.0 ? 0.0 : (freq > 1.0 ? 1.0 : freq);
     return freq < 1.0 - freq ? freq : 1.0 - freq;
+}
+
+int gather_and_tile(const ldsc_panel* panel, const uint64_t* d_bed_idx, uint64_t m,
+                    const int32_t* d_map, uint64_t m_pad, Workspace* ws, char* err,
+                    uint64_t err_len) {
+    const uint64_t n = panel->n;
+    const uint64_t n_pad = round_up(n, kKBlock);
+    const uint32_t nkb = (uint32_t)(n_pad / kKBlock);
+    hipStream_t st = ws->stream;
+    HIPC(ws->tiled.ensure((size_t)m_pad * n_pad / 4));
+    HIPC(ws->stats.ensure((size_t)m_pad * 16));
+    const uint8_t* src = panel->d_slabs;
+    uint64_t bps = panel->bps;
+    const uint64_t* rows = d_bed_idx;
+    if (d_map) {
+        // gather rows in the requested individual order, then tile them
+        const uint64_t bps_out = round_up((n + 3) / 4, 4);
+        if (panel->bps > 160 * 1024)
+            return fail(err, err_len, LDSC_ERR_INVALID_ARG,
+                        "individual subsets / sketching support at most 655360 FAM rows");
+        HIPC(ws->scratch.ensure((size_t)m * bps_out));
+        const size_t lds = round_up(panel->bps, 4);
+        if (lds > 64 * 1024)
+            HIPC(hipFuncSetAttribute((const void*)k_permute_rows,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        const uint32_t grid = (uint32_t)std::min<uint64_t>(m, 8192);
+        hipLaunchKernelGGL(k_permute_rows, dim3(grid), dim3(256), lds, st, panel->d_slabs,
+                           panel->bps, d_bed_idx, (uint32_t)m, d_map, (uint32_t)n, bps_out,
+                           ws->scratch.as<uint8_t>());
+        HIPC(hipGetLastError());
+        src = ws->scratch.as<uint8_t>();
+        bps = bps_out;
+        rows = nullptr;
+    }
+    hipLaunchKernelGGL(k_tile_stats, dim3((unsigned)(m_pad / kGroup)), dim3(256), 0, st, src, bps,
+                       (uint32_t)n, rows, (uint32_t)m, nkb, ws->tiled.as<uint32_t>(),
+                       ws->stats.as<uint32_t>());
+    HIPC(hipGetLastError());
+    return LDSC_OK;
 }
 
 int run_l2(const ldsc_panel* panel, const ldsc_snp* snps, uint64_t m, const ldsc_l2_config* cfg,
@@ -655,10 +580,16 @@ int run_l2(const ldsc_panel* panel, const ldsc_snp* snps, uint64_t m, const ldsc
     if (K < 1) return fail(err, err_len, LDSC_ERR_ANNOT, "annotation column count must be >= 1");
     if (!annot && K != 1)
         return fail(err, err_len, LDSC_ERR_ANNOT, "K must be 1 without an annotation matrix");
+    const uint64_t n = panel->n;
+    const uint64_t d = cfg->sketch_d;
+    if (d && !(d >= 3 && d < n))
+        return fail(err, err_len, LDSC_ERR_SKETCH_RANGE,
+                    "--sketch %llu is out of bounds (must be 3 <= d < n_indiv=%llu)",
+                    (unsigned long long)d, (unsigned long long)n);
     if (m == 0) {
         if (perf) {
             std::memset(perf, 0, sizeof(*perf));
-            perf->n_indiv = panel->n;
+            perf->n_indiv = n;
         }
         return LDSC_OK;
     }
@@ -670,11 +601,9 @@ int run_l2(const ldsc_panel* panel, const ldsc_snp* snps, uint64_t m, const ldsc
                         "SNP %llu has bed_idx %llu beyond the panel's %llu SNPs",
                         (unsigned long long)j, (unsigned long long)snps[j].bed_idx,
                         (unsigned long long)panel->m_bim);
-    const uint64_t n = panel->n;
-    if (cfg->sketch_d && !(cfg->sketch_d >= 3 && cfg->sketch_d < n))
-        return fail(err, err_len, LDSC_ERR_SKETCH_RANGE,
-                    "--sketch %llu is out of bounds (must be 3 <= d < n_indiv=%llu)",
-                    (unsigned long long)cfg->sketch_d, (unsigned long long)n);
+    if (d && d > 1000000)
+        return fail(err, err_len, LDSC_ERR_SKETCH_RANGE, "--sketch %llu too large",
+                    (unsigned long long)d);
 
     std::vector<uint64_t> lo;
     bool any_full = false;
@@ -688,35 +617,22 @@ int run_l2(const ldsc_panel* panel, const ldsc_snp* snps, uint64_t m, const ldsc
 
     HIPC(hipSetDevice(panel->device));
     hipStream_t st = ws->stream;
-    const double t_prep = now_s();
 
-    // per-SNP host inputs
+    SketchPlan* sp = nullptr;
+    if (d) {
+        rc = sketch_plan(panel, d, &sp, err, err_len);
+        if (rc) return rc;
+    }
+
     std::vector<uint64_t> bidx(m);
     for (uint64_t j = 0; j < m; ++j) bidx[j] = snps[j].bed_idx;
     std::vector<uint32_t> lo32(m_pad);
     for (uint64_t j = 0; j < m_pad; ++j) lo32[j] = j < m ? (uint32_t)lo[j] : (uint32_t)j;
+    const double t_prep = now_s();
 
-    if (cfg->sketch_d) {
-        rc = run_l2_sketch(panel, snps, m, cfg, lo, plan, annot, K, l2_out, maf_out, perf, ws,
-                           err, err_len);
-        if (rc == LDSC_OK && perf) perf->total_secs = now_s() - t_start;
-        if (rc == LDSC_OK && cb) {
-            ldsc_l2_progress pr{(m + cfg->chunk_size - 1) / cfg->chunk_size,
-                                (m + cfg->chunk_size - 1) / cfg->chunk_size, m, m,
-                                (now_s() - t_start) * 1e3};
-            cb(&pr, cb_ctx);
-        }
-        return rc;
-    }
-
-    const uint64_t n_pad = round_up(n, kKBlock);
-    const uint32_t nkb = (uint32_t)(n_pad / kKBlock);
-    const uint32_t ngroups = m_pad / kGroup;
-    HIPC(ws->tiled.ensure((size_t)m_pad * n_pad / 4));
-    HIPC(ws->stats.ensure((size_t)m_pad * 16));
     HIPC(ws->bed_idx.ensure((size_t)m * 8));
     HIPC(ws->lo.ensure((size_t)m_pad * 4));
-    HIPC(ws->info.ensure((size_t)m_pad * (8 + 8 + 16)));
+    HIPC(ws->info.ensure((size_t)m_pad * 48));
     HIPC(ws->aeff.ensure((size_t)m_pad * K * 8));
     HIPC(ws->tiles.ensure((size_t)plan.ntiles * 8));
     HIPC(ws->j0.ensure((size_t)plan.nb * 4));
@@ -724,10 +640,14 @@ int run_l2(const ldsc_panel* panel, const ldsc_snp* snps, uint64_t m, const ldsc
     HIPC(ws->rowp.ensure((size_t)plan.ntiles * K * kTile * 8));
     HIPC(ws->colp.ensure((size_t)plan.ntiles * K * kTile * 8));
     HIPC(ws->l2.ensure((size_t)m * K * 8));
-
+    if (d) {
+        HIPC(ws->sk_cols.ensure((size_t)m_pad * sp->d_pad * 4));
+        HIPC(ws->maf.ensure((size_t)m_pad * 8));  // kappa (maf-aware sketch)
+    }
     int2* d_cs = ws->info.as<int2>();
     double* d_rs = reinterpret_cast<double*>(ws->info.as<char>() + (size_t)m_pad * 8);
     float4* d_lut = reinterpret_cast<float4*>(ws->info.as<char>() + (size_t)m_pad * 16);
+    float2* d_musc = reinterpret_cast<float2*>(ws->info.as<char>() + (size_t)m_pad * 32);
 
     HIPC(hipMemcpyAsync(ws->bed_idx.p, bidx.data(), m * 8, hipMemcpyHostToDevice, st));
     HIPC(hipMemcpyAsync(ws->lo.p, lo32.data(), (size_t)m_pad * 4, hipMemcpyHostToDevice, st));
@@ -737,29 +657,77 @@ int run_l2(const ldsc_panel* panel, const ldsc_snp* snps, uint64_t m, const ldsc
     HIPC(hipMemcpyAsync(ws->tstart.p, plan.tstart.data(), (size_t)(plan.nb + 1) * 4,
                         hipMemcpyHostToDevice, st));
 
+    // 1. gather + tile + stats (+ CountSketch)
     HIPC(hipEventRecord(ws->ev[0], st));
-    hipLaunchKernelGGL(k_tile_stats, dim3(ngroups), dim3(256), 0, st, panel->d_slabs, panel->bps,
-                       panel->d_keep, (uint32_t)n, ws->bed_idx.as<uint64_t>(), (uint32_t)m, nkb,
-                       ws->tiled.as<uint32_t>(), ws->stats.as<uint32_t>());
-    HIPC(hipGetLastError());
+    rc = gather_and_tile(panel, ws->bed_idx.as<uint64_t>(), m, d ? sp->d_fam : panel->d_keep32,
+                         m_pad, ws, err, err_len);
+    if (rc) return rc;
     hipLaunchKernelGGL(k_snp_info, dim3((m_pad + 255) / 256), dim3(256), 0, st,
-                       ws->stats.as<uint32_t>(), m_pad, (uint32_t)n, d_cs, d_rs, d_lut);
+                       ws->stats.as<uint32_t>(), m_pad, (uint32_t)n, d_cs, d_rs, d_lut, d_musc);
     HIPC(hipGetLastError());
     HIPC(hipEventRecord(ws->ev[1], st));
+    const uint32_t nkb = (uint32_t)(round_up(n, kKBlock) / kKBlock);
+    if (d) {
+        HIPC(launch_sketch(sp, ws->tiled.as<uint32_t>(), nkb, d_musc, (uint32_t)m, m_pad, (double)n,
+                           ws->sk_cols.as<float>(), nullptr, nullptr, st));
+    }
+    HIPC(hipEventRecord(ws->ev[2], st));
 
-    // MAF (and pq-scaled annotations) on the host from the exact integer stats.
+    // 2. MAF (and pq-scaled annotations, kappa) on the host from the exact stats
     const bool use_f32 = cfg->precision == LDSC_PREC_FP32;
     const bool has_pq = !std::isnan(cfg->pq_exp);
+    const bool maf_aware = d && cfg->sketch_maf_aware;
     std::vector<uint32_t> hstats;
-    if (maf_out || has_pq) {
+    if (maf_out || has_pq || maf_aware) {
         hstats.resize((size_t)m_pad * 4);
         HIPC(hipMemcpyAsync(hstats.data(), ws->stats.p, (size_t)m_pad * 16, hipMemcpyDeviceToHost, st));
         HIPC(hipStreamSynchronize(st));
     }
     std::vector<double> maf(hstats.empty() ? 0 : m);
-    for (uint64_t j = 0; j < maf.size(); ++j) {
-        double v = maf_from_stats(hstats[4 * j], hstats[4 * j + 1]);
-        maf[j] = use_f32 ? (double)(float)v : v;  // normalize.rs:153 returns (f32)maf
+    if (!maf.empty()) {
+        // f32 exact path: (f32)maf (normalize.rs:153).  Sketch: f64 on the fused
+        // path of a BED-contiguous chunk (compute.rs:639), (f32)maf otherwise
+        // (compute.rs:1509-1511).
+        std::vector<uint8_t> round32(m, use_f32 && !d ? 1 : 0);
+        if (d) {
+            uint64_t run0 = 0;
+            while (run0 < m) {
+                uint64_t run1 = m;
+                if (cfg->per_chromosome) {
+                    run1 = run0 + 1;
+                    while (run1 < m && snps[run1].chr == snps[run0].chr) ++run1;
+                }
+                for (uint64_t s = run0; s < run1; s += cfg->chunk_size) {
+                    const uint64_t e = std::min<uint64_t>(s + cfg->chunk_size, run1);
+                    const bool contig = snps[e - 1].bed_idx == snps[s].bed_idx + (e - s) - 1;
+                    for (uint64_t j = s; j < e; ++j) round32[j] = contig ? 0 : 1;
+                }
+                run0 = run1;
+            }
+        }
+        for (uint64_t j = 0; j < m; ++j) {
+            const double v = maf_from_stats(hstats[4 * j], hstats[4 * j + 1]);
+            maf[j] = round32[j] ? (double)(float)v : v;
+        }
+    }
+    if (maf_aware) {
+        // per_snp_kappa (compute.rs:525-538) from the integer moments:
+        // sum g^3 = 8 n2 + het, sum g^4 = 16 n2 + het, n2 = (sum - het) / 2
+        std::vector<double> kap(m_pad, 0.0);
+        for (uint64_t j = 0; j < m; ++j) {
+            const uint32_t S = hstats[4 * j], C = hstats[4 * j + 1], SS = hstats[4 * j + 2],
+                           H = hstats[4 * j + 3];
+            const uint32_t n2 = (S - H) / 2;
+            const double cube = 8.0 * n2 + H, fourth = 16.0 * n2 + H;
+            if (C == 0) continue;
+            const double cf = (double)C, mean = (double)S / cf;
+            const double m4 = fourth - 4.0 * mean * cube + 6.0 * mean * mean * (double)SS -
+                              3.0 * std::pow(mean, 4) * cf;
+            const double css = (double)SS - cf * mean * mean;
+            kap[j] = css <= 0.0 ? 0.0 : m4 / (css * css);
+        }
+        HIPC(hipMemcpyAsync(ws->maf.p, kap.data(), kap.size() * 8, hipMemcpyHostToDevice, st));
+        HIPC(hipStreamSynchronize(st));
     }
     if (annot || has_pq) {
         std::vector<double> ae((size_t)m_pad * K, 0.0);
@@ -778,6 +746,7 @@ int run_l2(const ldsc_panel* panel, const ldsc_snp* snps, uint64_t m, const ldsc
         HIPC(hipGetLastError());
     }
 
+    // 3. band contraction
     BandArgs ba{};
     ba.tiled = ws->tiled.as<u32x4>();
     ba.tiles = ws->tiles.as<int2>();
@@ -788,7 +757,9 @@ int run_l2(const ldsc_panel* panel, const ldsc_snp* snps, uint64_t m, const ldsc
     ba.aeff = ws->aeff.as<double>();
     ba.rowp = ws->rowp.as<double>();
     ba.colp = ws->colp.as<double>();
-    ba.nkb = nkb;
+    ba.sk = d ? ws->sk_cols.as<float>() : nullptr;
+    ba.kappa = maf_aware ? ws->maf.as<double>() : nullptr;
+    ba.nkb = d ? (uint32_t)sp->nblocks : nkb;
     ba.ntiles = plan.ntiles;
     ba.per_xcd = (plan.ntiles + 7) / 8;
     ba.m = (uint32_t)m;
@@ -799,37 +770,45 @@ int run_l2(const ldsc_panel* panel, const ldsc_snp* snps, uint64_t m, const ldsc
     ba.r2u_a = 1.0 + 1.0 / denom;
     ba.r2u_b = -1.0 / denom;
     ba.n_inv_sq_r2u_a = n_inv * n_inv * ba.r2u_a;
+    ba.n_inv_sq = n_inv * n_inv;
+    ba.d_f = (double)d;
 
-    HIPC(hipEventRecord(ws->ev[2], st));
-    if (use_f32)
+    HIPC(hipEventRecord(ws->ev[3], st));
+    if (d)
+        HIPC(launch_band_sk(ba, st));
+    else if (use_f32)
         hipLaunchKernelGGL(k_band_f32, dim3(8 * ba.per_xcd), dim3(64), 0, st, ba);
     else
         hipLaunchKernelGGL(k_band_i8, dim3(8 * ba.per_xcd), dim3(64), 0, st, ba);
     HIPC(hipGetLastError());
-    HIPC(hipEventRecord(ws->ev[3], st));
+    HIPC(hipEventRecord(ws->ev[4], st));
+
+    // 4. deterministic reduction
     const uint64_t nred = m * K;
     hipLaunchKernelGGL(k_reduce, dim3((unsigned)((nred + 255) / 256)), dim3(256), 0, st,
                        ws->rowp.as<double>(), ws->colp.as<double>(), ws->aeff.as<double>(),
                        ws->j0.as<int>(), ws->tstart.as<int>(), plan.nb, (uint32_t)m, m_pad,
                        (uint32_t)K, ws->l2.as<double>());
     HIPC(hipGetLastError());
-    HIPC(hipEventRecord(ws->ev[4], st));
+    HIPC(hipEventRecord(ws->ev[5], st));
     HIPC(hipMemcpyAsync(l2_out, ws->l2.p, (size_t)m * K * 8, hipMemcpyDeviceToHost, st));
     HIPC(hipStreamSynchronize(st));
     if (maf_out) std::memcpy(maf_out, maf.data(), m * 8);
 
     if (perf) {
-        float t01 = 0, t23 = 0, t34 = 0;
+        float t01 = 0, t12 = 0, t34 = 0, t45 = 0;
         (void)hipEventElapsedTime(&t01, ws->ev[0], ws->ev[1]);
-        (void)hipEventElapsedTime(&t23, ws->ev[2], ws->ev[3]);
+        (void)hipEventElapsedTime(&t12, ws->ev[1], ws->ev[2]);
         (void)hipEventElapsedTime(&t34, ws->ev[3], ws->ev[4]);
+        (void)hipEventElapsedTime(&t45, ws->ev[4], ws->ev[5]);
         std::memset(perf, 0, sizeof(*perf));
         perf->m = m;
         perf->n_indiv = n;
         perf->bed_read_secs = t_prep - t_start;
         perf->norm_secs = t01 * 1e-3;
-        perf->bb_dot_secs = t23 * 1e-3;
-        perf->ring_store_secs = t34 * 1e-3;
+        perf->sketch_secs = t12 * 1e-3;
+        perf->bb_dot_secs = t34 * 1e-3;
+        perf->ring_store_secs = t45 * 1e-3;
         perf->total_secs = now_s() - t_start;
         perf->pairs_included = plan.pairs_included;
         perf->pairs_evaluated = (uint64_t)plan.ntiles * kTile * kTile;
@@ -889,6 +868,9 @@ extern "C" int ldsc_l2_open_panel(const uint8_t* bed, uint64_t bed_len, uint64_t
         p->keep.assign(keep_idx, keep_idx + n_keep);
         e = hipMalloc(&p->d_keep, n_keep * 8);
         if (e == hipSuccess) e = hipMemcpy(p->d_keep, keep_idx, n_keep * 8, hipMemcpyHostToDevice);
+        std::vector<int32_t> k32(keep_idx, keep_idx + n_keep);
+        if (e == hipSuccess) e = hipMalloc(&p->d_keep32, n_keep * 4);
+        if (e == hipSuccess) e = hipMemcpy(p->d_keep32, k32.data(), n_keep * 4, hipMemcpyHostToDevice);
     }
     if (e != hipSuccess) {
         ldsc_l2_close_panel(p);
@@ -903,6 +885,11 @@ extern "C" int ldsc_l2_close_panel(ldsc_panel* p) {
     if (!p) return LDSC_OK;
     if (p->d_slabs) (void)hipFree(p->d_slabs);
     if (p->d_keep) (void)hipFree(p->d_keep);
+    if (p->d_keep32) (void)hipFree(p->d_keep32);
+    for (SketchPlan* sp : p->sketch_plans) {
+        sp->release();
+        delete sp;
+    }
     delete p;
     return LDSC_OK;
 }
@@ -956,7 +943,7 @@ extern "C" int ldsc_l2_decode(const ldsc_panel* panel, const uint64_t* bed_idx, 
         double* d_rs = reinterpret_cast<double*>(ws->info.as<char>() + m * 8);
         float4* d_lut = reinterpret_cast<float4*>(ws->info.as<char>() + m * 16);
         hipLaunchKernelGGL(k_snp_info, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, st,
-                           ws->stats.as<uint32_t>(), (uint32_t)m, (uint32_t)n, d_cs, d_rs, d_lut);
+                           ws->stats.as<uint32_t>(), (uint32_t)m, (uint32_t)n, d_cs, d_rs, d_lut, nullptr);
         DCHK(hipGetLastError());
         char* sp = ws->scratch.as<char>();
         int8_t* d_dos = dosage ? reinterpret_cast<int8_t*>(sp) : nullptr;

@@ -1,16 +1,347 @@
-// sketch.hip -- CountSketch path (placeholder until the kernels land).
-#include "device.h"
+// sketch.hip -- the CountSketch path (--sketch d).
+//
+// Reference: CountSketchProj (src/l2/compute.rs:171-302), the fused
+// decode -> normalize -> signed scatter-add (countsketch_fused_project_f32,
+// compute.rs:601-770; scatter_one_column_f32_scalar, wasm_simd.rs:1691-1784)
+// and the quadratic bias inversion (compute.rs:947-998).
+//
+// MI355X form:
+//  * Individuals are put in bucket order (stable within a bucket) by the
+//    LDS-staged row gather, then tiled like the exact path.  A bucket's
+//    members are then one contiguous run of the K axis.
+//  * The signed bucket sums are a product with a sparse +-1 matrix P.  For
+//    each block of 32 buckets only the K-steps covering its members are
+//    visited; P's non-zero 32x32 tiles are precomputed as int8 MFMA
+//    A-operands.  v_mfma_i32_32x32x32_i8 then gives exact integers
+//      SG[b, j] = sum_{i in b} s_i g_ij,  SM[b, j] = sum_{i in b} s_i [present].
+//  * Bucket value y_b = inv_j (SG - mean_j SM) with the reference's f32 mean
+//    and inverse std (the same real number its f32 LUT scatter accumulates),
+//    renormalized so ||y||^2 = N, stored f32 in the lane order the
+//    contraction's MFMA reads.
+//  * The d-length contraction runs on v_mfma_f32_32x32x2f32 over 64x64 SNP
+//    tiles of the band, followed by the quadratic inversion, the unbiased
+//    r^2, masking and the annotation contraction shared with the exact path.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "band_common.h"
 
 namespace ldsc {
-struct BandPlan;
-int run_l2_sketch(const ldsc_panel*, const ldsc_snp*, uint64_t, const ldsc_l2_config*,
-                  const std::vector<uint64_t>&, const BandPlan&, const double*, uint64_t,
-                  double*, double*, ldsc_l2_perf*, Workspace*, char* err, uint64_t err_len) {
-    return fail(err, err_len, LDSC_ERR_INVALID_ARG, "--sketch is not available yet");
+
+void SketchPlan::release() {
+    if (d_fam) (void)hipFree(d_fam);
+    if (d_ptiles) (void)hipFree(d_ptiles);
+    if (d_blocks) (void)hipFree(d_blocks);
+    d_fam = nullptr;
+    d_ptiles = nullptr;
+    d_blocks = nullptr;
 }
+
+// Builds (once per panel and d) the bucket order of individuals and the
+// int8 P tiles.  Host work is O(N + d).
+int sketch_plan(const ldsc_panel* panel_c, uint64_t d, SketchPlan** out, char* err,
+                uint64_t err_len) {
+    ldsc_panel* panel = const_cast<ldsc_panel*>(panel_c);
+    std::lock_guard<std::mutex> lk(panel->sketch_mu);
+    for (SketchPlan* p : panel->sketch_plans)
+        if (p->d == d) {
+            *out = p;
+            return LDSC_OK;
+        }
+    const uint64_t n = panel->n;
+    std::vector<uint32_t> bucket(n);
+    std::vector<int8_t> sign(n);
+    countsketch_hash(n, d, 42, bucket.data(), sign.data());  // CountSketchProj::new(n, d, 42)
+    std::vector<uint64_t> off(d + 1, 0);
+    for (uint64_t i = 0; i < n; ++i) off[bucket[i] + 1]++;
+    for (uint64_t b = 0; b < d; ++b) off[b + 1] += off[b];
+    std::vector<uint64_t> fill(off.begin(), off.end() - 1);
+    std::vector<uint32_t> perm(n);
+    for (uint64_t i = 0; i < n; ++i) perm[fill[bucket[i]]++] = (uint32_t)i;
+    std::vector<int32_t> fam(n);
+    for (uint64_t k = 0; k < n; ++k)
+        fam[k] = panel->keep.empty() ? (int32_t)perm[k] : (int32_t)panel->keep[perm[k]];
+
+    auto* sp = new SketchPlan();
+    sp->d = d;
+    sp->d_pad = round_up(d, 32);
+    sp->nblocks = sp->d_pad / 32;
+    std::vector<int4> blocks(sp->nblocks);
+    uint64_t ntiles = 0;
+    for (uint64_t B = 0; B < sp->nblocks; ++B) {
+        const uint64_t lo = off[std::min<uint64_t>(32 * B, d)];
+        const uint64_t hi = off[std::min<uint64_t>(32 * B + 32, d)];
+        const uint64_t ks0 = lo / 32, ks1 = hi > lo ? (hi + 31) / 32 : ks0;
+        blocks[B] = make_int4((int)ntiles, (int)ks0, (int)(ks1 - ks0), 0);
+        ntiles += ks1 - ks0;
+    }
+    sp->ntiles = ntiles;
+    // A-operand tile of (bucket block B, K-step ks): lane l is bucket row
+    // 32B + l%32; its byte p is member ks*32 + (l/32)*16 + p.
+    std::vector<int8_t> tiles(std::max<uint64_t>(ntiles, 1) * 1024, 0);
+    for (uint64_t B = 0; B < sp->nblocks; ++B) {
+        const int4 bl = blocks[B];
+        for (int s = 0; s < bl.z; ++s) {
+            int8_t* t = tiles.data() + (size_t)(bl.x + s) * 1024;
+            const uint64_t ks = (uint64_t)bl.y + s;
+            for (int lane = 0; lane < 64; ++lane) {
+                const uint64_t b = 32 * B + (lane & 31);
+                for (int p = 0; p < 16; ++p) {
+                    const uint64_t k = ks * 32 + (lane >> 5) * 16 + p;
+                    if (k < n && bucket[perm[k]] == b) t[lane * 16 + p] = sign[perm[k]];
+                }
+            }
+        }
+    }
+    hipError_t e = hipSetDevice(panel->device);
+    if (e == hipSuccess) e = hipMalloc(&sp->d_fam, n * 4);
+    if (e == hipSuccess) e = hipMemcpy(sp->d_fam, fam.data(), n * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMalloc(&sp->d_ptiles, tiles.size());
+    if (e == hipSuccess) e = hipMemcpy(sp->d_ptiles, tiles.data(), tiles.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMalloc(&sp->d_blocks, blocks.size() * sizeof(int4));
+    if (e == hipSuccess)
+        e = hipMemcpy(sp->d_blocks, blocks.data(), blocks.size() * sizeof(int4), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        sp->release();
+        delete sp;
+        return fail(err, err_len, e == hipErrorOutOfMemory ? LDSC_ERR_OOM : LDSC_ERR_DEVICE,
+                    "sketch plan upload failed: %s", hipGetErrorString(e));
+    }
+    panel->sketch_plans.push_back(sp);
+    *out = sp;
+    return LDSC_OK;
+}
+
+// One wave per pair of SNP groups (64 SNPs).  RAW writes the integer bucket
+// sums (d x m column-major); otherwise normalized f32 sketch tiles:
+// sk[((g * nblocks + B) * 64 + lane) * 16 + r] holds bucket
+// 32B + 8(r/4) + 4(lane/32) + r%4 of SNP 32g + lane%32.
+template <bool RAW>
+__global__ __launch_bounds__(64) void k_sketch(const uint32_t* __restrict__ tiled, uint32_t nkb,
+                                                const v4i* __restrict__ ptiles,
+                                                const int4* __restrict__ blocks, uint32_t nblocks,
+                                                const float2* __restrict__ musc, uint32_t m,
+                                                uint32_t d, double n_norm, float* __restrict__ sk,
+                                                int32_t* __restrict__ sg_out,
+                                                int32_t* __restrict__ sm_out) {
+    const uint32_t gp = blockIdx.x;
+    const int lane = threadIdx.x;
+    const int h = lane >> 5;
+    const uint32_t* w0p = tiled + ((size_t)(2 * gp) * nkb * 64 + lane) * 4;
+    const uint32_t* w1p = w0p + (size_t)nkb * 256;
+    const uint32_t snp0 = 64 * gp + (lane & 31), snp1 = snp0 + 32;
+    const float2 ms0 = musc[snp0], ms1 = musc[snp1];
+    double nrm0 = 0.0, nrm1 = 0.0;
+    for (uint32_t B = 0; B < nblocks; ++B) {
+        const int4 blk = blocks[B];
+        v16i sg0, sm0, sg1, sm1;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) sg0[r] = sm0[r] = sg1[r] = sm1[r] = 0;
+        for (int s = 0; s < blk.z; ++s) {
+            const uint32_t ks = (uint32_t)blk.y + s;
+            const size_t off = (size_t)(ks >> 2) * 256 + (ks & 3);
+            const uint32_t w0 = w0p[off], w1 = w1p[off];
+            const v4i P = ptiles[(size_t)(blk.x + s) * 64 + lane];
+            v4i G0, M0, G1, M1;
+            expand_codes(w0, G0, M0);
+            expand_codes(w1, G1, M1);
+            sg0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(P, G0, sg0, 0, 0, 0);
+            sm0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(P, M0, sm0, 0, 0, 0);
+            sg1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(P, G1, sg1, 0, 0, 0);
+            sm1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(P, M1, sm1, 0, 0, 0);
+        }
+        if (RAW) {
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const uint32_t b = 32 * B + 8 * (r >> 2) + 4 * h + (r & 3);
+                if (b < d) {
+                    if (snp0 < m) {
+                        sg_out[(size_t)snp0 * d + b] = sg0[r];
+                        sm_out[(size_t)snp0 * d + b] = sm0[r];
+                    }
+                    if (snp1 < m) {
+                        sg_out[(size_t)snp1 * d + b] = sg1[r];
+                        sm_out[(size_t)snp1 * d + b] = sm1[r];
+                    }
+                }
+            }
+        } else {
+            float* t0 = sk + (((size_t)(2 * gp) * nblocks + B) * 64 + lane) * 16;
+            float* t1 = sk + (((size_t)(2 * gp + 1) * nblocks + B) * 64 + lane) * 16;
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const float y0 = (float)((double)ms0.y * ((double)sg0[r] - (double)ms0.x * (double)sm0[r]));
+                const float y1 = (float)((double)ms1.y * ((double)sg1[r] - (double)ms1.x * (double)sm1[r]));
+                nrm0 += (double)y0 * (double)y0;
+                nrm1 += (double)y1 * (double)y1;
+                t0[r] = y0;
+                t1[r] = y1;
+            }
+        }
+    }
+    if (!RAW) {
+        // ratio-estimator renorm: ||col||^2 = N (wasm_simd.rs:1774-1783)
+        nrm0 += __shfl_xor(nrm0, 32);
+        nrm1 += __shfl_xor(nrm1, 32);
+        const float sc0 = nrm0 > 0.0 ? (float)sqrt(n_norm / nrm0) : 1.0f;
+        const float sc1 = nrm1 > 0.0 ? (float)sqrt(n_norm / nrm1) : 1.0f;
+        for (uint32_t B = 0; B < nblocks; ++B) {
+            float4* t0 = reinterpret_cast<float4*>(sk + (((size_t)(2 * gp) * nblocks + B) * 64 + lane) * 16);
+            float4* t1 = reinterpret_cast<float4*>(sk + (((size_t)(2 * gp + 1) * nblocks + B) * 64 + lane) * 16);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                float4 a = t0[q], b = t1[q];
+                a.x *= sc0; a.y *= sc0; a.z *= sc0; a.w *= sc0;
+                b.x *= sc1; b.y *= sc1; b.z *= sc1; b.w *= sc1;
+                t0[q] = a;
+                t1[q] = b;
+            }
+        }
+    }
+}
+
+hipError_t launch_sketch(const SketchPlan* sp, const uint32_t* tiled, uint32_t nkb,
+                         const float2* musc, uint32_t m, uint32_t m_pad, double n, float* sk,
+                         int32_t* sg, int32_t* sm, hipStream_t st) {
+    const dim3 grid(m_pad / 64), block(64);
+    if (sg)
+        hipLaunchKernelGGL(k_sketch<true>, grid, block, 0, st, tiled, nkb,
+                           (const v4i*)sp->d_ptiles, sp->d_blocks, (uint32_t)sp->nblocks, musc, m,
+                           (uint32_t)sp->d, n, nullptr, sg, sm);
+    else
+        hipLaunchKernelGGL(k_sketch<false>, grid, block, 0, st, tiled, nkb,
+                           (const v4i*)sp->d_ptiles, sp->d_blocks, (uint32_t)sp->nblocks, musc, m,
+                           (uint32_t)sp->d, n, sk, nullptr, nullptr);
+    return hipGetLastError();
+}
+
+__device__ __forceinline__ float f4c(const float4& v, int c) {
+    return c == 0 ? v.x : (c == 1 ? v.y : (c == 2 ? v.z : v.w));
+}
+
+// d-length contraction of the sketched columns over one 64x64 band tile.
+__global__ __launch_bounds__(64) void k_band_sk(BandArgs a) {
+    __shared__ EpiShared sh;
+    __shared__ double kr[64], kc[64];
+    const int tile = tile_of_block(a);
+    if (tile < 0) return;
+    const int2 tl = a.tiles[tile];
+    const uint32_t I = (uint32_t)tl.x, J = (uint32_t)tl.y;
+    const int lane = threadIdx.x;
+    const size_t gstride = (size_t)a.nkb * 64 * 4;  // float4 per group
+    const float4* pa0 = reinterpret_cast<const float4*>(a.sk) + (size_t)(2 * I) * gstride + lane * 4;
+    const float4* pa1 = pa0 + gstride;
+    const float4* pb0 = reinterpret_cast<const float4*>(a.sk) + (size_t)(2 * J) * gstride + lane * 4;
+    const float4* pb1 = pb0 + gstride;
+
+    v16f acc[2][2];
+#pragma unroll
+    for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+        for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[bi][bj][r] = 0.0f;
+
+    for (uint32_t B = 0; B < a.nkb; ++B) {
+        const size_t o = (size_t)B * 256;
+        float4 xa0[4], xa1[4], xb0[4], xb1[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            xa0[q] = pa0[o + q];
+            xa1[q] = pa1[o + q];
+            xb0[q] = pb0[o + q];
+            xb1[q] = pb1[o + q];
+        }
+#pragma unroll
+        for (int s = 0; s < 16; ++s) {
+            const float fa0 = f4c(xa0[s >> 2], s & 3), fa1 = f4c(xa1[s >> 2], s & 3);
+            const float fb0 = f4c(xb0[s >> 2], s & 3), fb1 = f4c(xb1[s >> 2], s & 3);
+            acc[0][0] = __builtin_amdgcn_mfma_f32_32x32x2f32(fa0, fb0, acc[0][0], 0, 0, 0);
+            acc[0][1] = __builtin_amdgcn_mfma_f32_32x32x2f32(fa0, fb1, acc[0][1], 0, 0, 0);
+            acc[1][0] = __builtin_amdgcn_mfma_f32_32x32x2f32(fa1, fb0, acc[1][0], 0, 0, 0);
+            acc[1][1] = __builtin_amdgcn_mfma_f32_32x32x2f32(fa1, fb1, acc[1][1], 0, 0, 0);
+        }
+    }
+
+    stage_info(a, sh, I, J);
+    kr[lane] = a.kappa ? a.kappa[(size_t)I * kTile + lane] : 0.0;
+    kc[lane] = a.kappa ? a.kappa[(size_t)J * kTile + lane] : 0.0;
+    __syncthreads();
+    // quadratic_sketch_correction[_maf_aware] (compute.rs:947-998)
+    double r2u[2][2][16];
+#pragma unroll
+    for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+        for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int row = row_of(bi, r, lane), col = 32 * bj + (lane & 31);
+                const double v = (double)acc[bi][bj][r];
+                const double t = v * v * a.n_inv_sq;
+                const double dm3 = a.d_f - 3.0 - 2.0 * (kr[row] + kc[col]);
+                const double disc = dm3 * dm3 + 8.0 * (a.d_f * t - 1.0);
+                const double rs = disc >= 0.0 ? (-dm3 + sqrt(disc)) * 0.25 : 0.0;
+                const double val = rs * a.r2u_a + a.r2u_b;
+                r2u[bi][bj][r] = pair_included(a, sh, I, J, row, col) ? val : 0.0;
+            }
+    band_contract(a, sh, tile, I, J, r2u);
+}
+
+hipError_t launch_band_sk(const BandArgs& a, hipStream_t st) {
+    hipLaunchKernelGGL(k_band_sk, dim3(8 * a.per_xcd), dim3(64), 0, st, a);
+    return hipGetLastError();
+}
+
 }  // namespace ldsc
 
-extern "C" int ldsc_l2_sketch_buckets(const ldsc_panel*, const uint64_t*, uint64_t, uint64_t,
-                                      int32_t*, int32_t*, char* err, uint64_t err_len) {
-    return ldsc::fail(err, err_len, LDSC_ERR_INVALID_ARG, "sketch buckets not available yet");
+using namespace ldsc;
+
+extern "C" int ldsc_l2_sketch_buckets(const ldsc_panel* panel, const uint64_t* bed_idx, uint64_t m,
+                                      uint64_t d, int32_t* sg, int32_t* sm, char* err,
+                                      uint64_t err_len) {
+    if (!panel) return fail(err, err_len, LDSC_ERR_INVALID_ARG, "panel is NULL");
+    if (d < 1 || d > 1000000)
+        return fail(err, err_len, LDSC_ERR_SKETCH_RANGE, "sketch dimension %llu out of range",
+                    (unsigned long long)d);
+    if (m == 0) return LDSC_OK;
+    if (!bed_idx || !sg || !sm) return fail(err, err_len, LDSC_ERR_INVALID_ARG, "NULL array");
+    for (uint64_t j = 0; j < m; ++j)
+        if (bed_idx[j] >= panel->m_bim)
+            return fail(err, err_len, LDSC_ERR_INVALID_ARG, "bed_idx %llu out of range",
+                        (unsigned long long)bed_idx[j]);
+    if (m >= (1ull << 31)) return fail(err, err_len, LDSC_ERR_INVALID_ARG, "too many SNPs");
+    Workspace* ws = acquire_workspace(panel->device);
+    if (!ws) return fail(err, err_len, LDSC_ERR_DEVICE, "could not create a device workspace");
+    int rc = LDSC_OK;
+    do {
+        SketchPlan* sp = nullptr;
+        hipError_t e;
+#define SCHK(x) if ((e = (x)) != hipSuccess) { rc = fail(err, err_len, e == hipErrorOutOfMemory ? LDSC_ERR_OOM : LDSC_ERR_DEVICE, "%s: %s", #x, hipGetErrorString(e)); break; }
+        SCHK(hipSetDevice(panel->device));
+        rc = sketch_plan(panel, d, &sp, err, err_len);
+        if (rc) break;
+        const uint64_t m_pad = round_up(m, kTile);
+        hipStream_t st = ws->stream;
+        SCHK(ws->bed_idx.ensure(m * 8));
+        SCHK(ws->info.ensure(m_pad * 48));
+        SCHK(ws->sk_cols.ensure(m * d * 8));
+        SCHK(hipMemcpyAsync(ws->bed_idx.p, bed_idx, m * 8, hipMemcpyHostToDevice, st));
+        rc = gather_and_tile(panel, ws->bed_idx.as<uint64_t>(), m, sp->d_fam, m_pad, ws, err, err_len);
+        if (rc) break;
+        int32_t* d_sg = ws->sk_cols.as<int32_t>();
+        int32_t* d_sm = d_sg + m * d;
+        const uint32_t nkb = (uint32_t)(round_up(panel->n, kKBlock) / kKBlock);
+        // musc is unused by the integer form; pass any valid per-SNP array
+        SCHK(hipMemsetAsync(ws->info.p, 0, m_pad * 48, st));
+        SCHK(launch_sketch(sp, ws->tiled.as<uint32_t>(), nkb,
+                           reinterpret_cast<const float2*>(ws->info.as<char>()), (uint32_t)m,
+                           (uint32_t)m_pad, (double)panel->n, nullptr, d_sg, d_sm, st));
+        SCHK(hipMemcpyAsync(sg, d_sg, m * d * 4, hipMemcpyDeviceToHost, st));
+        SCHK(hipMemcpyAsync(sm, d_sm, m * d * 4, hipMemcpyDeviceToHost, st));
+        SCHK(hipStreamSynchronize(st));
+#undef SCHK
+    } while (0);
+    release_workspace(ws);
+    return rc;
 }

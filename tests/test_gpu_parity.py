@@ -1,0 +1,197 @@
+"""Device results against the CPU restatement beyond the basic window modes:
+bit-exact decode / stats / f32 standardization / integer sketch buckets, and
+l2 with --keep, --extract, annotations, pq, per-chromosome chunking, special
+columns and the CountSketch path."""
+import numpy as np
+import pytest
+
+import ldsc_amd as L
+from ldsc_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+TOL_EXACT = 1e-9     # int8 products + FP64 finish vs FP64 restatement (relative)
+TOL_F32 = 2e-3       # FP32 form vs f32 restatement (relative, floor 1)
+TOL_SKETCH = 2e-5    # exact-integer sketch vs f32 sequential scatter (relative, floor 1)
+
+
+def _rel(a, b):
+    return float(np.max(np.abs(a - b) / np.maximum(1.0, np.abs(b))))
+
+
+def _panel(n, m, seed, miss=0.02, nchr=1):
+    p = synth.make_panel(n=n, m=m, seed=seed, miss_rate=miss)
+    if nchr > 1:
+        p.chr[:] = 1 + (np.arange(m) * nchr) // m
+    return p
+
+
+@pytest.mark.parametrize("n", [1, 3, 4, 5, 127, 128, 129, 1000])
+def test_decode_bit_exact(oracle, n):
+    print(f"decode n={n}", flush=True)
+    p = _panel(n, 70, seed=n)
+    idx = np.array([5, 0, 69, 33, 33, 12], np.uint64)
+    keep_cases = [None]
+    if n >= 4:
+        keep_cases.append(np.array(sorted(set(range(n)) - {1, n // 2}))[::-1].copy())
+    for keep in keep_cases:
+        with L.Panel(p.bed_bytes(), n, p.m, keep=keep) as panel:
+            dos, miss, st, sf = L.decode(panel, idx)
+        edos = oracle.decode(p.bed_bytes(), n, idx, keep=keep)
+        assert np.array_equal(dos, edos)
+        assert np.array_equal(st, oracle.stats(p.bed_bytes(), n, idx, keep=keep))
+        assert np.array_equal(sf.view(np.uint32),
+                              oracle.std_f32(p.bed_bytes(), n, idx, keep=keep).view(np.uint32))
+        nn = edos.shape[0]
+        bits = (edos == -1)
+        words = (nn + 31) // 32
+        exp = np.zeros((words, len(idx)), np.uint32)
+        for i in range(nn):
+            exp[i // 32] |= (bits[i].astype(np.uint32) << np.uint32(i % 32))
+        assert np.array_equal(miss, exp)
+
+
+@pytest.mark.parametrize("n,d", [(1000, 50), (2049, 200), (3000, 3000), (777, 5)])
+def test_sketch_buckets_bit_exact(oracle, n, d):
+    print(f"sketch buckets n={n} d={d}", flush=True)
+    p = _panel(n, 96, seed=d)
+    idx = np.arange(96, dtype=np.uint64)[::-1].copy()
+    with L.Panel(p.bed_bytes(), n, p.m) as panel:
+        sg, sm = L.sketch_buckets(panel, idx, d)
+    esg, esm = oracle.sketch_int(p.bed_bytes(), n, idx, d)
+    assert np.array_equal(sg, esg) and np.array_equal(sm, esm)
+
+
+def test_sketch_buckets_with_keep(oracle):
+    n, d = 1200, 97
+    p = _panel(n, 40, seed=8)
+    keep = np.random.default_rng(1).permutation(n)[:900]
+    with L.Panel(p.bed_bytes(), n, p.m, keep=keep) as panel:
+        sg, sm = L.sketch_buckets(panel, np.arange(40), d)
+    esg, esm = oracle.sketch_int(p.bed_bytes(), n, np.arange(40), d, keep=keep)
+    assert np.array_equal(sg, esg) and np.array_equal(sm, esm)
+
+
+@pytest.mark.parametrize("use_f32", [False, True])
+def test_l2_keep_annot_pq(oracle, use_f32):
+    n, m = 613, 900
+    p = _panel(n, m, seed=21)
+    rng = np.random.default_rng(2)
+    keep = np.sort(rng.choice(n, 500, replace=False))
+    annot = synth.make_annot(m, seed=5, n_bin=3, n_cont=2)
+    annot[rng.random(m) < 0.01, 0] = np.nan
+    snps = L.make_snp_array(p.chr, p.cm, p.bp)
+    cfg = L.L2Config(mode=L.WindowMode.Kb(120.0), chunk_size=50, use_f32=use_f32, pq_exp=0.75)
+    with L.Panel(p.bed_bytes(), n, m, keep=keep) as panel:
+        got, maf, _ = L.compute_ldscore(panel, snps, cfg, annot=annot)
+    exp, emaf = oracle.l2(p.bed_bytes(), n, p.chr, p.cm, p.bp, mode=1, window=120.0, chunk=50,
+                          use_f32=use_f32, pq_exp=0.75, annot=np.nan_to_num(annot), keep=keep)
+    assert got.shape == (m, 5)
+    assert _rel(got, exp) < (TOL_F32 if use_f32 else TOL_EXACT)
+    assert np.array_equal(maf, emaf)
+
+
+def test_l2_extract_per_chromosome(oracle):
+    n, m = 400, 1200
+    p = _panel(n, m, seed=31, nchr=3)
+    sel = np.sort(np.random.default_rng(4).choice(m, 800, replace=False))
+    snps = L.make_snp_array(p.chr[sel], p.cm[sel], p.bp[sel], bed_idx=sel)
+    for per_chr in (False, True):
+        for masking in (False, True):
+            cfg = L.L2Config(mode=L.WindowMode.Cm(0.1), chunk_size=40, per_chromosome=per_chr,
+                             snp_level_masking=masking)
+            with L.Panel(p.bed_bytes(), n, m) as panel:
+                got, maf, _ = L.compute_ldscore(panel, snps, cfg)
+            exp, emaf = oracle.l2(p.bed_bytes(), n, p.chr[sel], p.cm[sel], p.bp[sel],
+                                  bed_idx=sel, mode=0, window=0.1, chunk=40, masking=masking,
+                                  per_chromosome=per_chr)
+            assert _rel(got, exp) < TOL_EXACT, (per_chr, masking)
+            assert np.array_equal(maf, emaf)
+
+
+def test_l2_special_columns(oracle):
+    """Identical, monomorphic and all-missing SNPs; N not a multiple of 4."""
+    n, m = 203, 130
+    p = _panel(n, m, seed=41)
+    body = p.body
+    body[10] = body[9]
+    body[20] = 0x00
+    body[30] = 0x55
+    body[40] = 0xFF
+    snps = L.make_snp_array(p.chr, p.cm, p.bp)
+    for use_f32 in (False, True):
+        cfg = L.L2Config(mode=L.WindowMode.Snp(20), chunk_size=16, use_f32=use_f32)
+        with L.Panel(p.bed_bytes(), n, m) as panel:
+            got, maf, _ = L.compute_ldscore(panel, snps, cfg)
+        exp, emaf = oracle.l2(p.bed_bytes(), n, p.chr, p.cm, p.bp, mode=2, window=20, chunk=16,
+                              use_f32=use_f32)
+        assert _rel(got, exp) < (TOL_F32 if use_f32 else TOL_EXACT)
+        assert np.array_equal(maf, emaf)
+        assert np.all(np.isfinite(got))
+
+
+@pytest.mark.parametrize("d,maf_aware", [(64, False), (300, False), (300, True), (1000, False)])
+def test_l2_sketch_matches_oracle(oracle, d, maf_aware):
+    n, m = 3000, 700
+    p = _panel(n, m, seed=d)
+    snps = L.make_snp_array(p.chr, p.cm, p.bp)
+    cfg = L.L2Config(mode=L.WindowMode.Cm(0.15), chunk_size=60, sketch=d,
+                     sketch_maf_aware=maf_aware, snp_level_masking=True)
+    print(f"sketch d={d} maf_aware={maf_aware}", flush=True)
+    with L.Panel(p.bed_bytes(), n, m) as panel:
+        got, maf, perf = L.compute_ldscore(panel, snps, cfg)
+    exp, emaf = oracle.l2(p.bed_bytes(), n, p.chr, p.cm, p.bp, mode=0, window=0.15, chunk=60,
+                          sketch=d, maf_aware=maf_aware, use_f32=True, masking=True)
+    print(f"  max rel {_rel(got, exp):.3e}", flush=True)
+    assert _rel(got, exp) < TOL_SKETCH
+    assert np.array_equal(maf, emaf)
+
+
+def test_sketch_errors_and_unmasked(oracle):
+    n, m = 500, 300
+    p = _panel(n, m, seed=77)
+    snps = L.make_snp_array(p.chr, p.cm, p.bp)
+    with L.Panel(p.bed_bytes(), n, m) as panel:
+        for bad in (2, n):
+            with pytest.raises(L.LdscError) as e:
+                L.compute_ldscore(panel, snps, L.L2Config(sketch=bad))
+            assert e.value.code == 4
+        cfg = L.L2Config(mode=L.WindowMode.Snp(40), chunk_size=25, sketch=100)
+        got, _, _ = L.compute_ldscore(panel, snps, cfg)
+    exp, _ = oracle.l2(p.bed_bytes(), n, p.chr, p.cm, p.bp, mode=2, window=40, chunk=25,
+                       sketch=100, use_f32=True)
+    assert _rel(got, exp) < TOL_SKETCH
+
+
+def test_l2_deterministic_and_chunk_invariant_when_masked():
+    n, m = 2000, 3000
+    p = _panel(n, m, seed=5)
+    snps = L.make_snp_array(p.chr, p.cm, p.bp)
+    with L.Panel(p.bed_bytes(), n, m) as panel:
+        a, _, _ = L.compute_ldscore(panel, snps, L.L2Config(mode=L.WindowMode.Cm(0.2),
+                                                             chunk_size=200,
+                                                             snp_level_masking=True))
+        b, _, _ = L.compute_ldscore(panel, snps, L.L2Config(mode=L.WindowMode.Cm(0.2),
+                                                             chunk_size=7,
+                                                             snp_level_masking=True))
+        c, _, _ = L.compute_ldscore(panel, snps, L.L2Config(mode=L.WindowMode.Cm(0.2),
+                                                             chunk_size=200,
+                                                             snp_level_masking=True))
+    assert np.array_equal(a, b) and np.array_equal(a, c)
+
+
+def test_int8_vs_fp32_at_panel_scale():
+    """N=50,000 x M=8,000 slice of the panel-scale workload: the two forms
+    agree to the FP32 tolerance and the result is reproducible bit for bit."""
+    n, m = 50_000, 8_000
+    p = _panel(n, m, seed=20261205, miss=0.01)
+    snps = L.make_snp_array(p.chr, p.cm, p.bp)
+    base = dict(mode=L.WindowMode.Cm(1.0), chunk_size=200, snp_level_masking=True)
+    with L.Panel(p.bed_bytes(), n, m) as panel:
+        a, _, perf = L.compute_ldscore(panel, snps, L.L2Config(**base))
+        b, _, _ = L.compute_ldscore(panel, snps, L.L2Config(**base, use_f32=True))
+        a2, _, _ = L.compute_ldscore(panel, snps, L.L2Config(**base))
+    assert np.array_equal(a, a2)
+    assert _rel(b, a) < TOL_F32
+    # l2 >= self term minus the bias of pairs: every SNP has its own 1
+    assert np.all(a > 0.5)
