@@ -1,14 +1,22 @@
-"""LD-score throughput on one MI355X.
+"""LD-score throughput and request latency on one MI355X.
 
-Workload: exact LD scores at panel scale -- synthetic founder-haplotype PLINK
-panel with N=50,000 individuals, M=100,000 SNPs on one chromosome, 1 %
-missing, --ld-wind-cm 1, --chunk-size 200, --snp-level-masking.  A step is
-one full ldsc_l2_compute over the resident panel.  The headline `value` is
-SNP-pairs/s of the int8-MFMA + FP64-finish form; the plain FP32 form runs in
-the same process and is reported beside it.
+Default run (one JSON line):
+  * headline -- exact LD scores at panel scale: synthetic founder-haplotype
+    PLINK panel, N=50,000 individuals x M=100,000 SNPs, 1 % missing,
+    --ld-wind-cm 1, --chunk-size 200, --snp-level-masking.  A step is one full
+    ldsc_l2_compute over the resident panel.  `value` = SNP-pairs/s of the
+    int8-MFMA + FP64-finish form; the plain FP32 form runs beside it.
+  * serving -- resident panels N=50,000 x M=1,000,000 and N=10,000 x
+    M=500,000; Poisson arrivals of contiguous SNP-range requests (size
+    log-uniform in [1,000, 50,000], window snps/kb/cM uniform, 0-10
+    annotation columns); p50/p99 arrival->result latency and aggregate
+    SNP-pairs/s.
+  * cpu_baseline -- the C restatement of the reference's f64 path on the
+    host cores, on a bounded slice of the same panel.
 
-Prints one JSON line.  `--gpus` above 1 is refused: this build targets one
-GPU.
+`--workload c3` (CountSketch, N=100,000 x M=200,000, d in {200, 1000, 5000})
+and `--workload c4` (partitioned, N=20,000 x M=500,000, K=50, kb 1000) print
+their own JSON lines.  `--gpus` above 1 is refused: this build targets one GPU.
 """
 from __future__ import annotations
 
@@ -25,6 +33,12 @@ sys.path.insert(0, ROOT)
 
 INT8_PEAK_TOPS = 5000.0    # MI355X dense int8 MFMA, TOP/s
 FP32_PEAK_TFLOPS = 157.3   # MI355X FP32 matrix, TFLOP/s
+HBM_PEAK_GBS = 8000.0      # MI355X HBM3E, GB/s
+METRIC = "SNP-pairs/s int8-MFMA vs FP32 plain (≤1e-6 rel); p50/p99 LD-request latency"
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
 
 
 def profiled_traffic(kernel: str):
@@ -41,8 +55,222 @@ def profiled_traffic(kernel: str):
     return None
 
 
-def log(*a):
-    print(*a, file=sys.stderr, flush=True)
+def timed_steps(L, panel, snps, cfg, steps, warmup, tag, annot=None):
+    for w in range(warmup):
+        L.compute_ldscore(panel, snps, cfg, annot=annot)
+        log(f"{tag} warmup {w + 1}/{warmup} done")
+    walls, perfs, out = [], [], None
+    for s in range(steps):
+        t = time.perf_counter()
+        l2, maf, perf = L.compute_ldscore(panel, snps, cfg, annot=annot)
+        walls.append(time.perf_counter() - t)
+        perfs.append(perf)
+        out = (l2, maf)
+        log(f"{tag} step {s + 1}/{steps}: {walls[-1] * 1e3:.2f} ms (band kernel "
+            f"{perf.bb_dot_secs * 1e3:.2f} ms, tile {perf.norm_secs * 1e3:.2f} ms, sketch "
+            f"{(perf.sketch_secs or 0) * 1e3:.2f} ms)")
+    return walls, perfs, out
+
+
+# ---------------------------------------------------------------------------
+# C2: headline
+# ---------------------------------------------------------------------------
+def leg_c2(args, L, synth):
+    seed = 20261205
+    t0 = time.perf_counter()
+    p = synth.make_panel(args.n, args.m, seed, miss_rate=0.01)
+    log(f"C2: generated panel N={p.n} M={p.m} in {time.perf_counter() - t0:.1f}s")
+    snps = L.make_snp_array(p.chr, p.cm, p.bp)
+    base = dict(mode=L.WindowMode.Cm(1.0), chunk_size=200, snp_level_masking=True)
+    cfg8, cfg32 = L.L2Config(**base), L.L2Config(**base, use_f32=True)
+    panel = L.Panel(p.bed_bytes(), p.n, p.m)
+    w8, pf8, (l2_8, _) = timed_steps(L, panel, snps, cfg8, args.steps, args.warmup, "int8")
+    w32, pf32, (l2_32, _) = timed_steps(L, panel, snps, cfg32, args.fp32_steps, 1, "fp32")
+    panel.close()
+    P = pf8[-1].pairs_included
+    t_step = float(np.mean(w8))
+    k8 = float(np.mean([q.bb_dot_secs for q in pf8]))
+    k32 = float(np.mean([q.bb_dot_secs for q in pf32]))
+    algo = 2.0 * p.n * P
+    res = {
+        "value": P / t_step, "ms_per_step": t_step * 1e3, "pairs": P,
+        "pairs_evaluated": pf8[-1].pairs_evaluated, "tiles": pf8[-1].tiles,
+        "fp32": {"value": P / float(np.mean(w32)), "unit": "SNP-pairs/s",
+                 "ms_per_step": float(np.mean(w32)) * 1e3, "band_kernel_ms": k32 * 1e3,
+                 "tflops_algorithmic": algo / k32 / 1e12,
+                 "frac_of_fp32_peak": algo / k32 / 1e12 / FP32_PEAK_TFLOPS,
+                 "max_rel_vs_int8": float(np.max(np.abs(l2_32 - l2_8) /
+                                                 np.maximum(1.0, np.abs(l2_8))))},
+        "tile_ms": float(np.mean([q.norm_secs for q in pf8])) * 1e3,
+        "reduce_ms": float(np.mean([q.ring_store_secs for q in pf8])) * 1e3,
+        "host_prep_ms": float(np.mean([q.bed_read_secs for q in pf8])) * 1e3,
+        "roofline": {"bound": "mfma", "kernel": "k_band_i8", "achieved": algo / k8 / 1e12,
+                     "peak": INT8_PEAK_TOPS, "unit": "TFLOP/s",
+                     "frac": algo / k8 / 1e12 / INT8_PEAK_TOPS,
+                     "traffic": profiled_traffic("k_band_i8"), "band_kernel_ms": k8 * 1e3,
+                     "ops_per_launch": algo, "note": "int8 ops; algorithmic 2*N*P"},
+        "ld_scored_snps_per_s": p.m / t_step,
+    }
+    if not args.skip_cpu:
+        from oracle import oracle as O
+
+        ms = min(args.cpu_sample_snps, p.m)
+        bed_s = np.concatenate([p.buf[:3], p.body[:ms].reshape(-1)])
+        t = time.perf_counter()
+        exp, _ = O.l2(bed_s, p.n, p.chr[:ms], p.cm[:ms], p.bp[:ms], mode=O.WINDOW_CM,
+                      window=1.0, chunk=200, masking=True, threads=args.cpu_threads)
+        cpu_wall = time.perf_counter() - t
+        with L.Panel(bed_s, p.n, ms) as ps:
+            got, _, perf_s = L.compute_ldscore(ps, snps[:ms], cfg8)
+            got32, _, _ = L.compute_ldscore(ps, snps[:ms], cfg32)
+        res["sample_accuracy"] = {
+            "int8_fp64_max_rel": float(np.max(np.abs(got - exp) / np.abs(exp))),
+            "fp32_max_abs": float(np.max(np.abs(got32 - exp))),
+            "fp32_max_rel": float(np.max(np.abs(got32 - exp) / np.abs(exp))),
+        }
+        res["cpu_baseline"] = {
+            "value": perf_s.pairs_included / cpu_wall, "unit": "SNP-pairs/s",
+            "cores": args.cpu_threads, "kind": "port",
+            "sample": f"first {ms} SNPs of the same panel (N={p.n}, cM 1, masked), "
+                      f"{perf_s.pairs_included} pairs, FP64 restatement, {cpu_wall:.1f}s",
+        }
+        log(f"cpu baseline {res['cpu_baseline']['value']:.3e} pairs/s; "
+            f"{res['sample_accuracy']}")
+    return res
+
+
+# ---------------------------------------------------------------------------
+# C5: serving
+# ---------------------------------------------------------------------------
+def leg_serving(args, L, synth):
+    from ldsc_amd.serve import Server
+
+    t0 = time.perf_counter()
+    specs = [(50_000, args.serve_m_big, 20261207), (10_000, args.serve_m_small, 20261208)]
+    panels, snp_sets, annots = [], [], []
+    for n, m, seed in specs:
+        p = synth.make_panel(n, m, seed, miss_rate=0.01)
+        panels.append(L.Panel(p.bed_bytes(), n, m))
+        snp_sets.append(L.make_snp_array(p.chr, p.cm, p.bp))
+        annots.append(synth.make_annot(m, seed, n_bin=8, n_cont=2))
+        del p
+        log(f"serving: panel N={n} M={m} resident ({time.perf_counter() - t0:.1f}s)")
+    rng = np.random.default_rng(20261209)
+    modes = [L.WindowMode.Snp(500), L.WindowMode.Kb(1000.0), L.WindowMode.Cm(1.0)]
+    reqs = []
+    for _ in range(args.serve_requests):
+        which = int(rng.integers(0, 2))
+        m = snp_sets[which].size
+        size = int(np.exp(rng.uniform(np.log(1000), np.log(50_000))))
+        start = int(rng.integers(0, m - size))
+        K = int(rng.integers(0, 11))
+        cfg = L.L2Config(mode=modes[int(rng.integers(0, 3))], chunk_size=200,
+                         snp_level_masking=True)
+        snps = snp_sets[which][start:start + size]
+        annot = None if K == 0 else np.ascontiguousarray(annots[which][start:start + size, :K])
+        pairs = int(np.sum(np.arange(size, dtype=np.int64) -
+                           L.pair_bounds(snps, cfg).astype(np.int64)))
+        reqs.append((which, snps, cfg, annot, pairs))
+    gaps = rng.exponential(1.0 / args.qps, size=len(reqs))
+    lats, tickets = [], []
+    with Server(hbm_budget_bytes=96 << 30, max_inflight=args.serve_inflight,
+                queue_capacity=4096) as srv:
+        warm = [srv.submit(panels[0], snp_sets[0][:60_000], reqs[0][2])
+                for _ in range(args.serve_inflight)]
+        for t in warm:
+            srv.wait(t)
+        log("serving: workers warm; starting the Poisson stream")
+        t_start = time.perf_counter()
+        t_next = t_start
+        for (which, snps, cfg, annot, _), gap in zip(reqs, gaps):
+            t_next += gap
+            delay = t_next - time.perf_counter()
+            if delay > 0:
+                time.sleep(delay)
+            tickets.append(srv.submit(panels[which], snps, cfg, annot))
+        for t in tickets:
+            lats.append(srv.wait(t))
+        t_end = time.perf_counter()
+    for pn in panels:
+        pn.close()
+    lats = np.array(lats)
+    total_pairs = float(sum(r[4] for r in reqs))
+    res = {
+        "requests": len(reqs), "offered_qps": args.qps,
+        "achieved_qps": len(reqs) / (t_end - t_start),
+        "p50_ms": float(np.percentile(lats, 50)) * 1e3,
+        "p99_ms": float(np.percentile(lats, 99)) * 1e3,
+        "mean_ms": float(lats.mean()) * 1e3,
+        "snp_pairs_per_s": total_pairs / (t_end - t_start),
+        "max_inflight": args.serve_inflight,
+        "panels": "N=50000 x M=%d and N=10000 x M=%d resident" % (args.serve_m_big,
+                                                                  args.serve_m_small),
+    }
+    log(f"serving: {res}")
+    return res
+
+
+# ---------------------------------------------------------------------------
+# C3: CountSketch
+# ---------------------------------------------------------------------------
+def leg_c3(args, L, synth):
+    n, m = 100_000, args.c3_m
+    p = synth.make_panel(n, m, 20261206, miss_rate=0.01)
+    snps = L.make_snp_array(p.chr, p.cm, p.bp)
+    out = {"workload": f"CountSketch N={n} x M={m}, cM 1, chunk 200, masked", "d": {}}
+    with L.Panel(p.bed_bytes(), n, m) as panel:
+        ex_cfg = L.L2Config(mode=L.WindowMode.Cm(1.0), snp_level_masking=True)
+        w, pf, (exact, _) = timed_steps(L, panel, snps, ex_cfg, 1, 1, "c3 exact")
+        out["exact_int8_ms"] = w[0] * 1e3
+        P = pf[0].pairs_included
+        for d in (200, 1000, 5000):
+            cfg = L.L2Config(mode=L.WindowMode.Cm(1.0), snp_level_masking=True, sketch=d)
+            w, pf, (sk, _) = timed_steps(L, panel, snps, cfg, args.steps, 1, f"c3 d={d}")
+            t = float(np.mean(w))
+            sk_s = float(np.mean([q.sketch_secs for q in pf]))
+            tile_s = float(np.mean([q.norm_secs for q in pf]))
+            band_s = float(np.mean([q.bb_dot_secs for q in pf]))
+            bytes_sk = m * ((n + 3) // 4) + 8 * n + 4 * m * d
+            out["d"][str(d)] = {
+                "snp_pairs_per_s": P / t, "ms_per_step": t * 1e3,
+                "gather_tile_ms": tile_s * 1e3, "sketch_kernel_ms": sk_s * 1e3,
+                "sketch_gbs_algorithmic": bytes_sk / sk_s / 1e9,
+                "sketch_frac_hbm": bytes_sk / sk_s / 1e9 / HBM_PEAK_GBS,
+                "contraction_ms": band_s * 1e3,
+                "contraction_tflops": 2.0 * d * P / band_s / 1e12,
+                "pearson_r_vs_exact": float(np.corrcoef(sk[:, 0], exact[:, 0])[0, 1]),
+                "max_abs_diff_vs_exact": float(np.max(np.abs(sk - exact))),
+            }
+            log(f"c3 d={d}: {out['d'][str(d)]}")
+    out["pairs"] = P
+    return out
+
+
+# ---------------------------------------------------------------------------
+# C4: partitioned LD scores
+# ---------------------------------------------------------------------------
+def leg_c4(args, L, synth):
+    n, m = 20_000, args.c4_m
+    p = synth.make_panel(n, m, 20261207 + 100, miss_rate=0.01)
+    snps = L.make_snp_array(p.chr, p.cm, p.bp)
+    annot = synth.make_annot(m, 20261210, n_bin=40, n_cont=10)
+    out = {"workload": f"partitioned N={n} x M={m}, K=50, kb 1000, chunk 200"}
+    res = {}
+    with L.Panel(p.bed_bytes(), n, m) as panel:
+        for tag, f32 in (("int8", False), ("fp32", True)):
+            cfg = L.L2Config(mode=L.WindowMode.Kb(1000.0), chunk_size=200, use_f32=f32)
+            w, pf, (l2, _) = timed_steps(L, panel, snps, cfg, args.steps, 1, f"c4 {tag}",
+                                         annot=annot)
+            P = pf[0].pairs_included
+            t = float(np.mean(w))
+            band = float(np.mean([q.bb_dot_secs for q in pf]))
+            out[tag] = {"snp_pairs_per_s": P / t, "ms_per_step": t * 1e3,
+                        "band_kernel_ms": band * 1e3, "pairs": P,
+                        "algorithmic_tops": 2.0 * n * P / band / 1e12}
+            res[tag] = l2
+        out["fp32_vs_int8_max_rel"] = float(np.max(np.abs(res["fp32"] - res["int8"]) /
+                                                   np.maximum(1.0, np.abs(res["int8"]))))
+    return out
 
 
 def main() -> int:
@@ -50,12 +278,21 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4"])
     ap.add_argument("--n", type=int, default=50_000)
     ap.add_argument("--m", type=int, default=100_000)
     ap.add_argument("--fp32-steps", type=int, default=2)
     ap.add_argument("--cpu-sample-snps", type=int, default=6000)
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--skip-serving", action="store_true")
+    ap.add_argument("--qps", type=float, default=200.0)
+    ap.add_argument("--serve-requests", type=int, default=400)
+    ap.add_argument("--serve-inflight", type=int, default=4)
+    ap.add_argument("--serve-m-big", type=int, default=1_000_000)
+    ap.add_argument("--serve-m-small", type=int, default=500_000)
+    ap.add_argument("--c3-m", type=int, default=200_000)
+    ap.add_argument("--c4-m", type=int, default=500_000)
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="only run a device-to-device copy of this many MiB (torch) to "
                          "calibrate HBM byte counters, then exit")
@@ -79,85 +316,25 @@ def main() -> int:
     import ldsc_amd as L
     from ldsc_amd import synth
 
-    seed = 20261205
-    t0 = time.perf_counter()
-    p = synth.make_panel(args.n, args.m, seed, miss_rate=0.01)
-    log(f"generated panel N={p.n} M={p.m} in {time.perf_counter() - t0:.1f}s")
-    snps = L.make_snp_array(p.chr, p.cm, p.bp)
-    cfg8 = L.L2Config(mode=L.WindowMode.Cm(1.0), chunk_size=200, snp_level_masking=True)
-    cfg32 = L.L2Config(mode=L.WindowMode.Cm(1.0), chunk_size=200, snp_level_masking=True,
-                       use_f32=True)
-    panel = L.Panel(p.bed_bytes(), p.n, p.m)
-    log("panel resident on device")
+    if args.workload == "c3":
+        print(json.dumps({"metric": "CountSketch LD scores", **leg_c3(args, L, synth)}),
+              flush=True)
+        return 0
+    if args.workload == "c4":
+        print(json.dumps({"metric": "partitioned LD scores (K=50)", **leg_c4(args, L, synth)}),
+              flush=True)
+        return 0
 
-    def run(cfg, steps, warmup, tag):
-        for w in range(warmup):
-            L.compute_ldscore(panel, snps, cfg)
-            log(f"{tag} warmup {w + 1}/{warmup} done")
-        walls, kern, outs = [], [], None
-        for s in range(steps):
-            t = time.perf_counter()
-            l2, maf, perf = L.compute_ldscore(panel, snps, cfg)
-            walls.append(time.perf_counter() - t)
-            kern.append(perf.bb_dot_secs)
-            outs = (l2, maf, perf)
-            log(f"{tag} step {s + 1}/{steps}: {walls[-1] * 1e3:.2f} ms "
-                f"(band kernel {perf.bb_dot_secs * 1e3:.2f} ms)")
-        return walls, kern, outs
-
-    w8, k8, (l2_8, maf8, perf8) = run(cfg8, args.steps, args.warmup, "int8")
-    w32, k32, (l2_32, _, perf32) = run(cfg32, args.fp32_steps, 1, "fp32")
-
-    P = perf8.pairs_included
-    t_step = float(np.mean(w8))
-    value = P / t_step
-    fp32_value = P / float(np.mean(w32))
-    algo_ops = 2.0 * p.n * P
-    k8m, k32m = float(np.mean(k8)), float(np.mean(k32))
-    achieved_tops = algo_ops / k8m / 1e12
-    rel32 = float(np.max(np.abs(l2_32 - l2_8) / np.maximum(1.0, np.abs(l2_8))))
-
-    # CPU baseline: the C restatement (f64, the reference default path) on the
-    # first `cpu-sample-snps` SNPs of the same panel, all pairs of that slice.
-    cpu = None
-    sample_err = None
-    if not args.skip_cpu:
-        from oracle import oracle as O
-
-        ms = min(args.cpu_sample_snps, p.m)
-        sub = p.body[:ms]
-        bed_s = np.concatenate([np.array([0x6C, 0x1B, 0x01], np.uint8), sub.reshape(-1)])
-        t = time.perf_counter()
-        exp, _ = O.l2(bed_s, p.n, p.chr[:ms], p.cm[:ms], p.bp[:ms], mode=O.WINDOW_CM,
-                      window=1.0, chunk=200, masking=True, threads=args.cpu_threads)
-        cpu_wall = time.perf_counter() - t
-        with L.Panel(bed_s, p.n, ms) as ps:
-            got, _, perf_s = L.compute_ldscore(ps, snps[:ms], cfg8)
-            got32, _, _ = L.compute_ldscore(ps, snps[:ms], cfg32)
-        sample_err = {
-            "int8_fp64_max_rel": float(np.max(np.abs(got - exp) / np.abs(exp))),
-            "fp32_max_abs": float(np.max(np.abs(got32 - exp))),
-            "fp32_max_rel": float(np.max(np.abs(got32 - exp) / np.abs(exp))),
-        }
-        cpu = {
-            "value": perf_s.pairs_included / cpu_wall,
-            "unit": "SNP-pairs/s",
-            "cores": args.cpu_threads,
-            "kind": "port",
-            "sample": f"first {ms} SNPs of the same panel (N={p.n}, cM 1, masked), "
-                      f"{perf_s.pairs_included} pairs, FP64 restatement, {cpu_wall:.1f}s",
-        }
-        log(f"cpu baseline: {cpu['value']:.3e} pairs/s over {cpu_wall:.1f}s; sample errors "
-            f"{sample_err}")
-
+    c2 = leg_c2(args, L, synth)
+    serving = None if args.skip_serving else leg_serving(args, L, synth)
     out = {
-        "metric": "SNP-pairs/s int8-MFMA vs FP32 plain (≤1e-6 rel); p50/p99 LD-request latency",
-        "value": value,
+        "metric": METRIC,
+        "value": c2["value"],
         "unit": "SNP-pairs/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": t_step * 1e3,
+        "ms_per_step": c2["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -165,25 +342,19 @@ def main() -> int:
         "data": "synthetic (seeded founder-haplotype generator)",
         "config": {"workload": "exact LD scores, N=50000 x M=100000, 1% missing, "
                                "--ld-wind-cm 1, chunk 200, --snp-level-masking",
-                   "n_indiv": p.n, "m_snps": p.m, "pairs": P,
-                   "pairs_evaluated": perf8.pairs_evaluated, "tiles": perf8.tiles},
-        "fp32_plain": {"value": fp32_value, "unit": "SNP-pairs/s",
-                       "ms_per_step": float(np.mean(w32)) * 1e3,
-                       "band_kernel_ms": k32m * 1e3,
-                       "tflops_algorithmic": algo_ops / k32m / 1e12,
-                       "frac_of_fp32_peak": algo_ops / k32m / 1e12 / FP32_PEAK_TFLOPS,
-                       "max_rel_vs_int8": rel32},
-        "int8_over_fp32": value / fp32_value,
-        "ld_scored_snps_per_s": p.m / t_step,
-        "roofline": {"bound": "mfma", "kernel": "k_band_i8", "achieved": achieved_tops,
-                     "peak": INT8_PEAK_TOPS, "unit": "TFLOP/s", "frac": achieved_tops / INT8_PEAK_TOPS,
-                     "traffic": profiled_traffic("k_band_i8"), "band_kernel_ms": k8m * 1e3,
-                     "ops_per_launch": algo_ops},
-        "cpu_baseline": cpu,
-        "sample_accuracy": sample_err,
+                   "n_indiv": args.n, "m_snps": args.m, "pairs": c2["pairs"],
+                   "pairs_evaluated": c2["pairs_evaluated"], "tiles": c2["tiles"]},
+        "fp32_plain": c2["fp32"],
+        "int8_over_fp32": c2["value"] / c2["fp32"]["value"],
+        "ld_scored_snps_per_s": c2["ld_scored_snps_per_s"],
+        "phases_ms": {"tile_stats": c2["tile_ms"], "band": c2["roofline"]["band_kernel_ms"],
+                      "reduce": c2["reduce_ms"], "host_prep": c2["host_prep_ms"]},
+        "roofline": c2["roofline"],
+        "cpu_baseline": c2.get("cpu_baseline"),
+        "sample_accuracy": c2.get("sample_accuracy"),
+        "serving": serving,
     }
     print(json.dumps(out), flush=True)
-    panel.close()
     return 0
 
 

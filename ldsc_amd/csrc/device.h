@@ -58,7 +58,7 @@ namespace ldsc {
 // Device-side CountSketch plan for one sketch dimension d (see sketch.hip).
 struct SketchPlan {
     uint64_t d = 0, d_pad = 0, nblocks = 0, ntiles = 0;
-    int32_t* d_fam = nullptr;   // FAM row of member k (members sorted by bucket)
+    int32_t* d_fam = nullptr;   // FAM row of each member (bucket order), lane-major map
     void* d_ptiles = nullptr;   // ntiles x 1 KiB signed int8 MFMA A-operand tiles
     int4* d_blocks = nullptr;   // per bucket block: first tile, first K-step, K-steps
     void release();
@@ -70,7 +70,7 @@ struct ldsc_panel {
     uint64_t n_fam = 0, m_bim = 0, bps = 0, n = 0;
     uint8_t* d_slabs = nullptr;  // m_bim x bps bytes, the .bed body
     int64_t* d_keep = nullptr;   // n FAM rows, or nullptr (all individuals)
-    int32_t* d_keep32 = nullptr; // the same rows as int32 (gather map)
+    int32_t* d_keep32 = nullptr; // the same rows as an int32 lane-major gather map
     std::vector<int64_t> keep;
     std::mutex sketch_mu;
     std::vector<ldsc::SketchPlan*> sketch_plans;
@@ -79,9 +79,10 @@ struct ldsc_panel {
 namespace ldsc {
 
 // Tiles the call's SNP rows into ws->tiled and counts their integer stats
-// into ws->stats.  `d_map` (nullable) lists, for each of the panel's N
+// into ws->stats.  `d_map` (nullable) gives, for each of the panel's N
 // individuals in the order wanted, its FAM row (--keep order, or the
-// CountSketch bucket order); it is applied by an LDS-staged row gather.
+// CountSketch bucket order) in the lane-major layout of lane_major_map; it
+// is applied by an LDS-staged row gather.
 int gather_and_tile(const ldsc_panel* panel, const uint64_t* d_bed_idx, uint64_t m,
                     const int32_t* d_map, uint64_t m_pad, Workspace* ws, char* err,
                     uint64_t err_len);

@@ -124,44 +124,53 @@ void release_workspace(Workspace* w) {
 // ------------------------------------------------------------------------
 // Kernel 1a: row gather in a chosen individual order (--keep, bucket order)
 // ------------------------------------------------------------------------
-// One SNP row at a time is staged in LDS (coalesced loads), then each thread
-// assembles 16 consecutive members of the output row from random LDS reads
-// and writes one dword.  Output rows are `bps_out` bytes (a multiple of 4);
-// members at or beyond `n` get code 01 (missing).
-__global__ __launch_bounds__(256) void k_permute_rows(const uint8_t* __restrict__ slabs,
-                                                       uint64_t bps_in,
+// R SNP rows at a time are staged in LDS (coalesced loads).  Each thread owns
+// output dword q (members 16q .. 16q+15): it loads its 16 source rows from the
+// lane-major map once (16 coalesced loads) and assembles that dword for all R
+// rows from random LDS reads.  Output rows are `bps_out` = 4 * n16 bytes;
+// members at or beyond N get code 01 (missing).
+template <int R>
+__global__ __launch_bounds__(512) void k_permute_rows(const uint8_t* __restrict__ slabs,
+                                                       uint64_t bps_in, uint32_t ldsrow,
                                                        const uint64_t* __restrict__ bed_idx,
-                                                       uint32_t m, const int32_t* __restrict__ map,
-                                                       uint32_t n, uint64_t bps_out,
-                                                       uint8_t* __restrict__ out) {
-    extern __shared__ uint8_t row[];
+                                                       uint32_t m, const int32_t* __restrict__ map_t,
+                                                       uint32_t n16, uint8_t* __restrict__ out) {
+    extern __shared__ uint8_t rows[];
     const int tid = threadIdx.x;
     const bool aligned = (bps_in & 3) == 0;
-    for (uint32_t j = blockIdx.x; j < m; j += gridDim.x) {
-        const uint8_t* src = slabs + bed_idx[j] * bps_in;
+    for (uint32_t j0 = blockIdx.x * R; j0 < m; j0 += gridDim.x * R) {
+        const uint32_t nr = m - j0 < (uint32_t)R ? m - j0 : (uint32_t)R;
         __syncthreads();
-        if (aligned) {
-            const uint32_t* s4 = reinterpret_cast<const uint32_t*>(src);
-            uint32_t* r4 = reinterpret_cast<uint32_t*>(row);
-            for (uint64_t q = tid; q < bps_in / 4; q += blockDim.x) r4[q] = s4[q];
-        } else {
-            for (uint64_t q = tid; q < bps_in; q += blockDim.x) row[q] = src[q];
+        for (uint32_t r = 0; r < nr; ++r) {
+            const uint8_t* src = slabs + bed_idx[j0 + r] * bps_in;
+            uint8_t* dst = rows + (size_t)r * ldsrow;
+            if (aligned) {
+                const uint32_t* s4 = reinterpret_cast<const uint32_t*>(src);
+                uint32_t* d4 = reinterpret_cast<uint32_t*>(dst);
+                for (uint64_t q = tid; q < bps_in / 4; q += blockDim.x) d4[q] = s4[q];
+            } else {
+                for (uint64_t q = tid; q < bps_in; q += blockDim.x) dst[q] = src[q];
+            }
         }
         __syncthreads();
-        uint32_t* dst = reinterpret_cast<uint32_t*>(out + (uint64_t)j * bps_out);
-        for (uint64_t q = tid; q < bps_out / 4; q += blockDim.x) {
-            uint32_t w = 0;
-            const uint64_t k0 = q * 16;
-#pragma unroll 4
-            for (uint32_t e = 0; e < 16; ++e) {
-                uint32_t code = 1u;
-                if (k0 + e < n) {
-                    const uint32_t r = (uint32_t)map[k0 + e];
-                    code = (row[r >> 2] >> (2 * (r & 3))) & 3u;
+        for (uint32_t q = tid; q < n16; q += blockDim.x) {
+            int32_t sr[16];
+#pragma unroll
+            for (int e = 0; e < 16; ++e) sr[e] = map_t[(size_t)e * n16 + q];
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                if ((uint32_t)r < nr) {
+                    const uint8_t* row = rows + (size_t)r * ldsrow;
+                    uint32_t w = 0;
+#pragma unroll
+                    for (int e = 0; e < 16; ++e) {
+                        const uint32_t code =
+                            sr[e] >= 0 ? (row[sr[e] >> 2] >> (2 * (sr[e] & 3))) & 3u : 1u;
+                        w |= code << (2 * e);
+                    }
+                    reinterpret_cast<uint32_t*>(out + (uint64_t)(j0 + r) * n16 * 4)[q] = w;
                 }
-                w |= code << (2 * e);
             }
-            dst[q] = w;
         }
     }
 }
@@ -262,10 +271,21 @@ __global__ void k_snp_info(const uint32_t* __restrict__ stats, uint32_t m_pad, u
     if (musc) musc[j] = make_float2(a32, i32);
 }
 
-// Fill A_eff with ones (no --annot, no pq) for rows < m, zero padding rows.
-__global__ void k_fill_ones(double* __restrict__ a, uint32_t m, uint32_t m_pad) {
-    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j < m_pad) a[j] = j < m ? 1.0 : 0.0;
+// A_eff in place (m_pad x K, column stride m_pad): the uploaded annotation
+// (or ones when `ones`), NaN -> 0 (read_annot, parse.rs:224), times pq_j
+// when given (compute.rs:1595-1616); padding rows are zero.
+__global__ void k_aeff(double* __restrict__ a, const double* __restrict__ pq, uint32_t m,
+                       uint32_t m_pad, uint32_t K, int ones) {
+    const uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (uint64_t)m_pad * K) return;
+    const uint32_t j = (uint32_t)(idx % m_pad);
+    double v = 0.0;
+    if (j < m) {
+        v = ones ? 1.0 : a[idx];
+        if (v != v) v = 0.0;
+        if (pq) v *= pq[j];
+    }
+    a[idx] = v;
 }
 
 __global__ __launch_bounds__(64) void k_band_i8(BandArgs a) {
@@ -545,20 +565,29 @@ int gather_and_tile(const ldsc_panel* panel, const uint64_t* d_bed_idx, uint64_t
     const uint64_t* rows = d_bed_idx;
     if (d_map) {
         // gather rows in the requested individual order, then tile them
-        const uint64_t bps_out = round_up((n + 3) / 4, 4);
-        if (panel->bps > 160 * 1024)
+        const uint32_t n16 = (uint32_t)((n + 15) / 16);
+        const uint64_t bps_out = 4ull * n16;
+        const uint32_t ldsrow = (uint32_t)round_up(panel->bps, 16);
+        if (ldsrow > 160 * 1024)
             return fail(err, err_len, LDSC_ERR_INVALID_ARG,
                         "individual subsets / sketching support at most 655360 FAM rows");
         HIPC(ws->scratch.ensure((size_t)m * bps_out));
-        const size_t lds = round_up(panel->bps, 4);
+        int R = (int)std::max<uint32_t>(1, (96 * 1024) / ldsrow);
+        R = R >= 8 ? 8 : (R >= 4 ? 4 : (R >= 2 ? 2 : 1));
+        const size_t lds = (size_t)R * ldsrow;
+        const void* kfn = R == 8 ? (const void*)k_permute_rows<8>
+                        : R == 4 ? (const void*)k_permute_rows<4>
+                        : R == 2 ? (const void*)k_permute_rows<2> : (const void*)k_permute_rows<1>;
         if (lds > 64 * 1024)
-            HIPC(hipFuncSetAttribute((const void*)k_permute_rows,
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-        const uint32_t grid = (uint32_t)std::min<uint64_t>(m, 8192);
-        hipLaunchKernelGGL(k_permute_rows, dim3(grid), dim3(256), lds, st, panel->d_slabs,
-                           panel->bps, d_bed_idx, (uint32_t)m, d_map, (uint32_t)n, bps_out,
-                           ws->scratch.as<uint8_t>());
-        HIPC(hipGetLastError());
+            HIPC(hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        const uint32_t grid = (uint32_t)std::min<uint64_t>((m + R - 1) / R, 4096);
+        const uint8_t* a0 = panel->d_slabs;
+        uint8_t* a7 = ws->scratch.as<uint8_t>();
+        uint64_t a1 = panel->bps;
+        uint32_t a4 = (uint32_t)m;
+        void* kargs[] = {(void*)&a0, (void*)&a1, (void*)&ldsrow, (void*)&d_bed_idx, (void*)&a4,
+                         (void*)&d_map, (void*)&n16, (void*)&a7};
+        HIPC(hipLaunchKernel(kfn, dim3(grid), dim3(512), kargs, lds, st));
         src = ws->scratch.as<uint8_t>();
         bps = bps_out;
         rows = nullptr;
@@ -729,20 +758,22 @@ int run_l2(const ldsc_panel* panel, const ldsc_snp* snps, uint64_t m, const ldsc
         HIPC(hipMemcpyAsync(ws->maf.p, kap.data(), kap.size() * 8, hipMemcpyHostToDevice, st));
         HIPC(hipStreamSynchronize(st));
     }
-    if (annot || has_pq) {
-        std::vector<double> ae((size_t)m_pad * K, 0.0);
-        for (uint64_t k = 0; k < K; ++k)
-            for (uint64_t j = 0; j < m; ++j) {
-                double a = annot ? annot[k * m + j] : 1.0;
-                if (std::isnan(a)) a = 0.0;  // read_annot maps NaN to 0 (parse.rs:224)
-                if (has_pq) a *= std::pow(maf[j] * (1.0 - maf[j]), cfg->pq_exp);
-                ae[k * m_pad + j] = a;
-            }
-        HIPC(hipMemcpyAsync(ws->aeff.p, ae.data(), ae.size() * 8, hipMemcpyHostToDevice, st));
-        HIPC(hipStreamSynchronize(st));  // `ae` is pageable and local
-    } else {
-        hipLaunchKernelGGL(k_fill_ones, dim3((m_pad + 255) / 256), dim3(256), 0, st,
-                           ws->aeff.as<double>(), (uint32_t)m, m_pad);
+    if (has_pq) {
+        // (maf (1 - maf))^S on the host with libm pow, like the reference
+        std::vector<double> pq(m);
+        for (uint64_t j = 0; j < m; ++j) pq[j] = std::pow(maf[j] * (1.0 - maf[j]), cfg->pq_exp);
+        HIPC(ws->annot.ensure(m * 8));
+        HIPC(hipMemcpyAsync(ws->annot.p, pq.data(), m * 8, hipMemcpyHostToDevice, st));
+        HIPC(hipStreamSynchronize(st));  // `pq` is pageable and local
+    }
+    if (annot)
+        HIPC(hipMemcpy2DAsync(ws->aeff.p, (size_t)m_pad * 8, annot, (size_t)m * 8, (size_t)m * 8,
+                              K, hipMemcpyHostToDevice, st));
+    {
+        const uint64_t tot = (uint64_t)m_pad * K;
+        hipLaunchKernelGGL(k_aeff, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st,
+                           ws->aeff.as<double>(), has_pq ? ws->annot.as<double>() : nullptr,
+                           (uint32_t)m, m_pad, (uint32_t)K, annot ? 0 : 1);
         HIPC(hipGetLastError());
     }
 
@@ -868,9 +899,10 @@ extern "C" int ldsc_l2_open_panel(const uint8_t* bed, uint64_t bed_len, uint64_t
         p->keep.assign(keep_idx, keep_idx + n_keep);
         e = hipMalloc(&p->d_keep, n_keep * 8);
         if (e == hipSuccess) e = hipMemcpy(p->d_keep, keep_idx, n_keep * 8, hipMemcpyHostToDevice);
-        std::vector<int32_t> k32(keep_idx, keep_idx + n_keep);
-        if (e == hipSuccess) e = hipMalloc(&p->d_keep32, n_keep * 4);
-        if (e == hipSuccess) e = hipMemcpy(p->d_keep32, k32.data(), n_keep * 4, hipMemcpyHostToDevice);
+        const std::vector<int32_t> k32 =
+            lane_major_map(std::vector<int32_t>(keep_idx, keep_idx + n_keep));
+        if (e == hipSuccess) e = hipMalloc(&p->d_keep32, k32.size() * 4);
+        if (e == hipSuccess) e = hipMemcpy(p->d_keep32, k32.data(), k32.size() * 4, hipMemcpyHostToDevice);
     }
     if (e != hipSuccess) {
         ldsc_l2_close_panel(p);

@@ -36,6 +36,16 @@ int check_config(const ldsc_l2_config* cfg, char* err, uint64_t err_len);
 void pair_bounds(const ldsc_snp* snps, uint64_t m, const ldsc_l2_config* cfg,
                  std::vector<uint64_t>& lo, bool* any_full);
 
+// Gather map in lane-major layout: map_t[e * n16 + q] is the FAM row of
+// output member 16q + e (-1 past the last member), so that the 16 loads a
+// thread makes for its output dword are each coalesced across the wave.
+inline std::vector<int32_t> lane_major_map(const std::vector<int32_t>& rows) {
+    const uint64_t n = rows.size(), n16 = (n + 15) / 16;
+    std::vector<int32_t> t(16 * n16, -1);
+    for (uint64_t k = 0; k < n; ++k) t[(k % 16) * n16 + k / 16] = rows[k];
+    return t;
+}
+
 // fastrand 2.3.0 wyrand stream of CountSketchProj::new (compute.rs:179-192).
 void countsketch_hash(uint64_t n, uint64_t d, uint64_t seed, uint32_t* bucket, int8_t* sign);
 

@@ -59,9 +59,10 @@ int sketch_plan(const ldsc_panel* panel_c, uint64_t d, SketchPlan** out, char* e
     std::vector<uint64_t> fill(off.begin(), off.end() - 1);
     std::vector<uint32_t> perm(n);
     for (uint64_t i = 0; i < n; ++i) perm[fill[bucket[i]]++] = (uint32_t)i;
-    std::vector<int32_t> fam(n);
+    std::vector<int32_t> fam_rows(n);
     for (uint64_t k = 0; k < n; ++k)
-        fam[k] = panel->keep.empty() ? (int32_t)perm[k] : (int32_t)panel->keep[perm[k]];
+        fam_rows[k] = panel->keep.empty() ? (int32_t)perm[k] : (int32_t)panel->keep[perm[k]];
+    const std::vector<int32_t> fam = lane_major_map(fam_rows);
 
     auto* sp = new SketchPlan();
     sp->d = d;
@@ -95,8 +96,8 @@ int sketch_plan(const ldsc_panel* panel_c, uint64_t d, SketchPlan** out, char* e
         }
     }
     hipError_t e = hipSetDevice(panel->device);
-    if (e == hipSuccess) e = hipMalloc(&sp->d_fam, n * 4);
-    if (e == hipSuccess) e = hipMemcpy(sp->d_fam, fam.data(), n * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMalloc(&sp->d_fam, fam.size() * 4);
+    if (e == hipSuccess) e = hipMemcpy(sp->d_fam, fam.data(), fam.size() * 4, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMalloc(&sp->d_ptiles, tiles.size());
     if (e == hipSuccess) e = hipMemcpy(sp->d_ptiles, tiles.data(), tiles.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMalloc(&sp->d_blocks, blocks.size() * sizeof(int4));

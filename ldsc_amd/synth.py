@@ -41,17 +41,19 @@ def _L():
 class SynthPanel:
     n: int
     m: int
-    body: np.ndarray     # m x ceil(n/4) packed .bed body (no header)
+    buf: np.ndarray      # the whole .bed file: 3-byte header + body
     chr: np.ndarray
     bp: np.ndarray
     cm: np.ndarray
     maf_true: np.ndarray
 
+    @property
+    def body(self) -> np.ndarray:
+        """m x ceil(n/4) packed SNP rows (a view into ``buf``)."""
+        return self.buf[3:].reshape(self.m, (self.n + 3) // 4)
+
     def bed_bytes(self) -> np.ndarray:
-        out = np.empty(self.body.size + 3, np.uint8)
-        out[:3] = (0x6C, 0x1B, 0x01)
-        out[3:] = self.body.reshape(-1)
-        return out
+        return self.buf
 
 
 def make_panel(n: int, m: int, seed: int, miss_rate: float = 0.01, n_founders: int = 20,
@@ -63,9 +65,12 @@ def make_panel(n: int, m: int, seed: int, miss_rate: float = 0.01, n_founders: i
     fnd = np.zeros(m, np.uint32)
     L.synth_snps(seed, m, n_founders, maf, bp, cm, fnd)
     bps = (n + 3) // 4
-    body = np.zeros(m * bps, np.uint8)
+    buf = np.empty(m * bps + 3, np.uint8)
+    buf[:3] = (0x6C, 0x1B, 0x01)
+    body = buf[3:]
+    body[:] = 0
     L.synth_bed(seed, n, m, n_founders, switch_p, miss_rate, fnd, body)
-    return SynthPanel(n, m, body.reshape(m, bps), np.ones(m, np.uint8), bp, cm, maf)
+    return SynthPanel(n, m, buf, np.ones(m, np.uint8), bp, cm, maf)
 
 
 def make_annot(m: int, seed: int, n_bin: int = 40, n_cont: int = 10,
