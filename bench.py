@@ -85,22 +85,26 @@ def leg_c2(args, L, synth):
     cfg8, cfg32 = L.L2Config(**base), L.L2Config(**base, use_f32=True)
     panel = L.Panel(p.bed_bytes(), p.n, p.m)
     w8, pf8, (l2_8, _) = timed_steps(L, panel, snps, cfg8, args.steps, args.warmup, "int8")
-    w32, pf32, (l2_32, _) = timed_steps(L, panel, snps, cfg32, args.fp32_steps, 1, "fp32")
+    if args.fp32_steps > 0:
+        w32, pf32, (l2_32, _) = timed_steps(L, panel, snps, cfg32, args.fp32_steps, 1, "fp32")
     panel.close()
     P = pf8[-1].pairs_included
     t_step = float(np.mean(w8))
     k8 = float(np.mean([q.bb_dot_secs for q in pf8]))
-    k32 = float(np.mean([q.bb_dot_secs for q in pf32]))
     algo = 2.0 * p.n * P
+    fp32 = None
+    if args.fp32_steps > 0:
+        k32 = float(np.mean([q.bb_dot_secs for q in pf32]))
+        fp32 = {"value": P / float(np.mean(w32)), "unit": "SNP-pairs/s",
+                "ms_per_step": float(np.mean(w32)) * 1e3, "band_kernel_ms": k32 * 1e3,
+                "tflops_algorithmic": algo / k32 / 1e12,
+                "frac_of_fp32_peak": algo / k32 / 1e12 / FP32_PEAK_TFLOPS,
+                "max_rel_vs_int8": float(np.max(np.abs(l2_32 - l2_8) /
+                                                np.maximum(1.0, np.abs(l2_8))))}
     res = {
         "value": P / t_step, "ms_per_step": t_step * 1e3, "pairs": P,
         "pairs_evaluated": pf8[-1].pairs_evaluated, "tiles": pf8[-1].tiles,
-        "fp32": {"value": P / float(np.mean(w32)), "unit": "SNP-pairs/s",
-                 "ms_per_step": float(np.mean(w32)) * 1e3, "band_kernel_ms": k32 * 1e3,
-                 "tflops_algorithmic": algo / k32 / 1e12,
-                 "frac_of_fp32_peak": algo / k32 / 1e12 / FP32_PEAK_TFLOPS,
-                 "max_rel_vs_int8": float(np.max(np.abs(l2_32 - l2_8) /
-                                                 np.maximum(1.0, np.abs(l2_8))))},
+        "fp32": fp32,
         "tile_ms": float(np.mean([q.norm_secs for q in pf8])) * 1e3,
         "reduce_ms": float(np.mean([q.ring_store_secs for q in pf8])) * 1e3,
         "host_prep_ms": float(np.mean([q.bed_read_secs for q in pf8])) * 1e3,
@@ -345,7 +349,7 @@ def main() -> int:
                    "n_indiv": args.n, "m_snps": args.m, "pairs": c2["pairs"],
                    "pairs_evaluated": c2["pairs_evaluated"], "tiles": c2["tiles"]},
         "fp32_plain": c2["fp32"],
-        "int8_over_fp32": c2["value"] / c2["fp32"]["value"],
+        "int8_over_fp32": c2["value"] / c2["fp32"]["value"] if c2["fp32"] else None,
         "ld_scored_snps_per_s": c2["ld_scored_snps_per_s"],
         "phases_ms": {"tile_stats": c2["tile_ms"], "band": c2["roofline"]["band_kernel_ms"],
                       "reduce": c2["reduce_ms"], "host_prep": c2["host_prep_ms"]},

@@ -37,6 +37,16 @@ struct DevBuf {
     T* as() const { return static_cast<T*>(p); }
 };
 
+// Grow-only pinned host staging buffer (hipHostMalloc).
+struct PinnedBuf {
+    void* p = nullptr;
+    size_t cap = 0;
+    hipError_t ensure(size_t bytes);
+    void release();
+    template <class T>
+    T* as() const { return static_cast<T*>(p); }
+};
+
 // One in-flight computation: its own stream, events and grow-only buffers.
 struct Workspace {
     int device = 0;
@@ -44,6 +54,7 @@ struct Workspace {
     hipEvent_t ev[8] = {};
     DevBuf tiled, stats, bed_idx, lo, info, aeff, annot, maf, tiles, j0, tstart, rowp, colp,
         l2, sk_bucket, sk_sign, sk_cols, sk_perm, sk_off, scratch;
+    PinnedBuf pinned;
     size_t bytes() const;
     hipError_t init(int dev);
     void destroy();

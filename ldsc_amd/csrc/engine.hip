@@ -59,6 +59,25 @@ void DevBuf::release() {
     cap = 0;
 }
 
+hipError_t PinnedBuf::ensure(size_t bytes) {
+    if (bytes <= cap && p) return hipSuccess;
+    release();
+    const size_t want = std::max<size_t>(bytes, 4096);
+    hipError_t e = hipHostMalloc(&p, want, hipHostMallocDefault);
+    if (e != hipSuccess) {
+        p = nullptr;
+        cap = 0;
+        return e;
+    }
+    cap = want;
+    return hipSuccess;
+}
+void PinnedBuf::release() {
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    cap = 0;
+}
+
 size_t Workspace::bytes() const {
     const DevBuf* all[] = {&tiled, &stats, &bed_idx, &lo, &info, &aeff, &annot, &maf, &tiles,
                            &j0, &tstart, &rowp, &colp, &l2, &sk_bucket, &sk_sign, &sk_cols,
@@ -86,6 +105,7 @@ void Workspace::destroy() {
                      &j0, &tstart, &rowp, &colp, &l2, &sk_bucket, &sk_sign, &sk_cols,
                      &sk_perm, &sk_off, &scratch};
     for (auto* b : all) b->release();
+    pinned.release();
     for (auto& x : ev)
         if (x) (void)hipEventDestroy(x);
     if (stream) (void)hipStreamDestroy(stream);
@@ -288,80 +308,162 @@ __global__ void k_aeff(double* __restrict__ a, const double* __restrict__ pq, ui
     a[idx] = v;
 }
 
-__global__ __launch_bounds__(64) void k_band_i8(BandArgs a) {
-    __shared__ EpiShared sh;
+// Epilogue storage of the two-wave band kernel: each wave stages one 16-row
+// quarter of its 32 x 64 r2u block at a time (~31 KiB per workgroup, so four
+// workgroups -- two waves per SIMD -- fit in a CU's LDS).
+struct EpiShared2 {
+    double R[2][16][65];
+    double ar[64][8];     // A_eff of the tile's row SNPs (chunk of 8 annotations)
+    double ac[64][8];     // A_eff of the tile's column SNPs
+    double cx[64][8];     // wave 1's column partials
+    int2 csr[64], csc[64];
+    double rsr[64], rsc[64];
+    uint32_t lor[64];
+};
+
+// r2u of wave w (rows 32w .. 32w+31 of the tile, all 64 columns, in MFMA
+// output order r2u[bj][r]) -> per-tile partial sums, fixed summation order.
+__device__ void band_contract2(const BandArgs& a, EpiShared2& sh, int tile, uint32_t I,
+                               uint32_t J, int w, const double (&r2u)[2][16]) {
+    const int lane = threadIdx.x & 63;
+    for (uint32_t kc0 = 0; kc0 < a.K; kc0 += 8) {
+        const uint32_t kcn = a.K - kc0 < 8 ? a.K - kc0 : 8;
+        __syncthreads();
+        for (uint32_t kk = 0; kk < kcn; ++kk) {
+            if (w == 0)
+                sh.ar[lane][kk] = a.aeff[(size_t)(kc0 + kk) * a.m_pad + (size_t)I * kTile + lane];
+            else
+                sh.ac[lane][kk] = a.aeff[(size_t)(kc0 + kk) * a.m_pad + (size_t)J * kTile + lane];
+        }
+        double cacc[8];
+#pragma unroll
+        for (int kk = 0; kk < 8; ++kk) cacc[kk] = 0.0;
+#pragma unroll
+        for (int Q = 0; Q < 2; ++Q) {
+            __syncthreads();
+#pragma unroll
+            for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+                for (int i = 0; i < 8; ++i) {
+                    const int r = 8 * Q + i;
+                    const int local = 8 * ((r >> 2) - 2 * Q) + 4 * (lane >> 5) + (r & 3);
+                    sh.R[w][local][32 * bj + (lane & 31)] = r2u[bj][r];
+                }
+            __syncthreads();
+            // row role: row lane%16 of the quarter, columns 16*(lane/16) .. +16
+            double racc[8];
+#pragma unroll
+            for (int kk = 0; kk < 8; ++kk) racc[kk] = 0.0;
+            const int rr = lane & 15, b0 = 16 * (lane >> 4);
+            for (int b = 0; b < 16; ++b) {
+                const double rv = sh.R[w][rr][b0 + b];
+#pragma unroll
+                for (int kk = 0; kk < 8; ++kk)
+                    if ((uint32_t)kk < kcn) racc[kk] += rv * sh.ac[b0 + b][kk];
+            }
+#pragma unroll
+            for (int kk = 0; kk < 8; ++kk)
+                if ((uint32_t)kk < kcn) {
+                    racc[kk] += __shfl_xor(racc[kk], 16);
+                    racc[kk] += __shfl_xor(racc[kk], 32);
+                }
+            if (lane < 16)
+                for (uint32_t kk = 0; kk < kcn; ++kk)
+                    a.rowp[((size_t)tile * a.K + kc0 + kk) * kTile + 32 * w + 16 * Q + lane] = racc[kk];
+            // column role: lane = column, the 16 rows of this quarter
+            for (int r = 0; r < 16; ++r) {
+                const double rv = sh.R[w][r][lane];
+#pragma unroll
+                for (int kk = 0; kk < 8; ++kk)
+                    if ((uint32_t)kk < kcn) cacc[kk] += rv * sh.ar[32 * w + 16 * Q + r][kk];
+            }
+        }
+        __syncthreads();
+        if (w == 1)
+            for (uint32_t kk = 0; kk < kcn; ++kk) sh.cx[lane][kk] = cacc[kk];
+        __syncthreads();
+        if (w == 0)
+            for (uint32_t kk = 0; kk < kcn; ++kk)
+                a.colp[((size_t)tile * a.K + kc0 + kk) * kTile + lane] = cacc[kk] + sh.cx[lane][kk];
+    }
+}
+
+// int8 form, two waves per 64x64 tile: wave w owns rows 32w .. 32w+31 (SNP
+// group 2I + w) against both column groups; 4 products x 2 blocks of
+// 32x32 int32 accumulators (128 registers), so two waves fit per SIMD and
+// one wave's code expansion overlaps the other's MFMAs.
+__global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_band_i8(BandArgs a) {
+    __shared__ EpiShared2 sh;
     const int tile = tile_of_block(a);
     if (tile < 0) return;
     const int2 tl = a.tiles[tile];
     const uint32_t I = (uint32_t)tl.x, J = (uint32_t)tl.y;
-    const int lane = threadIdx.x;
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const size_t gstride = (size_t)a.nkb * 64;
-    const u32x4* pa0 = a.tiled + (size_t)(2 * I) * gstride + lane;
-    const u32x4* pa1 = pa0 + gstride;
+    const u32x4* pa = a.tiled + (size_t)(2 * I + w) * gstride + lane;
     const u32x4* pb0 = a.tiled + (size_t)(2 * J) * gstride + lane;
     const u32x4* pb1 = pb0 + gstride;
 
-    v16i acc[4][2][2];
+    v16i acc[4][2];
 #pragma unroll
     for (int p = 0; p < 4; ++p)
 #pragma unroll
-        for (int bi = 0; bi < 2; ++bi)
+        for (int bj = 0; bj < 2; ++bj)
 #pragma unroll
-            for (int bj = 0; bj < 2; ++bj)
-#pragma unroll
-                for (int r = 0; r < 16; ++r) acc[p][bi][bj][r] = 0;
+            for (int r = 0; r < 16; ++r) acc[p][bj][r] = 0;
 
-    u32x4 ca0 = pa0[0], ca1 = pa1[0], cb0 = pb0[0], cb1 = pb1[0];
+    u32x4 ca = pa[0], cb0 = pb0[0], cb1 = pb1[0];
     for (uint32_t kb = 0; kb < a.nkb; ++kb) {
         const size_t nk = (size_t)(kb + 1 < a.nkb ? kb + 1 : kb) * 64;
-        const u32x4 na0 = pa0[nk], na1 = pa1[nk], nb0 = pb0[nk], nb1 = pb1[nk];
+        const u32x4 na = pa[nk], nb0 = pb0[nk], nb1 = pb1[nk];
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
-            v4i ga[2], ma[2], gb[2], mb[2];
-            expand_codes(ca0[t], ga[0], ma[0]);
-            expand_codes(ca1[t], ga[1], ma[1]);
+            v4i ga, ma, gb[2], mb[2];
+            expand_codes(ca[t], ga, ma);
             expand_codes(cb0[t], gb[0], mb[0]);
             expand_codes(cb1[t], gb[1], mb[1]);
 #pragma unroll
-            for (int bi = 0; bi < 2; ++bi)
-#pragma unroll
-                for (int bj = 0; bj < 2; ++bj) {
-                    acc[0][bi][bj] = __builtin_amdgcn_mfma_i32_32x32x32_i8(ga[bi], gb[bj], acc[0][bi][bj], 0, 0, 0);
-                    acc[1][bi][bj] = __builtin_amdgcn_mfma_i32_32x32x32_i8(ga[bi], mb[bj], acc[1][bi][bj], 0, 0, 0);
-                    acc[2][bi][bj] = __builtin_amdgcn_mfma_i32_32x32x32_i8(ma[bi], gb[bj], acc[2][bi][bj], 0, 0, 0);
-                    acc[3][bi][bj] = __builtin_amdgcn_mfma_i32_32x32x32_i8(ma[bi], mb[bj], acc[3][bi][bj], 0, 0, 0);
-                }
+            for (int bj = 0; bj < 2; ++bj) {
+                acc[0][bj] = __builtin_amdgcn_mfma_i32_32x32x32_i8(ga, gb[bj], acc[0][bj], 0, 0, 0);
+                acc[1][bj] = __builtin_amdgcn_mfma_i32_32x32x32_i8(ga, mb[bj], acc[1][bj], 0, 0, 0);
+                acc[2][bj] = __builtin_amdgcn_mfma_i32_32x32x32_i8(ma, gb[bj], acc[2][bj], 0, 0, 0);
+                acc[3][bj] = __builtin_amdgcn_mfma_i32_32x32x32_i8(ma, mb[bj], acc[3][bj], 0, 0, 0);
+            }
         }
-        ca0 = na0;
-        ca1 = na1;
+        ca = na;
         cb0 = nb0;
         cb1 = nb1;
     }
 
-    stage_info(a, sh, I, J);
+    if (w == 0) {
+        sh.csr[lane] = a.cs[(size_t)I * kTile + lane];
+        sh.rsr[lane] = a.rs[(size_t)I * kTile + lane];
+        sh.lor[lane] = a.lo[(size_t)I * kTile + lane];
+    } else {
+        sh.csc[lane] = a.cs[(size_t)J * kTile + lane];
+        sh.rsc[lane] = a.rs[(size_t)J * kTile + lane];
+    }
     __syncthreads();
     // FP64 finish from exact integers:
     //   T  = C_r (C_c GG - S_c GM) - S_r (C_c MG - S_c MM)  = C_r C_c * cov
     //   r  = T / sqrt(C_r Q_r C_c Q_c),  r2u = r^2 (1 + 1/D) - 1/D
-    double r2u[2][2][16];
+    double r2u[2][16];
 #pragma unroll
-    for (int bi = 0; bi < 2; ++bi)
+    for (int bj = 0; bj < 2; ++bj)
 #pragma unroll
-        for (int bj = 0; bj < 2; ++bj)
-#pragma unroll
-            for (int r = 0; r < 16; ++r) {
-                const int row = row_of(bi, r, lane), col = 32 * bj + (lane & 31);
-                const int2 cr = sh.csr[row], cc = sh.csc[col];
-                const double u = (double)cc.x * (double)acc[0][bi][bj][r] -
-                                 (double)cc.y * (double)acc[1][bi][bj][r];
-                const double v = (double)cc.x * (double)acc[2][bi][bj][r] -
-                                 (double)cc.y * (double)acc[3][bi][bj][r];
-                const double T = (double)cr.x * u - (double)cr.y * v;
-                const double rr = T * sh.rsr[row] * sh.rsc[col];
-                const double val = rr * rr * a.r2u_a + a.r2u_b;
-                r2u[bi][bj][r] = pair_included(a, sh, I, J, row, col) ? val : 0.0;
-            }
-    band_contract(a, sh, tile, I, J, r2u);
+        for (int r = 0; r < 16; ++r) {
+            const int row = 32 * w + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+            const int col = 32 * bj + (lane & 31);
+            const int2 cr = sh.csr[row], cc = sh.csc[col];
+            const double u = (double)cc.x * (double)acc[0][bj][r] - (double)cc.y * (double)acc[1][bj][r];
+            const double v = (double)cc.x * (double)acc[2][bj][r] - (double)cc.y * (double)acc[3][bj][r];
+            const double T = (double)cr.x * u - (double)cr.y * v;
+            const double rr = T * sh.rsr[row] * sh.rsc[col];
+            const double val = rr * rr * a.r2u_a + a.r2u_b;
+            const uint32_t jj = I * kTile + row, ii = J * kTile + col;
+            r2u[bj][r] = (ii < jj && jj < a.m && ii >= sh.lor[row]) ? val : 0.0;
+        }
+    band_contract2(a, sh, tile, I, J, w, r2u);
 }
 
 __global__ __launch_bounds__(64) void k_band_f32(BandArgs a) {
@@ -634,16 +736,7 @@ int run_l2(const ldsc_panel* panel, const ldsc_snp* snps, uint64_t m, const ldsc
         return fail(err, err_len, LDSC_ERR_SKETCH_RANGE, "--sketch %llu too large",
                     (unsigned long long)d);
 
-    std::vector<uint64_t> lo;
-    bool any_full = false;
-    pair_bounds(snps, m, cfg, lo, &any_full);
-    if (!cfg->yes_really && any_full)
-        printf("WARNING: LD window spans the entire chromosome. "
-               "Use --yes-really to silence this warning.\n");
-    BandPlan plan;
-    plan_band(lo, m, plan);
-    const uint32_t m_pad = plan.m_pad;
-
+    const uint32_t m_pad = (uint32_t)round_up(m, kTile);
     HIPC(hipSetDevice(panel->device));
     hipStream_t st = ws->stream;
 
@@ -653,22 +746,13 @@ int run_l2(const ldsc_panel* panel, const ldsc_snp* snps, uint64_t m, const ldsc
         if (rc) return rc;
     }
 
-    std::vector<uint64_t> bidx(m);
-    for (uint64_t j = 0; j < m; ++j) bidx[j] = snps[j].bed_idx;
-    std::vector<uint32_t> lo32(m_pad);
-    for (uint64_t j = 0; j < m_pad; ++j) lo32[j] = j < m ? (uint32_t)lo[j] : (uint32_t)j;
-    const double t_prep = now_s();
-
+    // 1. gather + tile + stats (+ CountSketch), queued before the host work
+    //    below so the device tiles while the host plans the band.
+    HIPC(ws->pinned.ensure((size_t)m * 8));
+    uint64_t* h_bidx = ws->pinned.as<uint64_t>();
+    for (uint64_t j = 0; j < m; ++j) h_bidx[j] = snps[j].bed_idx;
     HIPC(ws->bed_idx.ensure((size_t)m * 8));
-    HIPC(ws->lo.ensure((size_t)m_pad * 4));
     HIPC(ws->info.ensure((size_t)m_pad * 48));
-    HIPC(ws->aeff.ensure((size_t)m_pad * K * 8));
-    HIPC(ws->tiles.ensure((size_t)plan.ntiles * 8));
-    HIPC(ws->j0.ensure((size_t)plan.nb * 4));
-    HIPC(ws->tstart.ensure((size_t)(plan.nb + 1) * 4));
-    HIPC(ws->rowp.ensure((size_t)plan.ntiles * K * kTile * 8));
-    HIPC(ws->colp.ensure((size_t)plan.ntiles * K * kTile * 8));
-    HIPC(ws->l2.ensure((size_t)m * K * 8));
     if (d) {
         HIPC(ws->sk_cols.ensure((size_t)m_pad * sp->d_pad * 4));
         HIPC(ws->maf.ensure((size_t)m_pad * 8));  // kappa (maf-aware sketch)
@@ -677,16 +761,7 @@ int run_l2(const ldsc_panel* panel, const ldsc_snp* snps, uint64_t m, const ldsc
     double* d_rs = reinterpret_cast<double*>(ws->info.as<char>() + (size_t)m_pad * 8);
     float4* d_lut = reinterpret_cast<float4*>(ws->info.as<char>() + (size_t)m_pad * 16);
     float2* d_musc = reinterpret_cast<float2*>(ws->info.as<char>() + (size_t)m_pad * 32);
-
-    HIPC(hipMemcpyAsync(ws->bed_idx.p, bidx.data(), m * 8, hipMemcpyHostToDevice, st));
-    HIPC(hipMemcpyAsync(ws->lo.p, lo32.data(), (size_t)m_pad * 4, hipMemcpyHostToDevice, st));
-    HIPC(hipMemcpyAsync(ws->tiles.p, plan.tiles.data(), (size_t)plan.ntiles * 8,
-                        hipMemcpyHostToDevice, st));
-    HIPC(hipMemcpyAsync(ws->j0.p, plan.j0.data(), (size_t)plan.nb * 4, hipMemcpyHostToDevice, st));
-    HIPC(hipMemcpyAsync(ws->tstart.p, plan.tstart.data(), (size_t)(plan.nb + 1) * 4,
-                        hipMemcpyHostToDevice, st));
-
-    // 1. gather + tile + stats (+ CountSketch)
+    HIPC(hipMemcpyAsync(ws->bed_idx.p, h_bidx, m * 8, hipMemcpyHostToDevice, st));
     HIPC(hipEventRecord(ws->ev[0], st));
     rc = gather_and_tile(panel, ws->bed_idx.as<uint64_t>(), m, d ? sp->d_fam : panel->d_keep32,
                          m_pad, ws, err, err_len);
@@ -702,21 +777,59 @@ int run_l2(const ldsc_panel* panel, const ldsc_snp* snps, uint64_t m, const ldsc
     }
     HIPC(hipEventRecord(ws->ev[2], st));
 
-    // 2. MAF (and pq-scaled annotations, kappa) on the host from the exact stats
+    // 2. host: pair bounds and the band's tile list
+    std::vector<uint64_t> lo;
+    bool any_full = false;
+    pair_bounds(snps, m, cfg, lo, &any_full);
+    if (!cfg->yes_really && any_full)
+        printf("WARNING: LD window spans the entire chromosome. "
+               "Use --yes-really to silence this warning.\n");
+    BandPlan plan;
+    plan_band(lo, m, plan);
+    const size_t off_lo = 0, off_tiles = round_up((size_t)m_pad * 4, 16);
+    const size_t off_j0 = off_tiles + round_up((size_t)plan.ntiles * 8, 16);
+    const size_t off_ts = off_j0 + round_up((size_t)plan.nb * 4, 16);
+    const size_t off_st = off_ts + round_up((size_t)(plan.nb + 1) * 4, 16);
+    const size_t meta_bytes = off_st + (size_t)m_pad * 16;
+    // the staging buffer is reused (and may grow): wait for the bed_idx copy
+    HIPC(hipStreamSynchronize(st));
+    HIPC(ws->pinned.ensure(meta_bytes));
+    char* hp = ws->pinned.as<char>();
+    uint32_t* h_lo = reinterpret_cast<uint32_t*>(hp + off_lo);
+    for (uint64_t j = 0; j < m_pad; ++j) h_lo[j] = j < m ? (uint32_t)lo[j] : (uint32_t)j;
+    std::memcpy(hp + off_tiles, plan.tiles.data(), (size_t)plan.ntiles * 8);
+    std::memcpy(hp + off_j0, plan.j0.data(), (size_t)plan.nb * 4);
+    std::memcpy(hp + off_ts, plan.tstart.data(), (size_t)(plan.nb + 1) * 4);
+    const double t_prep = now_s();
+
+    HIPC(ws->lo.ensure((size_t)m_pad * 4));
+    HIPC(ws->aeff.ensure((size_t)m_pad * K * 8));
+    HIPC(ws->tiles.ensure((size_t)plan.ntiles * 8));
+    HIPC(ws->j0.ensure((size_t)plan.nb * 4));
+    HIPC(ws->tstart.ensure((size_t)(plan.nb + 1) * 4));
+    HIPC(ws->rowp.ensure((size_t)plan.ntiles * K * kTile * 8));
+    HIPC(ws->colp.ensure((size_t)plan.ntiles * K * kTile * 8));
+    HIPC(ws->l2.ensure((size_t)m * K * 8));
+    HIPC(hipMemcpyAsync(ws->lo.p, h_lo, (size_t)m_pad * 4, hipMemcpyHostToDevice, st));
+    HIPC(hipMemcpyAsync(ws->tiles.p, hp + off_tiles, (size_t)plan.ntiles * 8,
+                        hipMemcpyHostToDevice, st));
+    HIPC(hipMemcpyAsync(ws->j0.p, hp + off_j0, (size_t)plan.nb * 4, hipMemcpyHostToDevice, st));
+    HIPC(hipMemcpyAsync(ws->tstart.p, hp + off_ts, (size_t)(plan.nb + 1) * 4,
+                        hipMemcpyHostToDevice, st));
+
+    // 3. MAF (and pq-scaled annotations, kappa) on the host from the exact
+    //    stats; fetched now only when the device needs something derived
+    //    from them, else with the result at the end.
     const bool use_f32 = cfg->precision == LDSC_PREC_FP32;
     const bool has_pq = !std::isnan(cfg->pq_exp);
     const bool maf_aware = d && cfg->sketch_maf_aware;
-    std::vector<uint32_t> hstats;
-    if (maf_out || has_pq || maf_aware) {
-        hstats.resize((size_t)m_pad * 4);
-        HIPC(hipMemcpyAsync(hstats.data(), ws->stats.p, (size_t)m_pad * 16, hipMemcpyDeviceToHost, st));
-        HIPC(hipStreamSynchronize(st));
-    }
-    std::vector<double> maf(hstats.empty() ? 0 : m);
-    if (!maf.empty()) {
-        // f32 exact path: (f32)maf (normalize.rs:153).  Sketch: f64 on the fused
-        // path of a BED-contiguous chunk (compute.rs:639), (f32)maf otherwise
-        // (compute.rs:1509-1511).
+    uint32_t* h_stats = reinterpret_cast<uint32_t*>(hp + off_st);
+    std::vector<double> maf;
+    auto compute_maf = [&]() {
+        // f32 exact path: (f32)maf (normalize.rs:153).  Sketch: f64 on the
+        // fused path of a BED-contiguous chunk (compute.rs:639), (f32)maf
+        // otherwise (compute.rs:1509-1511).
+        maf.assign(m, 0.0);
         std::vector<uint8_t> round32(m, use_f32 && !d ? 1 : 0);
         if (d) {
             uint64_t run0 = 0;
@@ -726,26 +839,32 @@ int run_l2(const ldsc_panel* panel, const ldsc_snp* snps, uint64_t m, const ldsc
                     run1 = run0 + 1;
                     while (run1 < m && snps[run1].chr == snps[run0].chr) ++run1;
                 }
-                for (uint64_t s = run0; s < run1; s += cfg->chunk_size) {
-                    const uint64_t e = std::min<uint64_t>(s + cfg->chunk_size, run1);
-                    const bool contig = snps[e - 1].bed_idx == snps[s].bed_idx + (e - s) - 1;
-                    for (uint64_t j = s; j < e; ++j) round32[j] = contig ? 0 : 1;
+                for (uint64_t s0 = run0; s0 < run1; s0 += cfg->chunk_size) {
+                    const uint64_t e = std::min<uint64_t>(s0 + cfg->chunk_size, run1);
+                    const bool contig = snps[e - 1].bed_idx == snps[s0].bed_idx + (e - s0) - 1;
+                    for (uint64_t j = s0; j < e; ++j) round32[j] = contig ? 0 : 1;
                 }
                 run0 = run1;
             }
         }
         for (uint64_t j = 0; j < m; ++j) {
-            const double v = maf_from_stats(hstats[4 * j], hstats[4 * j + 1]);
+            const double v = maf_from_stats(h_stats[4 * j], h_stats[4 * j + 1]);
             maf[j] = round32[j] ? (double)(float)v : v;
         }
+    };
+    const bool early_stats = has_pq || maf_aware;
+    if (early_stats) {
+        HIPC(hipMemcpyAsync(h_stats, ws->stats.p, (size_t)m_pad * 16, hipMemcpyDeviceToHost, st));
+        HIPC(hipStreamSynchronize(st));
+        compute_maf();
     }
     if (maf_aware) {
         // per_snp_kappa (compute.rs:525-538) from the integer moments:
         // sum g^3 = 8 n2 + het, sum g^4 = 16 n2 + het, n2 = (sum - het) / 2
         std::vector<double> kap(m_pad, 0.0);
         for (uint64_t j = 0; j < m; ++j) {
-            const uint32_t S = hstats[4 * j], C = hstats[4 * j + 1], SS = hstats[4 * j + 2],
-                           H = hstats[4 * j + 3];
+            const uint32_t S = h_stats[4 * j], C = h_stats[4 * j + 1], SS = h_stats[4 * j + 2],
+                           H = h_stats[4 * j + 3];
             const uint32_t n2 = (S - H) / 2;
             const double cube = 8.0 * n2 + H, fourth = 16.0 * n2 + H;
             if (C == 0) continue;
@@ -810,7 +929,7 @@ int run_l2(const ldsc_panel* panel, const ldsc_snp* snps, uint64_t m, const ldsc
     else if (use_f32)
         hipLaunchKernelGGL(k_band_f32, dim3(8 * ba.per_xcd), dim3(64), 0, st, ba);
     else
-        hipLaunchKernelGGL(k_band_i8, dim3(8 * ba.per_xcd), dim3(64), 0, st, ba);
+        hipLaunchKernelGGL(k_band_i8, dim3(8 * ba.per_xcd), dim3(128), 0, st, ba);
     HIPC(hipGetLastError());
     HIPC(hipEventRecord(ws->ev[4], st));
 
@@ -822,9 +941,14 @@ int run_l2(const ldsc_panel* panel, const ldsc_snp* snps, uint64_t m, const ldsc
                        (uint32_t)K, ws->l2.as<double>());
     HIPC(hipGetLastError());
     HIPC(hipEventRecord(ws->ev[5], st));
+    if (maf_out && !early_stats)
+        HIPC(hipMemcpyAsync(h_stats, ws->stats.p, (size_t)m_pad * 16, hipMemcpyDeviceToHost, st));
     HIPC(hipMemcpyAsync(l2_out, ws->l2.p, (size_t)m * K * 8, hipMemcpyDeviceToHost, st));
     HIPC(hipStreamSynchronize(st));
-    if (maf_out) std::memcpy(maf_out, maf.data(), m * 8);
+    if (maf_out) {
+        if (!early_stats) compute_maf();
+        std::memcpy(maf_out, maf.data(), m * 8);
+    }
 
     if (perf) {
         float t01 = 0, t12 = 0, t34 = 0, t45 = 0;
