@@ -416,6 +416,9 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     for (uint32_t kb = 0; kb < a.nkb; ++kb) {
         const size_t nk = (size_t)(kb + 1 < a.nkb ? kb + 1 : kb) * 64;
         const u32x4 na = pa[nk], nb0 = pb0[nk], nb1 = pb1[nk];
+        // keep the next K-block's loads here, a full block of MFMA work ahead
+        // of their use (the scheduler would otherwise sink them to the use)
+        __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
             v4i ga, ma, gb[2], mb[2];
@@ -495,6 +498,7 @@ __global__ __launch_bounds__(64) void k_band_f32(BandArgs a) {
     for (uint32_t kb = 0; kb < a.nkb; ++kb) {
         const size_t nk = (size_t)(kb + 1 < a.nkb ? kb + 1 : kb) * 64;
         const u32x4 na0 = pa0[nk], na1 = pa1[nk], nb0 = pb0[nk], nb1 = pb1[nk];
+        __builtin_amdgcn_sched_barrier(0);  // keep the prefetch a block ahead
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
             const uint32_t wa0 = ca0[t], wa1 = ca1[t], wb0 = cb0[t], wb1 = cb1[t];

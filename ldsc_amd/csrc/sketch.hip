@@ -244,16 +244,25 @@ __global__ __launch_bounds__(64) void k_band_sk(BandArgs a) {
 #pragma unroll
             for (int r = 0; r < 16; ++r) acc[bi][bj][r] = 0.0f;
 
+    float4 xa0[4], xa1[4], xb0[4], xb1[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        xa0[q] = pa0[q];
+        xa1[q] = pa1[q];
+        xb0[q] = pb0[q];
+        xb1[q] = pb1[q];
+    }
     for (uint32_t B = 0; B < a.nkb; ++B) {
-        const size_t o = (size_t)B * 256;
-        float4 xa0[4], xa1[4], xb0[4], xb1[4];
+        const size_t o = (size_t)(B + 1 < a.nkb ? B + 1 : B) * 256;
+        float4 ya0[4], ya1[4], yb0[4], yb1[4];
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            xa0[q] = pa0[o + q];
-            xa1[q] = pa1[o + q];
-            xb0[q] = pb0[o + q];
-            xb1[q] = pb1[o + q];
+            ya0[q] = pa0[o + q];
+            ya1[q] = pa1[o + q];
+            yb0[q] = pb0[o + q];
+            yb1[q] = pb1[o + q];
         }
+        __builtin_amdgcn_sched_barrier(0);  // keep the prefetch a block ahead
 #pragma unroll
         for (int s = 0; s < 16; ++s) {
             const float fa0 = f4c(xa0[s >> 2], s & 3), fa1 = f4c(xa1[s >> 2], s & 3);
@@ -262,6 +271,13 @@ __global__ __launch_bounds__(64) void k_band_sk(BandArgs a) {
             acc[0][1] = __builtin_amdgcn_mfma_f32_32x32x2f32(fa0, fb1, acc[0][1], 0, 0, 0);
             acc[1][0] = __builtin_amdgcn_mfma_f32_32x32x2f32(fa1, fb0, acc[1][0], 0, 0, 0);
             acc[1][1] = __builtin_amdgcn_mfma_f32_32x32x2f32(fa1, fb1, acc[1][1], 0, 0, 0);
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            xa0[q] = ya0[q];
+            xa1[q] = ya1[q];
+            xb0[q] = yb0[q];
+            xb1[q] = yb1[q];
         }
     }
 
