@@ -87,7 +87,7 @@ def write_plink(prefix: str, panel: SynthPanel) -> None:
     panel.bed_bytes().tofile(prefix + ".bed")
     with open(prefix + ".bim", "w") as f:
         for j in range(panel.m):
-            f.write(f"{panel.chr[j]}\trs{j + 1}\t{panel.cm[j]!r}\t{panel.bp[j]}\tA\tG\n")
+            f.write(f"{int(panel.chr[j])}\trs{j + 1}\t{float(panel.cm[j])!r}\t{int(panel.bp[j])}\tA\tG\n")
     with open(prefix + ".fam", "w") as f:
         for i in range(panel.n):
             f.write(f"F{i}\tI{i}\t0\t0\t1\t-9\n")
