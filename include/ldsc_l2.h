@@ -42,14 +42,24 @@ enum {
 enum { LDSC_WINDOW_CM = 0, LDSC_WINDOW_KB = 1, LDSC_WINDOW_SNP = 2 };
 
 /* Arithmetic form of the exact path.
- *  INT8_FP64: exact int8-MFMA integer products (G·G, G·M, M·G, M·M with int32
- *             accumulation) finished in FP64.  Replaces the reference default
- *             f64 GEMM path (compute.rs:1755-1810, 2102-2179).
+ *  EXACT:     exact integer products (G·G, G·M, M·G, M·M) finished in FP64.
+ *             Replaces the reference default f64 GEMM path
+ *             (compute.rs:1755-1810, 2102-2179).  The products run on the
+ *             e2m1 (FP4) MFMA with f32 accumulation, exact for N <= 4,000,000
+ *             individuals, and on the int8 MFMA with int32 accumulation
+ *             beyond; both give the same integers, hence the same result.
  *  FP32:      standardized FP32 columns on FP32 MFMA, r² finished in FP64.
  *             Replaces `--fast-f32` (compute.rs:1668-1753, 1949-2100).
+ *  EXACT_I8:  EXACT with the int8 MFMA kernel at every N.
+ * LDSC_PREC_INT8_FP64 is the former name of EXACT.
  * The CountSketch path (sketch_d > 0) always sketches in FP32 like the
  * reference (`use_f32 = fast_f32 || sketch`, mod.rs:393). */
-enum { LDSC_PREC_INT8_FP64 = 0, LDSC_PREC_FP32 = 1 };
+enum {
+    LDSC_PREC_EXACT = 0,
+    LDSC_PREC_INT8_FP64 = 0,
+    LDSC_PREC_FP32 = 1,
+    LDSC_PREC_EXACT_I8 = 2
+};
 
 /* BimRecord (src/parse.rs:500-507): the fields the l2 path reads. */
 typedef struct {

@@ -21,7 +21,8 @@ ERR_NAMES = {
 }
 ERR_PENDING = 10
 WINDOW_CM, WINDOW_KB, WINDOW_SNP = 0, 1, 2
-PREC_INT8_FP64, PREC_FP32 = 0, 1
+PREC_EXACT, PREC_FP32, PREC_EXACT_I8 = 0, 1, 2
+PREC_INT8_FP64 = PREC_EXACT
 
 EXPORTED = [
     "ldsc_l2_open_panel", "ldsc_l2_close_panel", "ldsc_l2_panel_n_indiv", "ldsc_l2_compute",

@@ -8,6 +8,7 @@
 namespace ldsc {
 
 typedef int v4i __attribute__((ext_vector_type(4)));
+typedef int v8i __attribute__((ext_vector_type(8)));
 typedef int v16i __attribute__((ext_vector_type(16)));
 typedef float v16f __attribute__((ext_vector_type(16)));
 typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
@@ -146,13 +147,6 @@ __device__ __forceinline__ float pick_code(const float4& L, uint32_t code) {
 }
 
 
-struct BandPlan {
-    uint32_t m_pad = 0, nb = 0, ntiles = 0;
-    std::vector<int2> tiles;
-    std::vector<int> j0, tstart;
-    uint64_t pairs_included = 0;
-};
-
 inline void plan_band(const std::vector<uint64_t>& lo, uint64_t m, BandPlan& p) {
     p.m_pad = (uint32_t)round_up(m, kTile);
     p.nb = p.m_pad / kTile;
@@ -167,8 +161,6 @@ inline void plan_band(const std::vector<uint64_t>& lo, uint64_t m, BandPlan& p) 
         for (int J = J0; J <= (int)I; ++J) p.tiles.push_back(make_int2((int)I, J));
     }
     p.ntiles = (uint32_t)p.tiles.size();
-    p.pairs_included = 0;
-    for (uint64_t j = 0; j < m; ++j) p.pairs_included += j - lo[j];
 }
 
 // CountSketch (sketch.hip)

@@ -24,6 +24,9 @@ namespace ldsc {
 constexpr int kTile = 64;
 constexpr int kGroup = 32;
 constexpr int kKBlock = 128;
+// Largest N of the e2m1 exact kernel: its f32 accumulators hold G.G / 4 and
+// 4 M.M, multiples of 1/4 up to 4N, exactly while 4N <= 2^24.
+constexpr uint64_t kFp4MaxIndiv = 4000000;
 
 inline uint64_t round_up(uint64_t x, uint64_t a) { return (x + a - 1) / a * a; }
 
@@ -47,6 +50,24 @@ struct PinnedBuf {
     T* as() const { return static_cast<T*>(p); }
 };
 
+// Host-side band plan: tiles (I, J) of the 64x64 band, per block row I its
+// first column block j0[I] and its first tile tstart[I].
+struct BandPlan {
+    uint32_t m_pad = 0, nb = 0, ntiles = 0;
+    std::vector<int2> tiles;
+    std::vector<int> j0, tstart;
+    uint64_t pairs_included = 0;
+};
+
+// Host arrays of a call, kept (with their capacity) from call to call so that
+// a repeated call touches no freshly mapped pages.
+struct HostScratch {
+    std::vector<uint64_t> lo;
+    BandPlan plan;
+    std::vector<double> maf, aux;
+    std::vector<uint8_t> round32;
+};
+
 // One in-flight computation: its own stream, events and grow-only buffers.
 struct Workspace {
     int device = 0;
@@ -54,7 +75,9 @@ struct Workspace {
     hipEvent_t ev[8] = {};
     DevBuf tiled, stats, bed_idx, lo, info, aeff, annot, maf, tiles, j0, tstart, rowp, colp,
         l2, sk_bucket, sk_sign, sk_cols, sk_perm, sk_off, scratch;
-    PinnedBuf pinned;
+    PinnedBuf pinned;       // SNP row indices (upload in flight while the host plans)
+    PinnedBuf pinned_meta;  // pair bounds, band plan, stats read-back
+    HostScratch host;
     size_t bytes() const;
     hipError_t init(int dev);
     void destroy();
@@ -79,7 +102,8 @@ struct SketchPlan {
 struct ldsc_panel {
     int device = 0;
     uint64_t n_fam = 0, m_bim = 0, bps = 0, n = 0;
-    uint8_t* d_slabs = nullptr;  // m_bim x bps bytes, the .bed body
+    uint64_t stride = 0;         // bytes between resident rows: bps rounded up to 16
+    uint8_t* d_slabs = nullptr;  // m_bim rows of the .bed body, `stride` apart, zero padded
     int64_t* d_keep = nullptr;   // n FAM rows, or nullptr (all individuals)
     int32_t* d_keep32 = nullptr; // the same rows as an int32 lane-major gather map
     std::vector<int64_t> keep;
@@ -94,8 +118,9 @@ namespace ldsc {
 // individuals in the order wanted, its FAM row (--keep order, or the
 // CountSketch bucket order) in the lane-major layout of lane_major_map; it
 // is applied by an LDS-staged row gather.
+// `nib` selects the 4-bit tile form of the e2m1 band kernel (engine.hip).
 int gather_and_tile(const ldsc_panel* panel, const uint64_t* d_bed_idx, uint64_t m,
-                    const int32_t* d_map, uint64_t m_pad, Workspace* ws, char* err,
+                    const int32_t* d_map, uint64_t m_pad, bool nib, Workspace* ws, char* err,
                     uint64_t err_len);
 
 // The exact / sketch engine on workspace `ws` (which must belong to

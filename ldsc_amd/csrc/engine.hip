@@ -106,6 +106,7 @@ void Workspace::destroy() {
                      &sk_perm, &sk_off, &scratch};
     for (auto* b : all) b->release();
     pinned.release();
+    pinned_meta.release();
     for (auto& x : ev)
         if (x) (void)hipEventDestroy(x);
     if (stream) (void)hipStreamDestroy(stream);
@@ -151,26 +152,20 @@ void release_workspace(Workspace* w) {
 // members at or beyond N get code 01 (missing).
 template <int R>
 __global__ __launch_bounds__(512) void k_permute_rows(const uint8_t* __restrict__ slabs,
-                                                       uint64_t bps_in, uint32_t ldsrow,
+                                                       uint64_t pitch, uint32_t dwords,
+                                                       uint32_t ldsrow,
                                                        const uint64_t* __restrict__ bed_idx,
                                                        uint32_t m, const int32_t* __restrict__ map_t,
                                                        uint32_t n16, uint8_t* __restrict__ out) {
     extern __shared__ uint8_t rows[];
     const int tid = threadIdx.x;
-    const bool aligned = (bps_in & 3) == 0;
     for (uint32_t j0 = blockIdx.x * R; j0 < m; j0 += gridDim.x * R) {
         const uint32_t nr = m - j0 < (uint32_t)R ? m - j0 : (uint32_t)R;
         __syncthreads();
         for (uint32_t r = 0; r < nr; ++r) {
-            const uint8_t* src = slabs + bed_idx[j0 + r] * bps_in;
-            uint8_t* dst = rows + (size_t)r * ldsrow;
-            if (aligned) {
-                const uint32_t* s4 = reinterpret_cast<const uint32_t*>(src);
-                uint32_t* d4 = reinterpret_cast<uint32_t*>(dst);
-                for (uint64_t q = tid; q < bps_in / 4; q += blockDim.x) d4[q] = s4[q];
-            } else {
-                for (uint64_t q = tid; q < bps_in; q += blockDim.x) dst[q] = src[q];
-            }
+            const uint32_t* s4 = reinterpret_cast<const uint32_t*>(slabs + bed_idx[j0 + r] * pitch);
+            uint32_t* d4 = reinterpret_cast<uint32_t*>(rows + (size_t)r * ldsrow);
+            for (uint32_t q = tid; q < dwords; q += blockDim.x) d4[q] = s4[q];
         }
         __syncthreads();
         for (uint32_t q = tid; q < n16; q += blockDim.x) {
@@ -198,67 +193,154 @@ __global__ __launch_bounds__(512) void k_permute_rows(const uint8_t* __restrict_
 // ------------------------------------------------------------------------
 // Kernel 1b: tile + integer stats
 // ------------------------------------------------------------------------
-// Each workgroup owns one group of 32 SNPs and walks all K-blocks.  Thread
-// tau writes dword (lane = tau/4, t = tau%4) of each 1 KiB tile, so a
-// workgroup writes whole tiles with coalesced stores.  Row j of the source
-// is slabs + (bed_idx ? bed_idx[j] : j) * bps.
+// Workgroup (g, c) owns SNP group g (32 rows) and K-blocks
+// [c * kTsChunk, (c + 1) * kTsChunk).  Each pass stages 8 K-blocks of the 32
+// rows in LDS: wave w reads rows 8w .. 8w+7 with one dword per lane (256
+// contiguous bytes of a row per load), then every thread transposes the
+// dword it owns in each of the 8 output tiles and stores it (each wave
+// stores two whole 128-byte runs of a tile).  Integer stats of the chunk go
+// to their own slot; k_snp_info adds the chunks in a fixed order.  Rows are
+// `pitch` bytes apart and dword-aligned; dwords past the row's last
+// individual read as code 01 (missing).
+constexpr int kTsPass = 8;                 // K-blocks per LDS pass (64 dwords of a row)
+constexpr int kTsChunk = 8 * kTsPass;      // K-blocks per workgroup
+constexpr int kTsStride = 72;              // LDS row stride in dwords, 8 mod 32:
+                                           // the transposing reads hit 32 banks
+// Cells of individuals >= nv (the row's real count; 0 for a padding row)
+// of dword q read as code 01 (missing).
+__device__ __forceinline__ uint32_t ts_mask(uint32_t raw, uint64_t q, uint32_t nv) {
+    const int64_t left = (int64_t)nv - (int64_t)(q * 16);
+    const uint32_t keep = left >= 16 ? 0xFFFFFFFFu
+                        : left <= 0  ? 0u
+                                     : (1u << (2 * (uint32_t)left)) - 1u;
+    return (raw & keep) | (0x55555555u & ~keep);
+}
+
+// 4x4 transpose of 2-bit cells: individual 4f+k moves from bit 8f+2k to bit
+// 8k+2f (two delta swaps).
+__device__ __forceinline__ uint32_t transpose_cells(uint32_t x) {
+    uint32_t d = ((x >> 6) ^ x) & 0x00CC00CCu;
+    x ^= d ^ (d << 6);
+    d = ((x >> 12) ^ x) & 0x0000F0F0u;
+    return x ^ d ^ (d << 12);
+}
+
+// Number of K-block chunks (integer stat slots) of a tiling over n individuals.
+static inline uint32_t tile_parts(uint64_t n) {
+    const uint64_t nkb = round_up(n, kKBlock) / kKBlock;
+    return (uint32_t)((nkb + kTsChunk - 1) / kTsChunk);
+}
+
+// Nibble form of 8 codes (16 bits of a row dword): code of individual e at
+// bits 4e, 4e+1, then each nibble re-coded as the e2m1 pair source
+// 0110 (code 00, dosage 2), 0101 (10, dosage 1), 0100 (11, dosage 0),
+// 0000 (01, missing): masked with 0x3 it is G/2 in e2m1 (1.0, 0.5, 0, 0),
+// masked with 0x4 it is 2M (2.0 unless missing).
+__device__ __forceinline__ uint32_t nibble_codes(uint32_t x) {
+    x &= 0xFFFFu;
+    x = (x | (x << 8)) & 0x00FF00FFu;
+    x = (x | (x << 4)) & 0x0F0F0F0Fu;
+    x = (x | (x << 2)) & 0x33333333u;
+    const uint32_t L = x & 0x11111111u, H = (x >> 1) & 0x11111111u, nL = L ^ 0x11111111u;
+    return (nL & H) | ((nL & ~H & 0x11111111u) << 1) | ((nL | H) << 2);
+}
+
+// NIB = false: 2-bit tiles (1 KiB per group and K-block, device.h).
+// NIB = true: 4-bit tiles for the e2m1 MFMA (2 KiB per group and K-block):
+//   two 1 KiB K-steps of 64 individuals; in step u lane l holds SNP l%32
+//   and individuals u*64 + (l/32)*32 + p, p = 0..31, nibble p at bits
+//   4(p%8) of its dword p/8.
+template <bool NIB>
 __global__ __launch_bounds__(256) void k_tile_stats(const uint8_t* __restrict__ slabs,
-                                                     uint64_t bps, uint32_t n,
+                                                     uint64_t pitch, uint32_t n,
                                                      const uint64_t* __restrict__ bed_idx,
-                                                     uint32_t m, uint32_t nkb,
+                                                     uint32_t m, uint32_t nkb, uint32_t m_pad,
                                                      uint32_t* __restrict__ tiled,
-                                                     uint32_t* __restrict__ stats) {
-    __shared__ uint32_t sst[kGroup * 4];
-    const int tid = threadIdx.x;
-    if (tid < kGroup * 4) sst[tid] = 0;
-    __syncthreads();
-    const uint32_t g = blockIdx.x;
-    const int lane = tid >> 2, t = tid & 3;
-    const uint32_t snp = g * kGroup + (lane & 31);
-    const uint32_t h = (uint32_t)lane >> 5;
-    const bool valid = snp < m;
-    const uint8_t* slab = valid ? slabs + (bed_idx ? bed_idx[snp] : snp) * bps : nullptr;
-    const bool aligned = (bps & 3) == 0;
+                                                     uint32_t* __restrict__ part) {
+    __shared__ uint32_t lds[kGroup * kTsStride];
+    const int tid = threadIdx.x, wave = tid >> 6, lane_ld = tid & 63;
+    const uint32_t g = blockIdx.x, chunk = blockIdx.y;
+    const uint32_t kb_begin = chunk * kTsChunk;
+    const uint32_t kb_end = min(nkb, kb_begin + kTsChunk);
+    // load side: 8 rows per wave; loads are clamped to the row's last dword
+    // (in bounds) and masked when staged
+    const uint32_t* rowp[8];
+    uint32_t nv[8];
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+        const uint32_t snp = g * kGroup + 8 * wave + r;
+        const uint64_t src = snp < m ? (bed_idx ? bed_idx[snp] : snp) : 0;
+        rowp[r] = reinterpret_cast<const uint32_t*>(slabs + src * pitch);
+        nv[r] = snp < m ? n : 0;
+    }
+    const uint64_t qlast = (n - 1) / 16;
+    // store side: thread -> (SNP s, dword t, half h) of every tile; a
+    // half-wave holds 4 SNPs x 4 t x 2 h, so its LDS reads s*72 + 2t + h
+    // fall in 32 distinct banks
+    const int h = tid & 1, t = (tid >> 1) & 3;
+    const int s = 8 * wave + 4 * ((tid >> 5) & 1) + ((tid >> 3) & 3);
+    const int lane = s + 32 * h;
+    // 2-bit: dword t of the lane.  Nibble: row dword d = 2t + h of a K-block
+    // is step u = t/2, half t%2, nibble dwords 2h and 2h+1 of lane s + 32(t%2)
+    uint32_t* out = NIB ? tiled + (size_t)g * nkb * 512 + (t >> 1) * 256 + (s + 32 * (t & 1)) * 4 + 2 * h
+                        : tiled + ((size_t)g * nkb * 64 + lane) * 4 + t;
+    constexpr int kTileDw = NIB ? 512 : 256;
+    const uint32_t* src_lds = lds + s * kTsStride + 2 * t + h;
     uint32_t n2 = 0, n1 = 0, n0 = 0;
-    uint32_t* out = tiled + ((size_t)g * nkb * 64 + lane) * 4 + t;
-    for (uint32_t kb = 0; kb < nkb; ++kb) {
-        const uint32_t base = kb * kKBlock + t * 32 + h * 16;
-        uint32_t w = 0x55555555u;  // 16 x code 01 (missing)
-        if (valid && base < n) {
-            const uint32_t nval = n - base < 16 ? n - base : 16;
-            uint32_t raw = 0;
-            const uint64_t b0 = base >> 2;
-            if (aligned && b0 + 4 <= bps) {
-                raw = *reinterpret_cast<const uint32_t*>(slab + b0);
-            } else {
-                for (int q = 0; q < 4; ++q)
-                    if (b0 + q < bps) raw |= (uint32_t)slab[b0 + q] << (8 * q);
-            }
-            if (nval < 16) {
-                const uint32_t keepmask = (1u << (2 * nval)) - 1u;
-                raw = (raw & keepmask) | (0x55555555u & ~keepmask);
-            }
+    uint32_t v[8];
+    uint64_t q = (uint64_t)kb_begin * 8 + lane_ld;
+#pragma unroll
+    for (int r = 0; r < 8; ++r) v[r] = rowp[r][q < qlast ? q : qlast];
+    for (uint32_t kb0 = kb_begin; kb0 < kb_end; kb0 += kTsPass, q += 8 * kTsPass) {
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < 8; ++r) lds[(8 * wave + r) * kTsStride + lane_ld] = ts_mask(v[r], q, nv[r]);
+        __syncthreads();
+        if (kb0 + kTsPass < kb_end) {
+            const uint64_t qn = q + 8 * kTsPass;
+#pragma unroll
+            for (int r = 0; r < 8; ++r) v[r] = rowp[r][qn < qlast ? qn : qlast];
+        }
+        uint32_t x[kTsPass], y[kTsPass];
+#pragma unroll
+        for (int k = 0; k < kTsPass; ++k) {
+            const uint32_t raw = src_lds[8 * k];
             const uint32_t L = raw & 0x55555555u, H = (raw >> 1) & 0x55555555u;
             n2 += __popc(~(L | H) & 0x55555555u);  // 00: dosage 2
             n1 += __popc(~L & H);                   // 10: dosage 1 (het)
             n0 += __popc(L & H);                    // 11: dosage 0
-            // 4x4 transpose of 2-bit cells: individual 4f+k moves from bit
-            // 8f+2k to bit 8k+2f (two delta swaps).
-            uint32_t x = raw, s;
-            s = ((x >> 6) ^ x) & 0x00CC00CCu;
-            x ^= s ^ (s << 6);
-            s = ((x >> 12) ^ x) & 0x0000F0F0u;
-            x ^= s ^ (s << 12);
-            w = x;
+            if (NIB) {
+                x[k] = nibble_codes(raw);
+                y[k] = nibble_codes(raw >> 16);
+            } else {
+                x[k] = transpose_cells(raw);
+            }
         }
-        out[(size_t)kb * 256] = w;
+        // K-blocks past nkb (last pass) hold code 01 only: counted as
+        // nothing above, and not stored
+        uint32_t* o = out + (size_t)kb0 * kTileDw;
+#pragma unroll
+        for (int k = 0; k < kTsPass; ++k) {
+            if (kb0 + kTsPass <= kb_end || kb0 + k < kb_end) {
+                if (NIB)
+                    *reinterpret_cast<uint2*>(o + k * kTileDw) = make_uint2(x[k], y[k]);
+                else
+                    o[k * kTileDw] = x[k];
+            }
+        }
     }
-    atomicAdd(&sst[(lane & 31) * 4 + 0], 2 * n2 + n1);
-    atomicAdd(&sst[(lane & 31) * 4 + 1], n2 + n1 + n0);
-    atomicAdd(&sst[(lane & 31) * 4 + 2], 4 * n2 + n1);
-    atomicAdd(&sst[(lane & 31) * 4 + 3], n1);
-    __syncthreads();
-    if (tid < kGroup * 4) stats[(size_t)g * kGroup * 4 + tid] = sst[tid];
+    // the 8 threads of a SNP are consecutive: fold them, then one writes
+    uint32_t c[4] = {2 * n2 + n1, n2 + n1 + n0, 4 * n2 + n1, n1};
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        c[q] += __shfl_xor(c[q], 1);
+        c[q] += __shfl_xor(c[q], 2);
+        c[q] += __shfl_xor(c[q], 4);
+    }
+    if ((tid & 7) == 0) {
+        uint4* dst = reinterpret_cast<uint4*>(part) + (size_t)chunk * m_pad + g * kGroup + s;
+        *dst = make_uint4(c[0], c[1], c[2], c[3]);
+    }
 }
 
 // ------------------------------------------------------------------------
@@ -268,12 +350,24 @@ __global__ __launch_bounds__(256) void k_tile_stats(const uint8_t* __restrict__ 
 // C*Q = count^2 * centered_sum_sq).  f32 form: the 4-entry LUT of
 // normalize_col_f32_with_stats (normalize.rs:115-154) indexed by the PLINK
 // code: [(2-mean)inv, 0, (1-mean)inv, (0-mean)inv] in f32.
-__global__ void k_snp_info(const uint32_t* __restrict__ stats, uint32_t m_pad, uint32_t n,
+// `part` holds `nparts` slots of m_pad x 4 integer stats (K-block chunks);
+// their sum is written to `stats` when it is not `part` itself.
+__global__ void k_snp_info(const uint32_t* __restrict__ part, uint32_t nparts,
+                           uint32_t* __restrict__ stats, uint32_t m_pad, uint32_t n,
                            int2* __restrict__ cs, double* __restrict__ rs,
                            float4* __restrict__ lut, float2* __restrict__ musc) {
     const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= m_pad) return;
-    const uint32_t S = stats[4 * j + 0], C = stats[4 * j + 1], SS = stats[4 * j + 2];
+    uint4 a = reinterpret_cast<const uint4*>(part)[j];
+    for (uint32_t c = 1; c < nparts; ++c) {
+        const uint4 b = reinterpret_cast<const uint4*>(part)[(size_t)c * m_pad + j];
+        a.x += b.x;
+        a.y += b.y;
+        a.z += b.z;
+        a.w += b.w;
+    }
+    if (stats != part) reinterpret_cast<uint4*>(stats)[j] = a;
+    const uint32_t S = a.x, C = a.y, SS = a.z;
     cs[j] = make_int2((int)C, (int)S);
     const int64_t Q = (int64_t)C * (int64_t)SS - (int64_t)S * (int64_t)S;
     const int64_t V = (int64_t)C * Q;
@@ -388,6 +482,11 @@ __device__ void band_contract2(const BandArgs& a, EpiShared2& sh, int tile, uint
     }
 }
 
+template <class Acc>
+__device__ __forceinline__ void finish_2w(const BandArgs& a, EpiShared2& sh, int tile, uint32_t I,
+                                          uint32_t J, int w, int lane, const Acc (&acc)[4][2],
+                                          double s_gg, double s_mm);
+
 // int8 form, two waves per 64x64 tile: wave w owns rows 32w .. 32w+31 (SNP
 // group 2I + w) against both column groups; 4 products x 2 blocks of
 // 32x32 int32 accumulators (128 registers), so two waves fit per SIMD and
@@ -438,6 +537,18 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
         cb1 = nb1;
     }
 
+    finish_2w(a, sh, tile, I, J, w, lane, acc, 1.0, 1.0);
+}
+
+// FP64 finish of wave w of a two-wave tile from its exact products
+// (accumulator of product p times its scale is the integer G.G, G.M, M.G,
+// M.M), then the per-tile partial sums:
+//   T  = C_r (C_c GG - S_c GM) - S_r (C_c MG - S_c MM)  = C_r C_c * cov
+//   r  = T / sqrt(C_r Q_r C_c Q_c),  r2u = r^2 (1 + 1/D) - 1/D
+template <class Acc>
+__device__ __forceinline__ void finish_2w(const BandArgs& a, EpiShared2& sh, int tile, uint32_t I,
+                                          uint32_t J, int w, int lane, const Acc (&acc)[4][2],
+                                          double s_gg, double s_mm) {
     if (w == 0) {
         sh.csr[lane] = a.cs[(size_t)I * kTile + lane];
         sh.rsr[lane] = a.rs[(size_t)I * kTile + lane];
@@ -447,9 +558,6 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
         sh.rsc[lane] = a.rs[(size_t)J * kTile + lane];
     }
     __syncthreads();
-    // FP64 finish from exact integers:
-    //   T  = C_r (C_c GG - S_c GM) - S_r (C_c MG - S_c MM)  = C_r C_c * cov
-    //   r  = T / sqrt(C_r Q_r C_c Q_c),  r2u = r^2 (1 + 1/D) - 1/D
     double r2u[2][16];
 #pragma unroll
     for (int bj = 0; bj < 2; ++bj)
@@ -458,8 +566,11 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
             const int row = 32 * w + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
             const int col = 32 * bj + (lane & 31);
             const int2 cr = sh.csr[row], cc = sh.csc[col];
-            const double u = (double)cc.x * (double)acc[0][bj][r] - (double)cc.y * (double)acc[1][bj][r];
-            const double v = (double)cc.x * (double)acc[2][bj][r] - (double)cc.y * (double)acc[3][bj][r];
+            // exact integers (powers-of-two scales)
+            const double gg = (double)acc[0][bj][r] * s_gg, gm = (double)acc[1][bj][r];
+            const double mg = (double)acc[2][bj][r], mm = (double)acc[3][bj][r] * s_mm;
+            const double u = (double)cc.x * gg - (double)cc.y * gm;
+            const double v = (double)cc.x * mg - (double)cc.y * mm;
             const double T = (double)cr.x * u - (double)cr.y * v;
             const double rr = T * sh.rsr[row] * sh.rsc[col];
             const double val = rr * rr * a.r2u_a + a.r2u_b;
@@ -467,6 +578,117 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
             r2u[bj][r] = (ii < jj && jj < a.m && ii >= sh.lor[row]) ? val : 0.0;
         }
     band_contract2(a, sh, tile, I, J, w, r2u);
+}
+
+// e2m1 operands from a 4-bit tile dword quad (nibble form of k_tile_stats):
+// mask 0x3 gives G/2, mask 0x4 gives 2M; the MFMA reads 4 of the 8 dwords.
+__device__ __forceinline__ v8i fp4_operand(const u32x4& v, uint32_t mask) {
+    const v4i x = {(int)(v[0] & mask), (int)(v[1] & mask), (int)(v[2] & mask), (int)(v[3] & mask)};
+    return __builtin_shufflevector(x, x, 0, 1, 2, 3, -1, -1, -1, -1);
+}
+
+// scale bytes 127 (E8M0 2^0) on both operands: a plain e2m1 product
+#define LDSC_MFMA_FP4(A, B, C) \
+    __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(A, B, C, 4, 4, 0, 0x7f7f7f7f, 0, 0x7f7f7f7f)
+
+// K loop of one wave of the e2m1 kernel over the column groups it keeps
+// (U0, U1): per K-block two K-steps of 64 individuals, 4 products each.
+// Two register sets alternate (unrolled by two) so that each K-block's
+// loads are issued a whole block of MFMA work before their use and no
+// register copy waits on them.
+template <bool U0, bool U1>
+struct Fp4Ops {
+    u32x4 a[2], b0[2], b1[2];
+    __device__ __forceinline__ void load(const u32x4* pa, const u32x4* pb0, const u32x4* pb1,
+                                         size_t off) {
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            a[u] = pa[off + 64 * u];
+            if (U0) b0[u] = pb0[off + 64 * u];
+            if (U1) b1[u] = pb1[off + 64 * u];
+        }
+    }
+    __device__ __forceinline__ void mma(v16f (&acc)[4][2]) const {
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            const v8i ga = fp4_operand(a[u], 0x33333333u), ma = fp4_operand(a[u], 0x44444444u);
+            if (U0) {
+                const v8i gb = fp4_operand(b0[u], 0x33333333u), mb = fp4_operand(b0[u], 0x44444444u);
+                acc[0][0] = LDSC_MFMA_FP4(ga, gb, acc[0][0]);
+                acc[1][0] = LDSC_MFMA_FP4(ga, mb, acc[1][0]);
+                acc[2][0] = LDSC_MFMA_FP4(ma, gb, acc[2][0]);
+                acc[3][0] = LDSC_MFMA_FP4(ma, mb, acc[3][0]);
+            }
+            if (U1) {
+                const v8i gb = fp4_operand(b1[u], 0x33333333u), mb = fp4_operand(b1[u], 0x44444444u);
+                acc[0][1] = LDSC_MFMA_FP4(ga, gb, acc[0][1]);
+                acc[1][1] = LDSC_MFMA_FP4(ga, mb, acc[1][1]);
+                acc[2][1] = LDSC_MFMA_FP4(ma, gb, acc[2][1]);
+                acc[3][1] = LDSC_MFMA_FP4(ma, mb, acc[3][1]);
+            }
+        }
+    }
+};
+
+template <bool U0, bool U1>
+__device__ __forceinline__ void fp4_kloop(const u32x4* pa, const u32x4* pb0, const u32x4* pb1,
+                                          uint32_t nkb, v16f (&acc)[4][2]) {
+    Fp4Ops<U0, U1> s0, s1;
+    const uint32_t last = nkb - 1;
+    s0.load(pa, pb0, pb1, 0);
+    for (uint32_t kb = 0; kb < nkb; kb += 2) {
+        s1.load(pa, pb0, pb1, (size_t)min(kb + 1, last) * 128);
+        __builtin_amdgcn_sched_barrier(0);
+        s0.mma(acc);
+        s0.load(pa, pb0, pb1, (size_t)min(kb + 2, last) * 128);
+        __builtin_amdgcn_sched_barrier(0);
+        if (kb + 1 < nkb) s1.mma(acc);
+    }
+}
+
+// e2m1 form of the exact kernel (same tiles, waves and finish as k_band_i8):
+// operands G/2 and 2M in e2m1 on v_mfma_scale_f32_32x32x64_f8f6f4 (scale
+// 1), f32 accumulation.  Every product and partial sum is a multiple of 1/4
+// below 2^22 while N <= 4,000,000, so f32 holds the integer products
+// exactly: GG = 4 acc0, GM = acc1, MG = acc2, MM = acc3 / 4.  A wave skips
+// the column group of its tile that cannot hold an included pair (its
+// columns all precede the smallest pair bound of the wave's rows, or none
+// precedes its last row); those accumulators stay zero and the pair mask
+// drops every pair of the group.
+__global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_band_fp4(BandArgs a) {
+    __shared__ EpiShared2 sh;
+    const int tile = tile_of_block(a);
+    if (tile < 0) return;
+    const int2 tl = a.tiles[tile];
+    const uint32_t I = (uint32_t)tl.x, J = (uint32_t)tl.y;
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const size_t gstride = (size_t)a.nkb * 128;  // u32x4 per SNP group
+    const u32x4* pa = a.tiled + (size_t)(2 * I + w) * gstride + lane;
+    const u32x4* pb0 = a.tiled + (size_t)(2 * J) * gstride + lane;
+    const u32x4* pb1 = pb0 + gstride;
+
+    const uint32_t r0 = I * kTile + 32 * w, c0 = J * kTile;
+    uint32_t lmin = a.lo[r0 + (lane & 31)];
+#pragma unroll
+    for (int o = 16; o >= 1; o >>= 1) lmin = min(lmin, (uint32_t)__shfl_xor((int)lmin, o));
+    lmin = __builtin_amdgcn_readfirstlane(lmin);
+    const bool use0 = c0 + 31 >= lmin && c0 < r0 + 31;
+    const bool use1 = c0 + 63 >= lmin && c0 + 32 < r0 + 31;
+
+    v16f acc[4][2];
+#pragma unroll
+    for (int p = 0; p < 4; ++p)
+#pragma unroll
+        for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[p][bj][r] = 0.0f;
+    if (use0 && use1)
+        fp4_kloop<true, true>(pa, pb0, pb1, a.nkb, acc);
+    else if (use0)
+        fp4_kloop<true, false>(pa, pb0, pb1, a.nkb, acc);
+    else if (use1)
+        fp4_kloop<false, true>(pa, pb0, pb1, a.nkb, acc);
+    finish_2w(a, sh, tile, I, J, w, lane, acc, 4.0, 0.25);
 }
 
 __global__ __launch_bounds__(64) void k_band_f32(BandArgs a) {
@@ -640,8 +862,9 @@ size_t l2_workspace_bytes(const ldsc_panel* panel, uint64_t m, const ldsc_l2_con
     uint64_t tiles = nb * 16;
     if (cfg && cfg->window_mode == LDSC_WINDOW_SNP)
         tiles = nb * (2 + (uint64_t)(cfg->window_value + cfg->chunk_size) / kTile);
-    size_t bytes = m_pad * n_pad / 4;                           // tiled SNP slice
-    bytes += m_pad * (16 + 8 + 4 + 48) + m_pad * K * 8 + m * K * 8;  // stats, idx, lo, info, aeff, l2
+    size_t bytes = m_pad * n_pad / 2;                           // tiled SNP slice (4-bit form)
+    bytes += m_pad * 16 * (1 + tile_parts(panel->n));                // stats + chunk slots
+    bytes += m_pad * (8 + 4 + 48) + m_pad * K * 8 + m * K * 8;       // idx, lo, info, aeff, l2
     bytes += tiles * (kTile * K * 8 * 2 + 8);                   // tile partials + list
     const bool gather = cfg && (cfg->sketch_d || panel->d_keep32);
     if (gather) bytes += m * round_up((panel->n + 3) / 4, 4);   // permuted rows
@@ -658,16 +881,17 @@ static double maf_from_stats(uint32_t S, uint32_t C) {
 }
 
 int gather_and_tile(const ldsc_panel* panel, const uint64_t* d_bed_idx, uint64_t m,
-                    const int32_t* d_map, uint64_t m_pad, Workspace* ws, char* err,
+                    const int32_t* d_map, uint64_t m_pad, bool nib, Workspace* ws, char* err,
                     uint64_t err_len) {
     const uint64_t n = panel->n;
     const uint64_t n_pad = round_up(n, kKBlock);
     const uint32_t nkb = (uint32_t)(n_pad / kKBlock);
     hipStream_t st = ws->stream;
-    HIPC(ws->tiled.ensure((size_t)m_pad * n_pad / 4));
-    HIPC(ws->stats.ensure((size_t)m_pad * 16));
+    const uint32_t nparts = tile_parts(n);
+    HIPC(ws->tiled.ensure((size_t)m_pad * n_pad / (nib ? 2 : 4)));
+    HIPC(ws->stats.ensure((size_t)m_pad * 16 * (1 + nparts)));
     const uint8_t* src = panel->d_slabs;
-    uint64_t bps = panel->bps;
+    uint64_t pitch = panel->stride;
     const uint64_t* rows = d_bed_idx;
     if (d_map) {
         // gather rows in the requested individual order, then tile them
@@ -688,19 +912,20 @@ int gather_and_tile(const ldsc_panel* panel, const uint64_t* d_bed_idx, uint64_t
             HIPC(hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
         const uint32_t grid = (uint32_t)std::min<uint64_t>((m + R - 1) / R, 4096);
         const uint8_t* a0 = panel->d_slabs;
-        uint8_t* a7 = ws->scratch.as<uint8_t>();
-        uint64_t a1 = panel->bps;
-        uint32_t a4 = (uint32_t)m;
-        void* kargs[] = {(void*)&a0, (void*)&a1, (void*)&ldsrow, (void*)&d_bed_idx, (void*)&a4,
-                         (void*)&d_map, (void*)&n16, (void*)&a7};
+        uint8_t* a8 = ws->scratch.as<uint8_t>();
+        uint64_t a1 = panel->stride;
+        uint32_t a2 = (uint32_t)((panel->bps + 3) / 4), a5 = (uint32_t)m;
+        void* kargs[] = {(void*)&a0, (void*)&a1, (void*)&a2, (void*)&ldsrow, (void*)&d_bed_idx,
+                         (void*)&a5, (void*)&d_map, (void*)&n16, (void*)&a8};
         HIPC(hipLaunchKernel(kfn, dim3(grid), dim3(512), kargs, lds, st));
         src = ws->scratch.as<uint8_t>();
-        bps = bps_out;
+        pitch = bps_out;
         rows = nullptr;
     }
-    hipLaunchKernelGGL(k_tile_stats, dim3((unsigned)(m_pad / kGroup)), dim3(256), 0, st, src, bps,
-                       (uint32_t)n, rows, (uint32_t)m, nkb, ws->tiled.as<uint32_t>(),
-                       ws->stats.as<uint32_t>());
+    hipLaunchKernelGGL(nib ? k_tile_stats<true> : k_tile_stats<false>,
+                       dim3((unsigned)(m_pad / kGroup), nparts), dim3(256), 0, st, src, pitch,
+                       (uint32_t)n, rows, (uint32_t)m, nkb, (uint32_t)m_pad,
+                       ws->tiled.as<uint32_t>(), ws->stats.as<uint32_t>() + (size_t)m_pad * 4);
     HIPC(hipGetLastError());
     return LDSC_OK;
 }
@@ -767,10 +992,14 @@ int run_l2(const ldsc_panel* panel, const ldsc_snp* snps, uint64_t m, const ldsc
     float2* d_musc = reinterpret_cast<float2*>(ws->info.as<char>() + (size_t)m_pad * 32);
     HIPC(hipMemcpyAsync(ws->bed_idx.p, h_bidx, m * 8, hipMemcpyHostToDevice, st));
     HIPC(hipEventRecord(ws->ev[0], st));
+    const bool use_f32 = cfg->precision == LDSC_PREC_FP32;
+    // exact form: e2m1 MFMA while f32 holds the products exactly, int8 beyond
+    const bool use_fp4 = !d && cfg->precision == LDSC_PREC_EXACT && n <= kFp4MaxIndiv;
     rc = gather_and_tile(panel, ws->bed_idx.as<uint64_t>(), m, d ? sp->d_fam : panel->d_keep32,
-                         m_pad, ws, err, err_len);
+                         m_pad, use_fp4, ws, err, err_len);
     if (rc) return rc;
     hipLaunchKernelGGL(k_snp_info, dim3((m_pad + 255) / 256), dim3(256), 0, st,
+                       ws->stats.as<uint32_t>() + (size_t)m_pad * 4, tile_parts(n),
                        ws->stats.as<uint32_t>(), m_pad, (uint32_t)n, d_cs, d_rs, d_lut, d_musc);
     HIPC(hipGetLastError());
     HIPC(hipEventRecord(ws->ev[1], st));
@@ -781,26 +1010,42 @@ int run_l2(const ldsc_panel* panel, const ldsc_snp* snps, uint64_t m, const ldsc
     }
     HIPC(hipEventRecord(ws->ev[2], st));
 
-    // 2. host: pair bounds and the band's tile list
-    std::vector<uint64_t> lo;
+    // 2. host: pair bounds and the band's tile list, staged in a pinned
+    //    buffer of their own while the device tiles
+    std::vector<uint64_t>& lo = ws->host.lo;
     bool any_full = false;
     pair_bounds(snps, m, cfg, lo, &any_full);
     if (!cfg->yes_really && any_full)
         printf("WARNING: LD window spans the entire chromosome. "
                "Use --yes-really to silence this warning.\n");
-    BandPlan plan;
+    BandPlan& plan = ws->host.plan;
     plan_band(lo, m, plan);
     const size_t off_lo = 0, off_tiles = round_up((size_t)m_pad * 4, 16);
     const size_t off_j0 = off_tiles + round_up((size_t)plan.ntiles * 8, 16);
     const size_t off_ts = off_j0 + round_up((size_t)plan.nb * 4, 16);
     const size_t off_st = off_ts + round_up((size_t)(plan.nb + 1) * 4, 16);
     const size_t meta_bytes = off_st + (size_t)m_pad * 16;
-    // the staging buffer is reused (and may grow): wait for the bed_idx copy
-    HIPC(hipStreamSynchronize(st));
-    HIPC(ws->pinned.ensure(meta_bytes));
-    char* hp = ws->pinned.as<char>();
+    // nothing in flight reads pinned_meta (the previous call ended with a
+    // synchronize), so it may grow here
+    HIPC(ws->pinned_meta.ensure(meta_bytes));
+    char* hp = ws->pinned_meta.as<char>();
     uint32_t* h_lo = reinterpret_cast<uint32_t*>(hp + off_lo);
-    for (uint64_t j = 0; j < m_pad; ++j) h_lo[j] = j < m ? (uint32_t)lo[j] : (uint32_t)j;
+    {
+        const int T = task_count(m_pad);
+        std::vector<uint64_t> part(T, 0);
+        parallel_tasks(T, [&](int t) {
+            const uint64_t a = (uint64_t)m_pad * t / T, b = (uint64_t)m_pad * (t + 1) / T;
+            uint64_t pairs = 0;
+            for (uint64_t j = a; j < b; ++j) {
+                const uint64_t l = j < m ? lo[j] : j;
+                h_lo[j] = (uint32_t)l;
+                pairs += j - l;
+            }
+            part[t] = pairs;
+        });
+        plan.pairs_included = 0;
+        for (uint64_t v : part) plan.pairs_included += v;
+    }
     std::memcpy(hp + off_tiles, plan.tiles.data(), (size_t)plan.ntiles * 8);
     std::memcpy(hp + off_j0, plan.j0.data(), (size_t)plan.nb * 4);
     std::memcpy(hp + off_ts, plan.tstart.data(), (size_t)(plan.nb + 1) * 4);
@@ -824,17 +1069,17 @@ int run_l2(const ldsc_panel* panel, const ldsc_snp* snps, uint64_t m, const ldsc
     // 3. MAF (and pq-scaled annotations, kappa) on the host from the exact
     //    stats; fetched now only when the device needs something derived
     //    from them, else with the result at the end.
-    const bool use_f32 = cfg->precision == LDSC_PREC_FP32;
     const bool has_pq = !std::isnan(cfg->pq_exp);
     const bool maf_aware = d && cfg->sketch_maf_aware;
     uint32_t* h_stats = reinterpret_cast<uint32_t*>(hp + off_st);
-    std::vector<double> maf;
+    std::vector<double>& maf = ws->host.maf;
     auto compute_maf = [&]() {
         // f32 exact path: (f32)maf (normalize.rs:153).  Sketch: f64 on the
         // fused path of a BED-contiguous chunk (compute.rs:639), (f32)maf
         // otherwise (compute.rs:1509-1511).
-        maf.assign(m, 0.0);
-        std::vector<uint8_t> round32(m, use_f32 && !d ? 1 : 0);
+        maf.resize(m);
+        std::vector<uint8_t>& round32 = ws->host.round32;
+        round32.assign(m, use_f32 && !d ? 1 : 0);
         if (d) {
             uint64_t run0 = 0;
             while (run0 < m) {
@@ -865,7 +1110,8 @@ int run_l2(const ldsc_panel* panel, const ldsc_snp* snps, uint64_t m, const ldsc
     if (maf_aware) {
         // per_snp_kappa (compute.rs:525-538) from the integer moments:
         // sum g^3 = 8 n2 + het, sum g^4 = 16 n2 + het, n2 = (sum - het) / 2
-        std::vector<double> kap(m_pad, 0.0);
+        std::vector<double>& kap = ws->host.aux;
+        kap.assign(m_pad, 0.0);
         for (uint64_t j = 0; j < m; ++j) {
             const uint32_t S = h_stats[4 * j], C = h_stats[4 * j + 1], SS = h_stats[4 * j + 2],
                            H = h_stats[4 * j + 3];
@@ -883,7 +1129,8 @@ int run_l2(const ldsc_panel* panel, const ldsc_snp* snps, uint64_t m, const ldsc
     }
     if (has_pq) {
         // (maf (1 - maf))^S on the host with libm pow, like the reference
-        std::vector<double> pq(m);
+        std::vector<double>& pq = ws->host.aux;
+        pq.resize(m);
         for (uint64_t j = 0; j < m; ++j) pq[j] = std::pow(maf[j] * (1.0 - maf[j]), cfg->pq_exp);
         HIPC(ws->annot.ensure(m * 8));
         HIPC(hipMemcpyAsync(ws->annot.p, pq.data(), m * 8, hipMemcpyHostToDevice, st));
@@ -932,6 +1179,8 @@ int run_l2(const ldsc_panel* panel, const ldsc_snp* snps, uint64_t m, const ldsc
         HIPC(launch_band_sk(ba, st));
     else if (use_f32)
         hipLaunchKernelGGL(k_band_f32, dim3(8 * ba.per_xcd), dim3(64), 0, st, ba);
+    else if (use_fp4)
+        hipLaunchKernelGGL(k_band_fp4, dim3(8 * ba.per_xcd), dim3(128), 0, st, ba);
     else
         hipLaunchKernelGGL(k_band_i8, dim3(8 * ba.per_xcd), dim3(128), 0, st, ba);
     HIPC(hipGetLastError());
@@ -1019,10 +1268,15 @@ extern "C" int ldsc_l2_open_panel(const uint8_t* bed, uint64_t bed_len, uint64_t
     p->n_fam = n_fam;
     p->m_bim = m_bim;
     p->bps = bps;
+    p->stride = round_up(bps, 16);
     p->n = n;
+    // rows 16-byte aligned and padded to whole dwords, so that device reads
+    // of a row are aligned vector loads that never leave the allocation
     hipError_t e = hipGetDevice(&p->device);
-    if (e == hipSuccess) e = hipMalloc(&p->d_slabs, bps * m_bim);
-    if (e == hipSuccess) e = hipMemcpy(p->d_slabs, bed + 3, bps * m_bim, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMalloc(&p->d_slabs, p->stride * m_bim);
+    if (e == hipSuccess) e = hipMemset(p->d_slabs, 0, p->stride * m_bim);
+    if (e == hipSuccess)
+        e = hipMemcpy2D(p->d_slabs, p->stride, bed + 3, bps, bps, m_bim, hipMemcpyHostToDevice);
     if (e == hipSuccess && keep_idx) {
         p->keep.assign(keep_idx, keep_idx + n_keep);
         e = hipMalloc(&p->d_keep, n_keep * 8);
@@ -1096,14 +1350,15 @@ extern "C" int ldsc_l2_decode(const ldsc_panel* panel, const uint64_t* bed_idx, 
                                 (std_f32 ? n * m * 4 : 0) + 256));
         DCHK(hipMemcpyAsync(ws->bed_idx.p, bed_idx, m * 8, hipMemcpyHostToDevice, st));
         hipLaunchKernelGGL(k_stats_only, dim3((unsigned)m), dim3(256), 0, st, panel->d_slabs,
-                           panel->bps, panel->d_keep, (uint32_t)n, ws->bed_idx.as<uint64_t>(),
+                           panel->stride, panel->d_keep, (uint32_t)n, ws->bed_idx.as<uint64_t>(),
                            ws->stats.as<uint32_t>());
         DCHK(hipGetLastError());
         int2* d_cs = ws->info.as<int2>();
         double* d_rs = reinterpret_cast<double*>(ws->info.as<char>() + m * 8);
         float4* d_lut = reinterpret_cast<float4*>(ws->info.as<char>() + m * 16);
         hipLaunchKernelGGL(k_snp_info, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, st,
-                           ws->stats.as<uint32_t>(), (uint32_t)m, (uint32_t)n, d_cs, d_rs, d_lut, nullptr);
+                           ws->stats.as<uint32_t>(), 1u, ws->stats.as<uint32_t>(), (uint32_t)m,
+                           (uint32_t)n, d_cs, d_rs, d_lut, nullptr);
         DCHK(hipGetLastError());
         char* sp = ws->scratch.as<char>();
         int8_t* d_dos = dosage ? reinterpret_cast<int8_t*>(sp) : nullptr;
@@ -1116,7 +1371,7 @@ extern "C" int ldsc_l2_decode(const ldsc_panel* panel, const uint64_t* bed_idx, 
         if (dosage || miss || std_f32) {
             const uint32_t gx = (uint32_t)((round_up(n, 64)) / 64);
             hipLaunchKernelGGL(k_decode, dim3(gx, (unsigned)m), dim3(64), 0, st, panel->d_slabs,
-                               panel->bps, panel->d_keep, (uint32_t)n, ws->bed_idx.as<uint64_t>(),
+                               panel->stride, panel->d_keep, (uint32_t)n, ws->bed_idx.as<uint64_t>(),
                                (uint32_t)m, d_lut, d_dos, d_miss, d_std);
             DCHK(hipGetLastError());
         }

@@ -1,50 +1,185 @@
 // host_windows.cpp -- host-side index work of the l2 path: LD windows, the
 // per-SNP pair bound that encodes the reference's chunk/ring/mask inclusion
 // rule, and the seeded CountSketch hash.
+#include <algorithm>
+#include <atomic>
 #include <cmath>
+#include <condition_variable>
 #include <cstring>
+#include <mutex>
+#include <thread>
 
 #include "internal.h"
 
 namespace ldsc {
 
-// get_block_lefts_f64 (src/l2/window.rs:15-28): first j with |c_i - c_j| <= W,
-// advancing while the distance is strictly greater.
-static void block_lefts_f64(const double* coords, uint64_t m, double max_dist,
-                            uint64_t* out) {
-    uint64_t j = 0;
-    for (uint64_t i = 0; i < m; ++i) {
-        while (j < m && std::fabs(coords[i] - coords[j]) > max_dist) ++j;
+// ------------------------------------------------------------------------
+// Host thread pool
+// ------------------------------------------------------------------------
+namespace {
+class TaskPool {
+  public:
+    TaskPool() {
+        unsigned hw = std::thread::hardware_concurrency();
+        const int nt = (int)std::min<unsigned>(hw > 1 ? hw - 1 : 0, 7);
+        for (int i = 0; i < nt; ++i) std::thread([this] { worker(); }).detach();
+        nthreads_ = nt;
+    }
+    int threads() const { return nthreads_; }
+    void run(int n, const std::function<void(int)>& f) {
+        std::unique_lock<std::mutex> busy(run_mu_, std::try_to_lock);
+        if (!busy.owns_lock() || nthreads_ == 0 || n <= 1) {
+            for (int i = 0; i < n; ++i) f(i);
+            return;
+        }
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            job_ = &f;
+            ntasks_ = n;
+            next_.store(0);
+            active_ = nthreads_;
+            ++gen_;
+        }
+        cv_.notify_all();
+        for (int i; (i = next_.fetch_add(1)) < n;) f(i);
+        std::unique_lock<std::mutex> lk(mu_);
+        done_cv_.wait(lk, [this] { return active_ == 0; });
+        job_ = nullptr;
+    }
+
+  private:
+    void worker() {
+        uint64_t seen = 0;
+        for (;;) {
+            const std::function<void(int)>* f;
+            int n;
+            {
+                std::unique_lock<std::mutex> lk(mu_);
+                cv_.wait(lk, [&] { return gen_ != seen; });
+                seen = gen_;
+                f = job_;
+                n = ntasks_;
+            }
+            for (int i; (i = next_.fetch_add(1)) < n;) (*f)(i);
+            std::lock_guard<std::mutex> lk(mu_);
+            if (--active_ == 0) done_cv_.notify_one();
+        }
+    }
+    std::mutex run_mu_, mu_;
+    std::condition_variable cv_, done_cv_;
+    const std::function<void(int)>* job_ = nullptr;
+    int ntasks_ = 0, active_ = 0, nthreads_ = 0;
+    std::atomic<int> next_{0};
+    uint64_t gen_ = 0;
+};
+
+TaskPool& pool() {
+    static TaskPool* p = new TaskPool();  // never destroyed: workers outlive main
+    return *p;
+}
+}  // namespace
+
+void parallel_tasks(int n, const std::function<void(int)>& f) { pool().run(n, f); }
+
+int task_count(uint64_t items) {
+    if (items < 32768) return 1;
+    return (int)std::min<uint64_t>((uint64_t)pool().threads() + 1, items / 16384);
+}
+
+// ------------------------------------------------------------------------
+// LD windows
+// ------------------------------------------------------------------------
+static inline double coord_of(const ldsc_snp& s, bool cm) {
+    return cm ? s.cm : (double)s.bp / 1000.0;
+}
+
+// get_block_lefts_f64 (src/l2/window.rs:15-28) over i in [i0, i1) of the
+// chromosome run [start, end): first j with |c_i - c_j| <= W, advancing
+// while the distance is strictly greater.  A run whose coordinates are
+// non-decreasing makes "distance > W" a prefix property of j <= i, so the
+// scan may begin anywhere with a binary search for its first j and agrees
+// with the scan from the run's start; other runs must start at i0 = start.
+static void block_lefts_range(const ldsc_snp* snps, uint64_t start, uint64_t i0, uint64_t i1,
+                              uint64_t end, bool cm, double W, bool sorted, uint64_t* out) {
+    uint64_t j = start;
+    if (sorted && i0 > start) {
+        const double ci = coord_of(snps[i0], cm);
+        uint64_t lo = start, hi = i0;  // first j in [start, i0] with !(|ci - cj| > W)
+        while (lo < hi) {
+            const uint64_t mid = lo + (hi - lo) / 2;
+            if (std::fabs(ci - coord_of(snps[mid], cm)) > W) lo = mid + 1;
+            else hi = mid;
+        }
+        j = lo;
+    }
+    for (uint64_t i = i0; i < i1; ++i) {
+        const double ci = coord_of(snps[i], cm);
+        while (j < end && std::fabs(ci - coord_of(snps[j], cm)) > W) ++j;
         out[i] = j;
     }
 }
 
+struct Run {
+    uint64_t start, end;
+};
+
 bool block_lefts(const ldsc_snp* snps, uint64_t m, int mode, double value,
                  std::vector<uint64_t>& bl) {
-    bl.assign(m, 0);
-    bool any_full = false;
-    std::vector<double> coords;
-    std::vector<uint64_t> local;
-    uint64_t start = 0;
-    while (start < m) {  // contiguous runs of one chromosome code (window.rs:41-49)
-        uint64_t end = start + 1;
-        while (end < m && snps[end].chr == snps[start].chr) ++end;
-        const uint64_t len = end - start;
-        if (mode == LDSC_WINDOW_SNP) {
-            const uint64_t half = (uint64_t)value;
-            if (half >= (len > 0 ? len - 1 : 0)) any_full = true;
-            for (uint64_t i = 0; i < len; ++i) bl[start + i] = start + (i > half ? i - half : 0);
-        } else {
-            coords.resize(len);
-            local.resize(len);
-            for (uint64_t i = 0; i < len; ++i)
-                coords[i] = mode == LDSC_WINDOW_CM ? snps[start + i].cm
-                                                   : (double)snps[start + i].bp / 1000.0;
-            block_lefts_f64(coords.data(), len, value, local.data());
-            if (local[len - 1] == 0) any_full = true;
-            for (uint64_t i = 0; i < len; ++i) bl[start + i] = start + local[i];
+    bl.resize(m);
+    if (m == 0) return false;
+    const bool cm = mode == LDSC_WINDOW_CM;
+    // contiguous runs of one chromosome code (window.rs:41-49)
+    std::vector<Run> runs;
+    uint64_t s0 = 0;
+    for (uint64_t k = 1; k < m; ++k)
+        if (snps[k].chr != snps[k - 1].chr) {
+            runs.push_back({s0, k});
+            s0 = k;
         }
-        start = end;
+    runs.push_back({s0, m});
+    const int T = task_count(m);
+    bool any_full = false;
+    if (mode == LDSC_WINDOW_SNP) {
+        const uint64_t half = (uint64_t)value;
+        for (const Run& r : runs)
+            if (half >= r.end - r.start - 1) any_full = true;
+        parallel_tasks(T, [&](int t) {
+            const uint64_t a = m * t / T, b = m * (t + 1) / T;
+            size_t r = 0;
+            for (uint64_t i = a; i < b; ++i) {
+                while (runs[r].end <= i) ++r;
+                const uint64_t li = i - runs[r].start;
+                bl[i] = runs[r].start + (li > half ? li - half : 0);
+            }
+        });
+        return any_full;
+    }
+    // Each task scans its share of every run as if the run's coordinates
+    // were non-decreasing, and checks that they are (NaN counts as not);
+    // runs found otherwise are scanned again from their start afterwards.
+    std::vector<std::vector<size_t>> unsorted_by_task(T);
+    parallel_tasks(T, [&](int t) {
+        const uint64_t a = m * t / T, b = m * (t + 1) / T;
+        for (size_t r = 0; r < runs.size(); ++r) {
+            const Run& R = runs[r];
+            if (R.end <= a || R.start >= b) continue;
+            const uint64_t i0 = std::max(a, R.start), i1 = std::min(b, R.end);
+            for (uint64_t k = std::max(i0, R.start + 1); k < i1; ++k)
+                if (!(coord_of(snps[k], cm) >= coord_of(snps[k - 1], cm))) {
+                    unsorted_by_task[t].push_back(r);
+                    break;
+                }
+            block_lefts_range(snps, R.start, i0, i1, R.end, cm, value, true, bl.data());
+        }
+    });
+    std::vector<uint8_t> unsorted(runs.size(), 0);
+    for (const auto& v : unsorted_by_task)
+        for (size_t r : v) unsorted[r] = 1;
+    for (size_t r = 0; r < runs.size(); ++r) {
+        const Run& R = runs[r];
+        if (unsorted[r])
+            block_lefts_range(snps, R.start, R.start, R.end, R.end, cm, value, false, bl.data());
+        if (bl[R.end - 1] == R.start) any_full = true;
     }
     return any_full;
 }
@@ -61,7 +196,8 @@ int check_config(const ldsc_l2_config* cfg, char* err, uint64_t err_len) {
                     cfg->window_mode);
     if (!(cfg->window_value >= 0.0))
         return fail(err, err_len, LDSC_ERR_INVALID_ARG, "window value must be >= 0");
-    if (cfg->precision != LDSC_PREC_INT8_FP64 && cfg->precision != LDSC_PREC_FP32)
+    if (cfg->precision != LDSC_PREC_EXACT && cfg->precision != LDSC_PREC_FP32 &&
+        cfg->precision != LDSC_PREC_EXACT_I8)
         return fail(err, err_len, LDSC_ERR_INVALID_ARG, "unknown precision %d",
                     cfg->precision);
     return LDSC_OK;
@@ -76,15 +212,14 @@ int check_config(const ldsc_l2_config* cfg, char* err, uint64_t err_len) {
 // zeroes i < bl[j]; a_left(chunk_start(j)) <= bl[j], so masked lo[j] = bl[j].
 void pair_bounds(const ldsc_snp* snps, uint64_t m, const ldsc_l2_config* cfg,
                  std::vector<uint64_t>& lo, bool* any_full) {
-    std::vector<uint64_t> bl;
-    bool full = block_lefts(snps, m, cfg->window_mode, cfg->window_value, bl);
+    // lo starts as block_left and is rewritten in place chunk by chunk: each
+    // chunk reads block_left only at its own start, before writing over it,
+    // so chunks are independent tasks (owned by the task holding their start)
+    const bool full = block_lefts(snps, m, cfg->window_mode, cfg->window_value, lo);
     if (any_full) *any_full = full;
-    lo.assign(m, 0);
-    if (cfg->snp_level_masking) {
-        lo = bl;
-        return;
-    }
+    if (cfg->snp_level_masking || m == 0) return;
     const uint64_t c = cfg->chunk_size;
+    std::vector<std::pair<uint64_t, uint64_t>> runs;  // chunk grids
     uint64_t run0 = 0;
     while (run0 < m) {
         uint64_t run1 = m;
@@ -92,17 +227,28 @@ void pair_bounds(const ldsc_snp* snps, uint64_t m, const ldsc_l2_config* cfg,
             run1 = run0 + 1;
             while (run1 < m && snps[run1].chr == snps[run0].chr) ++run1;
         }
-        for (uint64_t s = run0; s < run1; s += c) {
-            const uint64_t sl = s - run0;  // chunk start, local to the run
-            const uint64_t dist = s - bl[s];
-            uint64_t bsz = dist == 0 ? 0 : ((dist + c - 1) / c) * c;
-            if (bsz > sl) bsz = sl;
-            const uint64_t a_left = s - bsz;
-            const uint64_t e = s + c < run1 ? s + c : run1;
-            for (uint64_t j = s; j < e; ++j) lo[j] = a_left;
-        }
+        runs.push_back({run0, run1});
         run0 = run1;
     }
+    const int T = task_count(m);
+    parallel_tasks(T, [&](int t) {
+        const uint64_t a = m * t / T, b = m * (t + 1) / T;
+        for (const auto& rr : runs) {
+            if (rr.second <= a || rr.first >= b) continue;
+            // first chunk start of the run at or after a
+            uint64_t s = rr.first;
+            if (a > s) s += (a - s + c - 1) / c * c;
+            for (; s < rr.second && s < b; s += c) {
+                const uint64_t sl = s - rr.first;  // chunk start, local to the run
+                const uint64_t dist = s - lo[s];
+                uint64_t bsz = dist == 0 ? 0 : ((dist + c - 1) / c) * c;
+                if (bsz > sl) bsz = sl;
+                const uint64_t a_left = s - bsz;
+                const uint64_t e = s + c < rr.second ? s + c : rr.second;
+                for (uint64_t j = s; j < e; ++j) lo[j] = a_left;
+            }
+        }
+    });
 }
 
 // fastrand 2.3.0 (wyrand, published algorithm): gen_u64 adds 0x2d358dccaa6c78a5

@@ -4,6 +4,7 @@
 #include <cstdarg>
 #include <cstdint>
 #include <cstdio>
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -21,6 +22,14 @@ inline int fail(char* err, uint64_t err_len, int code, const char* fmt, ...) {
     }
     return code;
 }
+
+// Runs f(0) .. f(n-1) on a small pool of persistent host threads and the
+// calling thread; returns when all are done.  Calls that find the pool busy
+// (another thread's call in progress) run their tasks inline.
+void parallel_tasks(int n, const std::function<void(int)>& f);
+
+// Number of tasks worth splitting `items` independent items into.
+int task_count(uint64_t items);
 
 // get_block_lefts_by_chr (src/l2/window.rs:32-90).  Returns any_full_chr_window.
 bool block_lefts(const ldsc_snp* snps, uint64_t m, int mode, double value,

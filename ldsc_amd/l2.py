@@ -49,8 +49,10 @@ class L2Config:
     """Knobs of one LD-score computation (mirrors ``L2Config``).
 
     ``use_f32`` selects the FP32 standardized-column form (``--fast-f32``);
-    otherwise the exact form runs as int8 integer products with an FP64
-    finish (the reference's default f64 path).  ``per_chromosome`` restarts
+    otherwise the exact form runs as exact integer products with an FP64
+    finish (the reference's default f64 path): on the e2m1 MFMA for up to
+    4,000,000 individuals, on the int8 MFMA beyond or when
+    ``exact_kernel="i8"``.  ``per_chromosome`` restarts
     the chunk grid at every chromosome run, as ``run()`` does without
     ``--global-pass``; the reference's library API is one global pass.
     """
@@ -65,13 +67,17 @@ class L2Config:
     pq_exp: Optional[float] = None
     verbose_timing: bool = False
     per_chromosome: bool = False
+    exact_kernel: str = "auto"
 
     def to_c(self) -> N.Config:
         c = N.Config()
         c.window_mode = self.mode.kind
         c.window_value = self.mode.value
         c.chunk_size = int(self.chunk_size)
-        c.precision = N.PREC_FP32 if self.use_f32 else N.PREC_INT8_FP64
+        if self.exact_kernel not in ("auto", "i8"):
+            raise ValueError(f"exact_kernel must be 'auto' or 'i8', got {self.exact_kernel!r}")
+        c.precision = (N.PREC_FP32 if self.use_f32 else
+                       N.PREC_EXACT_I8 if self.exact_kernel == "i8" else N.PREC_EXACT)
         c.sketch_d = int(self.sketch or 0)
         c.sketch_maf_aware = int(bool(self.sketch_maf_aware))
         c.snp_level_masking = int(bool(self.snp_level_masking))

@@ -7,9 +7,10 @@ from ldsc_amd import synth
 
 pytestmark = pytest.mark.gpu
 
-# int8 products are exact and finished in FP64: agreement with the FP64
-# restatement is to rounding.  The FP32 form is compared with the f32
-# restatement of --fast-f32 (different summation order).
+# The exact form's integer products (e2m1 or int8 MFMA) are exact and
+# finished in FP64: agreement with the FP64 restatement is to rounding.  The
+# FP32 form is compared with the f32 restatement of --fast-f32 (different
+# summation order).
 TOL_EXACT = 1e-9
 TOL_F32 = 2e-3
 
@@ -35,14 +36,16 @@ def test_reference_smoke_case():
     ("cm", 0.3, True, 50), ("cm", 0.3, False, 50), ("kb", 250.0, False, 37),
     ("snp", 90, True, 64), ("snp", 90, False, 200),
 ])
-@pytest.mark.parametrize("use_f32", [False, True])
-def test_l2_matches_oracle(oracle, mode, window, masking, chunk, use_f32):
+@pytest.mark.parametrize("form", ["exact", "exact_i8", "f32"])
+def test_l2_matches_oracle(oracle, mode, window, masking, chunk, form):
     p = synth.make_panel(n=517, m=1500, seed=11, miss_rate=0.02)
     md = {"cm": L.WindowMode.Cm, "kb": L.WindowMode.Kb, "snp": L.WindowMode.Snp}[mode](window)
     om = {"cm": oracle.WINDOW_CM, "kb": oracle.WINDOW_KB, "snp": oracle.WINDOW_SNP}[mode]
     snps = L.make_snp_array(p.chr, p.cm, p.bp)
-    cfg = L.L2Config(mode=md, chunk_size=chunk, use_f32=use_f32, snp_level_masking=masking)
-    print(f"case mode={mode} masking={masking} chunk={chunk} f32={use_f32}", flush=True)
+    use_f32 = form == "f32"
+    cfg = L.L2Config(mode=md, chunk_size=chunk, use_f32=use_f32, snp_level_masking=masking,
+                     exact_kernel="i8" if form == "exact_i8" else "auto")
+    print(f"case mode={mode} masking={masking} chunk={chunk} form={form}", flush=True)
     with L.Panel(p.bed_bytes(), p.n, p.m) as panel:
         got, maf, perf = L.compute_ldscore(panel, snps, cfg)
     exp, emaf = oracle.l2(p.bed_bytes(), p.n, p.chr, p.cm, p.bp, mode=om, window=window,

@@ -10,7 +10,7 @@ from ldsc_amd import synth
 
 pytestmark = pytest.mark.gpu
 
-TOL_EXACT = 1e-9     # int8 products + FP64 finish vs FP64 restatement (relative)
+TOL_EXACT = 1e-9     # exact integer products + FP64 finish vs FP64 restatement (relative)
 TOL_F32 = 2e-3       # FP32 form vs f32 restatement (relative, floor 1)
 TOL_SKETCH = 2e-5    # exact-integer sketch vs f32 sequential scatter (relative, floor 1)
 
@@ -180,9 +180,51 @@ def test_l2_deterministic_and_chunk_invariant_when_masked():
     assert np.array_equal(a, b) and np.array_equal(a, c)
 
 
-def test_int8_vs_fp32_at_panel_scale():
-    """N=50,000 x M=8,000 slice of the panel-scale workload: the two forms
-    agree to the FP32 tolerance and the result is reproducible bit for bit."""
+def test_exact_kernels_identical():
+    """The e2m1 and int8 MFMA kernels compute the same integer products, so
+    their LD scores agree bit for bit: N around the K-block edges, M off the
+    tile edges, --keep, several annotations with pq, --extract across
+    chromosome runs with and without masking, special columns."""
+    def both(panel, snps, annot=None, **kw):
+        a, ma, _ = L.compute_ldscore(panel, snps, L.L2Config(**kw), annot=annot)
+        b, mb, _ = L.compute_ldscore(panel, snps, L.L2Config(**kw, exact_kernel="i8"), annot=annot)
+        return a, b, ma, mb
+
+    for n, m in ((1, 40), (3, 65), (127, 130), (128, 64), (129, 200), (1000, 777)):
+        print(f"exact kernels n={n} m={m}", flush=True)
+        p = _panel(n, m, seed=n + m)
+        snps = L.make_snp_array(p.chr, p.cm, p.bp)
+        with L.Panel(p.bed_bytes(), n, m) as panel:
+            a, b, ma, mb = both(panel, snps, mode=L.WindowMode.Snp(50), chunk_size=30)
+        assert np.array_equal(a, b) and np.array_equal(ma, mb), (n, m)
+    n, m = 613, 900
+    p = _panel(n, m, seed=21, nchr=3)
+    body = p.body
+    body[10] = body[9]
+    body[20] = 0x00
+    body[30] = 0x55
+    body[40] = 0xFF
+    rng = np.random.default_rng(2)
+    keep = np.sort(rng.choice(n, 500, replace=False))
+    annot = synth.make_annot(m, seed=5, n_bin=3, n_cont=2)
+    snps = L.make_snp_array(p.chr, p.cm, p.bp)
+    sel = np.sort(rng.choice(m, 700, replace=False))
+    snps_sel = L.make_snp_array(p.chr[sel], p.cm[sel], p.bp[sel], bed_idx=sel)
+    with L.Panel(p.bed_bytes(), n, m, keep=keep) as panel:
+        a, b, _, _ = both(panel, snps, annot=annot, mode=L.WindowMode.Kb(120.0), chunk_size=50,
+                          pq_exp=0.75)
+        assert np.array_equal(a, b)
+        for per_chr in (False, True):
+            for masking in (False, True):
+                a, b, _, _ = both(panel, snps_sel, mode=L.WindowMode.Cm(0.1), chunk_size=40,
+                                  per_chromosome=per_chr, snp_level_masking=masking)
+                assert np.array_equal(a, b), (per_chr, masking)
+
+
+def test_exact_and_fp32_at_panel_scale():
+    """N=50,000 x M=8,000 slice of the panel-scale workload: the e2m1 and
+    int8 exact kernels agree bit for bit, the FP32 form agrees to its
+    tolerance, and the result is reproducible bit for bit."""
     n, m = 50_000, 8_000
     p = _panel(n, m, seed=20261205, miss=0.01)
     snps = L.make_snp_array(p.chr, p.cm, p.bp)
@@ -191,7 +233,9 @@ def test_int8_vs_fp32_at_panel_scale():
         a, _, perf = L.compute_ldscore(panel, snps, L.L2Config(**base))
         b, _, _ = L.compute_ldscore(panel, snps, L.L2Config(**base, use_f32=True))
         a2, _, _ = L.compute_ldscore(panel, snps, L.L2Config(**base))
+        c, _, _ = L.compute_ldscore(panel, snps, L.L2Config(**base, exact_kernel="i8"))
     assert np.array_equal(a, a2)
+    assert np.array_equal(a, c)
     assert _rel(b, a) < TOL_F32
     # l2 >= self term minus the bias of pairs: every SNP has its own 1
     assert np.all(a > 0.5)

@@ -52,6 +52,33 @@ def test_block_left_unsorted_coordinates(oracle):
     assert np.array_equal(got, exp)
 
 
+def test_block_left_large_split_scan(oracle):
+    """Lists long enough for the host to split the scan into tasks: runs
+    of several chromosomes, tied coordinates, one run with a position out of
+    order and one NaN cM entry (those runs scan from their start), against
+    the single forward scan of the restatement."""
+    rng = np.random.default_rng(11)
+    m = 200_000
+    chr_ = np.repeat(np.arange(1, 6, dtype=np.uint8), m // 5)
+    bp = np.cumsum(rng.integers(0, 3000, size=m)).astype(np.uint32)  # gaps of 0 tie
+    cm = bp * 1e-6
+    bp_bad = bp.copy()
+    bp_bad[50_000] = bp_bad[49_000]  # chromosome 2 out of order (kb mode)
+    cm_bad = cm.copy()
+    cm_bad[130_000] = np.nan         # chromosome 4 with a NaN (cM mode)
+    cases = [(0, 1.0, cm, bp), (1, 40.0, cm, bp_bad), (0, 0.5, cm_bad, bp), (2, 700, cm, bp)]
+    kinds = [L.WindowMode.Cm, L.WindowMode.Kb, L.WindowMode.Snp]
+    for mode, w, c, b in cases:
+        snps = L.make_snp_array(chr_, c, b)
+        got, full = L.block_left(snps, kinds[mode](w))
+        exp, efull = oracle.block_left(chr_, c, b, mode, w)
+        assert np.array_equal(got, exp) and full == efull, (mode, w)
+        for per_chr in (False, True):
+            cfg = L.L2Config(mode=kinds[mode](w), chunk_size=200, per_chromosome=per_chr)
+            assert np.array_equal(L.pair_bounds(snps, cfg),
+                                  _a_left_rule(exp, m, chr_, 200, per_chr)), (mode, per_chr)
+
+
 def _a_left_rule(bl, m, chr_, chunk, per_chr):
     lo = np.zeros(m, np.uint64)
     run0 = 0
