@@ -5,7 +5,10 @@ Default run (one JSON line):
     PLINK panel, N=50,000 individuals x M=100,000 SNPs, 1 % missing,
     --ld-wind-cm 1, --chunk-size 200, --snp-level-masking.  A step is one full
     ldsc_l2_compute over the resident panel.  `value` = SNP-pairs/s of the
-    int8-MFMA + FP64-finish form; the plain FP32 form runs beside it.
+    exact form (exact integer products G.G, G.M, M.G, M.M + FP64 finish),
+    whose products run on the e2m1 MFMA with f32 accumulation; the same form
+    on the int8 MFMA (int32 accumulation, identical results) and the plain
+    FP32 form run beside it.
   * serving -- resident panels N=50,000 x M=1,000,000 and N=10,000 x
     M=500,000; Poisson arrivals of contiguous SNP-range requests (size
     log-uniform in [1,000, 50,000], window snps/kb/cM uniform, 0-10
@@ -32,6 +35,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 INT8_PEAK_TOPS = 5000.0    # MI355X dense int8 MFMA, TOP/s
+FP4_PEAK_TFLOPS = 10100.0  # MI355X dense FP4 (e2m1) MFMA, TFLOP/s
 FP32_PEAK_TFLOPS = 157.3   # MI355X FP32 matrix, TFLOP/s
 HBM_PEAK_GBS = 8000.0      # MI355X HBM3E, GB/s
 METRIC = "SNP-pairs/s int8-MFMA vs FP32 plain (≤1e-6 rel); p50/p99 LD-request latency"
@@ -44,7 +48,7 @@ def log(*a):
 def profiled_traffic(kernel: str):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 summary
     (FETCH_SIZE/WRITE_SIZE scaled by a copy-kernel calibration), or None."""
-    path = os.path.join(ROOT, "profiles", "k_band_i8_c2.md")
+    path = os.path.join(ROOT, "profiles", f"{kernel}_c2.md")
     try:
         for line in open(path):
             parts = line.split()
@@ -83,8 +87,11 @@ def leg_c2(args, L, synth):
     snps = L.make_snp_array(p.chr, p.cm, p.bp)
     base = dict(mode=L.WindowMode.Cm(1.0), chunk_size=200, snp_level_masking=True)
     cfg8, cfg32 = L.L2Config(**base), L.L2Config(**base, use_f32=True)
+    cfg_i8 = L.L2Config(**base, exact_kernel="i8")
     panel = L.Panel(p.bed_bytes(), p.n, p.m)
-    w8, pf8, (l2_8, _) = timed_steps(L, panel, snps, cfg8, args.steps, args.warmup, "int8")
+    w8, pf8, (l2_8, _) = timed_steps(L, panel, snps, cfg8, args.steps, args.warmup, "exact")
+    if args.i8_steps > 0:
+        wi8, pfi8, (l2_i8, _) = timed_steps(L, panel, snps, cfg_i8, args.i8_steps, 1, "exact-int8")
     if args.fp32_steps > 0:
         w32, pf32, (l2_32, _) = timed_steps(L, panel, snps, cfg32, args.fp32_steps, 1, "fp32")
     panel.close()
@@ -99,20 +106,36 @@ def leg_c2(args, L, synth):
                 "ms_per_step": float(np.mean(w32)) * 1e3, "band_kernel_ms": k32 * 1e3,
                 "tflops_algorithmic": algo / k32 / 1e12,
                 "frac_of_fp32_peak": algo / k32 / 1e12 / FP32_PEAK_TFLOPS,
-                "max_rel_vs_int8": float(np.max(np.abs(l2_32 - l2_8) /
+                "max_rel_vs_exact": float(np.max(np.abs(l2_32 - l2_8) /
                                                 np.maximum(1.0, np.abs(l2_8))))}
+    exact_i8 = None
+    if args.i8_steps > 0:
+        ki8 = float(np.mean([q.bb_dot_secs for q in pfi8]))
+        exact_i8 = {"value": P / float(np.mean(wi8)), "unit": "SNP-pairs/s",
+                    "ms_per_step": float(np.mean(wi8)) * 1e3, "band_kernel_ms": ki8 * 1e3,
+                    "kernel": "k_band_i8", "achieved": algo / ki8 / 1e12,
+                    "peak": INT8_PEAK_TOPS, "unit_peak": "TOP/s",
+                    "frac": algo / ki8 / 1e12 / INT8_PEAK_TOPS,
+                    "identical_to_e2m1_form": bool(np.array_equal(l2_i8, l2_8))}
+    fp4 = p.n <= 4_000_000
+    kname = "k_band_fp4" if fp4 else "k_band_i8"
+    peak = FP4_PEAK_TFLOPS if fp4 else INT8_PEAK_TOPS
     res = {
         "value": P / t_step, "ms_per_step": t_step * 1e3, "pairs": P,
         "pairs_evaluated": pf8[-1].pairs_evaluated, "tiles": pf8[-1].tiles,
-        "fp32": fp32,
+        "fp32": fp32, "exact_int8": exact_i8,
         "tile_ms": float(np.mean([q.norm_secs for q in pf8])) * 1e3,
         "reduce_ms": float(np.mean([q.ring_store_secs for q in pf8])) * 1e3,
         "host_prep_ms": float(np.mean([q.bed_read_secs for q in pf8])) * 1e3,
-        "roofline": {"bound": "mfma", "kernel": "k_band_i8", "achieved": algo / k8 / 1e12,
-                     "peak": INT8_PEAK_TOPS, "unit": "TFLOP/s",
-                     "frac": algo / k8 / 1e12 / INT8_PEAK_TOPS,
-                     "traffic": profiled_traffic("k_band_i8"), "band_kernel_ms": k8 * 1e3,
-                     "ops_per_launch": algo, "note": "int8 ops; algorithmic 2*N*P"},
+        "library_ms": float(np.mean([q.total_secs for q in pf8])) * 1e3,
+        "roofline": {"bound": "mfma", "kernel": kname, "achieved": algo / k8 / 1e12,
+                     "peak": peak, "unit": "TFLOP/s",
+                     "frac": algo / k8 / 1e12 / peak,
+                     "traffic": profiled_traffic(kname), "band_kernel_ms": k8 * 1e3,
+                     "ops_per_launch": algo,
+                     "note": "algorithmic 2*N*P per launch (one N-long dot product per "
+                             "included pair); peak = dense " +
+                             ("e2m1 (FP4)" if fp4 else "int8") + " MFMA"},
         "ld_scored_snps_per_s": p.m / t_step,
     }
     if not args.skip_cpu:
@@ -128,7 +151,7 @@ def leg_c2(args, L, synth):
             got, _, perf_s = L.compute_ldscore(ps, snps[:ms], cfg8)
             got32, _, _ = L.compute_ldscore(ps, snps[:ms], cfg32)
         res["sample_accuracy"] = {
-            "int8_fp64_max_rel": float(np.max(np.abs(got - exp) / np.abs(exp))),
+            "exact_vs_fp64_max_rel": float(np.max(np.abs(got - exp) / np.abs(exp))),
             "fp32_max_abs": float(np.max(np.abs(got32 - exp))),
             "fp32_max_rel": float(np.max(np.abs(got32 - exp) / np.abs(exp))),
         }
@@ -261,8 +284,10 @@ def leg_c4(args, L, synth):
     out = {"workload": f"partitioned N={n} x M={m}, K=50, kb 1000, chunk 200"}
     res = {}
     with L.Panel(p.bed_bytes(), n, m) as panel:
-        for tag, f32 in (("int8", False), ("fp32", True)):
-            cfg = L.L2Config(mode=L.WindowMode.Kb(1000.0), chunk_size=200, use_f32=f32)
+        for tag, f32, kern in (("exact", False, "auto"), ("exact_int8", False, "i8"),
+                               ("fp32", True, "auto")):
+            cfg = L.L2Config(mode=L.WindowMode.Kb(1000.0), chunk_size=200, use_f32=f32,
+                             exact_kernel=kern)
             w, pf, (l2, _) = timed_steps(L, panel, snps, cfg, args.steps, 1, f"c4 {tag}",
                                          annot=annot)
             P = pf[0].pairs_included
@@ -272,8 +297,9 @@ def leg_c4(args, L, synth):
                         "band_kernel_ms": band * 1e3, "pairs": P,
                         "algorithmic_tops": 2.0 * n * P / band / 1e12}
             res[tag] = l2
-        out["fp32_vs_int8_max_rel"] = float(np.max(np.abs(res["fp32"] - res["int8"]) /
-                                                   np.maximum(1.0, np.abs(res["int8"]))))
+        out["fp32_vs_exact_max_rel"] = float(np.max(np.abs(res["fp32"] - res["exact"]) /
+                                                    np.maximum(1.0, np.abs(res["exact"]))))
+        out["exact_int8_identical"] = bool(np.array_equal(res["exact"], res["exact_int8"]))
     return out
 
 
@@ -286,6 +312,7 @@ def main() -> int:
     ap.add_argument("--n", type=int, default=50_000)
     ap.add_argument("--m", type=int, default=100_000)
     ap.add_argument("--fp32-steps", type=int, default=2)
+    ap.add_argument("--i8-steps", type=int, default=3)
     ap.add_argument("--cpu-sample-snps", type=int, default=6000)
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--skip-cpu", action="store_true")
@@ -342,17 +369,19 @@ def main() -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "int8 (int32 acc) + f64 finish",
+        "dtype": "exact integer products on e2m1 MFMA (f32 acc) + f64 finish",
         "data": "synthetic (seeded founder-haplotype generator)",
         "config": {"workload": "exact LD scores, N=50000 x M=100000, 1% missing, "
                                "--ld-wind-cm 1, chunk 200, --snp-level-masking",
                    "n_indiv": args.n, "m_snps": args.m, "pairs": c2["pairs"],
                    "pairs_evaluated": c2["pairs_evaluated"], "tiles": c2["tiles"]},
         "fp32_plain": c2["fp32"],
-        "int8_over_fp32": c2["value"] / c2["fp32"]["value"] if c2["fp32"] else None,
+        "exact_int8": c2["exact_int8"],
+        "exact_over_fp32": c2["value"] / c2["fp32"]["value"] if c2["fp32"] else None,
         "ld_scored_snps_per_s": c2["ld_scored_snps_per_s"],
         "phases_ms": {"tile_stats": c2["tile_ms"], "band": c2["roofline"]["band_kernel_ms"],
-                      "reduce": c2["reduce_ms"], "host_prep": c2["host_prep_ms"]},
+                      "reduce": c2["reduce_ms"], "host_prep": c2["host_prep_ms"],
+                      "library_call": c2["library_ms"]},
         "roofline": c2["roofline"],
         "cpu_baseline": c2.get("cpu_baseline"),
         "sample_accuracy": c2.get("sample_accuracy"),

@@ -45,7 +45,7 @@ enum { LDSC_WINDOW_CM = 0, LDSC_WINDOW_KB = 1, LDSC_WINDOW_SNP = 2 };
  *  EXACT:     exact integer products (G·G, G·M, M·G, M·M) finished in FP64.
  *             Replaces the reference default f64 GEMM path
  *             (compute.rs:1755-1810, 2102-2179).  The products run on the
- *             e2m1 (FP4) MFMA with f32 accumulation, exact for N <= 4,000,000
+ *             e2m1 (FP4) MFMA with f32 accumulation, exact for N <= 1,800,000
  *             individuals, and on the int8 MFMA with int32 accumulation
  *             beyond; both give the same integers, hence the same result.
  *  FP32:      standardized FP32 columns on FP32 MFMA, r² finished in FP64.

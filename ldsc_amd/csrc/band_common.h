@@ -30,6 +30,7 @@ struct BandArgs {
     const double* kappa;   // per-SNP kurtosis of the MAF-aware sketch or nullptr
     uint32_t nkb, ntiles, per_xcd, m, m_pad, K;
     double r2u_a, r2u_b, n_inv_sq_r2u_a, n_inv_sq, d_f;
+    double n_pad;          // individuals of the tiles, padding included
 };
 
 // Tile -> XCD mapping: workgroups are dealt to the 8 XCDs round-robin, so

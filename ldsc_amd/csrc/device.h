@@ -24,9 +24,10 @@ namespace ldsc {
 constexpr int kTile = 64;
 constexpr int kGroup = 32;
 constexpr int kKBlock = 128;
-// Largest N of the e2m1 exact kernel: its f32 accumulators hold G.G / 4 and
-// 4 M.M, multiples of 1/4 up to 4N, exactly while 4N <= 2^24.
-constexpr uint64_t kFp4MaxIndiv = 4000000;
+// Largest N of the e2m1 exact kernel: its f32 accumulators hold sums of
+// products of v/2 and q/2 (v <= 3), multiples of 1/4 up to 2.25 N_pad,
+// exactly while 9 N_pad <= 2^24.
+constexpr uint64_t kFp4MaxIndiv = 1800000;
 
 inline uint64_t round_up(uint64_t x, uint64_t a) { return (x + a - 1) / a * a; }
 
@@ -73,8 +74,8 @@ struct Workspace {
     int device = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev[8] = {};
-    DevBuf tiled, stats, bed_idx, lo, info, aeff, annot, maf, tiles, j0, tstart, rowp, colp,
-        l2, sk_bucket, sk_sign, sk_cols, sk_perm, sk_off, scratch;
+    DevBuf tiled, stats, bed_idx, lo, info, aeff, annot, maf, mafd, tiles, j0, tstart, rowp,
+        colp, l2, sk_bucket, sk_sign, sk_cols, sk_perm, sk_off, scratch;
     PinnedBuf pinned;       // SNP row indices (upload in flight while the host plans)
     PinnedBuf pinned_meta;  // pair bounds, band plan, stats read-back
     HostScratch host;
@@ -118,9 +119,10 @@ namespace ldsc {
 // individuals in the order wanted, its FAM row (--keep order, or the
 // CountSketch bucket order) in the lane-major layout of lane_major_map; it
 // is applied by an LDS-staged row gather.
-// `nib` selects the 4-bit tile form of the e2m1 band kernel (engine.hip).
+// `vcode` selects the dosage-value tile form of the e2m1 band kernel
+// (engine.hip) instead of the PLINK-code form above.
 int gather_and_tile(const ldsc_panel* panel, const uint64_t* d_bed_idx, uint64_t m,
-                    const int32_t* d_map, uint64_t m_pad, bool nib, Workspace* ws, char* err,
+                    const int32_t* d_map, uint64_t m_pad, bool vcode, Workspace* ws, char* err,
                     uint64_t err_len);
 
 // The exact / sketch engine on workspace `ws` (which must belong to

@@ -79,7 +79,7 @@ void PinnedBuf::release() {
 }
 
 size_t Workspace::bytes() const {
-    const DevBuf* all[] = {&tiled, &stats, &bed_idx, &lo, &info, &aeff, &annot, &maf, &tiles,
+    const DevBuf* all[] = {&tiled, &stats, &bed_idx, &lo, &info, &aeff, &annot, &maf, &mafd, &tiles,
                            &j0, &tstart, &rowp, &colp, &l2, &sk_bucket, &sk_sign, &sk_cols,
                            &sk_perm, &sk_off, &scratch};
     size_t s = 0;
@@ -101,7 +101,7 @@ hipError_t Workspace::init(int dev) {
 }
 
 void Workspace::destroy() {
-    DevBuf* all[] = {&tiled, &stats, &bed_idx, &lo, &info, &aeff, &annot, &maf, &tiles,
+    DevBuf* all[] = {&tiled, &stats, &bed_idx, &lo, &info, &aeff, &annot, &maf, &mafd, &tiles,
                      &j0, &tstart, &rowp, &colp, &l2, &sk_bucket, &sk_sign, &sk_cols,
                      &sk_perm, &sk_off, &scratch};
     for (auto* b : all) b->release();
@@ -231,26 +231,22 @@ static inline uint32_t tile_parts(uint64_t n) {
     return (uint32_t)((nkb + kTsChunk - 1) / kTsChunk);
 }
 
-// Nibble form of 8 codes (16 bits of a row dword): code of individual e at
-// bits 4e, 4e+1, then each nibble re-coded as the e2m1 pair source
-// 0110 (code 00, dosage 2), 0101 (10, dosage 1), 0100 (11, dosage 0),
-// 0000 (01, missing): masked with 0x3 it is G/2 in e2m1 (1.0, 0.5, 0, 0),
-// masked with 0x4 it is 2M (2.0 unless missing).
-__device__ __forceinline__ uint32_t nibble_codes(uint32_t x) {
-    x &= 0xFFFFu;
-    x = (x | (x << 8)) & 0x00FF00FFu;
-    x = (x | (x << 4)) & 0x0F0F0F0Fu;
-    x = (x | (x << 2)) & 0x33333333u;
-    const uint32_t L = x & 0x11111111u, H = (x >> 1) & 0x11111111u, nL = L ^ 0x11111111u;
-    return (nL & H) | ((nL & ~H & 0x11111111u) << 1) | ((nL | H) << 2);
+// Dosage-value code of 16 cells: PLINK code 00 (dosage 2) -> 2, 10
+// (dosage 1) -> 1, 11 (dosage 0) -> 0, 01 (missing) -> 3, i.e. bit 0 =
+// L xor H, bit 1 = not H.
+__device__ __forceinline__ uint32_t value_codes(uint32_t raw) {
+    const uint32_t L = raw & 0x55555555u, H = (raw >> 1) & 0x55555555u;
+    return (L ^ H) | ((H ^ 0x55555555u) << 1);
 }
 
-// NIB = false: 2-bit tiles (1 KiB per group and K-block, device.h).
-// NIB = true: 4-bit tiles for the e2m1 MFMA (2 KiB per group and K-block):
-//   two 1 KiB K-steps of 64 individuals; in step u lane l holds SNP l%32
-//   and individuals u*64 + (l/32)*32 + p, p = 0..31, nibble p at bits
-//   4(p%8) of its dword p/8.
-template <bool NIB>
+// VCODE = false: PLINK-code tiles (device.h), for the int8, FP32 and
+//   CountSketch kernels.
+// VCODE = true: dosage-value tiles of the e2m1 kernel, also 1 KiB per group
+//   and K-block: lane l (SNP l%32) holds 4 dwords, dword 2u + i covering
+//   individuals u*64 + (l/32)*32 + 16i + f in its 2-bit fields f (value
+//   codes, missing and padding = 3), i.e. row dword d = 2t + h of a K-block
+//   lands unchanged but re-coded in lane s + 32(t%2), dword 2(t/2) + h.
+template <bool VCODE>
 __global__ __launch_bounds__(256) void k_tile_stats(const uint8_t* __restrict__ slabs,
                                                      uint64_t pitch, uint32_t n,
                                                      const uint64_t* __restrict__ bed_idx,
@@ -280,11 +276,9 @@ __global__ __launch_bounds__(256) void k_tile_stats(const uint8_t* __restrict__ 
     const int h = tid & 1, t = (tid >> 1) & 3;
     const int s = 8 * wave + 4 * ((tid >> 5) & 1) + ((tid >> 3) & 3);
     const int lane = s + 32 * h;
-    // 2-bit: dword t of the lane.  Nibble: row dword d = 2t + h of a K-block
-    // is step u = t/2, half t%2, nibble dwords 2h and 2h+1 of lane s + 32(t%2)
-    uint32_t* out = NIB ? tiled + (size_t)g * nkb * 512 + (t >> 1) * 256 + (s + 32 * (t & 1)) * 4 + 2 * h
-                        : tiled + ((size_t)g * nkb * 64 + lane) * 4 + t;
-    constexpr int kTileDw = NIB ? 512 : 256;
+    uint32_t* out = VCODE ? tiled + ((size_t)g * nkb * 64 + s + 32 * (t & 1)) * 4 + 2 * (t >> 1) + h
+                          : tiled + ((size_t)g * nkb * 64 + lane) * 4 + t;
+    constexpr int kTileDw = 256;
     const uint32_t* src_lds = lds + s * kTsStride + 2 * t + h;
     uint32_t n2 = 0, n1 = 0, n0 = 0;
     uint32_t v[8];
@@ -301,7 +295,7 @@ __global__ __launch_bounds__(256) void k_tile_stats(const uint8_t* __restrict__ 
 #pragma unroll
             for (int r = 0; r < 8; ++r) v[r] = rowp[r][qn < qlast ? qn : qlast];
         }
-        uint32_t x[kTsPass], y[kTsPass];
+        uint32_t x[kTsPass];
 #pragma unroll
         for (int k = 0; k < kTsPass; ++k) {
             const uint32_t raw = src_lds[8 * k];
@@ -309,24 +303,18 @@ __global__ __launch_bounds__(256) void k_tile_stats(const uint8_t* __restrict__ 
             n2 += __popc(~(L | H) & 0x55555555u);  // 00: dosage 2
             n1 += __popc(~L & H);                   // 10: dosage 1 (het)
             n0 += __popc(L & H);                    // 11: dosage 0
-            if (NIB) {
-                x[k] = nibble_codes(raw);
-                y[k] = nibble_codes(raw >> 16);
-            } else {
-                x[k] = transpose_cells(raw);
-            }
+            x[k] = VCODE ? value_codes(raw) : transpose_cells(raw);
         }
         // K-blocks past nkb (last pass) hold code 01 only: counted as
         // nothing above, and not stored
         uint32_t* o = out + (size_t)kb0 * kTileDw;
+        if (kb0 + kTsPass <= kb_end) {
 #pragma unroll
-        for (int k = 0; k < kTsPass; ++k) {
-            if (kb0 + kTsPass <= kb_end || kb0 + k < kb_end) {
-                if (NIB)
-                    *reinterpret_cast<uint2*>(o + k * kTileDw) = make_uint2(x[k], y[k]);
-                else
-                    o[k * kTileDw] = x[k];
-            }
+            for (int k = 0; k < kTsPass; ++k) o[k * kTileDw] = x[k];
+        } else {
+#pragma unroll
+            for (int k = 0; k < kTsPass; ++k)
+                if (kb0 + k < kb_end) o[k * kTileDw] = x[k];
         }
     }
     // the 8 threads of a SNP are consecutive: fold them, then one writes
@@ -352,10 +340,13 @@ __global__ __launch_bounds__(256) void k_tile_stats(const uint8_t* __restrict__ 
 // code: [(2-mean)inv, 0, (1-mean)inv, (0-mean)inv] in f32.
 // `part` holds `nparts` slots of m_pad x 4 integer stats (K-block chunks);
 // their sum is written to `stats` when it is not `part` itself.
+// `maf` (nullable): the MAF of normalize.rs:71-81 from the same integers,
+// rounded through f32 when `maf32` (the FP32 form, normalize.rs:153).
 __global__ void k_snp_info(const uint32_t* __restrict__ part, uint32_t nparts,
                            uint32_t* __restrict__ stats, uint32_t m_pad, uint32_t n,
                            int2* __restrict__ cs, double* __restrict__ rs,
-                           float4* __restrict__ lut, float2* __restrict__ musc) {
+                           float4* __restrict__ lut, float2* __restrict__ musc,
+                           double* __restrict__ maf, int maf32) {
     const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= m_pad) return;
     uint4 a = reinterpret_cast<const uint4*>(part)[j];
@@ -383,6 +374,12 @@ __global__ void k_snp_info(const uint32_t* __restrict__ part, uint32_t nparts,
     const float w0 = a32 == 0.0f ? 0.0f : -(a32 * i32);
     lut[j] = make_float4((2.0f - a32) * i32, 0.0f, (1.0f - a32) * i32, w0);
     if (musc) musc[j] = make_float2(a32, i32);
+    if (maf) {
+        double freq = avg / 2.0;
+        freq = freq < 0.0 ? 0.0 : (freq > 1.0 ? 1.0 : freq);
+        const double v = freq < 1.0 - freq ? freq : 1.0 - freq;
+        maf[j] = maf32 ? (double)(float)v : v;
+    }
 }
 
 // A_eff in place (m_pad x K, column stride m_pad): the uploaded annotation
@@ -482,10 +479,9 @@ __device__ void band_contract2(const BandArgs& a, EpiShared2& sh, int tile, uint
     }
 }
 
-template <class Acc>
+template <bool VCODE, class Acc>
 __device__ __forceinline__ void finish_2w(const BandArgs& a, EpiShared2& sh, int tile, uint32_t I,
-                                          uint32_t J, int w, int lane, const Acc (&acc)[4][2],
-                                          double s_gg, double s_mm);
+                                          uint32_t J, int w, int lane, const Acc (&acc)[4][2]);
 
 // int8 form, two waves per 64x64 tile: wave w owns rows 32w .. 32w+31 (SNP
 // group 2I + w) against both column groups; 4 products x 2 blocks of
@@ -537,18 +533,18 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
         cb1 = nb1;
     }
 
-    finish_2w(a, sh, tile, I, J, w, lane, acc, 1.0, 1.0);
+    finish_2w<false>(a, sh, tile, I, J, w, lane, acc);
 }
 
-// FP64 finish of wave w of a two-wave tile from its exact products
-// (accumulator of product p times its scale is the integer G.G, G.M, M.G,
-// M.M), then the per-tile partial sums:
+// FP64 finish of wave w of a two-wave tile from its exact products, then
+// the per-tile partial sums.  The accumulators hold G.G, G.M, M.G, M.M
+// (int8 kernel) or X.X, X.Q, Q.X, Q.Q (e2m1 kernel, VCODE), from which the
+// same four integers follow exactly; then
 //   T  = C_r (C_c GG - S_c GM) - S_r (C_c MG - S_c MM)  = C_r C_c * cov
 //   r  = T / sqrt(C_r Q_r C_c Q_c),  r2u = r^2 (1 + 1/D) - 1/D
-template <class Acc>
+template <bool VCODE, class Acc>
 __device__ __forceinline__ void finish_2w(const BandArgs& a, EpiShared2& sh, int tile, uint32_t I,
-                                          uint32_t J, int w, int lane, const Acc (&acc)[4][2],
-                                          double s_gg, double s_mm) {
+                                          uint32_t J, int w, int lane, const Acc (&acc)[4][2]) {
     if (w == 0) {
         sh.csr[lane] = a.cs[(size_t)I * kTile + lane];
         sh.rsr[lane] = a.rs[(size_t)I * kTile + lane];
@@ -566,9 +562,20 @@ __device__ __forceinline__ void finish_2w(const BandArgs& a, EpiShared2& sh, int
             const int row = 32 * w + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
             const int col = 32 * bj + (lane & 31);
             const int2 cr = sh.csr[row], cc = sh.csc[col];
-            // exact integers (powers-of-two scales)
-            const double gg = (double)acc[0][bj][r] * s_gg, gm = (double)acc[1][bj][r];
-            const double mg = (double)acc[2][bj][r], mm = (double)acc[3][bj][r] * s_mm;
+            double gg, gm, mg, mm;  // exact integers
+            if (VCODE) {
+                const double vv = 4.0 * (double)acc[0][bj][r], vq = 4.0 * (double)acc[1][bj][r];
+                const double qv = 4.0 * (double)acc[2][bj][r], qq = 4.0 * (double)acc[3][bj][r];
+                gg = vv - 3.0 * vq - 3.0 * qv + 9.0 * qq;
+                gm = (double)cr.y - vq + 3.0 * qq;
+                mg = (double)cc.y - qv + 3.0 * qq;
+                mm = (double)cr.x + (double)cc.x - a.n_pad + qq;
+            } else {
+                gg = (double)acc[0][bj][r];
+                gm = (double)acc[1][bj][r];
+                mg = (double)acc[2][bj][r];
+                mm = (double)acc[3][bj][r];
+            }
             const double u = (double)cc.x * gg - (double)cc.y * gm;
             const double v = (double)cc.x * mg - (double)cc.y * mm;
             const double T = (double)cr.x * u - (double)cr.y * v;
@@ -580,81 +587,97 @@ __device__ __forceinline__ void finish_2w(const BandArgs& a, EpiShared2& sh, int
     band_contract2(a, sh, tile, I, J, w, r2u);
 }
 
-// e2m1 operands from a 4-bit tile dword quad (nibble form of k_tile_stats):
-// mask 0x3 gives G/2, mask 0x4 gives 2M; the MFMA reads 4 of the 8 dwords.
-__device__ __forceinline__ v8i fp4_operand(const u32x4& v, uint32_t mask) {
-    const v4i x = {(int)(v[0] & mask), (int)(v[1] & mask), (int)(v[2] & mask), (int)(v[3] & mask)};
-    return __builtin_shufflevector(x, x, 0, 1, 2, 3, -1, -1, -1, -1);
+// e2m1 operands of one K-step from the lane's two value-code dwords w0, w1
+// (32 individuals): X = v/2 (nibble = v in its low bits: 0, 0.5, 1, 1.5) and
+// Q = q/2 (q = 1 iff v = 3: missing or padding).  Fields 2n and 2n+1 of a
+// dword go to nibble n of two operand dwords; the same placement on both
+// operands keeps every individual paired with itself.  The MFMA reads 4 of
+// the 8 operand dwords.
+__device__ __forceinline__ void vcode_operands(uint32_t w0, uint32_t w1, v8i& X, v8i& Q) {
+    // shifts of the 64-bit pair (w1:w0): the bits w1 pushes into the top of
+    // w0 land outside every mask below
+    const uint64_t w = ((uint64_t)w1 << 32) | w0;
+    const uint64_t s1 = w >> 1, s2 = w >> 2, s3 = w >> 3;
+    const uint32_t a2 = (uint32_t)s2, b2 = (uint32_t)(s2 >> 32);
+    const v4i x = {(int)(w0 & 0x33333333u), (int)(a2 & 0x33333333u),
+                   (int)(w1 & 0x33333333u), (int)(b2 & 0x33333333u)};
+    const v4i q = {(int)(w0 & (uint32_t)s1 & 0x11111111u), (int)(a2 & (uint32_t)s3 & 0x11111111u),
+                   (int)(w1 & (uint32_t)(s1 >> 32) & 0x11111111u),
+                   (int)(b2 & (uint32_t)(s3 >> 32) & 0x11111111u)};
+    X = __builtin_shufflevector(x, x, 0, 1, 2, 3, -1, -1, -1, -1);
+    Q = __builtin_shufflevector(q, q, 0, 1, 2, 3, -1, -1, -1, -1);
 }
 
 // scale bytes 127 (E8M0 2^0) on both operands: a plain e2m1 product
 #define LDSC_MFMA_FP4(A, B, C) \
     __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(A, B, C, 4, 4, 0, 0x7f7f7f7f, 0, 0x7f7f7f7f)
 
-// K loop of one wave of the e2m1 kernel over the column groups it keeps
-// (U0, U1): per K-block two K-steps of 64 individuals, 4 products each.
-// Two register sets alternate (unrolled by two) so that each K-block's
-// loads are issued a whole block of MFMA work before their use and no
-// register copy waits on them.
+// One K-block (two K-steps of 64 individuals) of a wave of the e2m1 kernel
+// over the column groups it keeps (U0, U1): products X.X, X.Q, Q.X, Q.Q.
 template <bool U0, bool U1>
-struct Fp4Ops {
-    u32x4 a[2], b0[2], b1[2];
-    __device__ __forceinline__ void load(const u32x4* pa, const u32x4* pb0, const u32x4* pb1,
-                                         size_t off) {
+__device__ __forceinline__ void vcode_mma(const u32x4& ca, const u32x4& cb0, const u32x4& cb1,
+                                          v16f (&acc)[4][2]) {
 #pragma unroll
-        for (int u = 0; u < 2; ++u) {
-            a[u] = pa[off + 64 * u];
-            if (U0) b0[u] = pb0[off + 64 * u];
-            if (U1) b1[u] = pb1[off + 64 * u];
+    for (int u = 0; u < 2; ++u) {
+        v8i xa, qa, xb, qb;
+        vcode_operands(ca[2 * u], ca[2 * u + 1], xa, qa);
+        if (U0) {
+            vcode_operands(cb0[2 * u], cb0[2 * u + 1], xb, qb);
+            acc[0][0] = LDSC_MFMA_FP4(xa, xb, acc[0][0]);
+            acc[1][0] = LDSC_MFMA_FP4(xa, qb, acc[1][0]);
+            acc[2][0] = LDSC_MFMA_FP4(qa, xb, acc[2][0]);
+            acc[3][0] = LDSC_MFMA_FP4(qa, qb, acc[3][0]);
         }
-    }
-    __device__ __forceinline__ void mma(v16f (&acc)[4][2]) const {
-#pragma unroll
-        for (int u = 0; u < 2; ++u) {
-            const v8i ga = fp4_operand(a[u], 0x33333333u), ma = fp4_operand(a[u], 0x44444444u);
-            if (U0) {
-                const v8i gb = fp4_operand(b0[u], 0x33333333u), mb = fp4_operand(b0[u], 0x44444444u);
-                acc[0][0] = LDSC_MFMA_FP4(ga, gb, acc[0][0]);
-                acc[1][0] = LDSC_MFMA_FP4(ga, mb, acc[1][0]);
-                acc[2][0] = LDSC_MFMA_FP4(ma, gb, acc[2][0]);
-                acc[3][0] = LDSC_MFMA_FP4(ma, mb, acc[3][0]);
-            }
-            if (U1) {
-                const v8i gb = fp4_operand(b1[u], 0x33333333u), mb = fp4_operand(b1[u], 0x44444444u);
-                acc[0][1] = LDSC_MFMA_FP4(ga, gb, acc[0][1]);
-                acc[1][1] = LDSC_MFMA_FP4(ga, mb, acc[1][1]);
-                acc[2][1] = LDSC_MFMA_FP4(ma, gb, acc[2][1]);
-                acc[3][1] = LDSC_MFMA_FP4(ma, mb, acc[3][1]);
-            }
+        if (U1) {
+            vcode_operands(cb1[2 * u], cb1[2 * u + 1], xb, qb);
+            acc[0][1] = LDSC_MFMA_FP4(xa, xb, acc[0][1]);
+            acc[1][1] = LDSC_MFMA_FP4(xa, qb, acc[1][1]);
+            acc[2][1] = LDSC_MFMA_FP4(qa, xb, acc[2][1]);
+            acc[3][1] = LDSC_MFMA_FP4(qa, qb, acc[3][1]);
         }
-    }
-};
-
-template <bool U0, bool U1>
-__device__ __forceinline__ void fp4_kloop(const u32x4* pa, const u32x4* pb0, const u32x4* pb1,
-                                          uint32_t nkb, v16f (&acc)[4][2]) {
-    Fp4Ops<U0, U1> s0, s1;
-    const uint32_t last = nkb - 1;
-    s0.load(pa, pb0, pb1, 0);
-    for (uint32_t kb = 0; kb < nkb; kb += 2) {
-        s1.load(pa, pb0, pb1, (size_t)min(kb + 1, last) * 128);
-        __builtin_amdgcn_sched_barrier(0);
-        s0.mma(acc);
-        s0.load(pa, pb0, pb1, (size_t)min(kb + 2, last) * 128);
-        __builtin_amdgcn_sched_barrier(0);
-        if (kb + 1 < nkb) s1.mma(acc);
     }
 }
 
-// e2m1 form of the exact kernel (same tiles, waves and finish as k_band_i8):
-// operands G/2 and 2M in e2m1 on v_mfma_scale_f32_32x32x64_f8f6f4 (scale
-// 1), f32 accumulation.  Every product and partial sum is a multiple of 1/4
-// below 2^22 while N <= 4,000,000, so f32 holds the integer products
-// exactly: GG = 4 acc0, GM = acc1, MG = acc2, MM = acc3 / 4.  A wave skips
-// the column group of its tile that cannot hold an included pair (its
-// columns all precede the smallest pair bound of the wave's rows, or none
-// precedes its last row); those accumulators stay zero and the pair mask
-// drops every pair of the group.
+// K loop with two alternating register sets (unrolled by two): each
+// K-block's loads are issued a whole block of MFMA work before their use and
+// no register copy waits on them.
+// pa, pb0, pb1 are the wave-uniform starts of the three SNP groups; each
+// lane adds its own 16-byte slot (scalar base + lane offset addressing).
+template <bool U0, bool U1>
+__device__ __forceinline__ void vcode_kloop(const u32x4* pa, const u32x4* pb0, const u32x4* pb1,
+                                            uint32_t nkb, int lane, v16f (&acc)[4][2]) {
+    const uint32_t last = nkb - 1;
+    u32x4 a0 = pa[lane], b00 = U0 ? pb0[lane] : u32x4{}, b10 = U1 ? pb1[lane] : u32x4{};
+    u32x4 a1, b01, b11;
+    for (uint32_t kb = 0; kb < nkb; kb += 2) {
+        const size_t o1 = (size_t)min(kb + 1, last) * 64;
+        a1 = (pa + o1)[lane];
+        if (U0) b01 = (pb0 + o1)[lane];
+        if (U1) b11 = (pb1 + o1)[lane];
+        __builtin_amdgcn_sched_barrier(0);
+        vcode_mma<U0, U1>(a0, b00, b10, acc);
+        const size_t o2 = (size_t)min(kb + 2, last) * 64;
+        a0 = (pa + o2)[lane];
+        if (U0) b00 = (pb0 + o2)[lane];
+        if (U1) b10 = (pb1 + o2)[lane];
+        __builtin_amdgcn_sched_barrier(0);
+        if (kb + 1 < nkb) vcode_mma<U0, U1>(a1, b01, b11, acc);
+    }
+}
+
+// e2m1 form of the exact kernel (same 64x64 tiles, two waves, finish and
+// partial sums as k_band_i8).  Tiles hold 2-bit dosage-value codes v (3 =
+// missing or padding); with q = [v = 3] the genotype is G = v - 3q and the
+// presence M = 1 - q, so the four exact products follow from X.X, X.Q,
+// Q.X, Q.Q on v_mfma_scale_f32_32x32x64_f8f6f4 (scale 1, f32
+// accumulation, exact while 9 N_pad <= 2^24):
+//   VV = 4 acc0, VQ = 4 acc1, QV = 4 acc2, QQ = 4 acc3
+//   GG = VV - 3 VQ - 3 QV + 9 QQ      GM = S_r - VQ + 3 QQ
+//   MG = S_c - QV + 3 QQ              MM = C_r + C_c - N_pad + QQ
+// A wave skips a column group of its tile that cannot hold an included pair
+// (all its columns precede the smallest pair bound of the wave's rows, or
+// none precedes its last row): those accumulators stay zero and the pair
+// mask drops every pair of the group.
 __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_band_fp4(BandArgs a) {
     __shared__ EpiShared2 sh;
     const int tile = tile_of_block(a);
@@ -662,9 +685,9 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     const int2 tl = a.tiles[tile];
     const uint32_t I = (uint32_t)tl.x, J = (uint32_t)tl.y;
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const size_t gstride = (size_t)a.nkb * 128;  // u32x4 per SNP group
-    const u32x4* pa = a.tiled + (size_t)(2 * I + w) * gstride + lane;
-    const u32x4* pb0 = a.tiled + (size_t)(2 * J) * gstride + lane;
+    const size_t gstride = (size_t)a.nkb * 64;
+    const u32x4* pa = a.tiled + (size_t)(2 * I + __builtin_amdgcn_readfirstlane(w)) * gstride;
+    const u32x4* pb0 = a.tiled + (size_t)(2 * J) * gstride;
     const u32x4* pb1 = pb0 + gstride;
 
     const uint32_t r0 = I * kTile + 32 * w, c0 = J * kTile;
@@ -683,12 +706,12 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
 #pragma unroll
             for (int r = 0; r < 16; ++r) acc[p][bj][r] = 0.0f;
     if (use0 && use1)
-        fp4_kloop<true, true>(pa, pb0, pb1, a.nkb, acc);
+        vcode_kloop<true, true>(pa, pb0, pb1, a.nkb, lane, acc);
     else if (use0)
-        fp4_kloop<true, false>(pa, pb0, pb1, a.nkb, acc);
+        vcode_kloop<true, false>(pa, pb0, pb1, a.nkb, lane, acc);
     else if (use1)
-        fp4_kloop<false, true>(pa, pb0, pb1, a.nkb, acc);
-    finish_2w(a, sh, tile, I, J, w, lane, acc, 4.0, 0.25);
+        vcode_kloop<false, true>(pa, pb0, pb1, a.nkb, lane, acc);
+    finish_2w<true>(a, sh, tile, I, J, w, lane, acc);
 }
 
 __global__ __launch_bounds__(64) void k_band_f32(BandArgs a) {
@@ -862,9 +885,9 @@ size_t l2_workspace_bytes(const ldsc_panel* panel, uint64_t m, const ldsc_l2_con
     uint64_t tiles = nb * 16;
     if (cfg && cfg->window_mode == LDSC_WINDOW_SNP)
         tiles = nb * (2 + (uint64_t)(cfg->window_value + cfg->chunk_size) / kTile);
-    size_t bytes = m_pad * n_pad / 2;                           // tiled SNP slice (4-bit form)
+    size_t bytes = m_pad * n_pad / 4;                           // tiled SNP slice
     bytes += m_pad * 16 * (1 + tile_parts(panel->n));                // stats + chunk slots
-    bytes += m_pad * (8 + 4 + 48) + m_pad * K * 8 + m * K * 8;       // idx, lo, info, aeff, l2
+    bytes += m_pad * (8 + 4 + 48 + 8) + m_pad * K * 8 + m * K * 8;   // idx, lo, info, maf, aeff, l2
     bytes += tiles * (kTile * K * 8 * 2 + 8);                   // tile partials + list
     const bool gather = cfg && (cfg->sketch_d || panel->d_keep32);
     if (gather) bytes += m * round_up((panel->n + 3) / 4, 4);   // permuted rows
@@ -881,14 +904,14 @@ static double maf_from_stats(uint32_t S, uint32_t C) {
 }
 
 int gather_and_tile(const ldsc_panel* panel, const uint64_t* d_bed_idx, uint64_t m,
-                    const int32_t* d_map, uint64_t m_pad, bool nib, Workspace* ws, char* err,
+                    const int32_t* d_map, uint64_t m_pad, bool vcode, Workspace* ws, char* err,
                     uint64_t err_len) {
     const uint64_t n = panel->n;
     const uint64_t n_pad = round_up(n, kKBlock);
     const uint32_t nkb = (uint32_t)(n_pad / kKBlock);
     hipStream_t st = ws->stream;
     const uint32_t nparts = tile_parts(n);
-    HIPC(ws->tiled.ensure((size_t)m_pad * n_pad / (nib ? 2 : 4)));
+    HIPC(ws->tiled.ensure((size_t)m_pad * n_pad / 4));
     HIPC(ws->stats.ensure((size_t)m_pad * 16 * (1 + nparts)));
     const uint8_t* src = panel->d_slabs;
     uint64_t pitch = panel->stride;
@@ -922,7 +945,7 @@ int gather_and_tile(const ldsc_panel* panel, const uint64_t* d_bed_idx, uint64_t
         pitch = bps_out;
         rows = nullptr;
     }
-    hipLaunchKernelGGL(nib ? k_tile_stats<true> : k_tile_stats<false>,
+    hipLaunchKernelGGL(vcode ? k_tile_stats<true> : k_tile_stats<false>,
                        dim3((unsigned)(m_pad / kGroup), nparts), dim3(256), 0, st, src, pitch,
                        (uint32_t)n, rows, (uint32_t)m, nkb, (uint32_t)m_pad,
                        ws->tiled.as<uint32_t>(), ws->stats.as<uint32_t>() + (size_t)m_pad * 4);
@@ -998,9 +1021,14 @@ int run_l2(const ldsc_panel* panel, const ldsc_snp* snps, uint64_t m, const ldsc
     rc = gather_and_tile(panel, ws->bed_idx.as<uint64_t>(), m, d ? sp->d_fam : panel->d_keep32,
                          m_pad, use_fp4, ws, err, err_len);
     if (rc) return rc;
+    // MAF on the device for the exact and FP32 forms; the sketch form's
+    // rounding depends on chunk contiguity and is finished on the host
+    const bool dev_maf = !d;
+    if (dev_maf) HIPC(ws->mafd.ensure((size_t)m_pad * 8));
     hipLaunchKernelGGL(k_snp_info, dim3((m_pad + 255) / 256), dim3(256), 0, st,
                        ws->stats.as<uint32_t>() + (size_t)m_pad * 4, tile_parts(n),
-                       ws->stats.as<uint32_t>(), m_pad, (uint32_t)n, d_cs, d_rs, d_lut, d_musc);
+                       ws->stats.as<uint32_t>(), m_pad, (uint32_t)n, d_cs, d_rs, d_lut, d_musc,
+                       dev_maf ? ws->mafd.as<double>() : nullptr, use_f32 ? 1 : 0);
     HIPC(hipGetLastError());
     HIPC(hipEventRecord(ws->ev[1], st));
     const uint32_t nkb = (uint32_t)(round_up(n, kKBlock) / kKBlock);
@@ -1024,7 +1052,11 @@ int run_l2(const ldsc_panel* panel, const ldsc_snp* snps, uint64_t m, const ldsc
     const size_t off_j0 = off_tiles + round_up((size_t)plan.ntiles * 8, 16);
     const size_t off_ts = off_j0 + round_up((size_t)plan.nb * 4, 16);
     const size_t off_st = off_ts + round_up((size_t)(plan.nb + 1) * 4, 16);
-    const size_t meta_bytes = off_st + (size_t)m_pad * 16;
+    const size_t off_maf = off_st + (size_t)m_pad * 16;
+    const size_t off_l2 = off_maf + (size_t)m_pad * 8;
+    // results come back through pinned memory unless they are large
+    const bool stage_l2 = (size_t)m * K * 8 <= (64u << 20);
+    const size_t meta_bytes = off_l2 + (stage_l2 ? (size_t)m * K * 8 : 0);
     // nothing in flight reads pinned_meta (the previous call ended with a
     // synchronize), so it may grow here
     HIPC(ws->pinned_meta.ensure(meta_bytes));
@@ -1072,8 +1104,9 @@ int run_l2(const ldsc_panel* panel, const ldsc_snp* snps, uint64_t m, const ldsc
     const bool has_pq = !std::isnan(cfg->pq_exp);
     const bool maf_aware = d && cfg->sketch_maf_aware;
     uint32_t* h_stats = reinterpret_cast<uint32_t*>(hp + off_st);
+    double* h_maf = reinterpret_cast<double*>(hp + off_maf);
     std::vector<double>& maf = ws->host.maf;
-    auto compute_maf = [&]() {
+    auto compute_maf = [&]() {  // sketch form: from the integer stats on the host
         // f32 exact path: (f32)maf (normalize.rs:153).  Sketch: f64 on the
         // fused path of a BED-contiguous chunk (compute.rs:639), (f32)maf
         // otherwise (compute.rs:1509-1511).
@@ -1103,9 +1136,15 @@ int run_l2(const ldsc_panel* panel, const ldsc_snp* snps, uint64_t m, const ldsc
     };
     const bool early_stats = has_pq || maf_aware;
     if (early_stats) {
-        HIPC(hipMemcpyAsync(h_stats, ws->stats.p, (size_t)m_pad * 16, hipMemcpyDeviceToHost, st));
-        HIPC(hipStreamSynchronize(st));
-        compute_maf();
+        if (dev_maf) {
+            HIPC(hipMemcpyAsync(h_maf, ws->mafd.p, m * 8, hipMemcpyDeviceToHost, st));
+            HIPC(hipStreamSynchronize(st));
+            maf.assign(h_maf, h_maf + m);
+        } else {
+            HIPC(hipMemcpyAsync(h_stats, ws->stats.p, (size_t)m_pad * 16, hipMemcpyDeviceToHost, st));
+            HIPC(hipStreamSynchronize(st));
+            compute_maf();
+        }
     }
     if (maf_aware) {
         // per_snp_kappa (compute.rs:525-538) from the integer moments:
@@ -1173,6 +1212,7 @@ int run_l2(const ldsc_panel* panel, const ldsc_snp* snps, uint64_t m, const ldsc
     ba.n_inv_sq_r2u_a = n_inv * n_inv * ba.r2u_a;
     ba.n_inv_sq = n_inv * n_inv;
     ba.d_f = (double)d;
+    ba.n_pad = (double)round_up(n, kKBlock);
 
     HIPC(hipEventRecord(ws->ev[3], st));
     if (d)
@@ -1194,13 +1234,26 @@ int run_l2(const ldsc_panel* panel, const ldsc_snp* snps, uint64_t m, const ldsc
                        (uint32_t)K, ws->l2.as<double>());
     HIPC(hipGetLastError());
     HIPC(hipEventRecord(ws->ev[5], st));
-    if (maf_out && !early_stats)
-        HIPC(hipMemcpyAsync(h_stats, ws->stats.p, (size_t)m_pad * 16, hipMemcpyDeviceToHost, st));
-    HIPC(hipMemcpyAsync(l2_out, ws->l2.p, (size_t)m * K * 8, hipMemcpyDeviceToHost, st));
+    if (maf_out && !early_stats) {
+        if (dev_maf)
+            HIPC(hipMemcpyAsync(h_maf, ws->mafd.p, m * 8, hipMemcpyDeviceToHost, st));
+        else
+            HIPC(hipMemcpyAsync(h_stats, ws->stats.p, (size_t)m_pad * 16, hipMemcpyDeviceToHost, st));
+    }
+    double* h_l2 = reinterpret_cast<double*>(hp + off_l2);
+    HIPC(hipMemcpyAsync(stage_l2 ? h_l2 : l2_out, ws->l2.p, (size_t)m * K * 8,
+                        hipMemcpyDeviceToHost, st));
     HIPC(hipStreamSynchronize(st));
+    if (stage_l2) std::memcpy(l2_out, h_l2, (size_t)m * K * 8);
     if (maf_out) {
-        if (!early_stats) compute_maf();
-        std::memcpy(maf_out, maf.data(), m * 8);
+        if (early_stats)
+            std::memcpy(maf_out, maf.data(), m * 8);
+        else if (dev_maf)
+            std::memcpy(maf_out, h_maf, m * 8);
+        else {
+            compute_maf();
+            std::memcpy(maf_out, maf.data(), m * 8);
+        }
     }
 
     if (perf) {
@@ -1358,7 +1411,7 @@ extern "C" int ldsc_l2_decode(const ldsc_panel* panel, const uint64_t* bed_idx, 
         float4* d_lut = reinterpret_cast<float4*>(ws->info.as<char>() + m * 16);
         hipLaunchKernelGGL(k_snp_info, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, st,
                            ws->stats.as<uint32_t>(), 1u, ws->stats.as<uint32_t>(), (uint32_t)m,
-                           (uint32_t)n, d_cs, d_rs, d_lut, nullptr);
+                           (uint32_t)n, d_cs, d_rs, d_lut, nullptr, nullptr, 0);
         DCHK(hipGetLastError());
         char* sp = ws->scratch.as<char>();
         int8_t* d_dos = dosage ? reinterpret_cast<int8_t*>(sp) : nullptr;

@@ -51,7 +51,7 @@ class L2Config:
     ``use_f32`` selects the FP32 standardized-column form (``--fast-f32``);
     otherwise the exact form runs as exact integer products with an FP64
     finish (the reference's default f64 path): on the e2m1 MFMA for up to
-    4,000,000 individuals, on the int8 MFMA beyond or when
+    1,800,000 individuals, on the int8 MFMA beyond or when
     ``exact_kernel="i8"``.  ``per_chromosome`` restarts
     the chunk grid at every chromosome run, as ``run()`` does without
     ``--global-pass``; the reference's library API is one global pass.
