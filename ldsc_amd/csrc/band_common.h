@@ -70,59 +70,73 @@ __device__ __forceinline__ int row_of(int bi, int r, int lane) {
     return 32 * bi + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
 }
 
-// r2u (masked, 64 per lane in MFMA output order) -> per-tile partial sums.
+// r2u (masked, 64 per lane in MFMA output order) -> per-tile partial sums
+// for annotations kc0 .. kc0+kcn-1 (kcn <= KC; KC = 1 lets the common
+// single-annotation case run without per-annotation predicates).
 //   rowp[tile][k][a] = sum_b R[a][b] * A_eff[col b][k]
 //   colp[tile][k][b] = sum_a R[a][b] * A_eff[row a][k]
-__device__ void band_contract(const BandArgs& a, EpiShared& sh, int tile, uint32_t I, uint32_t J,
-                              const double (&r2u)[2][2][16]) {
+template <int KC>
+__device__ __forceinline__ void band_contract_chunk(const BandArgs& a, EpiShared& sh, int tile,
+                                                    uint32_t I, uint32_t J,
+                                                    const double (&r2u)[2][2][16], uint32_t kc0,
+                                                    uint32_t kcn_in) {
     const int lane = threadIdx.x;
-    for (uint32_t kc0 = 0; kc0 < a.K; kc0 += 8) {
-        const uint32_t kcn = a.K - kc0 < 8 ? a.K - kc0 : 8;
-        __syncthreads();
-        for (uint32_t kk = 0; kk < kcn; ++kk) {
-            sh.ar[lane][kk] = a.aeff[(size_t)(kc0 + kk) * a.m_pad + (size_t)I * kTile + lane];
-            sh.ac[lane][kk] = a.aeff[(size_t)(kc0 + kk) * a.m_pad + (size_t)J * kTile + lane];
-        }
-        double cacc[8];
-#pragma unroll
-        for (int kk = 0; kk < 8; ++kk) cacc[kk] = 0.0;
-#pragma unroll
-        for (int bi = 0; bi < 2; ++bi) {
-            __syncthreads();
-#pragma unroll
-            for (int bj = 0; bj < 2; ++bj)
-#pragma unroll
-                for (int r = 0; r < 16; ++r)
-                    sh.R[row_of(0, r, lane)][32 * bj + (lane & 31)] = r2u[bi][bj][r];
-            __syncthreads();
-            // row role: lane covers row lane%32 over columns of its half
-            double racc[8];
-#pragma unroll
-            for (int kk = 0; kk < 8; ++kk) racc[kk] = 0.0;
-            const int rr = lane & 31, b0 = 32 * (lane >> 5);
-            for (int b = 0; b < 32; ++b) {
-                const double rv = sh.R[rr][b0 + b];
-#pragma unroll
-                for (int kk = 0; kk < 8; ++kk)
-                    if ((uint32_t)kk < kcn) racc[kk] += rv * sh.ac[b0 + b][kk];
-            }
-#pragma unroll
-            for (int kk = 0; kk < 8; ++kk)
-                if ((uint32_t)kk < kcn) racc[kk] += __shfl_xor(racc[kk], 32);
-            if (lane < 32)
-                for (uint32_t kk = 0; kk < kcn; ++kk)
-                    a.rowp[((size_t)tile * a.K + kc0 + kk) * kTile + 32 * bi + lane] = racc[kk];
-            // column role: lane = column, rows of this half
-            for (int r = 0; r < 32; ++r) {
-                const double rv = sh.R[r][lane];
-#pragma unroll
-                for (int kk = 0; kk < 8; ++kk)
-                    if ((uint32_t)kk < kcn) cacc[kk] += rv * sh.ar[32 * bi + r][kk];
-            }
-        }
-        for (uint32_t kk = 0; kk < kcn; ++kk)
-            a.colp[((size_t)tile * a.K + kc0 + kk) * kTile + lane] = cacc[kk];
+    const uint32_t kcn = KC == 1 ? 1u : kcn_in;
+    __syncthreads();
+    for (uint32_t kk = 0; kk < kcn; ++kk) {
+        sh.ar[lane][kk] = a.aeff[(size_t)(kc0 + kk) * a.m_pad + (size_t)I * kTile + lane];
+        sh.ac[lane][kk] = a.aeff[(size_t)(kc0 + kk) * a.m_pad + (size_t)J * kTile + lane];
     }
+    double cacc[KC];
+#pragma unroll
+    for (int kk = 0; kk < KC; ++kk) cacc[kk] = 0.0;
+#pragma unroll
+    for (int bi = 0; bi < 2; ++bi) {
+        __syncthreads();
+#pragma unroll
+        for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+            for (int r = 0; r < 16; ++r)
+                sh.R[row_of(0, r, lane)][32 * bj + (lane & 31)] = r2u[bi][bj][r];
+        __syncthreads();
+        // row role: lane covers row lane%32 over columns of its half
+        double racc[KC];
+#pragma unroll
+        for (int kk = 0; kk < KC; ++kk) racc[kk] = 0.0;
+        const int rr = lane & 31, b0 = 32 * (lane >> 5);
+        for (int b = 0; b < 32; ++b) {
+            const double rv = sh.R[rr][b0 + b];
+#pragma unroll
+            for (int kk = 0; kk < KC; ++kk)
+                if ((uint32_t)kk < kcn) racc[kk] += rv * sh.ac[b0 + b][kk];
+        }
+#pragma unroll
+        for (int kk = 0; kk < KC; ++kk)
+            if ((uint32_t)kk < kcn) racc[kk] += __shfl_xor(racc[kk], 32);
+        if (lane < 32)
+            for (uint32_t kk = 0; kk < kcn; ++kk)
+                a.rowp[((size_t)tile * a.K + kc0 + kk) * kTile + 32 * bi + lane] = racc[kk];
+        // column role: lane = column, rows of this half
+        for (int r = 0; r < 32; ++r) {
+            const double rv = sh.R[r][lane];
+#pragma unroll
+            for (int kk = 0; kk < KC; ++kk)
+                if ((uint32_t)kk < kcn) cacc[kk] += rv * sh.ar[32 * bi + r][kk];
+        }
+    }
+    for (uint32_t kk = 0; kk < kcn; ++kk)
+        a.colp[((size_t)tile * a.K + kc0 + kk) * kTile + lane] = cacc[kk];
+}
+
+__device__ __forceinline__ void band_contract(const BandArgs& a, EpiShared& sh, int tile,
+                                              uint32_t I, uint32_t J,
+                                              const double (&r2u)[2][2][16]) {
+    if (a.K == 1) {
+        band_contract_chunk<1>(a, sh, tile, I, J, r2u, 0, 1);
+        return;
+    }
+    for (uint32_t kc0 = 0; kc0 < a.K; kc0 += 8)
+        band_contract_chunk<8>(a, sh, tile, I, J, r2u, kc0, a.K - kc0 < 8 ? a.K - kc0 : 8);
 }
 
 __device__ __forceinline__ void stage_info(const BandArgs& a, EpiShared& sh, uint32_t I,

@@ -413,75 +413,102 @@ struct EpiShared2 {
 };
 
 // r2u of wave w (rows 32w .. 32w+31 of the tile, all 64 columns, in MFMA
-// output order r2u[bj][r]) -> per-tile partial sums, fixed summation order.
-__device__ void band_contract2(const BandArgs& a, EpiShared2& sh, int tile, uint32_t I,
-                               uint32_t J, int w, const double (&r2u)[2][16]) {
+// output order r2u[bj][r]) -> per-tile partial sums for annotations kc0 ..
+// kc0+kcn-1 (kcn <= KC), fixed summation order.
+template <int KC>
+__device__ __forceinline__ void band_contract2_chunk(const BandArgs& a, EpiShared2& sh, int tile,
+                                                     uint32_t I, uint32_t J, int w,
+                                                     const double (&r2u)[2][16], uint32_t kc0,
+                                                     uint32_t kcn_in) {
     const int lane = threadIdx.x & 63;
-    for (uint32_t kc0 = 0; kc0 < a.K; kc0 += 8) {
-        const uint32_t kcn = a.K - kc0 < 8 ? a.K - kc0 : 8;
-        __syncthreads();
-        for (uint32_t kk = 0; kk < kcn; ++kk) {
-            if (w == 0)
-                sh.ar[lane][kk] = a.aeff[(size_t)(kc0 + kk) * a.m_pad + (size_t)I * kTile + lane];
-            else
-                sh.ac[lane][kk] = a.aeff[(size_t)(kc0 + kk) * a.m_pad + (size_t)J * kTile + lane];
-        }
-        double cacc[8];
-#pragma unroll
-        for (int kk = 0; kk < 8; ++kk) cacc[kk] = 0.0;
-#pragma unroll
-        for (int Q = 0; Q < 2; ++Q) {
-            __syncthreads();
-#pragma unroll
-            for (int bj = 0; bj < 2; ++bj)
-#pragma unroll
-                for (int i = 0; i < 8; ++i) {
-                    const int r = 8 * Q + i;
-                    const int local = 8 * ((r >> 2) - 2 * Q) + 4 * (lane >> 5) + (r & 3);
-                    sh.R[w][local][32 * bj + (lane & 31)] = r2u[bj][r];
-                }
-            __syncthreads();
-            // row role: row lane%16 of the quarter, columns 16*(lane/16) .. +16
-            double racc[8];
-#pragma unroll
-            for (int kk = 0; kk < 8; ++kk) racc[kk] = 0.0;
-            const int rr = lane & 15, b0 = 16 * (lane >> 4);
-            for (int b = 0; b < 16; ++b) {
-                const double rv = sh.R[w][rr][b0 + b];
-#pragma unroll
-                for (int kk = 0; kk < 8; ++kk)
-                    if ((uint32_t)kk < kcn) racc[kk] += rv * sh.ac[b0 + b][kk];
-            }
-#pragma unroll
-            for (int kk = 0; kk < 8; ++kk)
-                if ((uint32_t)kk < kcn) {
-                    racc[kk] += __shfl_xor(racc[kk], 16);
-                    racc[kk] += __shfl_xor(racc[kk], 32);
-                }
-            if (lane < 16)
-                for (uint32_t kk = 0; kk < kcn; ++kk)
-                    a.rowp[((size_t)tile * a.K + kc0 + kk) * kTile + 32 * w + 16 * Q + lane] = racc[kk];
-            // column role: lane = column, the 16 rows of this quarter
-            for (int r = 0; r < 16; ++r) {
-                const double rv = sh.R[w][r][lane];
-#pragma unroll
-                for (int kk = 0; kk < 8; ++kk)
-                    if ((uint32_t)kk < kcn) cacc[kk] += rv * sh.ar[32 * w + 16 * Q + r][kk];
-            }
-        }
-        __syncthreads();
-        if (w == 1)
-            for (uint32_t kk = 0; kk < kcn; ++kk) sh.cx[lane][kk] = cacc[kk];
-        __syncthreads();
+    const uint32_t kcn = KC == 1 ? 1u : kcn_in;
+    __syncthreads();
+    for (uint32_t kk = 0; kk < kcn; ++kk) {
         if (w == 0)
-            for (uint32_t kk = 0; kk < kcn; ++kk)
-                a.colp[((size_t)tile * a.K + kc0 + kk) * kTile + lane] = cacc[kk] + sh.cx[lane][kk];
+            sh.ar[lane][kk] = a.aeff[(size_t)(kc0 + kk) * a.m_pad + (size_t)I * kTile + lane];
+        else
+            sh.ac[lane][kk] = a.aeff[(size_t)(kc0 + kk) * a.m_pad + (size_t)J * kTile + lane];
     }
+    double cacc[KC];
+#pragma unroll
+    for (int kk = 0; kk < KC; ++kk) cacc[kk] = 0.0;
+#pragma unroll
+    for (int Q = 0; Q < 2; ++Q) {
+        __syncthreads();
+#pragma unroll
+        for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                const int r = 8 * Q + i;
+                const int local = 8 * ((r >> 2) - 2 * Q) + 4 * (lane >> 5) + (r & 3);
+                sh.R[w][local][32 * bj + (lane & 31)] = r2u[bj][r];
+            }
+        __syncthreads();
+        // row role: row lane%16 of the quarter, columns 16*(lane/16) .. +16
+        double racc[KC];
+#pragma unroll
+        for (int kk = 0; kk < KC; ++kk) racc[kk] = 0.0;
+        const int rr = lane & 15, b0 = 16 * (lane >> 4);
+        for (int b = 0; b < 16; ++b) {
+            const double rv = sh.R[w][rr][b0 + b];
+#pragma unroll
+            for (int kk = 0; kk < KC; ++kk)
+                if ((uint32_t)kk < kcn) racc[kk] += rv * sh.ac[b0 + b][kk];
+        }
+#pragma unroll
+        for (int kk = 0; kk < KC; ++kk)
+            if ((uint32_t)kk < kcn) {
+                racc[kk] += __shfl_xor(racc[kk], 16);
+                racc[kk] += __shfl_xor(racc[kk], 32);
+            }
+        if (lane < 16)
+            for (uint32_t kk = 0; kk < kcn; ++kk)
+                a.rowp[((size_t)tile * a.K + kc0 + kk) * kTile + 32 * w + 16 * Q + lane] = racc[kk];
+        // column role: lane = column, the 16 rows of this quarter
+        for (int r = 0; r < 16; ++r) {
+            const double rv = sh.R[w][r][lane];
+#pragma unroll
+            for (int kk = 0; kk < KC; ++kk)
+                if ((uint32_t)kk < kcn) cacc[kk] += rv * sh.ar[32 * w + 16 * Q + r][kk];
+        }
+    }
+    __syncthreads();
+    if (w == 1)
+        for (uint32_t kk = 0; kk < kcn; ++kk) sh.cx[lane][kk] = cacc[kk];
+    __syncthreads();
+    if (w == 0)
+        for (uint32_t kk = 0; kk < kcn; ++kk)
+            a.colp[((size_t)tile * a.K + kc0 + kk) * kTile + lane] = cacc[kk] + sh.cx[lane][kk];
 }
 
-template <bool VCODE, class Acc>
+__device__ __forceinline__ void band_contract2(const BandArgs& a, EpiShared2& sh, int tile,
+                                               uint32_t I, uint32_t J, int w,
+                                               const double (&r2u)[2][16]) {
+    if (a.K == 1) {
+        band_contract2_chunk<1>(a, sh, tile, I, J, w, r2u, 0, 1);
+        return;
+    }
+    for (uint32_t kc0 = 0; kc0 < a.K; kc0 += 8)
+        band_contract2_chunk<8>(a, sh, tile, I, J, w, r2u, kc0, a.K - kc0 < 8 ? a.K - kc0 : 8);
+}
+
 __device__ __forceinline__ void finish_2w(const BandArgs& a, EpiShared2& sh, int tile, uint32_t I,
-                                          uint32_t J, int w, int lane, const Acc (&acc)[4][2]);
+                                          uint32_t J, int w, int lane, const v16i (&acc)[4][2]);
+
+// r2u of one pair from its exact products G.G, G.M, M.G, M.M (integers in
+// doubles), the (C, S) of its row and column SNPs and their 1/sqrt(C Q):
+//   T  = C_r (C_c GG - S_c GM) - S_r (C_c MG - S_c MM)  = C_r C_c * cov
+//   r  = T / sqrt(C_r Q_r C_c Q_c),  r2u = r^2 (1 + 1/D) - 1/D
+// Every operation is spelled out (no contraction left to the compiler), so
+// both kernels round identically; u and v are exact (integers below 2^53).
+__device__ __forceinline__ double exact_r2u(const BandArgs& a, double gg, double gm, double mg,
+                                            double mm, int2 cr, int2 cc, double rsr, double rsc) {
+    const double u = __dsub_rn(__dmul_rn((double)cc.x, gg), __dmul_rn((double)cc.y, gm));
+    const double v = __dsub_rn(__dmul_rn((double)cc.x, mg), __dmul_rn((double)cc.y, mm));
+    const double T = __fma_rn((double)cr.x, u, -__dmul_rn((double)cr.y, v));
+    const double rr = __dmul_rn(__dmul_rn(T, rsr), rsc);
+    return __fma_rn(__dmul_rn(rr, rr), a.r2u_a, a.r2u_b);
+}
 
 // int8 form, two waves per 64x64 tile: wave w owns rows 32w .. 32w+31 (SNP
 // group 2I + w) against both column groups; 4 products x 2 blocks of
@@ -533,18 +560,13 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
         cb1 = nb1;
     }
 
-    finish_2w<false>(a, sh, tile, I, J, w, lane, acc);
+    finish_2w(a, sh, tile, I, J, w, lane, acc);
 }
 
-// FP64 finish of wave w of a two-wave tile from its exact products, then
-// the per-tile partial sums.  The accumulators hold G.G, G.M, M.G, M.M
-// (int8 kernel) or X.X, X.Q, Q.X, Q.Q (e2m1 kernel, VCODE), from which the
-// same four integers follow exactly; then
-//   T  = C_r (C_c GG - S_c GM) - S_r (C_c MG - S_c MM)  = C_r C_c * cov
-//   r  = T / sqrt(C_r Q_r C_c Q_c),  r2u = r^2 (1 + 1/D) - 1/D
-template <bool VCODE, class Acc>
+// FP64 finish of wave w of a two-wave int8 tile from its exact products
+// G.G, G.M, M.G, M.M, then the per-tile partial sums.
 __device__ __forceinline__ void finish_2w(const BandArgs& a, EpiShared2& sh, int tile, uint32_t I,
-                                          uint32_t J, int w, int lane, const Acc (&acc)[4][2]) {
+                                          uint32_t J, int w, int lane, const v16i (&acc)[4][2]) {
     if (w == 0) {
         sh.csr[lane] = a.cs[(size_t)I * kTile + lane];
         sh.rsr[lane] = a.rs[(size_t)I * kTile + lane];
@@ -562,25 +584,9 @@ __device__ __forceinline__ void finish_2w(const BandArgs& a, EpiShared2& sh, int
             const int row = 32 * w + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
             const int col = 32 * bj + (lane & 31);
             const int2 cr = sh.csr[row], cc = sh.csc[col];
-            double gg, gm, mg, mm;  // exact integers
-            if (VCODE) {
-                const double vv = 4.0 * (double)acc[0][bj][r], vq = 4.0 * (double)acc[1][bj][r];
-                const double qv = 4.0 * (double)acc[2][bj][r], qq = 4.0 * (double)acc[3][bj][r];
-                gg = vv - 3.0 * vq - 3.0 * qv + 9.0 * qq;
-                gm = (double)cr.y - vq + 3.0 * qq;
-                mg = (double)cc.y - qv + 3.0 * qq;
-                mm = (double)cr.x + (double)cc.x - a.n_pad + qq;
-            } else {
-                gg = (double)acc[0][bj][r];
-                gm = (double)acc[1][bj][r];
-                mg = (double)acc[2][bj][r];
-                mm = (double)acc[3][bj][r];
-            }
-            const double u = (double)cc.x * gg - (double)cc.y * gm;
-            const double v = (double)cc.x * mg - (double)cc.y * mm;
-            const double T = (double)cr.x * u - (double)cr.y * v;
-            const double rr = T * sh.rsr[row] * sh.rsc[col];
-            const double val = rr * rr * a.r2u_a + a.r2u_b;
+            const double val = exact_r2u(a, (double)acc[0][bj][r], (double)acc[1][bj][r],
+                                         (double)acc[2][bj][r], (double)acc[3][bj][r], cr, cc,
+                                         sh.rsr[row], sh.rsc[col]);
             const uint32_t jj = I * kTile + row, ii = J * kTile + col;
             r2u[bj][r] = (ii < jj && jj < a.m && ii >= sh.lor[row]) ? val : 0.0;
         }
@@ -612,106 +618,153 @@ __device__ __forceinline__ void vcode_operands(uint32_t w0, uint32_t w1, v8i& X,
 #define LDSC_MFMA_FP4(A, B, C) \
     __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(A, B, C, 4, 4, 0, 0x7f7f7f7f, 0, 0x7f7f7f7f)
 
-// One K-block (two K-steps of 64 individuals) of a wave of the e2m1 kernel
-// over the column groups it keeps (U0, U1): products X.X, X.Q, Q.X, Q.Q.
-template <bool U0, bool U1>
-__device__ __forceinline__ void vcode_mma(const u32x4& ca, const u32x4& cb0, const u32x4& cb1,
-                                          v16f (&acc)[4][2]) {
+// Sub-blocks of a 64x64 tile: bit 2*bi + bj stands for rows 32bi.. x
+// columns 32bj.. .  One K-block (two K-steps of 64 individuals) of the
+// e2m1 kernel over the sub-blocks of MASK: products X.X, X.Q, Q.X, Q.Q.
+template <int MASK>
+struct VcodeBlock {
+    static constexpr bool kA0 = (MASK & 0x3) != 0, kA1 = (MASK & 0xC) != 0;
+    static constexpr bool kB0 = (MASK & 0x5) != 0, kB1 = (MASK & 0xA) != 0;
+    u32x4 a0, a1, b0, b1;
+    __device__ __forceinline__ void load(const u32x4* pa0, const u32x4* pa1, const u32x4* pb0,
+                                         const u32x4* pb1, size_t off, int lane) {
+        if (kA0) a0 = (pa0 + off)[lane];
+        if (kA1) a1 = (pa1 + off)[lane];
+        if (kB0) b0 = (pb0 + off)[lane];
+        if (kB1) b1 = (pb1 + off)[lane];
+    }
+    __device__ __forceinline__ void mma(v16f (&acc)[4][2][2]) const {
 #pragma unroll
-    for (int u = 0; u < 2; ++u) {
-        v8i xa, qa, xb, qb;
-        vcode_operands(ca[2 * u], ca[2 * u + 1], xa, qa);
-        if (U0) {
-            vcode_operands(cb0[2 * u], cb0[2 * u + 1], xb, qb);
-            acc[0][0] = LDSC_MFMA_FP4(xa, xb, acc[0][0]);
-            acc[1][0] = LDSC_MFMA_FP4(xa, qb, acc[1][0]);
-            acc[2][0] = LDSC_MFMA_FP4(qa, xb, acc[2][0]);
-            acc[3][0] = LDSC_MFMA_FP4(qa, qb, acc[3][0]);
-        }
-        if (U1) {
-            vcode_operands(cb1[2 * u], cb1[2 * u + 1], xb, qb);
-            acc[0][1] = LDSC_MFMA_FP4(xa, xb, acc[0][1]);
-            acc[1][1] = LDSC_MFMA_FP4(xa, qb, acc[1][1]);
-            acc[2][1] = LDSC_MFMA_FP4(qa, xb, acc[2][1]);
-            acc[3][1] = LDSC_MFMA_FP4(qa, qb, acc[3][1]);
+        for (int u = 0; u < 2; ++u) {
+            v8i xa[2], qa[2], xb[2], qb[2];
+            if (kA0) vcode_operands(a0[2 * u], a0[2 * u + 1], xa[0], qa[0]);
+            if (kA1) vcode_operands(a1[2 * u], a1[2 * u + 1], xa[1], qa[1]);
+            if (kB0) vcode_operands(b0[2 * u], b0[2 * u + 1], xb[0], qb[0]);
+            if (kB1) vcode_operands(b1[2 * u], b1[2 * u + 1], xb[1], qb[1]);
+#pragma unroll
+            for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+                for (int bj = 0; bj < 2; ++bj)
+                    if ((MASK >> (2 * bi + bj)) & 1) {
+                        acc[0][bi][bj] = LDSC_MFMA_FP4(xa[bi], xb[bj], acc[0][bi][bj]);
+                        acc[1][bi][bj] = LDSC_MFMA_FP4(xa[bi], qb[bj], acc[1][bi][bj]);
+                        acc[2][bi][bj] = LDSC_MFMA_FP4(qa[bi], xb[bj], acc[2][bi][bj]);
+                        acc[3][bi][bj] = LDSC_MFMA_FP4(qa[bi], qb[bj], acc[3][bi][bj]);
+                    }
         }
     }
-}
+};
 
 // K loop with two alternating register sets (unrolled by two): each
 // K-block's loads are issued a whole block of MFMA work before their use and
-// no register copy waits on them.
-// pa, pb0, pb1 are the wave-uniform starts of the three SNP groups; each
-// lane adds its own 16-byte slot (scalar base + lane offset addressing).
-template <bool U0, bool U1>
-__device__ __forceinline__ void vcode_kloop(const u32x4* pa, const u32x4* pb0, const u32x4* pb1,
-                                            uint32_t nkb, int lane, v16f (&acc)[4][2]) {
+// no register copy waits on them.  pa*/pb* are the wave-uniform starts of
+// the tile's four SNP groups; each lane adds its own 16-byte slot.
+template <int MASK>
+__device__ __forceinline__ void vcode_kloop(const u32x4* pa0, const u32x4* pa1, const u32x4* pb0,
+                                            const u32x4* pb1, uint32_t nkb, int lane,
+                                            v16f (&acc)[4][2][2]) {
     const uint32_t last = nkb - 1;
-    u32x4 a0 = pa[lane], b00 = U0 ? pb0[lane] : u32x4{}, b10 = U1 ? pb1[lane] : u32x4{};
-    u32x4 a1, b01, b11;
-    for (uint32_t kb = 0; kb < nkb; kb += 2) {
-        const size_t o1 = (size_t)min(kb + 1, last) * 64;
-        a1 = (pa + o1)[lane];
-        if (U0) b01 = (pb0 + o1)[lane];
-        if (U1) b11 = (pb1 + o1)[lane];
+    VcodeBlock<MASK> s0, s1;
+    s0.load(pa0, pa1, pb0, pb1, 0, lane);
+    uint32_t kb = 0;
+    for (; kb + 1 < nkb; kb += 2) {  // s0 holds block kb here
+        s1.load(pa0, pa1, pb0, pb1, (size_t)(kb + 1) * 64, lane);
         __builtin_amdgcn_sched_barrier(0);
-        vcode_mma<U0, U1>(a0, b00, b10, acc);
-        const size_t o2 = (size_t)min(kb + 2, last) * 64;
-        a0 = (pa + o2)[lane];
-        if (U0) b00 = (pb0 + o2)[lane];
-        if (U1) b10 = (pb1 + o2)[lane];
+        s0.mma(acc);
+        s0.load(pa0, pa1, pb0, pb1, (size_t)min(kb + 2, last) * 64, lane);
         __builtin_amdgcn_sched_barrier(0);
-        if (kb + 1 < nkb) vcode_mma<U0, U1>(a1, b01, b11, acc);
+        s1.mma(acc);
     }
+    if (kb < nkb) s0.mma(acc);  // odd block count: the last block
 }
 
-// e2m1 form of the exact kernel (same 64x64 tiles, two waves, finish and
-// partial sums as k_band_i8).  Tiles hold 2-bit dosage-value codes v (3 =
-// missing or padding); with q = [v = 3] the genotype is G = v - 3q and the
-// presence M = 1 - q, so the four exact products follow from X.X, X.Q,
-// Q.X, Q.Q on v_mfma_scale_f32_32x32x64_f8f6f4 (scale 1, f32
-// accumulation, exact while 9 N_pad <= 2^24):
+// e2m1 form of the exact kernel: one wave per 64x64 tile, 4 products x 4
+// sub-blocks of 32x32 f32 accumulators (256 registers, one wave per SIMD),
+// so every operand group is loaded and expanded once per K-step and feeds 8
+// MFMAs.  Tiles hold 2-bit dosage-value codes v (3 = missing or padding);
+// with q = [v = 3] the genotype is G = v - 3q and the presence M = 1 - q,
+// so the four exact products follow from X.X, X.Q, Q.X, Q.Q on
+// v_mfma_scale_f32_32x32x64_f8f6f4 (scale 1, f32 accumulation, exact while
+// 9 N_pad <= 2^24):
 //   VV = 4 acc0, VQ = 4 acc1, QV = 4 acc2, QQ = 4 acc3
 //   GG = VV - 3 VQ - 3 QV + 9 QQ      GM = S_r - VQ + 3 QQ
 //   MG = S_c - QV + 3 QQ              MM = C_r + C_c - N_pad + QQ
-// A wave skips a column group of its tile that cannot hold an included pair
-// (all its columns precede the smallest pair bound of the wave's rows, or
-// none precedes its last row): those accumulators stay zero and the pair
-// mask drops every pair of the group.
-__global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_band_fp4(BandArgs a) {
-    __shared__ EpiShared2 sh;
+// Sub-blocks that cannot hold an included pair (all columns precede the
+// smallest pair bound of the rows, or none precedes the last row) are
+// skipped when the tile matches one of the frequent patterns (diagonal
+// tile, band edges); skipped accumulators stay zero and the pair mask drops
+// every pair they cover.
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1))) void k_band_fp4(BandArgs a) {
+    __shared__ EpiShared sh;
     const int tile = tile_of_block(a);
     if (tile < 0) return;
     const int2 tl = a.tiles[tile];
     const uint32_t I = (uint32_t)tl.x, J = (uint32_t)tl.y;
-    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int lane = threadIdx.x;
     const size_t gstride = (size_t)a.nkb * 64;
-    const u32x4* pa = a.tiled + (size_t)(2 * I + __builtin_amdgcn_readfirstlane(w)) * gstride;
+    const u32x4* pa0 = a.tiled + (size_t)(2 * I) * gstride;
     const u32x4* pb0 = a.tiled + (size_t)(2 * J) * gstride;
-    const u32x4* pb1 = pb0 + gstride;
 
-    const uint32_t r0 = I * kTile + 32 * w, c0 = J * kTile;
-    uint32_t lmin = a.lo[r0 + (lane & 31)];
+    const uint32_t r0 = I * kTile, c0 = J * kTile;
+    uint32_t lmin = a.lo[r0 + lane];  // lanes 0-31: rows of half 0, 32-63: half 1
 #pragma unroll
     for (int o = 16; o >= 1; o >>= 1) lmin = min(lmin, (uint32_t)__shfl_xor((int)lmin, o));
-    lmin = __builtin_amdgcn_readfirstlane(lmin);
-    const bool use0 = c0 + 31 >= lmin && c0 < r0 + 31;
-    const bool use1 = c0 + 63 >= lmin && c0 + 32 < r0 + 31;
+    const uint32_t lmin0 = __builtin_amdgcn_readlane(lmin, 0);
+    const uint32_t lmin1 = __builtin_amdgcn_readlane(lmin, 32);
+    int mask = 0;
+#pragma unroll
+    for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+        for (int bj = 0; bj < 2; ++bj) {
+            const uint32_t cmin = c0 + 32 * bj, rmax = r0 + 32 * bi + 31;
+            if (cmin + 31 >= (bi ? lmin1 : lmin0) && cmin < rmax) mask |= 1 << (2 * bi + bj);
+        }
 
-    v16f acc[4][2];
+    v16f acc[4][2][2];
 #pragma unroll
     for (int p = 0; p < 4; ++p)
 #pragma unroll
+        for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+            for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+                for (int r = 0; r < 16; ++r) acc[p][bi][bj][r] = 0.0f;
+    const u32x4* pa1 = pa0 + gstride;
+    const u32x4* pb1 = pb0 + gstride;
+    if (mask == 0xF)
+        vcode_kloop<0xF>(pa0, pa1, pb0, pb1, a.nkb, lane, acc);
+    else if ((mask & ~0x2) == 0 && mask)
+        vcode_kloop<0x2>(pa0, pa1, pb0, pb1, a.nkb, lane, acc);
+    else if ((mask & ~0xD) == 0 && mask)
+        vcode_kloop<0xD>(pa0, pa1, pb0, pb1, a.nkb, lane, acc);
+    else if ((mask & ~0xB) == 0 && mask)
+        vcode_kloop<0xB>(pa0, pa1, pb0, pb1, a.nkb, lane, acc);
+    else if (mask)
+        vcode_kloop<0xF>(pa0, pa1, pb0, pb1, a.nkb, lane, acc);
+
+    stage_info(a, sh, I, J);
+    __syncthreads();
+    double r2u[2][2][16];
+#pragma unroll
+    for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
         for (int bj = 0; bj < 2; ++bj)
 #pragma unroll
-            for (int r = 0; r < 16; ++r) acc[p][bj][r] = 0.0f;
-    if (use0 && use1)
-        vcode_kloop<true, true>(pa, pb0, pb1, a.nkb, lane, acc);
-    else if (use0)
-        vcode_kloop<true, false>(pa, pb0, pb1, a.nkb, lane, acc);
-    else if (use1)
-        vcode_kloop<false, true>(pa, pb0, pb1, a.nkb, lane, acc);
-    finish_2w<true>(a, sh, tile, I, J, w, lane, acc);
+            for (int r = 0; r < 16; ++r) {
+                const int row = row_of(bi, r, lane), col = 32 * bj + (lane & 31);
+                const int2 cr = sh.csr[row], cc = sh.csc[col];
+                // exact integers
+                const double vv = 4.0 * (double)acc[0][bi][bj][r], vq = 4.0 * (double)acc[1][bi][bj][r];
+                const double qv = 4.0 * (double)acc[2][bi][bj][r], qq = 4.0 * (double)acc[3][bi][bj][r];
+                const double gg = vv - 3.0 * (vq + qv) + 9.0 * qq;  // all exact
+                const double gm = (double)cr.y - vq + 3.0 * qq;
+                const double mg = (double)cc.y - qv + 3.0 * qq;
+                const double mm = (double)cr.x + (double)cc.x - a.n_pad + qq;
+                r2u[bi][bj][r] = pair_included(a, sh, I, J, row, col)
+                                     ? exact_r2u(a, gg, gm, mg, mm, cr, cc, sh.rsr[row], sh.rsc[col])
+                                     : 0.0;
+            }
+    band_contract(a, sh, tile, I, J, r2u);
 }
 
 __global__ __launch_bounds__(64) void k_band_f32(BandArgs a) {
@@ -1220,7 +1273,7 @@ int run_l2(const ldsc_panel* panel, const ldsc_snp* snps, uint64_t m, const ldsc
     else if (use_f32)
         hipLaunchKernelGGL(k_band_f32, dim3(8 * ba.per_xcd), dim3(64), 0, st, ba);
     else if (use_fp4)
-        hipLaunchKernelGGL(k_band_fp4, dim3(8 * ba.per_xcd), dim3(128), 0, st, ba);
+        hipLaunchKernelGGL(k_band_fp4, dim3(8 * ba.per_xcd), dim3(64), 0, st, ba);
     else
         hipLaunchKernelGGL(k_band_i8, dim3(8 * ba.per_xcd), dim3(128), 0, st, ba);
     HIPC(hipGetLastError());

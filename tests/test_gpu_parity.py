@@ -180,9 +180,15 @@ def test_l2_deterministic_and_chunk_invariant_when_masked():
     assert np.array_equal(a, b) and np.array_equal(a, c)
 
 
-def test_exact_kernels_identical():
-    """The e2m1 and int8 MFMA kernels compute the same integer products, so
-    their LD scores agree bit for bit: N around the K-block edges, M off the
+# The e2m1 and int8 kernels form the same integer products and the same
+# per-pair FP64 r2u; only the order in which a tile's r2u are summed into its
+# partial sums differs, so their LD scores agree to a few ulps.  A wrong
+# integer anywhere moves an LD score by far more than this.
+TOL_KERNELS = 1e-12
+
+
+def test_exact_kernels_agree():
+    """e2m1 vs int8 MFMA exact kernels: N around the K-block edges, M off the
     tile edges, --keep, several annotations with pq, --extract across
     chromosome runs with and without masking, special columns."""
     def both(panel, snps, annot=None, **kw):
@@ -196,7 +202,7 @@ def test_exact_kernels_identical():
         snps = L.make_snp_array(p.chr, p.cm, p.bp)
         with L.Panel(p.bed_bytes(), n, m) as panel:
             a, b, ma, mb = both(panel, snps, mode=L.WindowMode.Snp(50), chunk_size=30)
-        assert np.array_equal(a, b) and np.array_equal(ma, mb), (n, m)
+        assert _rel(a, b) < TOL_KERNELS and np.array_equal(ma, mb), (n, m, _rel(a, b))
     n, m = 613, 900
     p = _panel(n, m, seed=21, nchr=3)
     body = p.body
@@ -213,17 +219,17 @@ def test_exact_kernels_identical():
     with L.Panel(p.bed_bytes(), n, m, keep=keep) as panel:
         a, b, _, _ = both(panel, snps, annot=annot, mode=L.WindowMode.Kb(120.0), chunk_size=50,
                           pq_exp=0.75)
-        assert np.array_equal(a, b)
+        assert _rel(a, b) < TOL_KERNELS
         for per_chr in (False, True):
             for masking in (False, True):
                 a, b, _, _ = both(panel, snps_sel, mode=L.WindowMode.Cm(0.1), chunk_size=40,
                                   per_chromosome=per_chr, snp_level_masking=masking)
-                assert np.array_equal(a, b), (per_chr, masking)
+                assert _rel(a, b) < TOL_KERNELS, (per_chr, masking)
 
 
 def test_exact_and_fp32_at_panel_scale():
     """N=50,000 x M=8,000 slice of the panel-scale workload: the e2m1 and
-    int8 exact kernels agree bit for bit, the FP32 form agrees to its
+    int8 exact kernels agree to a few ulps, the FP32 form agrees to its
     tolerance, and the result is reproducible bit for bit."""
     n, m = 50_000, 8_000
     p = _panel(n, m, seed=20261205, miss=0.01)
@@ -235,7 +241,7 @@ def test_exact_and_fp32_at_panel_scale():
         a2, _, _ = L.compute_ldscore(panel, snps, L.L2Config(**base))
         c, _, _ = L.compute_ldscore(panel, snps, L.L2Config(**base, exact_kernel="i8"))
     assert np.array_equal(a, a2)
-    assert np.array_equal(a, c)
+    assert _rel(c, a) < TOL_KERNELS
     assert _rel(b, a) < TOL_F32
     # l2 >= self term minus the bias of pairs: every SNP has its own 1
     assert np.all(a > 0.5)
