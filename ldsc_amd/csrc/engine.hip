@@ -619,12 +619,16 @@ __device__ __forceinline__ void vcode_operands(uint32_t w0, uint32_t w1, v8i& X,
     __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(A, B, C, 4, 4, 0, 0x7f7f7f7f, 0, 0x7f7f7f7f)
 
 // Sub-blocks of a 64x64 tile: bit 2*bi + bj stands for rows 32bi.. x
-// columns 32bj.. .  One K-block (two K-steps of 64 individuals) of the
-// e2m1 kernel over the sub-blocks of MASK: products X.X, X.Q, Q.X, Q.Q.
+// columns 32bj.. .  VcodeBlock holds one K-block (two K-steps of 64
+// individuals) of the raw value codes of the groups MASK needs; VcodeOps the
+// e2m1 operands X, Q of one K-step, whose MFMAs are X.X, X.Q, Q.X, Q.Q.
 template <int MASK>
 struct VcodeBlock {
     static constexpr bool kA0 = (MASK & 0x3) != 0, kA1 = (MASK & 0xC) != 0;
     static constexpr bool kB0 = (MASK & 0x5) != 0, kB1 = (MASK & 0xA) != 0;
+    static constexpr int kGroups = (int)kA0 + (int)kA1 + (int)kB0 + (int)kB1;
+    static constexpr int kMfma = 4 * (((MASK >> 0) & 1) + ((MASK >> 1) & 1) + ((MASK >> 2) & 1) +
+                                      ((MASK >> 3) & 1));
     u32x4 a0, a1, b0, b1;
     __device__ __forceinline__ void load(const u32x4* pa0, const u32x4* pa1, const u32x4* pb0,
                                          const u32x4* pb1, size_t off, int lane) {
@@ -633,49 +637,81 @@ struct VcodeBlock {
         if (kB0) b0 = (pb0 + off)[lane];
         if (kB1) b1 = (pb1 + off)[lane];
     }
+};
+
+template <int MASK>
+struct VcodeOps {
+    using B = VcodeBlock<MASK>;
+    v8i xa[2], qa[2], xb[2], qb[2];
+    __device__ __forceinline__ void expand(const B& blk, int u) {
+        if (B::kA0) vcode_operands(blk.a0[2 * u], blk.a0[2 * u + 1], xa[0], qa[0]);
+        if (B::kA1) vcode_operands(blk.a1[2 * u], blk.a1[2 * u + 1], xa[1], qa[1]);
+        if (B::kB0) vcode_operands(blk.b0[2 * u], blk.b0[2 * u + 1], xb[0], qb[0]);
+        if (B::kB1) vcode_operands(blk.b1[2 * u], blk.b1[2 * u + 1], xb[1], qb[1]);
+    }
     __device__ __forceinline__ void mma(v16f (&acc)[4][2][2]) const {
 #pragma unroll
-        for (int u = 0; u < 2; ++u) {
-            v8i xa[2], qa[2], xb[2], qb[2];
-            if (kA0) vcode_operands(a0[2 * u], a0[2 * u + 1], xa[0], qa[0]);
-            if (kA1) vcode_operands(a1[2 * u], a1[2 * u + 1], xa[1], qa[1]);
-            if (kB0) vcode_operands(b0[2 * u], b0[2 * u + 1], xb[0], qb[0]);
-            if (kB1) vcode_operands(b1[2 * u], b1[2 * u + 1], xb[1], qb[1]);
+        for (int bi = 0; bi < 2; ++bi)
 #pragma unroll
-            for (int bi = 0; bi < 2; ++bi)
-#pragma unroll
-                for (int bj = 0; bj < 2; ++bj)
-                    if ((MASK >> (2 * bi + bj)) & 1) {
-                        acc[0][bi][bj] = LDSC_MFMA_FP4(xa[bi], xb[bj], acc[0][bi][bj]);
-                        acc[1][bi][bj] = LDSC_MFMA_FP4(xa[bi], qb[bj], acc[1][bi][bj]);
-                        acc[2][bi][bj] = LDSC_MFMA_FP4(qa[bi], xb[bj], acc[2][bi][bj]);
-                        acc[3][bi][bj] = LDSC_MFMA_FP4(qa[bi], qb[bj], acc[3][bi][bj]);
-                    }
-        }
+            for (int bj = 0; bj < 2; ++bj)
+                if ((MASK >> (2 * bi + bj)) & 1) {
+                    acc[0][bi][bj] = LDSC_MFMA_FP4(xa[bi], xb[bj], acc[0][bi][bj]);
+                    acc[1][bi][bj] = LDSC_MFMA_FP4(xa[bi], qb[bj], acc[1][bi][bj]);
+                    acc[2][bi][bj] = LDSC_MFMA_FP4(qa[bi], xb[bj], acc[2][bi][bj]);
+                    acc[3][bi][bj] = LDSC_MFMA_FP4(qa[bi], qb[bj], acc[3][bi][bj]);
+                }
     }
 };
 
-// K loop with two alternating register sets (unrolled by two): each
-// K-block's loads are issued a whole block of MFMA work before their use and
-// no register copy waits on them.  pa*/pb* are the wave-uniform starts of
-// the tile's four SNP groups; each lane adds its own 16-byte slot.
+// One pipeline phase: the MFMAs of the expanded step `cur` run while the
+// next step is expanded into `nxt`; the scheduler is asked to alternate one
+// MFMA with a share of the expansion's VALU work so that the VALU issues in
+// the shadow of the matrix pipe.
+template <int MASK>
+__device__ __forceinline__ void vcode_phase(VcodeOps<MASK>& nxt, const VcodeBlock<MASK>& src,
+                                            int u, const VcodeOps<MASK>& cur,
+                                            v16f (&acc)[4][2][2]) {
+    using B = VcodeBlock<MASK>;
+    nxt.expand(src, u);
+    cur.mma(acc);
+    constexpr int kValu = (14 * B::kGroups + B::kMfma - 1) / B::kMfma;
+#pragma unroll
+    for (int i = 0; i < B::kMfma; ++i) {
+        __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);      // one MFMA
+        __builtin_amdgcn_sched_group_barrier(0x002, kValu, 0);  // then VALU
+    }
+    __builtin_amdgcn_sched_barrier(0);
+}
+
+// K loop, software-pipelined at the K-step level and unrolled by two K-blocks
+// (raw sets sA, sB and operand sets e0, e1 alternate statically, so no
+// register copies): each K-block's loads are issued a block ahead of their
+// expansion, each K-step is expanded while the previous one's MFMAs run.
+// pa*/pb* are the wave-uniform starts of the tile's four SNP groups; each
+// lane adds its own 16-byte slot.
 template <int MASK>
 __device__ __forceinline__ void vcode_kloop(const u32x4* pa0, const u32x4* pa1, const u32x4* pb0,
                                             const u32x4* pb1, uint32_t nkb, int lane,
                                             v16f (&acc)[4][2][2]) {
     const uint32_t last = nkb - 1;
-    VcodeBlock<MASK> s0, s1;
-    s0.load(pa0, pa1, pb0, pb1, 0, lane);
+    VcodeBlock<MASK> sA, sB;
+    VcodeOps<MASK> e0, e1;
+    sA.load(pa0, pa1, pb0, pb1, 0, lane);
+    sB.load(pa0, pa1, pb0, pb1, (size_t)min(1u, last) * 64, lane);
+    e0.expand(sA, 0);
     uint32_t kb = 0;
-    for (; kb + 1 < nkb; kb += 2) {  // s0 holds block kb here
-        s1.load(pa0, pa1, pb0, pb1, (size_t)(kb + 1) * 64, lane);
-        __builtin_amdgcn_sched_barrier(0);
-        s0.mma(acc);
-        s0.load(pa0, pa1, pb0, pb1, (size_t)min(kb + 2, last) * 64, lane);
-        __builtin_amdgcn_sched_barrier(0);
-        s1.mma(acc);
+    for (; kb + 1 < nkb; kb += 2) {  // e0 = step (kb, 0), sA = block kb, sB = block kb+1
+        vcode_phase(e1, sA, 1, e0, acc);
+        sA.load(pa0, pa1, pb0, pb1, (size_t)min(kb + 2, last) * 64, lane);
+        vcode_phase(e0, sB, 0, e1, acc);
+        vcode_phase(e1, sB, 1, e0, acc);
+        sB.load(pa0, pa1, pb0, pb1, (size_t)min(kb + 3, last) * 64, lane);
+        vcode_phase(e0, sA, 0, e1, acc);
     }
-    if (kb < nkb) s0.mma(acc);  // odd block count: the last block
+    if (kb < nkb) {  // odd block count: e0 = step (kb, 0), sA = block kb
+        vcode_phase(e1, sA, 1, e0, acc);
+        e1.mma(acc);
+    }
 }
 
 // e2m1 form of the exact kernel: one wave per 64x64 tile, 4 products x 4
