@@ -683,33 +683,45 @@ __device__ __forceinline__ void vcode_phase(VcodeOps<MASK>& nxt, const VcodeBloc
     __builtin_amdgcn_sched_barrier(0);
 }
 
-// K loop, software-pipelined at the K-step level and unrolled by two K-blocks
-// (raw sets sA, sB and operand sets e0, e1 alternate statically, so no
-// register copies): each K-block's loads are issued a block ahead of their
-// expansion, each K-step is expanded while the previous one's MFMAs run.
-// pa*/pb* are the wave-uniform starts of the tile's four SNP groups; each
-// lane adds its own 16-byte slot.
+// K loop, software-pipelined at the K-step level and unrolled by three
+// K-blocks: raw sets sA, sB, sC hold blocks kb, kb+1, kb+2 and operand sets
+// e0, e1 alternate statically (no register copies).  Each raw set is
+// reloaded right after its second K-step is expanded, 2.5 blocks of MFMA
+// work before its next use; each K-step is expanded while the previous
+// one's MFMAs run.  pa*/pb* are the wave-uniform starts of the tile's four
+// SNP groups; each lane adds its own 16-byte slot.
 template <int MASK>
 __device__ __forceinline__ void vcode_kloop(const u32x4* pa0, const u32x4* pa1, const u32x4* pb0,
                                             const u32x4* pb1, uint32_t nkb, int lane,
                                             v16f (&acc)[4][2][2]) {
     const uint32_t last = nkb - 1;
-    VcodeBlock<MASK> sA, sB;
+    VcodeBlock<MASK> sA, sB, sC;
     VcodeOps<MASK> e0, e1;
+    auto at = [&](uint32_t k) { return (size_t)min(k, last) * 64; };
     sA.load(pa0, pa1, pb0, pb1, 0, lane);
-    sB.load(pa0, pa1, pb0, pb1, (size_t)min(1u, last) * 64, lane);
+    sB.load(pa0, pa1, pb0, pb1, at(1), lane);
+    sC.load(pa0, pa1, pb0, pb1, at(2), lane);
     e0.expand(sA, 0);
     uint32_t kb = 0;
-    for (; kb + 1 < nkb; kb += 2) {  // e0 = step (kb, 0), sA = block kb, sB = block kb+1
+    for (; kb + 2 < nkb; kb += 3) {  // e0 = step (kb, 0); sA, sB, sC = blocks kb .. kb+2
         vcode_phase(e1, sA, 1, e0, acc);
-        sA.load(pa0, pa1, pb0, pb1, (size_t)min(kb + 2, last) * 64, lane);
+        sA.load(pa0, pa1, pb0, pb1, at(kb + 3), lane);
         vcode_phase(e0, sB, 0, e1, acc);
         vcode_phase(e1, sB, 1, e0, acc);
-        sB.load(pa0, pa1, pb0, pb1, (size_t)min(kb + 3, last) * 64, lane);
+        sB.load(pa0, pa1, pb0, pb1, at(kb + 4), lane);
+        vcode_phase(e0, sC, 0, e1, acc);
+        vcode_phase(e1, sC, 1, e0, acc);
+        sC.load(pa0, pa1, pb0, pb1, at(kb + 5), lane);
         vcode_phase(e0, sA, 0, e1, acc);
     }
-    if (kb < nkb) {  // odd block count: e0 = step (kb, 0), sA = block kb
+    // 0, 1 or 2 blocks left: e0 = step (kb, 0), sA = block kb, sB = block kb+1
+    if (kb + 1 == nkb) {
         vcode_phase(e1, sA, 1, e0, acc);
+        e1.mma(acc);
+    } else if (kb + 2 == nkb) {
+        vcode_phase(e1, sA, 1, e0, acc);
+        vcode_phase(e0, sB, 0, e1, acc);
+        vcode_phase(e1, sB, 1, e0, acc);
         e1.mma(acc);
     }
 }
