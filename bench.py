@@ -17,8 +17,8 @@ Default run (one JSON line):
   * cpu_baseline -- the C restatement of the reference's f64 path on the
     host cores, on a bounded slice of the same panel.
 
-`--workload c3` (CountSketch, N=100,000 x M=200,000, d in {200, 1000, 5000})
-and `--workload c4` (partitioned, N=20,000 x M=500,000, K=50, kb 1000) print
+`--workload sketch` (CountSketch, N=100,000 x M=200,000, d in {200, 1000, 5000})
+and `--workload partitioned` (partitioned, N=20,000 x M=500,000, K=50, kb 1000) print
 their own JSON lines.  `--gpus` above 1 is refused: this build targets one GPU.
 """
 from __future__ import annotations
@@ -48,7 +48,7 @@ def log(*a):
 def profiled_traffic(kernel: str):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 summary
     (FETCH_SIZE/WRITE_SIZE scaled by a copy-kernel calibration), or None."""
-    path = os.path.join(ROOT, "profiles", f"{kernel}_c2.md")
+    path = os.path.join(ROOT, "profiles", "k_band_fp4_50k_x_100k.md")
     try:
         for line in open(path):
             parts = line.split()
@@ -77,13 +77,13 @@ def timed_steps(L, panel, snps, cfg, steps, warmup, tag, annot=None):
 
 
 # ---------------------------------------------------------------------------
-# C2: headline
+# Headline: exact LD scores at panel scale
 # ---------------------------------------------------------------------------
-def leg_c2(args, L, synth):
+def leg_panel(args, L, synth):
     seed = 20261205
     t0 = time.perf_counter()
     p = synth.make_panel(args.n, args.m, seed, miss_rate=0.01)
-    log(f"C2: generated panel N={p.n} M={p.m} in {time.perf_counter() - t0:.1f}s")
+    log(f"panel: generated N={p.n} M={p.m} in {time.perf_counter() - t0:.1f}s")
     snps = L.make_snp_array(p.chr, p.cm, p.bp)
     base = dict(mode=L.WindowMode.Cm(1.0), chunk_size=200, snp_level_masking=True)
     cfg8, cfg32 = L.L2Config(**base), L.L2Config(**base, use_f32=True)
@@ -194,7 +194,7 @@ def leg_c2(args, L, synth):
 
 
 # ---------------------------------------------------------------------------
-# C5: serving
+# Serving: mixed request stream on resident panels
 # ---------------------------------------------------------------------------
 def leg_serving(args, L, synth):
     from ldsc_amd.serve import Server
@@ -265,21 +265,21 @@ def leg_serving(args, L, synth):
 
 
 # ---------------------------------------------------------------------------
-# C3: CountSketch
+# CountSketch
 # ---------------------------------------------------------------------------
-def leg_c3(args, L, synth):
-    n, m = 100_000, args.c3_m
+def leg_sketch(args, L, synth):
+    n, m = 100_000, args.sketch_m
     p = synth.make_panel(n, m, 20261206, miss_rate=0.01)
     snps = L.make_snp_array(p.chr, p.cm, p.bp)
     out = {"workload": f"CountSketch N={n} x M={m}, cM 1, chunk 200, masked", "d": {}}
     with L.Panel(p.bed_bytes(), n, m) as panel:
         ex_cfg = L.L2Config(mode=L.WindowMode.Cm(1.0), snp_level_masking=True)
-        w, pf, (exact, _) = timed_steps(L, panel, snps, ex_cfg, 1, 1, "c3 exact")
+        w, pf, (exact, _) = timed_steps(L, panel, snps, ex_cfg, 1, 1, "sketch exact")
         out["exact_int8_ms"] = w[0] * 1e3
         P = pf[0].pairs_included
         for d in (200, 1000, 5000):
             cfg = L.L2Config(mode=L.WindowMode.Cm(1.0), snp_level_masking=True, sketch=d)
-            w, pf, (sk, _) = timed_steps(L, panel, snps, cfg, args.steps, 1, f"c3 d={d}")
+            w, pf, (sk, _) = timed_steps(L, panel, snps, cfg, args.steps, 1, f"sketch d={d}")
             t = float(np.mean(w))
             sk_s = float(np.mean([q.sketch_secs for q in pf]))
             tile_s = float(np.mean([q.norm_secs for q in pf]))
@@ -295,16 +295,16 @@ def leg_c3(args, L, synth):
                 "pearson_r_vs_exact": float(np.corrcoef(sk[:, 0], exact[:, 0])[0, 1]),
                 "max_abs_diff_vs_exact": float(np.max(np.abs(sk - exact))),
             }
-            log(f"c3 d={d}: {out['d'][str(d)]}")
+            log(f"sketch d={d}: {out['d'][str(d)]}")
     out["pairs"] = P
     return out
 
 
 # ---------------------------------------------------------------------------
-# C4: partitioned LD scores
+# Partitioned LD scores
 # ---------------------------------------------------------------------------
-def leg_c4(args, L, synth):
-    n, m = 20_000, args.c4_m
+def leg_partitioned(args, L, synth):
+    n, m = 20_000, args.partitioned_m
     p = synth.make_panel(n, m, 20261207 + 100, miss_rate=0.01)
     snps = L.make_snp_array(p.chr, p.cm, p.bp)
     annot = synth.make_annot(m, 20261210, n_bin=40, n_cont=10)
@@ -315,7 +315,7 @@ def leg_c4(args, L, synth):
                                ("fp32", True, "auto")):
             cfg = L.L2Config(mode=L.WindowMode.Kb(1000.0), chunk_size=200, use_f32=f32,
                              exact_kernel=kern)
-            w, pf, (l2, _) = timed_steps(L, panel, snps, cfg, args.steps, 1, f"c4 {tag}",
+            w, pf, (l2, _) = timed_steps(L, panel, snps, cfg, args.steps, 1, f"partitioned {tag}",
                                          annot=annot)
             P = pf[0].pairs_included
             t = float(np.mean(w))
@@ -336,7 +336,7 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4"])
+    ap.add_argument("--workload", default="panel", choices=["panel", "sketch", "partitioned"])
     ap.add_argument("--n", type=int, default=50_000)
     ap.add_argument("--m", type=int, default=100_000)
     ap.add_argument("--fp32-steps", type=int, default=2)
@@ -350,8 +350,8 @@ def main() -> int:
     ap.add_argument("--serve-inflight", type=int, default=4)
     ap.add_argument("--serve-m-big", type=int, default=1_000_000)
     ap.add_argument("--serve-m-small", type=int, default=500_000)
-    ap.add_argument("--c3-m", type=int, default=200_000)
-    ap.add_argument("--c4-m", type=int, default=500_000)
+    ap.add_argument("--sketch-m", type=int, default=200_000)
+    ap.add_argument("--partitioned-m", type=int, default=500_000)
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="only run a device-to-device copy of this many MiB (torch) to "
                          "calibrate HBM byte counters, then exit")
@@ -375,25 +375,25 @@ def main() -> int:
     import ldsc_amd as L
     from ldsc_amd import synth
 
-    if args.workload == "c3":
-        print(json.dumps({"metric": "CountSketch LD scores", **leg_c3(args, L, synth)}),
+    if args.workload == "sketch":
+        print(json.dumps({"metric": "CountSketch LD scores", **leg_sketch(args, L, synth)}),
               flush=True)
         return 0
-    if args.workload == "c4":
-        print(json.dumps({"metric": "partitioned LD scores (K=50)", **leg_c4(args, L, synth)}),
+    if args.workload == "partitioned":
+        print(json.dumps({"metric": "partitioned LD scores (K=50)", **leg_partitioned(args, L, synth)}),
               flush=True)
         return 0
 
-    c2 = leg_c2(args, L, synth)
+    pan = leg_panel(args, L, synth)
     serving = None if args.skip_serving else leg_serving(args, L, synth)
     out = {
         "metric": METRIC,
-        "value": c2["value"],
+        "value": pan["value"],
         "unit": "SNP-pairs/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": c2["ms_per_step"],
+        "ms_per_step": pan["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -401,18 +401,18 @@ def main() -> int:
         "data": "synthetic (seeded founder-haplotype generator)",
         "config": {"workload": "exact LD scores, N=50000 x M=100000, 1% missing, "
                                "--ld-wind-cm 1, chunk 200, --snp-level-masking",
-                   "n_indiv": args.n, "m_snps": args.m, "pairs": c2["pairs"],
-                   "pairs_evaluated": c2["pairs_evaluated"], "tiles": c2["tiles"]},
-        "fp32_plain": c2["fp32"],
-        "exact_int8": c2["exact_int8"],
-        "exact_over_fp32": c2["value"] / c2["fp32"]["value"] if c2["fp32"] else None,
-        "ld_scored_snps_per_s": c2["ld_scored_snps_per_s"],
-        "phases_ms": {"tile_stats": c2["tile_ms"], "band": c2["roofline"]["band_kernel_ms"],
-                      "reduce": c2["reduce_ms"], "host_prep": c2["host_prep_ms"],
-                      "library_call": c2["library_ms"]},
-        "roofline": c2["roofline"],
-        "cpu_baseline": c2.get("cpu_baseline"),
-        "sample_accuracy": c2.get("sample_accuracy"),
+                   "n_indiv": args.n, "m_snps": args.m, "pairs": pan["pairs"],
+                   "pairs_evaluated": pan["pairs_evaluated"], "tiles": pan["tiles"]},
+        "fp32_plain": pan["fp32"],
+        "exact_int8": pan["exact_int8"],
+        "exact_over_fp32": pan["value"] / pan["fp32"]["value"] if pan["fp32"] else None,
+        "ld_scored_snps_per_s": pan["ld_scored_snps_per_s"],
+        "phases_ms": {"tile_stats": pan["tile_ms"], "band": pan["roofline"]["band_kernel_ms"],
+                      "reduce": pan["reduce_ms"], "host_prep": pan["host_prep_ms"],
+                      "library_call": pan["library_ms"]},
+        "roofline": pan["roofline"],
+        "cpu_baseline": pan.get("cpu_baseline"),
+        "sample_accuracy": pan.get("sample_accuracy"),
         "serving": serving,
     }
     print(json.dumps(out), flush=True)

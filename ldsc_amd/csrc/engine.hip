@@ -630,12 +630,21 @@ struct VcodeBlock {
     static constexpr int kMfma = 4 * (((MASK >> 0) & 1) + ((MASK >> 1) & 1) + ((MASK >> 2) & 1) +
                                       ((MASK >> 3) & 1));
     u32x4 a0, a1, b0, b1;
+    // Block k; with GUARD, zeros past the last block (value code 0 in every
+    // cell adds nothing to any product), so the K loop can run in whole
+    // unrolled steps.
+    template <bool GUARD>
     __device__ __forceinline__ void load(const u32x4* pa0, const u32x4* pa1, const u32x4* pb0,
-                                         const u32x4* pb1, size_t off, int lane) {
-        if (kA0) a0 = (pa0 + off)[lane];
-        if (kA1) a1 = (pa1 + off)[lane];
-        if (kB0) b0 = (pb0 + off)[lane];
-        if (kB1) b1 = (pb1 + off)[lane];
+                                         const u32x4* pb1, uint32_t k, uint32_t nkb, int lane) {
+        if (!GUARD || k < nkb) {
+            const size_t off = (size_t)k * 64;
+            if (kA0) a0 = (pa0 + off)[lane];
+            if (kA1) a1 = (pa1 + off)[lane];
+            if (kB0) b0 = (pb0 + off)[lane];
+            if (kB1) b1 = (pb1 + off)[lane];
+        } else {
+            a0 = a1 = b0 = b1 = u32x4{0u, 0u, 0u, 0u};
+        }
     }
 };
 
@@ -688,42 +697,44 @@ __device__ __forceinline__ void vcode_phase(VcodeOps<MASK>& nxt, const VcodeBloc
 // e0, e1 alternate statically (no register copies).  Each raw set is
 // reloaded right after its second K-step is expanded, 2.5 blocks of MFMA
 // work before its next use; each K-step is expanded while the previous
-// one's MFMAs run.  pa*/pb* are the wave-uniform starts of the tile's four
-// SNP groups; each lane adds its own 16-byte slot.
+// one's MFMAs run.  The block count is rounded up to a multiple of three
+// with zero blocks (at most two, < 1 % of the work at panel scale), so there
+// is no separate tail.  pa*/pb* are the wave-uniform starts of the tile's
+// four SNP groups; each lane adds its own 16-byte slot.
+template <int MASK, bool GUARD>
+__device__ __forceinline__ void vcode_triple(VcodeBlock<MASK>& sA, VcodeBlock<MASK>& sB,
+                                             VcodeBlock<MASK>& sC, VcodeOps<MASK>& e0,
+                                             VcodeOps<MASK>& e1, const u32x4* pa0,
+                                             const u32x4* pa1, const u32x4* pb0,
+                                             const u32x4* pb1, uint32_t kb, uint32_t nkb,
+                                             int lane, v16f (&acc)[4][2][2]) {
+    vcode_phase(e1, sA, 1, e0, acc);
+    sA.template load<GUARD>(pa0, pa1, pb0, pb1, kb + 3, nkb, lane);
+    vcode_phase(e0, sB, 0, e1, acc);
+    vcode_phase(e1, sB, 1, e0, acc);
+    sB.template load<GUARD>(pa0, pa1, pb0, pb1, kb + 4, nkb, lane);
+    vcode_phase(e0, sC, 0, e1, acc);
+    vcode_phase(e1, sC, 1, e0, acc);
+    sC.template load<GUARD>(pa0, pa1, pb0, pb1, kb + 5, nkb, lane);
+    vcode_phase(e0, sA, 0, e1, acc);
+}
+
 template <int MASK>
 __device__ __forceinline__ void vcode_kloop(const u32x4* pa0, const u32x4* pa1, const u32x4* pb0,
                                             const u32x4* pb1, uint32_t nkb, int lane,
                                             v16f (&acc)[4][2][2]) {
-    const uint32_t last = nkb - 1;
     VcodeBlock<MASK> sA, sB, sC;
     VcodeOps<MASK> e0, e1;
-    auto at = [&](uint32_t k) { return (size_t)min(k, last) * 64; };
-    sA.load(pa0, pa1, pb0, pb1, 0, lane);
-    sB.load(pa0, pa1, pb0, pb1, at(1), lane);
-    sC.load(pa0, pa1, pb0, pb1, at(2), lane);
+    sA.template load<true>(pa0, pa1, pb0, pb1, 0, nkb, lane);
+    sB.template load<true>(pa0, pa1, pb0, pb1, 1, nkb, lane);
+    sC.template load<true>(pa0, pa1, pb0, pb1, 2, nkb, lane);
     e0.expand(sA, 0);
+    // e0 = step (kb, 0); sA, sB, sC = blocks kb .. kb+2
     uint32_t kb = 0;
-    for (; kb + 2 < nkb; kb += 3) {  // e0 = step (kb, 0); sA, sB, sC = blocks kb .. kb+2
-        vcode_phase(e1, sA, 1, e0, acc);
-        sA.load(pa0, pa1, pb0, pb1, at(kb + 3), lane);
-        vcode_phase(e0, sB, 0, e1, acc);
-        vcode_phase(e1, sB, 1, e0, acc);
-        sB.load(pa0, pa1, pb0, pb1, at(kb + 4), lane);
-        vcode_phase(e0, sC, 0, e1, acc);
-        vcode_phase(e1, sC, 1, e0, acc);
-        sC.load(pa0, pa1, pb0, pb1, at(kb + 5), lane);
-        vcode_phase(e0, sA, 0, e1, acc);
-    }
-    // 0, 1 or 2 blocks left: e0 = step (kb, 0), sA = block kb, sB = block kb+1
-    if (kb + 1 == nkb) {
-        vcode_phase(e1, sA, 1, e0, acc);
-        e1.mma(acc);
-    } else if (kb + 2 == nkb) {
-        vcode_phase(e1, sA, 1, e0, acc);
-        vcode_phase(e0, sB, 0, e1, acc);
-        vcode_phase(e1, sB, 1, e0, acc);
-        e1.mma(acc);
-    }
+    for (; kb + 5 < nkb; kb += 3)  // every block it loads exists
+        vcode_triple<MASK, false>(sA, sB, sC, e0, e1, pa0, pa1, pb0, pb1, kb, nkb, lane, acc);
+    for (; kb < nkb; kb += 3)      // the last one or two steps, zero blocks past the end
+        vcode_triple<MASK, true>(sA, sB, sC, e0, e1, pa0, pa1, pb0, pb1, kb, nkb, lane, acc);
 }
 
 // e2m1 form of the exact kernel: one wave per 64x64 tile, 4 products x 4
