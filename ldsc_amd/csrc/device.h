@@ -73,11 +73,13 @@ struct HostScratch {
 struct Workspace {
     int device = 0;
     hipStream_t stream = nullptr;
+    hipStream_t copy_stream = nullptr;  // read-backs that overlap the band kernel
     hipEvent_t ev[8] = {};
     DevBuf tiled, stats, bed_idx, lo, info, aeff, annot, maf, mafd, tiles, j0, tstart, rowp,
         colp, l2, sk_bucket, sk_sign, sk_cols, sk_perm, sk_off, scratch;
     PinnedBuf pinned;       // SNP row indices (upload in flight while the host plans)
     PinnedBuf pinned_meta;  // pair bounds, band plan, stats read-back
+    PinnedBuf pinned_maf;   // MAF read-back (during the band kernel)
     HostScratch host;
     size_t bytes() const;
     hipError_t init(int dev);
@@ -114,16 +116,17 @@ struct ldsc_panel {
 
 namespace ldsc {
 
-// Tiles the call's SNP rows into ws->tiled and counts their integer stats
+// Tiles the call's SNP rows (d_bed_idx[j], or base + j when d_bed_idx is
+// null) into ws->tiled and counts their integer stats
 // into ws->stats.  `d_map` (nullable) gives, for each of the panel's N
 // individuals in the order wanted, its FAM row (--keep order, or the
 // CountSketch bucket order) in the lane-major layout of lane_major_map; it
 // is applied by an LDS-staged row gather.
 // `vcode` selects the dosage-value tile form of the e2m1 band kernel
 // (engine.hip) instead of the PLINK-code form above.
-int gather_and_tile(const ldsc_panel* panel, const uint64_t* d_bed_idx, uint64_t m,
-                    const int32_t* d_map, uint64_t m_pad, bool vcode, Workspace* ws, char* err,
-                    uint64_t err_len);
+int gather_and_tile(const ldsc_panel* panel, const uint64_t* d_bed_idx, uint64_t base,
+                    uint64_t m, const int32_t* d_map, uint64_t m_pad, bool vcode, Workspace* ws,
+                    char* err, uint64_t err_len);
 
 // The exact / sketch engine on workspace `ws` (which must belong to
 // panel->device).  Synchronous: outputs are written when it returns.

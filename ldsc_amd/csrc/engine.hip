@@ -93,6 +93,8 @@ hipError_t Workspace::init(int dev) {
     if (e != hipSuccess) return e;
     e = hipStreamCreateWithFlags(&stream, hipStreamNonBlocking);
     if (e != hipSuccess) return e;
+    e = hipStreamCreateWithFlags(&copy_stream, hipStreamNonBlocking);
+    if (e != hipSuccess) return e;
     for (auto& x : ev) {
         e = hipEventCreate(&x);
         if (e != hipSuccess) return e;
@@ -107,9 +109,12 @@ void Workspace::destroy() {
     for (auto* b : all) b->release();
     pinned.release();
     pinned_meta.release();
+    pinned_maf.release();
     for (auto& x : ev)
         if (x) (void)hipEventDestroy(x);
     if (stream) (void)hipStreamDestroy(stream);
+    if (copy_stream) (void)hipStreamDestroy(copy_stream);
+    copy_stream = nullptr;
     stream = nullptr;
 }
 
@@ -155,7 +160,8 @@ __global__ __launch_bounds__(512) void k_permute_rows(const uint8_t* __restrict_
                                                        uint64_t pitch, uint32_t dwords,
                                                        uint32_t ldsrow,
                                                        const uint64_t* __restrict__ bed_idx,
-                                                       uint32_t m, const int32_t* __restrict__ map_t,
+                                                       uint64_t base, uint32_t m,
+                                                       const int32_t* __restrict__ map_t,
                                                        uint32_t n16, uint8_t* __restrict__ out) {
     extern __shared__ uint8_t rows[];
     const int tid = threadIdx.x;
@@ -163,7 +169,8 @@ __global__ __launch_bounds__(512) void k_permute_rows(const uint8_t* __restrict_
         const uint32_t nr = m - j0 < (uint32_t)R ? m - j0 : (uint32_t)R;
         __syncthreads();
         for (uint32_t r = 0; r < nr; ++r) {
-            const uint32_t* s4 = reinterpret_cast<const uint32_t*>(slabs + bed_idx[j0 + r] * pitch);
+            const uint64_t row = bed_idx ? bed_idx[j0 + r] : base + j0 + r;
+            const uint32_t* s4 = reinterpret_cast<const uint32_t*>(slabs + row * pitch);
             uint32_t* d4 = reinterpret_cast<uint32_t*>(rows + (size_t)r * ldsrow);
             for (uint32_t q = tid; q < dwords; q += blockDim.x) d4[q] = s4[q];
         }
@@ -250,7 +257,8 @@ template <bool VCODE>
 __global__ __launch_bounds__(256) void k_tile_stats(const uint8_t* __restrict__ slabs,
                                                      uint64_t pitch, uint32_t n,
                                                      const uint64_t* __restrict__ bed_idx,
-                                                     uint32_t m, uint32_t nkb, uint32_t m_pad,
+                                                     uint64_t base, uint32_t m, uint32_t nkb,
+                                                     uint32_t m_pad,
                                                      uint32_t* __restrict__ tiled,
                                                      uint32_t* __restrict__ part) {
     __shared__ uint32_t lds[kGroup * kTsStride];
@@ -265,7 +273,7 @@ __global__ __launch_bounds__(256) void k_tile_stats(const uint8_t* __restrict__ 
 #pragma unroll
     for (int r = 0; r < 8; ++r) {
         const uint32_t snp = g * kGroup + 8 * wave + r;
-        const uint64_t src = snp < m ? (bed_idx ? bed_idx[snp] : snp) : 0;
+        const uint64_t src = snp < m ? (bed_idx ? bed_idx[snp] : base + snp) : base;
         rowp[r] = reinterpret_cast<const uint32_t*>(slabs + src * pitch);
         nv[r] = snp < m ? n : 0;
     }
@@ -1015,9 +1023,9 @@ static double maf_from_stats(uint32_t S, uint32_t C) {
     return freq < 1.0 - freq ? freq : 1.0 - freq;
 }
 
-int gather_and_tile(const ldsc_panel* panel, const uint64_t* d_bed_idx, uint64_t m,
-                    const int32_t* d_map, uint64_t m_pad, bool vcode, Workspace* ws, char* err,
-                    uint64_t err_len) {
+int gather_and_tile(const ldsc_panel* panel, const uint64_t* d_bed_idx, uint64_t base,
+                    uint64_t m, const int32_t* d_map, uint64_t m_pad, bool vcode, Workspace* ws,
+                    char* err, uint64_t err_len) {
     const uint64_t n = panel->n;
     const uint64_t n_pad = round_up(n, kKBlock);
     const uint32_t nkb = (uint32_t)(n_pad / kKBlock);
@@ -1050,16 +1058,18 @@ int gather_and_tile(const ldsc_panel* panel, const uint64_t* d_bed_idx, uint64_t
         uint8_t* a8 = ws->scratch.as<uint8_t>();
         uint64_t a1 = panel->stride;
         uint32_t a2 = (uint32_t)((panel->bps + 3) / 4), a5 = (uint32_t)m;
+        uint64_t a_base = base;
         void* kargs[] = {(void*)&a0, (void*)&a1, (void*)&a2, (void*)&ldsrow, (void*)&d_bed_idx,
-                         (void*)&a5, (void*)&d_map, (void*)&n16, (void*)&a8};
+                         (void*)&a_base, (void*)&a5, (void*)&d_map, (void*)&n16, (void*)&a8};
         HIPC(hipLaunchKernel(kfn, dim3(grid), dim3(512), kargs, lds, st));
         src = ws->scratch.as<uint8_t>();
         pitch = bps_out;
         rows = nullptr;
+        base = 0;
     }
     hipLaunchKernelGGL(vcode ? k_tile_stats<true> : k_tile_stats<false>,
                        dim3((unsigned)(m_pad / kGroup), nparts), dim3(256), 0, st, src, pitch,
-                       (uint32_t)n, rows, (uint32_t)m, nkb, (uint32_t)m_pad,
+                       (uint32_t)n, rows, base, (uint32_t)m, nkb, (uint32_t)m_pad,
                        ws->tiled.as<uint32_t>(), ws->stats.as<uint32_t>() + (size_t)m_pad * 4);
     HIPC(hipGetLastError());
     return LDSC_OK;
@@ -1090,12 +1100,16 @@ int run_l2(const ldsc_panel* panel, const ldsc_snp* snps, uint64_t m, const ldsc
     }
     if (!snps || !l2_out) return fail(err, err_len, LDSC_ERR_INVALID_ARG, "NULL SNP or output array");
     if (m >= (1ull << 31)) return fail(err, err_len, LDSC_ERR_INVALID_ARG, "too many SNPs");
-    for (uint64_t j = 0; j < m; ++j)
+    bool contiguous = true;  // rows base .. base+m-1 of the panel need no index upload
+    const uint64_t base = snps[0].bed_idx;
+    for (uint64_t j = 0; j < m; ++j) {
         if (snps[j].bed_idx >= panel->m_bim)
             return fail(err, err_len, LDSC_ERR_INVALID_ARG,
                         "SNP %llu has bed_idx %llu beyond the panel's %llu SNPs",
                         (unsigned long long)j, (unsigned long long)snps[j].bed_idx,
                         (unsigned long long)panel->m_bim);
+        contiguous = contiguous && snps[j].bed_idx == base + j;
+    }
     if (d && d > 1000000)
         return fail(err, err_len, LDSC_ERR_SKETCH_RANGE, "--sketch %llu too large",
                     (unsigned long long)d);
@@ -1112,10 +1126,13 @@ int run_l2(const ldsc_panel* panel, const ldsc_snp* snps, uint64_t m, const ldsc
 
     // 1. gather + tile + stats (+ CountSketch), queued before the host work
     //    below so the device tiles while the host plans the band.
-    HIPC(ws->pinned.ensure((size_t)m * 8));
-    uint64_t* h_bidx = ws->pinned.as<uint64_t>();
-    for (uint64_t j = 0; j < m; ++j) h_bidx[j] = snps[j].bed_idx;
-    HIPC(ws->bed_idx.ensure((size_t)m * 8));
+    uint64_t* h_bidx = nullptr;
+    if (!contiguous) {
+        HIPC(ws->pinned.ensure((size_t)m * 8));
+        h_bidx = ws->pinned.as<uint64_t>();
+        for (uint64_t j = 0; j < m; ++j) h_bidx[j] = snps[j].bed_idx;
+        HIPC(ws->bed_idx.ensure((size_t)m * 8));
+    }
     HIPC(ws->info.ensure((size_t)m_pad * 48));
     if (d) {
         HIPC(ws->sk_cols.ensure((size_t)m_pad * sp->d_pad * 4));
@@ -1125,13 +1142,15 @@ int run_l2(const ldsc_panel* panel, const ldsc_snp* snps, uint64_t m, const ldsc
     double* d_rs = reinterpret_cast<double*>(ws->info.as<char>() + (size_t)m_pad * 8);
     float4* d_lut = reinterpret_cast<float4*>(ws->info.as<char>() + (size_t)m_pad * 16);
     float2* d_musc = reinterpret_cast<float2*>(ws->info.as<char>() + (size_t)m_pad * 32);
-    HIPC(hipMemcpyAsync(ws->bed_idx.p, h_bidx, m * 8, hipMemcpyHostToDevice, st));
+    if (!contiguous)
+        HIPC(hipMemcpyAsync(ws->bed_idx.p, h_bidx, m * 8, hipMemcpyHostToDevice, st));
     HIPC(hipEventRecord(ws->ev[0], st));
     const bool use_f32 = cfg->precision == LDSC_PREC_FP32;
     // exact form: e2m1 MFMA while f32 holds the products exactly, int8 beyond
     const bool use_fp4 = !d && cfg->precision == LDSC_PREC_EXACT && n <= kFp4MaxIndiv;
-    rc = gather_and_tile(panel, ws->bed_idx.as<uint64_t>(), m, d ? sp->d_fam : panel->d_keep32,
-                         m_pad, use_fp4, ws, err, err_len);
+    rc = gather_and_tile(panel, contiguous ? nullptr : ws->bed_idx.as<uint64_t>(),
+                         contiguous ? base : 0, m, d ? sp->d_fam : panel->d_keep32, m_pad, use_fp4,
+                         ws, err, err_len);
     if (rc) return rc;
     // MAF on the device for the exact and FP32 forms; the sketch form's
     // rounding depends on chunk contiguity and is finished on the host
@@ -1142,6 +1161,19 @@ int run_l2(const ldsc_panel* panel, const ldsc_snp* snps, uint64_t m, const ldsc
                        ws->stats.as<uint32_t>(), m_pad, (uint32_t)n, d_cs, d_rs, d_lut, d_musc,
                        dev_maf ? ws->mafd.as<double>() : nullptr, use_f32 ? 1 : 0);
     HIPC(hipGetLastError());
+    // the MAF goes back to the host on a second stream while the band runs
+    const bool has_pq = !std::isnan(cfg->pq_exp);
+    const bool maf_aware = d && cfg->sketch_maf_aware;
+    const bool early_stats = has_pq || maf_aware;
+    const bool early_maf = dev_maf && maf_out && !early_stats;
+    if (early_maf) {
+        HIPC(ws->pinned_maf.ensure((size_t)m * 8));
+        HIPC(hipEventRecord(ws->ev[6], st));
+        HIPC(hipStreamWaitEvent(ws->copy_stream, ws->ev[6], 0));
+        HIPC(hipMemcpyAsync(ws->pinned_maf.p, ws->mafd.p, m * 8, hipMemcpyDeviceToHost,
+                            ws->copy_stream));
+        HIPC(hipEventRecord(ws->ev[7], ws->copy_stream));
+    }
     HIPC(hipEventRecord(ws->ev[1], st));
     const uint32_t nkb = (uint32_t)(round_up(n, kKBlock) / kKBlock);
     if (d) {
@@ -1213,8 +1245,6 @@ int run_l2(const ldsc_panel* panel, const ldsc_snp* snps, uint64_t m, const ldsc
     // 3. MAF (and pq-scaled annotations, kappa) on the host from the exact
     //    stats; fetched now only when the device needs something derived
     //    from them, else with the result at the end.
-    const bool has_pq = !std::isnan(cfg->pq_exp);
-    const bool maf_aware = d && cfg->sketch_maf_aware;
     uint32_t* h_stats = reinterpret_cast<uint32_t*>(hp + off_st);
     double* h_maf = reinterpret_cast<double*>(hp + off_maf);
     std::vector<double>& maf = ws->host.maf;
@@ -1246,7 +1276,6 @@ int run_l2(const ldsc_panel* panel, const ldsc_snp* snps, uint64_t m, const ldsc
             maf[j] = round32[j] ? (double)(float)v : v;
         }
     };
-    const bool early_stats = has_pq || maf_aware;
     if (early_stats) {
         if (dev_maf) {
             HIPC(hipMemcpyAsync(h_maf, ws->mafd.p, m * 8, hipMemcpyDeviceToHost, st));
@@ -1346,26 +1375,20 @@ int run_l2(const ldsc_panel* panel, const ldsc_snp* snps, uint64_t m, const ldsc
                        (uint32_t)K, ws->l2.as<double>());
     HIPC(hipGetLastError());
     HIPC(hipEventRecord(ws->ev[5], st));
-    if (maf_out && !early_stats) {
-        if (dev_maf)
-            HIPC(hipMemcpyAsync(h_maf, ws->mafd.p, m * 8, hipMemcpyDeviceToHost, st));
-        else
-            HIPC(hipMemcpyAsync(h_stats, ws->stats.p, (size_t)m_pad * 16, hipMemcpyDeviceToHost, st));
+    if (early_maf) {  // copy it out while the band kernel runs
+        HIPC(hipEventSynchronize(ws->ev[7]));
+        std::memcpy(maf_out, ws->pinned_maf.p, m * 8);
+    } else if (maf_out && !early_stats) {
+        HIPC(hipMemcpyAsync(h_stats, ws->stats.p, (size_t)m_pad * 16, hipMemcpyDeviceToHost, st));
     }
     double* h_l2 = reinterpret_cast<double*>(hp + off_l2);
     HIPC(hipMemcpyAsync(stage_l2 ? h_l2 : l2_out, ws->l2.p, (size_t)m * K * 8,
                         hipMemcpyDeviceToHost, st));
     HIPC(hipStreamSynchronize(st));
     if (stage_l2) std::memcpy(l2_out, h_l2, (size_t)m * K * 8);
-    if (maf_out) {
-        if (early_stats)
-            std::memcpy(maf_out, maf.data(), m * 8);
-        else if (dev_maf)
-            std::memcpy(maf_out, h_maf, m * 8);
-        else {
-            compute_maf();
-            std::memcpy(maf_out, maf.data(), m * 8);
-        }
+    if (maf_out && !early_maf) {
+        if (!early_stats) compute_maf();  // sketch form: from the integer stats
+        std::memcpy(maf_out, maf.data(), m * 8);
     }
 
     if (perf) {

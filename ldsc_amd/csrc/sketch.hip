@@ -344,8 +344,8 @@ extern "C" int ldsc_l2_sketch_buckets(const ldsc_panel* panel, const uint64_t* b
         SCHK(ws->info.ensure(m_pad * 48));
         SCHK(ws->sk_cols.ensure(m * d * 8));
         SCHK(hipMemcpyAsync(ws->bed_idx.p, bed_idx, m * 8, hipMemcpyHostToDevice, st));
-        rc = gather_and_tile(panel, ws->bed_idx.as<uint64_t>(), m, sp->d_fam, m_pad, false, ws, err,
-                             err_len);
+        rc = gather_and_tile(panel, ws->bed_idx.as<uint64_t>(), 0, m, sp->d_fam, m_pad, false, ws,
+                             err, err_len);
         if (rc) break;
         int32_t* d_sg = ws->sk_cols.as<int32_t>();
         int32_t* d_sm = d_sg + m * d;
