@@ -182,8 +182,9 @@ inline void plan_band(const std::vector<uint64_t>& lo, uint64_t m, BandPlan& p) 
 int sketch_plan(const ldsc_panel* panel, uint64_t d, SketchPlan** out, char* err,
                 uint64_t err_len);
 hipError_t launch_sketch(const SketchPlan* sp, const uint32_t* tiled, uint32_t nkb,
-                         const float2* musc, uint32_t m, uint32_t m_pad, double n, float* sk,
-                         int32_t* sg, int32_t* sm, hipStream_t st);
+                         const float2* musc, uint32_t m, uint32_t m_pad, double n, double* norm,
+                         float* sk, int32_t* sg, int32_t* sm, hipStream_t st);
+size_t sketch_norm_bytes(const SketchPlan* sp, uint64_t m_pad);
 hipError_t launch_band_sk(const BandArgs& a, hipStream_t st);
 
 }  // namespace ldsc

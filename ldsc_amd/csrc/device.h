@@ -76,7 +76,7 @@ struct Workspace {
     hipStream_t copy_stream = nullptr;  // read-backs that overlap the band kernel
     hipEvent_t ev[8] = {};
     DevBuf tiled, stats, bed_idx, lo, info, aeff, annot, maf, mafd, tiles, j0, tstart, rowp,
-        colp, l2, sk_bucket, sk_sign, sk_cols, sk_perm, sk_off, scratch;
+        colp, l2, sk_cols, sk_norm, scratch;
     PinnedBuf pinned;       // SNP row indices (upload in flight while the host plans)
     PinnedBuf pinned_meta;  // pair bounds, band plan, stats read-back
     PinnedBuf pinned_maf;   // MAF read-back (during the band kernel)
@@ -121,12 +121,15 @@ namespace ldsc {
 // into ws->stats.  `d_map` (nullable) gives, for each of the panel's N
 // individuals in the order wanted, its FAM row (--keep order, or the
 // CountSketch bucket order) in the lane-major layout of lane_major_map; it
-// is applied by an LDS-staged row gather.
+// is applied by an LDS-staged row gather.  *parts receives the number of
+// integer-stat slots written after ws->stats' first m_pad x 4 words.  With
+// `tile` false and a map, only the gathered rows are produced (ws->scratch,
+// ceil(N/16) dwords per SNP, SNP-major) with their stats in slot 0.
 // `vcode` selects the dosage-value tile form of the e2m1 band kernel
 // (engine.hip) instead of the PLINK-code form above.
 int gather_and_tile(const ldsc_panel* panel, const uint64_t* d_bed_idx, uint64_t base,
-                    uint64_t m, const int32_t* d_map, uint64_t m_pad, bool vcode, Workspace* ws,
-                    char* err, uint64_t err_len);
+                    uint64_t m, const int32_t* d_map, uint64_t m_pad, bool vcode, bool tile,
+                    Workspace* ws, uint32_t* parts, char* err, uint64_t err_len);
 
 // The exact / sketch engine on workspace `ws` (which must belong to
 // panel->device).  Synchronous: outputs are written when it returns.

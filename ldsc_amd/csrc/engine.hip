@@ -80,8 +80,7 @@ void PinnedBuf::release() {
 
 size_t Workspace::bytes() const {
     const DevBuf* all[] = {&tiled, &stats, &bed_idx, &lo, &info, &aeff, &annot, &maf, &mafd, &tiles,
-                           &j0, &tstart, &rowp, &colp, &l2, &sk_bucket, &sk_sign, &sk_cols,
-                           &sk_perm, &sk_off, &scratch};
+                           &j0, &tstart, &rowp, &colp, &l2, &sk_cols, &sk_norm, &scratch};
     size_t s = 0;
     for (auto* b : all) s += b->cap;
     return s;
@@ -104,8 +103,7 @@ hipError_t Workspace::init(int dev) {
 
 void Workspace::destroy() {
     DevBuf* all[] = {&tiled, &stats, &bed_idx, &lo, &info, &aeff, &annot, &maf, &mafd, &tiles,
-                     &j0, &tstart, &rowp, &colp, &l2, &sk_bucket, &sk_sign, &sk_cols,
-                     &sk_perm, &sk_off, &scratch};
+                     &j0, &tstart, &rowp, &colp, &l2, &sk_cols, &sk_norm, &scratch};
     for (auto* b : all) b->release();
     pinned.release();
     pinned_meta.release();
@@ -336,6 +334,122 @@ __global__ __launch_bounds__(256) void k_tile_stats(const uint8_t* __restrict__ 
     if ((tid & 7) == 0) {
         uint4* dst = reinterpret_cast<uint4*>(part) + (size_t)chunk * m_pad + g * kGroup + s;
         *dst = make_uint4(c[0], c[1], c[2], c[3]);
+    }
+}
+
+// ------------------------------------------------------------------------
+// Kernel 1c: four-row gather through byte-interleaved LDS
+// ------------------------------------------------------------------------
+// For --keep subsets and the CountSketch bucket order: four SNP rows per
+// pass are staged in LDS byte-interleaved (word k holds byte k of the four
+// rows), so one 4-byte LDS read fetches a member's code in all four rows.
+// Thread q builds output dword q (members 16q .. 16q+15, map_t lane-major,
+// -1 past the last member -> code 01) of the four rows, with the next
+// dword's 16 map entries already loading, and stores the four dwords into
+// the gathered rows (`n16` dwords each, coalesced across the wave).  The
+// rows' integer stats (sum, count, sum_sq, het) go to part[j] as uint4.
+__device__ __forceinline__ void transpose_bytes4(uint32_t d0, uint32_t d1, uint32_t d2,
+                                                 uint32_t d3, uint32_t (&o)[4]) {
+    // o[k] byte r = byte k of d_r
+    const uint32_t lo01 = __builtin_amdgcn_perm(d1, d0, 0x05010400u);  // d0.b0 d1.b0 d0.b1 d1.b1
+    const uint32_t hi01 = __builtin_amdgcn_perm(d1, d0, 0x07030602u);  // d0.b2 d1.b2 d0.b3 d1.b3
+    const uint32_t lo23 = __builtin_amdgcn_perm(d3, d2, 0x05010400u);
+    const uint32_t hi23 = __builtin_amdgcn_perm(d3, d2, 0x07030602u);
+    o[0] = __builtin_amdgcn_perm(lo23, lo01, 0x05040100u);
+    o[1] = __builtin_amdgcn_perm(lo23, lo01, 0x07060302u);
+    o[2] = __builtin_amdgcn_perm(hi23, hi01, 0x05040100u);
+    o[3] = __builtin_amdgcn_perm(hi23, hi01, 0x07060302u);
+}
+
+__global__ __launch_bounds__(512) void k_gather4(const uint8_t* __restrict__ slabs, uint64_t pitch,
+                                                  uint32_t src_dw,
+                                                  const uint64_t* __restrict__ bed_idx,
+                                                  uint64_t base, uint32_t m, uint32_t m_pad,
+                                                  const int32_t* __restrict__ map_t, uint32_t n16,
+                                                  uint32_t* __restrict__ out,
+                                                  uint32_t* __restrict__ part) {
+    extern __shared__ uint32_t lds[];  // 4 * src_dw words
+    __shared__ uint32_t red[8][12];
+    const int tid = threadIdx.x, wv = tid >> 6, ln = tid & 63;
+    for (uint32_t rg = blockIdx.x; 4 * rg < m_pad; rg += gridDim.x) {
+        const uint32_t j0 = 4 * rg;
+        const uint32_t* rp[4];
+        bool valid[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            valid[r] = j0 + r < m;
+            const uint64_t row = valid[r] ? (bed_idx ? bed_idx[j0 + r] : base + j0 + r) : 0;
+            rp[r] = reinterpret_cast<const uint32_t*>(slabs + row * pitch);
+        }
+        __syncthreads();  // the previous pass is done with lds
+        for (uint32_t q = tid; q < src_dw; q += 512) {
+            uint32_t o[4];
+            transpose_bytes4(valid[0] ? rp[0][q] : 0u, valid[1] ? rp[1][q] : 0u,
+                             valid[2] ? rp[2][q] : 0u, valid[3] ? rp[3][q] : 0u, o);
+            reinterpret_cast<uint4*>(lds)[q] = make_uint4(o[0], o[1], o[2], o[3]);
+        }
+        __syncthreads();
+        uint32_t cnt[4][3];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) cnt[r][0] = cnt[r][1] = cnt[r][2] = 0;
+        int32_t sr[16];
+        uint32_t q = tid;
+        if (q < n16)
+#pragma unroll
+            for (int e = 0; e < 16; ++e) sr[e] = map_t[(size_t)e * n16 + q];
+        for (; q < n16; q += 512) {
+            int32_t cur[16];
+#pragma unroll
+            for (int e = 0; e < 16; ++e) cur[e] = sr[e];
+            if (q + 512 < n16)
+#pragma unroll
+                for (int e = 0; e < 16; ++e) sr[e] = map_t[(size_t)e * n16 + q + 512];
+            uint32_t acc[4] = {0u, 0u, 0u, 0u};
+#pragma unroll
+            for (int e = 0; e < 16; ++e) {
+                const int32_t s = cur[e];
+                uint32_t c4 = 0x01010101u;  // missing in all four rows
+                if (s >= 0) c4 = (lds[s >> 2] >> (2 * (s & 3))) & 0x03030303u;
+                acc[e >> 2] |= c4 << (2 * (e & 3));
+            }
+            uint32_t o[4];
+            transpose_bytes4(acc[0], acc[1], acc[2], acc[3], o);  // o[r] = row r's dword
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const uint32_t L = o[r] & 0x55555555u, H = (o[r] >> 1) & 0x55555555u;
+                cnt[r][0] += __popc(~(L | H) & 0x55555555u);  // 00: dosage 2
+                cnt[r][1] += __popc(~L & H);                   // 10: dosage 1
+                cnt[r][2] += __popc(L & H);                    // 11: dosage 0
+                if (valid[r]) out[(size_t)(j0 + r) * n16 + q] = o[r];
+            }
+        }
+        // block sums of the 4 x 3 counts: waves first, then across waves
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                uint32_t v = cnt[r][c];
+#pragma unroll
+                for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o);
+                cnt[r][c] = v;
+            }
+        if (ln == 0)
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+#pragma unroll
+                for (int c = 0; c < 3; ++c) red[wv][3 * r + c] = cnt[r][c];
+        __syncthreads();
+        if (tid < 4) {
+            uint32_t n2 = 0, n1 = 0, n0 = 0;
+            for (int w = 0; w < 8; ++w) {
+                n2 += red[w][3 * tid + 0];
+                n1 += red[w][3 * tid + 1];
+                n0 += red[w][3 * tid + 2];
+            }
+            const bool v = j0 + tid < m;  // padding rows: nothing
+            reinterpret_cast<uint4*>(part)[j0 + tid] =
+                v ? make_uint4(2 * n2 + n1, n2 + n1 + n0, 4 * n2 + n1, n1) : make_uint4(0, 0, 0, 0);
+        }
     }
 }
 
@@ -1011,7 +1125,11 @@ size_t l2_workspace_bytes(const ldsc_panel* panel, uint64_t m, const ldsc_l2_con
     bytes += tiles * (kTile * K * 8 * 2 + 8);                   // tile partials + list
     const bool gather = cfg && (cfg->sketch_d || panel->d_keep32);
     if (gather) bytes += m * round_up((panel->n + 3) / 4, 4);   // permuted rows
-    if (cfg && cfg->sketch_d) bytes += m_pad * round_up(cfg->sketch_d, 32) * 4 + m_pad * 8;
+    if (cfg && cfg->sketch_d) {
+        const uint64_t nblocks = round_up(cfg->sketch_d, 32) / 32;
+        bytes += m_pad * nblocks * 32 * 4 + m_pad * 8;  // sketch tiles, kappa
+        bytes += m_pad * 8 * ((nblocks + 3) / 4);        // norm shares
+    }
     return bytes;
 }
 
@@ -1024,19 +1142,42 @@ static double maf_from_stats(uint32_t S, uint32_t C) {
 }
 
 int gather_and_tile(const ldsc_panel* panel, const uint64_t* d_bed_idx, uint64_t base,
-                    uint64_t m, const int32_t* d_map, uint64_t m_pad, bool vcode, Workspace* ws,
-                    char* err, uint64_t err_len) {
+                    uint64_t m, const int32_t* d_map, uint64_t m_pad, bool vcode, bool tile,
+                    Workspace* ws, uint32_t* parts, char* err, uint64_t err_len) {
     const uint64_t n = panel->n;
     const uint64_t n_pad = round_up(n, kKBlock);
     const uint32_t nkb = (uint32_t)(n_pad / kKBlock);
     hipStream_t st = ws->stream;
     const uint32_t nparts = tile_parts(n);
-    HIPC(ws->tiled.ensure((size_t)m_pad * n_pad / 4));
     HIPC(ws->stats.ensure((size_t)m_pad * 16 * (1 + nparts)));
+    *parts = nparts;
     const uint8_t* src = panel->d_slabs;
     uint64_t pitch = panel->stride;
     const uint64_t* rows = d_bed_idx;
-    if (d_map) {
+    const uint32_t src_dw = (uint32_t)((panel->bps + 3) / 4);
+    if (d_map && (size_t)src_dw * 16 <= 160 * 1024) {
+        // gather four rows at a time through byte-interleaved LDS
+        const uint32_t n16 = (uint32_t)((n + 15) / 16);
+        HIPC(ws->scratch.ensure((size_t)m * n16 * 4));
+        const size_t lds = (size_t)src_dw * 16;
+        if (lds > 64 * 1024)
+            HIPC(hipFuncSetAttribute((const void*)k_gather4,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        const uint32_t grid = (uint32_t)std::min<uint64_t>((m_pad + 3) / 4, 2048);
+        hipLaunchKernelGGL(k_gather4, dim3(grid), dim3(512), lds, st, panel->d_slabs,
+                           panel->stride, src_dw, d_bed_idx, base, (uint32_t)m, (uint32_t)m_pad,
+                           d_map, n16,
+                           ws->scratch.as<uint32_t>(), ws->stats.as<uint32_t>() + (size_t)m_pad * 4);
+        HIPC(hipGetLastError());
+        if (!tile) {  // the caller reads the gathered rows; stats are in slot 0
+            *parts = 1;
+            return LDSC_OK;
+        }
+        src = ws->scratch.as<uint8_t>();
+        pitch = 4ull * n16;
+        rows = nullptr;
+        base = 0;
+    } else if (d_map) {
         // gather rows in the requested individual order, then tile them
         const uint32_t n16 = (uint32_t)((n + 15) / 16);
         const uint64_t bps_out = 4ull * n16;
@@ -1067,6 +1208,7 @@ int gather_and_tile(const ldsc_panel* panel, const uint64_t* d_bed_idx, uint64_t
         rows = nullptr;
         base = 0;
     }
+    HIPC(ws->tiled.ensure((size_t)m_pad * n_pad / 4));
     hipLaunchKernelGGL(vcode ? k_tile_stats<true> : k_tile_stats<false>,
                        dim3((unsigned)(m_pad / kGroup), nparts), dim3(256), 0, st, src, pitch,
                        (uint32_t)n, rows, base, (uint32_t)m, nkb, (uint32_t)m_pad,
@@ -1148,16 +1290,17 @@ int run_l2(const ldsc_panel* panel, const ldsc_snp* snps, uint64_t m, const ldsc
     const bool use_f32 = cfg->precision == LDSC_PREC_FP32;
     // exact form: e2m1 MFMA while f32 holds the products exactly, int8 beyond
     const bool use_fp4 = !d && cfg->precision == LDSC_PREC_EXACT && n <= kFp4MaxIndiv;
+    uint32_t parts = 1;
     rc = gather_and_tile(panel, contiguous ? nullptr : ws->bed_idx.as<uint64_t>(),
                          contiguous ? base : 0, m, d ? sp->d_fam : panel->d_keep32, m_pad, use_fp4,
-                         ws, err, err_len);
+                         /*tile=*/true, ws, &parts, err, err_len);
     if (rc) return rc;
     // MAF on the device for the exact and FP32 forms; the sketch form's
     // rounding depends on chunk contiguity and is finished on the host
     const bool dev_maf = !d;
     if (dev_maf) HIPC(ws->mafd.ensure((size_t)m_pad * 8));
     hipLaunchKernelGGL(k_snp_info, dim3((m_pad + 255) / 256), dim3(256), 0, st,
-                       ws->stats.as<uint32_t>() + (size_t)m_pad * 4, tile_parts(n),
+                       ws->stats.as<uint32_t>() + (size_t)m_pad * 4, parts,
                        ws->stats.as<uint32_t>(), m_pad, (uint32_t)n, d_cs, d_rs, d_lut, d_musc,
                        dev_maf ? ws->mafd.as<double>() : nullptr, use_f32 ? 1 : 0);
     HIPC(hipGetLastError());
@@ -1177,8 +1320,10 @@ int run_l2(const ldsc_panel* panel, const ldsc_snp* snps, uint64_t m, const ldsc
     HIPC(hipEventRecord(ws->ev[1], st));
     const uint32_t nkb = (uint32_t)(round_up(n, kKBlock) / kKBlock);
     if (d) {
-        HIPC(launch_sketch(sp, ws->tiled.as<uint32_t>(), nkb, d_musc, (uint32_t)m, m_pad, (double)n,
-                           ws->sk_cols.as<float>(), nullptr, nullptr, st));
+        HIPC(ws->sk_norm.ensure(sketch_norm_bytes(sp, m_pad)));
+        HIPC(launch_sketch(sp, ws->tiled.as<uint32_t>(), nkb, d_musc, (uint32_t)m, m_pad,
+                           (double)n, ws->sk_norm.as<double>(), ws->sk_cols.as<float>(), nullptr,
+                           nullptr, st));
     }
     HIPC(hipEventRecord(ws->ev[2], st));
 

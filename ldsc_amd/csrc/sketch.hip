@@ -114,19 +114,36 @@ int sketch_plan(const ldsc_panel* panel_c, uint64_t d, SketchPlan** out, char* e
     return LDSC_OK;
 }
 
-// One wave per pair of SNP groups (64 SNPs).  RAW writes the integer bucket
-// sums (d x m column-major); otherwise normalized f32 sketch tiles:
-// sk[((g * nblocks + B) * 64 + lane) * 16 + r] holds bucket
-// 32B + 8(r/4) + 4(lane/32) + r%4 of SNP 32g + lane%32.
+// Bucket blocks per wave of k_sketch: the waves of a SNP pair-group split the
+// d/32 bucket blocks so that the whole chip works at small d too.
+constexpr uint32_t kSkChunk = 4;
+
+static inline uint32_t sketch_chunks(const SketchPlan* sp) {
+    return (uint32_t)((sp->nblocks + kSkChunk - 1) / kSkChunk);
+}
+
+size_t sketch_norm_bytes(const SketchPlan* sp, uint64_t m_pad) {
+    return (size_t)sketch_chunks(sp) * m_pad * 8;
+}
+
+// Wave (gp, c) handles the SNP groups 2gp, 2gp+1 (64 SNPs) and bucket
+// blocks [c * kSkChunk, (c+1) * kSkChunk).  Within a block it walks the
+// block's K-steps with the next K-step's loads in flight.  RAW writes the
+// integer bucket sums (d x m column-major); otherwise unscaled f32 bucket
+// values into the sketch tiles sk[((g * nblocks + B) * 64 + lane) * 16 + r]
+// (bucket 32B + 8(r/4) + 4(lane/32) + r%4 of SNP 32g + lane%32) and the
+// wave's share of each SNP's squared norm into norm[c * m_pad + snp];
+// k_sketch_scale then renormalizes.
 template <bool RAW>
 __global__ __launch_bounds__(64) void k_sketch(const uint32_t* __restrict__ tiled, uint32_t nkb,
                                                 const v4i* __restrict__ ptiles,
                                                 const int4* __restrict__ blocks, uint32_t nblocks,
                                                 const float2* __restrict__ musc, uint32_t m,
-                                                uint32_t d, double n_norm, float* __restrict__ sk,
+                                                uint32_t m_pad, uint32_t d,
+                                                double* __restrict__ norm, float* __restrict__ sk,
                                                 int32_t* __restrict__ sg_out,
                                                 int32_t* __restrict__ sm_out) {
-    const uint32_t gp = blockIdx.x;
+    const uint32_t gp = blockIdx.x, c = blockIdx.y;
     const int lane = threadIdx.x;
     const int h = lane >> 5;
     const uint32_t* w0p = tiled + ((size_t)(2 * gp) * nkb * 64 + lane) * 4;
@@ -134,23 +151,38 @@ __global__ __launch_bounds__(64) void k_sketch(const uint32_t* __restrict__ tile
     const uint32_t snp0 = 64 * gp + (lane & 31), snp1 = snp0 + 32;
     const float2 ms0 = musc[snp0], ms1 = musc[snp1];
     double nrm0 = 0.0, nrm1 = 0.0;
-    for (uint32_t B = 0; B < nblocks; ++B) {
+    const uint32_t B1 = min(nblocks, (c + 1) * kSkChunk);
+    for (uint32_t B = c * kSkChunk; B < B1; ++B) {
         const int4 blk = blocks[B];
         v16i sg0, sm0, sg1, sm1;
 #pragma unroll
         for (int r = 0; r < 16; ++r) sg0[r] = sm0[r] = sg1[r] = sm1[r] = 0;
-        for (int s = 0; s < blk.z; ++s) {
-            const uint32_t ks = (uint32_t)blk.y + s;
+        uint32_t w0 = 0, w1 = 0;
+        v4i P = {0, 0, 0, 0};
+        if (blk.z > 0) {
+            const uint32_t ks = (uint32_t)blk.y;
             const size_t off = (size_t)(ks >> 2) * 256 + (ks & 3);
-            const uint32_t w0 = w0p[off], w1 = w1p[off];
-            const v4i P = ptiles[(size_t)(blk.x + s) * 64 + lane];
+            w0 = w0p[off];
+            w1 = w1p[off];
+            P = ptiles[(size_t)blk.x * 64 + lane];
+        }
+        for (int s = 0; s < blk.z; ++s) {
+            const uint32_t cw0 = w0, cw1 = w1;
+            const v4i cP = P;
+            if (s + 1 < blk.z) {  // next K-step's operands in flight
+                const uint32_t ks = (uint32_t)blk.y + s + 1;
+                const size_t off = (size_t)(ks >> 2) * 256 + (ks & 3);
+                w0 = w0p[off];
+                w1 = w1p[off];
+                P = ptiles[(size_t)(blk.x + s + 1) * 64 + lane];
+            }
             v4i G0, M0, G1, M1;
-            expand_codes(w0, G0, M0);
-            expand_codes(w1, G1, M1);
-            sg0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(P, G0, sg0, 0, 0, 0);
-            sm0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(P, M0, sm0, 0, 0, 0);
-            sg1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(P, G1, sg1, 0, 0, 0);
-            sm1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(P, M1, sm1, 0, 0, 0);
+            expand_codes(cw0, G0, M0);
+            expand_codes(cw1, G1, M1);
+            sg0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(cP, G0, sg0, 0, 0, 0);
+            sm0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(cP, M0, sm0, 0, 0, 0);
+            sg1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(cP, G1, sg1, 0, 0, 0);
+            sm1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(cP, M1, sm1, 0, 0, 0);
         }
         if (RAW) {
 #pragma unroll
@@ -182,38 +214,58 @@ __global__ __launch_bounds__(64) void k_sketch(const uint32_t* __restrict__ tile
         }
     }
     if (!RAW) {
-        // ratio-estimator renorm: ||col||^2 = N (wasm_simd.rs:1774-1783)
         nrm0 += __shfl_xor(nrm0, 32);
         nrm1 += __shfl_xor(nrm1, 32);
-        const float sc0 = nrm0 > 0.0 ? (float)sqrt(n_norm / nrm0) : 1.0f;
-        const float sc1 = nrm1 > 0.0 ? (float)sqrt(n_norm / nrm1) : 1.0f;
-        for (uint32_t B = 0; B < nblocks; ++B) {
-            float4* t0 = reinterpret_cast<float4*>(sk + (((size_t)(2 * gp) * nblocks + B) * 64 + lane) * 16);
-            float4* t1 = reinterpret_cast<float4*>(sk + (((size_t)(2 * gp + 1) * nblocks + B) * 64 + lane) * 16);
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                float4 a = t0[q], b = t1[q];
-                a.x *= sc0; a.y *= sc0; a.z *= sc0; a.w *= sc0;
-                b.x *= sc1; b.y *= sc1; b.z *= sc1; b.w *= sc1;
-                t0[q] = a;
-                t1[q] = b;
-            }
+        if (h == 0) {
+            norm[(size_t)c * m_pad + snp0] = nrm0;
+            norm[(size_t)c * m_pad + snp1] = nrm1;
         }
     }
 }
 
+// Ratio-estimator renorm ||col||^2 = N (wasm_simd.rs:1774-1783): the squared
+// norm is the chunk shares added in a fixed order; thread (tile (g, B),
+// lane) scales its 16 values.
+__global__ __launch_bounds__(64) void k_sketch_scale(float* __restrict__ sk,
+                                                      const double* __restrict__ norm,
+                                                      uint32_t nchunks, uint32_t nblocks,
+                                                      uint32_t m_pad, double n_norm) {
+    const uint32_t g = blockIdx.x / nblocks, B = blockIdx.x % nblocks;
+    const int lane = threadIdx.x;
+    const uint32_t snp = 32 * g + (lane & 31);
+    double nrm = 0.0;
+    for (uint32_t c = 0; c < nchunks; ++c) nrm += norm[(size_t)c * m_pad + snp];
+    const float sc = nrm > 0.0 ? (float)sqrt(n_norm / nrm) : 1.0f;
+    float4* t = reinterpret_cast<float4*>(sk + (((size_t)g * nblocks + B) * 64 + lane) * 16);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        float4 v = t[q];
+        v.x *= sc;
+        v.y *= sc;
+        v.z *= sc;
+        v.w *= sc;
+        t[q] = v;
+    }
+}
+
 hipError_t launch_sketch(const SketchPlan* sp, const uint32_t* tiled, uint32_t nkb,
-                         const float2* musc, uint32_t m, uint32_t m_pad, double n, float* sk,
-                         int32_t* sg, int32_t* sm, hipStream_t st) {
-    const dim3 grid(m_pad / 64), block(64);
-    if (sg)
+                         const float2* musc, uint32_t m, uint32_t m_pad, double n, double* norm,
+                         float* sk, int32_t* sg, int32_t* sm, hipStream_t st) {
+    const uint32_t nch = sketch_chunks(sp), nb = (uint32_t)sp->nblocks;
+    const dim3 grid(m_pad / 64, nch), block(64);
+    if (sg) {
         hipLaunchKernelGGL(k_sketch<true>, grid, block, 0, st, tiled, nkb,
-                           (const v4i*)sp->d_ptiles, sp->d_blocks, (uint32_t)sp->nblocks, musc, m,
-                           (uint32_t)sp->d, n, nullptr, sg, sm);
-    else
-        hipLaunchKernelGGL(k_sketch<false>, grid, block, 0, st, tiled, nkb,
-                           (const v4i*)sp->d_ptiles, sp->d_blocks, (uint32_t)sp->nblocks, musc, m,
-                           (uint32_t)sp->d, n, sk, nullptr, nullptr);
+                           (const v4i*)sp->d_ptiles, sp->d_blocks, nb, musc, m, m_pad,
+                           (uint32_t)sp->d, nullptr, nullptr, sg, sm);
+        return hipGetLastError();
+    }
+    hipLaunchKernelGGL(k_sketch<false>, grid, block, 0, st, tiled, nkb, (const v4i*)sp->d_ptiles,
+                       sp->d_blocks, nb, musc, m, m_pad, (uint32_t)sp->d, norm, sk, nullptr,
+                       nullptr);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_sketch_scale, dim3((m_pad / 32) * nb), dim3(64), 0, st, sk, norm, nch,
+                       nb, m_pad, n);
     return hipGetLastError();
 }
 
@@ -344,17 +396,18 @@ extern "C" int ldsc_l2_sketch_buckets(const ldsc_panel* panel, const uint64_t* b
         SCHK(ws->info.ensure(m_pad * 48));
         SCHK(ws->sk_cols.ensure(m * d * 8));
         SCHK(hipMemcpyAsync(ws->bed_idx.p, bed_idx, m * 8, hipMemcpyHostToDevice, st));
-        rc = gather_and_tile(panel, ws->bed_idx.as<uint64_t>(), 0, m, sp->d_fam, m_pad, false, ws,
-                             err, err_len);
+        uint32_t parts = 0;
+        rc = gather_and_tile(panel, ws->bed_idx.as<uint64_t>(), 0, m, sp->d_fam, m_pad, false,
+                             /*tile=*/true, ws, &parts, err, err_len);
         if (rc) break;
         int32_t* d_sg = ws->sk_cols.as<int32_t>();
         int32_t* d_sm = d_sg + m * d;
-        const uint32_t nkb = (uint32_t)(round_up(panel->n, kKBlock) / kKBlock);
         // musc is unused by the integer form; pass any valid per-SNP array
         SCHK(hipMemsetAsync(ws->info.p, 0, m_pad * 48, st));
+        const uint32_t nkb = (uint32_t)(round_up(panel->n, kKBlock) / kKBlock);
         SCHK(launch_sketch(sp, ws->tiled.as<uint32_t>(), nkb,
                            reinterpret_cast<const float2*>(ws->info.as<char>()), (uint32_t)m,
-                           (uint32_t)m_pad, (double)panel->n, nullptr, d_sg, d_sm, st));
+                           (uint32_t)m_pad, (double)panel->n, nullptr, nullptr, d_sg, d_sm, st));
         SCHK(hipMemcpyAsync(sg, d_sg, m * d * 4, hipMemcpyDeviceToHost, st));
         SCHK(hipMemcpyAsync(sm, d_sm, m * d * 4, hipMemcpyDeviceToHost, st));
         SCHK(hipStreamSynchronize(st));
