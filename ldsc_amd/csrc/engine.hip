@@ -361,6 +361,9 @@ __device__ __forceinline__ void transpose_bytes4(uint32_t d0, uint32_t d1, uint3
     o[3] = __builtin_amdgcn_perm(hi23, hi01, 0x07060302u);
 }
 
+// PF > 0: each thread keeps its PF source dwords per row of the next pass
+// in registers, loaded while the current pass gathers (src_dw <= 512 PF).
+template <int PF>
 __global__ __launch_bounds__(512) void k_gather4(const uint8_t* __restrict__ slabs, uint64_t pitch,
                                                   uint32_t src_dw,
                                                   const uint64_t* __restrict__ bed_idx,
@@ -371,24 +374,54 @@ __global__ __launch_bounds__(512) void k_gather4(const uint8_t* __restrict__ sla
     extern __shared__ uint32_t lds[];  // 4 * src_dw words
     __shared__ uint32_t red[8][12];
     const int tid = threadIdx.x, wv = tid >> 6, ln = tid & 63;
-    for (uint32_t rg = blockIdx.x; 4 * rg < m_pad; rg += gridDim.x) {
-        const uint32_t j0 = 4 * rg;
-        const uint32_t* rp[4];
-        bool valid[4];
+    auto row_ptr = [&](uint32_t j) {
+        const uint64_t row = j < m ? (bed_idx ? bed_idx[j] : base + j) : 0;
+        return reinterpret_cast<const uint32_t*>(slabs + row * pitch);
+    };
+    uint32_t pre[PF > 0 ? PF : 1][4];
+    auto fetch = [&](uint32_t j0) {
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-            valid[r] = j0 + r < m;
-            const uint64_t row = valid[r] ? (bed_idx ? bed_idx[j0 + r] : base + j0 + r) : 0;
-            rp[r] = reinterpret_cast<const uint32_t*>(slabs + row * pitch);
+            const uint32_t* rp = row_ptr(j0 + r);
+            const bool v = j0 + r < m;
+#pragma unroll
+            for (int i = 0; i < PF; ++i) {
+                const uint32_t q = tid + 512 * i;
+                pre[i][r] = (v && q < src_dw) ? rp[q] : 0u;
+            }
         }
+    };
+    if (PF > 0 && 4 * blockIdx.x < m_pad) fetch(4 * blockIdx.x);
+    for (uint32_t rg = blockIdx.x; 4 * rg < m_pad; rg += gridDim.x) {
+        const uint32_t j0 = 4 * rg;
+        bool valid[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) valid[r] = j0 + r < m;
         __syncthreads();  // the previous pass is done with lds
-        for (uint32_t q = tid; q < src_dw; q += 512) {
-            uint32_t o[4];
-            transpose_bytes4(valid[0] ? rp[0][q] : 0u, valid[1] ? rp[1][q] : 0u,
-                             valid[2] ? rp[2][q] : 0u, valid[3] ? rp[3][q] : 0u, o);
-            reinterpret_cast<uint4*>(lds)[q] = make_uint4(o[0], o[1], o[2], o[3]);
+        if (PF > 0) {
+#pragma unroll
+            for (int i = 0; i < PF; ++i) {
+                const uint32_t q = tid + 512 * i;
+                if (q < src_dw) {
+                    uint32_t o[4];
+                    transpose_bytes4(pre[i][0], pre[i][1], pre[i][2], pre[i][3], o);
+                    reinterpret_cast<uint4*>(lds)[q] = make_uint4(o[0], o[1], o[2], o[3]);
+                }
+            }
+        } else {
+            const uint32_t* rp[4];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) rp[r] = row_ptr(j0 + r);
+            for (uint32_t q = tid; q < src_dw; q += 512) {
+                uint32_t o[4];
+                transpose_bytes4(valid[0] ? rp[0][q] : 0u, valid[1] ? rp[1][q] : 0u,
+                                 valid[2] ? rp[2][q] : 0u, valid[3] ? rp[3][q] : 0u, o);
+                reinterpret_cast<uint4*>(lds)[q] = make_uint4(o[0], o[1], o[2], o[3]);
+            }
         }
         __syncthreads();
+        // the next pass's rows load while this one gathers
+        if (PF > 0 && 4 * (rg + gridDim.x) < m_pad) fetch(4 * (rg + gridDim.x));
         uint32_t cnt[4][3];
 #pragma unroll
         for (int r = 0; r < 4; ++r) cnt[r][0] = cnt[r][1] = cnt[r][2] = 0;
@@ -1160,15 +1193,21 @@ int gather_and_tile(const ldsc_panel* panel, const uint64_t* d_bed_idx, uint64_t
         const uint32_t n16 = (uint32_t)((n + 15) / 16);
         HIPC(ws->scratch.ensure((size_t)m * n16 * 4));
         const size_t lds = (size_t)src_dw * 16;
+        const void* kfn = src_dw <= 2048 ? (const void*)k_gather4<4>
+                        : src_dw <= 4096 ? (const void*)k_gather4<8>
+                        : src_dw <= 8192 ? (const void*)k_gather4<16> : (const void*)k_gather4<0>;
         if (lds > 64 * 1024)
-            HIPC(hipFuncSetAttribute((const void*)k_gather4,
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+            HIPC(hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
         const uint32_t grid = (uint32_t)std::min<uint64_t>((m_pad + 3) / 4, 2048);
-        hipLaunchKernelGGL(k_gather4, dim3(grid), dim3(512), lds, st, panel->d_slabs,
-                           panel->stride, src_dw, d_bed_idx, base, (uint32_t)m, (uint32_t)m_pad,
-                           d_map, n16,
-                           ws->scratch.as<uint32_t>(), ws->stats.as<uint32_t>() + (size_t)m_pad * 4);
-        HIPC(hipGetLastError());
+        const uint32_t a_m = (uint32_t)m, a_mp = (uint32_t)m_pad;
+        uint32_t* a_out = ws->scratch.as<uint32_t>();
+        uint32_t* a_part = ws->stats.as<uint32_t>() + (size_t)m_pad * 4;
+        const uint8_t* a_sl = panel->d_slabs;
+        const uint64_t a_pitch = panel->stride;
+        void* kargs[] = {(void*)&a_sl, (void*)&a_pitch, (void*)&src_dw, (void*)&d_bed_idx,
+                         (void*)&base, (void*)&a_m, (void*)&a_mp, (void*)&d_map, (void*)&n16,
+                         (void*)&a_out, (void*)&a_part};
+        HIPC(hipLaunchKernel(kfn, dim3(grid), dim3(512), kargs, lds, st));
         if (!tile) {  // the caller reads the gathered rows; stats are in slot 0
             *parts = 1;
             return LDSC_OK;
