@@ -58,9 +58,14 @@ __device__ __forceinline__ void expand_codes(uint32_t w, v4i& G, v4i& M) {
 }
 
 struct EpiShared {
-    double R[32][65];     // one 32-row half of the tile's masked r2u
-    double ar[64][8];     // A_eff of the tile's row SNPs (chunk of 8 annotations)
-    double ac[64][8];     // A_eff of the tile's column SNPs
+    union {
+        struct {              // one annotation: one 32-row half of R at a time
+            double R[32][65];
+            double ar[64];    // A_eff of the tile's row SNPs
+            double ac[64];    // A_eff of the tile's column SNPs
+        } one;
+        double Rf[64][65];    // several annotations: the whole tile's masked r2u
+    } u;
     int2 csr[64], csc[64];
     double rsr[64], rsc[64];
     uint32_t lor[64];
@@ -70,26 +75,25 @@ __device__ __forceinline__ int row_of(int bi, int r, int lane) {
     return 32 * bi + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
 }
 
-// r2u (masked, 64 per lane in MFMA output order) -> per-tile partial sums
-// for annotations kc0 .. kc0+kcn-1 (kcn <= KC; KC = 1 lets the common
-// single-annotation case run without per-annotation predicates).
-//   rowp[tile][k][a] = sum_b R[a][b] * A_eff[col b][k]
-//   colp[tile][k][b] = sum_a R[a][b] * A_eff[row a][k]
-template <int KC>
-__device__ __forceinline__ void band_contract_chunk(const BandArgs& a, EpiShared& sh, int tile,
-                                                    uint32_t I, uint32_t J,
-                                                    const double (&r2u)[2][2][16], uint32_t kc0,
-                                                    uint32_t kcn_in) {
+// SNP stride of A_eff in HBM (m_pad rows of aeff_stride(K) doubles,
+// annotations contiguous): 1 for a single annotation, else K rounded up to
+// the 16 annotations one FP64 MFMA pass covers (padding columns are zero).
+__host__ __device__ inline uint32_t aeff_stride(uint32_t K) {
+    return K == 1 ? 1u : (K + 15u) & ~15u;
+}
+
+// One annotation: r2u (masked, 64 per lane in MFMA output order) ->
+//   rowp[tile][0][a] = sum_b R[a][b] * A_eff[col b]
+//   colp[tile][0][b] = sum_a R[a][b] * A_eff[row a]
+// on the vector FP64 units, one 32-row half of R through LDS at a time.
+__device__ __forceinline__ void band_contract_one(const BandArgs& a, EpiShared& sh, int tile,
+                                                  uint32_t I, uint32_t J,
+                                                  const double (&r2u)[2][2][16]) {
     const int lane = threadIdx.x;
-    const uint32_t kcn = KC == 1 ? 1u : kcn_in;
     __syncthreads();
-    for (uint32_t kk = 0; kk < kcn; ++kk) {
-        sh.ar[lane][kk] = a.aeff[(size_t)(kc0 + kk) * a.m_pad + (size_t)I * kTile + lane];
-        sh.ac[lane][kk] = a.aeff[(size_t)(kc0 + kk) * a.m_pad + (size_t)J * kTile + lane];
-    }
-    double cacc[KC];
-#pragma unroll
-    for (int kk = 0; kk < KC; ++kk) cacc[kk] = 0.0;
+    sh.u.one.ar[lane] = a.aeff[(size_t)I * kTile + lane];
+    sh.u.one.ac[lane] = a.aeff[(size_t)J * kTile + lane];
+    double cacc = 0.0;
 #pragma unroll
     for (int bi = 0; bi < 2; ++bi) {
         __syncthreads();
@@ -97,46 +101,113 @@ __device__ __forceinline__ void band_contract_chunk(const BandArgs& a, EpiShared
         for (int bj = 0; bj < 2; ++bj)
 #pragma unroll
             for (int r = 0; r < 16; ++r)
-                sh.R[row_of(0, r, lane)][32 * bj + (lane & 31)] = r2u[bi][bj][r];
+                sh.u.one.R[row_of(0, r, lane)][32 * bj + (lane & 31)] = r2u[bi][bj][r];
         __syncthreads();
         // row role: lane covers row lane%32 over columns of its half
-        double racc[KC];
-#pragma unroll
-        for (int kk = 0; kk < KC; ++kk) racc[kk] = 0.0;
+        double racc = 0.0;
         const int rr = lane & 31, b0 = 32 * (lane >> 5);
-        for (int b = 0; b < 32; ++b) {
-            const double rv = sh.R[rr][b0 + b];
-#pragma unroll
-            for (int kk = 0; kk < KC; ++kk)
-                if ((uint32_t)kk < kcn) racc[kk] += rv * sh.ac[b0 + b][kk];
-        }
-#pragma unroll
-        for (int kk = 0; kk < KC; ++kk)
-            if ((uint32_t)kk < kcn) racc[kk] += __shfl_xor(racc[kk], 32);
-        if (lane < 32)
-            for (uint32_t kk = 0; kk < kcn; ++kk)
-                a.rowp[((size_t)tile * a.K + kc0 + kk) * kTile + 32 * bi + lane] = racc[kk];
+        for (int b = 0; b < 32; ++b) racc += sh.u.one.R[rr][b0 + b] * sh.u.one.ac[b0 + b];
+        racc += __shfl_xor(racc, 32);
+        if (lane < 32) a.rowp[(size_t)tile * kTile + 32 * bi + lane] = racc;
         // column role: lane = column, rows of this half
-        for (int r = 0; r < 32; ++r) {
-            const double rv = sh.R[r][lane];
-#pragma unroll
-            for (int kk = 0; kk < KC; ++kk)
-                if ((uint32_t)kk < kcn) cacc[kk] += rv * sh.ar[32 * bi + r][kk];
-        }
+        for (int r = 0; r < 32; ++r) cacc += sh.u.one.R[r][lane] * sh.u.one.ar[32 * bi + r];
     }
-    for (uint32_t kk = 0; kk < kcn; ++kk)
-        a.colp[((size_t)tile * a.K + kc0 + kk) * kTile + lane] = cacc[kk];
+    a.colp[(size_t)tile * kTile + lane] = cacc;
 }
 
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+// Several annotations: R (the tile's masked r2u, row-major in LDS) against
+// the A_eff rows of the tile's SNPs on the FP64 MFMA, 16 annotations per
+// pass:
+//   ROWS: rowp[tile][k][a] = sum_b R[a][b] * A_eff[col b][k]
+//   COLS: colp[tile][k][b] = sum_a R[a][b] * A_eff[row a][k]
+// v_mfma_f64_16x16x4f64: lane l supplies A[l%16][l/16] and B[l/16][l%16]
+// and holds D[4i + l/16][l%16] in its register i = 0..3.  Each 16x16 output block sums
+// its 64 terms in a fixed order (16 steps of 4), so results are
+// reproducible and equal whichever wave computes them.
+template <bool ROWS, bool COLS>
+__device__ __forceinline__ void band_contract_mfma(const BandArgs& a, const double (*Rf)[65],
+                                                   int tile, uint32_t I, uint32_t J) {
+    const int lane = threadIdx.x & 63, li = lane & 15, lk = lane >> 4;
+    const uint32_t Kp = aeff_stride(a.K);
+    const double* ac = a.aeff + (size_t)(J * kTile + lk) * Kp + li;
+    const double* ar = a.aeff + (size_t)(I * kTile + lk) * Kp + li;
+    for (uint32_t k0 = 0; k0 < a.K; k0 += 16) {
+        double bc[16], br[16];
+#pragma unroll
+        for (int s = 0; s < 16; ++s) {
+            if (ROWS) bc[s] = ac[(size_t)4 * s * Kp + k0];
+            if (COLS) br[s] = ar[(size_t)4 * s * Kp + k0];
+        }
+        v4d dr[4], dc[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            dr[q] = v4d{0.0, 0.0, 0.0, 0.0};
+            dc[q] = v4d{0.0, 0.0, 0.0, 0.0};
+        }
+        // A operands of step s+1 are read from LDS while step s's MFMAs
+        // run; the barrier keeps the scheduler from hoisting further reads
+        // (which would only add live registers)
+        double xr[4], xc[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            xr[q] = ROWS ? Rf[16 * q + li][lk] : 0.0;
+            xc[q] = COLS ? Rf[lk][16 * q + li] : 0.0;
+        }
+#pragma unroll
+        for (int s = 0; s < 16; ++s) {
+            double yr[4], yc[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                yr[q] = xr[q];
+                yc[q] = xc[q];
+                if (s + 1 < 16) {
+                    if (ROWS) xr[q] = Rf[16 * q + li][4 * (s + 1) + lk];
+                    if (COLS) xc[q] = Rf[4 * (s + 1) + lk][16 * q + li];
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                if (ROWS) dr[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(yr[q], bc[s], dr[q], 0, 0, 0);
+                if (COLS) dc[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(yc[q], br[s], dc[q], 0, 0, 0);
+            }
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        const uint32_t k = k0 + li;
+        if (k < a.K) {
+            const size_t o = ((size_t)tile * a.K + k) * kTile + lk;
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    if (ROWS) a.rowp[o + 16 * q + 4 * i] = dr[q][i];
+                    if (COLS) a.colp[o + 16 * q + 4 * i] = dc[q][i];
+                }
+        }
+    }
+}
+
+// r2u (masked, 64 per lane in MFMA output order) -> the tile's partial
+// sums for every annotation (single-wave kernels).
 __device__ __forceinline__ void band_contract(const BandArgs& a, EpiShared& sh, int tile,
                                               uint32_t I, uint32_t J,
                                               const double (&r2u)[2][2][16]) {
     if (a.K == 1) {
-        band_contract_chunk<1>(a, sh, tile, I, J, r2u, 0, 1);
+        band_contract_one(a, sh, tile, I, J, r2u);
         return;
     }
-    for (uint32_t kc0 = 0; kc0 < a.K; kc0 += 8)
-        band_contract_chunk<8>(a, sh, tile, I, J, r2u, kc0, a.K - kc0 < 8 ? a.K - kc0 : 8);
+    const int lane = threadIdx.x;
+    __syncthreads();
+#pragma unroll
+    for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+        for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+            for (int r = 0; r < 16; ++r)
+                sh.u.Rf[row_of(bi, r, lane)][32 * bj + (lane & 31)] = r2u[bi][bj][r];
+    __syncthreads();
+    band_contract_mfma<true, true>(a, sh.u.Rf, tile, I, J);
 }
 
 __device__ __forceinline__ void stage_info(const BandArgs& a, EpiShared& sh, uint32_t I,

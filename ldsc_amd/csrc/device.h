@@ -75,8 +75,8 @@ struct Workspace {
     hipStream_t stream = nullptr;
     hipStream_t copy_stream = nullptr;  // read-backs that overlap the band kernel
     hipEvent_t ev[8] = {};
-    DevBuf tiled, stats, bed_idx, lo, info, aeff, annot, maf, mafd, tiles, j0, tstart, rowp,
-        colp, l2, sk_cols, sk_norm, scratch;
+    DevBuf tiled, stats, bed_idx, lo, info, aeff, annot, pq, maf, mafd, tiles, j0, tstart,
+        rowp, colp, l2, sk_cols, sk_norm, scratch;
     PinnedBuf pinned;       // SNP row indices (upload in flight while the host plans)
     PinnedBuf pinned_meta;  // pair bounds, band plan, stats read-back
     PinnedBuf pinned_maf;   // MAF read-back (during the band kernel)

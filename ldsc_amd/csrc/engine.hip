@@ -79,8 +79,8 @@ void PinnedBuf::release() {
 }
 
 size_t Workspace::bytes() const {
-    const DevBuf* all[] = {&tiled, &stats, &bed_idx, &lo, &info, &aeff, &annot, &maf, &mafd, &tiles,
-                           &j0, &tstart, &rowp, &colp, &l2, &sk_cols, &sk_norm, &scratch};
+    const DevBuf* all[] = {&tiled, &stats, &bed_idx, &lo, &info, &aeff, &annot, &pq, &maf, &mafd,
+                           &tiles, &j0, &tstart, &rowp, &colp, &l2, &sk_cols, &sk_norm, &scratch};
     size_t s = 0;
     for (auto* b : all) s += b->cap;
     return s;
@@ -102,8 +102,8 @@ hipError_t Workspace::init(int dev) {
 }
 
 void Workspace::destroy() {
-    DevBuf* all[] = {&tiled, &stats, &bed_idx, &lo, &info, &aeff, &annot, &maf, &mafd, &tiles,
-                     &j0, &tstart, &rowp, &colp, &l2, &sk_cols, &sk_norm, &scratch};
+    DevBuf* all[] = {&tiled, &stats, &bed_idx, &lo, &info, &aeff, &annot, &pq, &maf, &mafd,
+                     &tiles, &j0, &tstart, &rowp, &colp, &l2, &sk_cols, &sk_norm, &scratch};
     for (auto* b : all) b->release();
     pinned.release();
     pinned_meta.release();
@@ -537,56 +537,81 @@ __global__ void k_snp_info(const uint32_t* __restrict__ part, uint32_t nparts,
     }
 }
 
-// A_eff in place (m_pad x K, column stride m_pad): the uploaded annotation
-// (or ones when `ones`), NaN -> 0 (read_annot, parse.rs:224), times pq_j
-// when given (compute.rs:1595-1616); padding rows are zero.
-__global__ void k_aeff(double* __restrict__ a, const double* __restrict__ pq, uint32_t m,
-                       uint32_t m_pad, uint32_t K, int ones) {
-    const uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= (uint64_t)m_pad * K) return;
-    const uint32_t j = (uint32_t)(idx % m_pad);
-    double v = 0.0;
-    if (j < m) {
-        v = ones ? 1.0 : a[idx];
-        if (v != v) v = 0.0;
-        if (pq) v *= pq[j];
+// A_eff (m_pad rows of Kp = aeff_stride(K) doubles, annotations contiguous)
+// from the uploaded annotation `src` (column-major m x K, column stride m;
+// ones when null): NaN -> 0 (read_annot, parse.rs:224), times pq_j when
+// given (compute.rs:1595-1616); padding rows and columns are zero.  A
+// 64-SNP x 16-annotation block goes through LDS, so that both the reads
+// (along SNPs) and the writes (along annotations) are coalesced.
+__global__ __launch_bounds__(256) void k_aeff(double* __restrict__ out, const double* __restrict__ src,
+                                              const double* __restrict__ pq, uint32_t m, uint32_t K,
+                                              uint32_t Kp) {
+    __shared__ double blk[16][65];
+    const uint32_t j0 = blockIdx.x * 64, k0 = blockIdx.y * 16, t = threadIdx.x;
+    if (Kp == 1) {  // one annotation: 64 SNPs, no transposition
+        if (t < 64) {
+            const uint32_t j = j0 + t;
+            double v = 0.0;
+            if (j < m) {
+                v = src ? src[j] : 1.0;
+                if (v != v) v = 0.0;
+                if (pq) v *= pq[j];
+            }
+            out[j] = v;
+        }
+        return;
     }
-    a[idx] = v;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const uint32_t kk = t / 64 + 4 * i, jj = t % 64, j = j0 + jj, k = k0 + kk;
+        double v = 0.0;
+        if (j < m && k < K) {
+            v = src ? src[(size_t)k * m + j] : 1.0;
+            if (v != v) v = 0.0;
+            if (pq) v *= pq[j];
+        }
+        blk[kk][jj] = v;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const uint32_t kk = t % 16, jj = t / 16 + 16 * i;
+        out[(size_t)(j0 + jj) * Kp + k0 + kk] = blk[kk][jj];
+    }
 }
 
-// Epilogue storage of the two-wave band kernel: each wave stages one 16-row
-// quarter of its 32 x 64 r2u block at a time (~31 KiB per workgroup, so four
-// workgroups -- two waves per SIMD -- fit in a CU's LDS).
+// Epilogue storage of the two-wave band kernel.  One annotation: each wave
+// stages one 16-row quarter of its 32 x 64 r2u block at a time.  Several:
+// the whole tile's r2u for the FP64 MFMA contraction.  ~35 KiB per
+// workgroup, so four workgroups (two waves per SIMD) fit in a CU's LDS.
 struct EpiShared2 {
-    double R[2][16][65];
-    double ar[64][8];     // A_eff of the tile's row SNPs (chunk of 8 annotations)
-    double ac[64][8];     // A_eff of the tile's column SNPs
-    double cx[64][8];     // wave 1's column partials
+    union {
+        struct {
+            double R[2][16][65];
+            double ar[64];    // A_eff of the tile's row SNPs
+            double ac[64];    // A_eff of the tile's column SNPs
+            double cx[64];    // wave 1's column partials
+        } one;
+        double Rf[64][65];
+    } u;
     int2 csr[64], csc[64];
     double rsr[64], rsc[64];
     uint32_t lor[64];
 };
 
-// r2u of wave w (rows 32w .. 32w+31 of the tile, all 64 columns, in MFMA
-// output order r2u[bj][r]) -> per-tile partial sums for annotations kc0 ..
-// kc0+kcn-1 (kcn <= KC), fixed summation order.
-template <int KC>
-__device__ __forceinline__ void band_contract2_chunk(const BandArgs& a, EpiShared2& sh, int tile,
-                                                     uint32_t I, uint32_t J, int w,
-                                                     const double (&r2u)[2][16], uint32_t kc0,
-                                                     uint32_t kcn_in) {
+// One annotation: r2u of wave w (rows 32w .. 32w+31 of the tile, all 64
+// columns, in MFMA output order r2u[bj][r]) -> the tile's partial sums,
+// fixed summation order.
+__device__ __forceinline__ void band_contract2_one(const BandArgs& a, EpiShared2& sh, int tile,
+                                                   uint32_t I, uint32_t J, int w,
+                                                   const double (&r2u)[2][16]) {
     const int lane = threadIdx.x & 63;
-    const uint32_t kcn = KC == 1 ? 1u : kcn_in;
     __syncthreads();
-    for (uint32_t kk = 0; kk < kcn; ++kk) {
-        if (w == 0)
-            sh.ar[lane][kk] = a.aeff[(size_t)(kc0 + kk) * a.m_pad + (size_t)I * kTile + lane];
-        else
-            sh.ac[lane][kk] = a.aeff[(size_t)(kc0 + kk) * a.m_pad + (size_t)J * kTile + lane];
-    }
-    double cacc[KC];
-#pragma unroll
-    for (int kk = 0; kk < KC; ++kk) cacc[kk] = 0.0;
+    if (w == 0)
+        sh.u.one.ar[lane] = a.aeff[(size_t)I * kTile + lane];
+    else
+        sh.u.one.ac[lane] = a.aeff[(size_t)J * kTile + lane];
+    double cacc = 0.0;
 #pragma unroll
     for (int Q = 0; Q < 2; ++Q) {
         __syncthreads();
@@ -596,55 +621,48 @@ __device__ __forceinline__ void band_contract2_chunk(const BandArgs& a, EpiShare
             for (int i = 0; i < 8; ++i) {
                 const int r = 8 * Q + i;
                 const int local = 8 * ((r >> 2) - 2 * Q) + 4 * (lane >> 5) + (r & 3);
-                sh.R[w][local][32 * bj + (lane & 31)] = r2u[bj][r];
+                sh.u.one.R[w][local][32 * bj + (lane & 31)] = r2u[bj][r];
             }
         __syncthreads();
         // row role: row lane%16 of the quarter, columns 16*(lane/16) .. +16
-        double racc[KC];
-#pragma unroll
-        for (int kk = 0; kk < KC; ++kk) racc[kk] = 0.0;
+        double racc = 0.0;
         const int rr = lane & 15, b0 = 16 * (lane >> 4);
-        for (int b = 0; b < 16; ++b) {
-            const double rv = sh.R[w][rr][b0 + b];
-#pragma unroll
-            for (int kk = 0; kk < KC; ++kk)
-                if ((uint32_t)kk < kcn) racc[kk] += rv * sh.ac[b0 + b][kk];
-        }
-#pragma unroll
-        for (int kk = 0; kk < KC; ++kk)
-            if ((uint32_t)kk < kcn) {
-                racc[kk] += __shfl_xor(racc[kk], 16);
-                racc[kk] += __shfl_xor(racc[kk], 32);
-            }
-        if (lane < 16)
-            for (uint32_t kk = 0; kk < kcn; ++kk)
-                a.rowp[((size_t)tile * a.K + kc0 + kk) * kTile + 32 * w + 16 * Q + lane] = racc[kk];
+        for (int b = 0; b < 16; ++b) racc += sh.u.one.R[w][rr][b0 + b] * sh.u.one.ac[b0 + b];
+        racc += __shfl_xor(racc, 16);
+        racc += __shfl_xor(racc, 32);
+        if (lane < 16) a.rowp[(size_t)tile * kTile + 32 * w + 16 * Q + lane] = racc;
         // column role: lane = column, the 16 rows of this quarter
-        for (int r = 0; r < 16; ++r) {
-            const double rv = sh.R[w][r][lane];
-#pragma unroll
-            for (int kk = 0; kk < KC; ++kk)
-                if ((uint32_t)kk < kcn) cacc[kk] += rv * sh.ar[32 * w + 16 * Q + r][kk];
-        }
+        for (int r = 0; r < 16; ++r)
+            cacc += sh.u.one.R[w][r][lane] * sh.u.one.ar[32 * w + 16 * Q + r];
     }
     __syncthreads();
-    if (w == 1)
-        for (uint32_t kk = 0; kk < kcn; ++kk) sh.cx[lane][kk] = cacc[kk];
+    if (w == 1) sh.u.one.cx[lane] = cacc;
     __syncthreads();
-    if (w == 0)
-        for (uint32_t kk = 0; kk < kcn; ++kk)
-            a.colp[((size_t)tile * a.K + kc0 + kk) * kTile + lane] = cacc[kk] + sh.cx[lane][kk];
+    if (w == 0) a.colp[(size_t)tile * kTile + lane] = cacc + sh.u.one.cx[lane];
 }
 
 __device__ __forceinline__ void band_contract2(const BandArgs& a, EpiShared2& sh, int tile,
                                                uint32_t I, uint32_t J, int w,
                                                const double (&r2u)[2][16]) {
     if (a.K == 1) {
-        band_contract2_chunk<1>(a, sh, tile, I, J, w, r2u, 0, 1);
+        band_contract2_one(a, sh, tile, I, J, w, r2u);
         return;
     }
-    for (uint32_t kc0 = 0; kc0 < a.K; kc0 += 8)
-        band_contract2_chunk<8>(a, sh, tile, I, J, w, r2u, kc0, a.K - kc0 < 8 ? a.K - kc0 : 8);
+    // several annotations: both waves stage their rows, then wave 0 forms
+    // the row partials and wave 1 the column partials
+    const int lane = threadIdx.x & 63;
+    __syncthreads();
+#pragma unroll
+    for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+        for (int r = 0; r < 16; ++r)
+            sh.u.Rf[32 * w + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3)][32 * bj + (lane & 31)] =
+                r2u[bj][r];
+    __syncthreads();
+    if (w == 0)
+        band_contract_mfma<true, false>(a, sh.u.Rf, tile, I, J);
+    else
+        band_contract_mfma<false, true>(a, sh.u.Rf, tile, I, J);
 }
 
 __device__ __forceinline__ void finish_2w(const BandArgs& a, EpiShared2& sh, int tile, uint32_t I,
@@ -1056,13 +1074,13 @@ __global__ __launch_bounds__(64) void k_band_f32(BandArgs a) {
 // ------------------------------------------------------------------------
 __global__ void k_reduce(const double* __restrict__ rowp, const double* __restrict__ colp,
                          const double* __restrict__ aeff, const int* __restrict__ j0,
-                         const int* __restrict__ tstart, uint32_t nb, uint32_t m,
-                         uint32_t m_pad, uint32_t K, double* __restrict__ l2) {
+                         const int* __restrict__ tstart, uint32_t nb, uint32_t m, uint32_t K,
+                         double* __restrict__ l2) {
     const uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (gid >= (uint64_t)m * K) return;
     const uint32_t k = (uint32_t)(gid / m), j = (uint32_t)(gid % m);
     const uint32_t I = j / kTile, r = j % kTile;
-    double s = aeff[(size_t)k * m_pad + j];  // self pair, r2u(j, j) = 1 (compute.rs:1658)
+    double s = aeff[(size_t)j * aeff_stride(K) + k];  // self pair, r2u(j, j) = 1 (compute.rs:1658)
     for (int J = j0[I]; J <= (int)I; ++J) {
         const size_t t = (size_t)tstart[I] + (J - j0[I]);
         s += rowp[(t * K + k) * kTile + r];
@@ -1154,7 +1172,9 @@ size_t l2_workspace_bytes(const ldsc_panel* panel, uint64_t m, const ldsc_l2_con
         tiles = nb * (2 + (uint64_t)(cfg->window_value + cfg->chunk_size) / kTile);
     size_t bytes = m_pad * n_pad / 4;                           // tiled SNP slice
     bytes += m_pad * 16 * (1 + tile_parts(panel->n));                // stats + chunk slots
-    bytes += m_pad * (8 + 4 + 48 + 8) + m_pad * K * 8 + m * K * 8;   // idx, lo, info, maf, aeff, l2
+    bytes += m_pad * (8 + 4 + 48 + 8);                          // idx, lo, info, maf
+    bytes += m_pad * aeff_stride((uint32_t)K) * 8 + m * K * 8;  // A_eff, l2
+    if (K > 1) bytes += m * K * 8;                              // uploaded annotation
     bytes += tiles * (kTile * K * 8 * 2 + 8);                   // tile partials + list
     const bool gather = cfg && (cfg->sketch_d || panel->d_keep32);
     if (gather) bytes += m * round_up((panel->n + 3) / 4, 4);   // permuted rows
@@ -1412,7 +1432,8 @@ int run_l2(const ldsc_panel* panel, const ldsc_snp* snps, uint64_t m, const ldsc
     const double t_prep = now_s();
 
     HIPC(ws->lo.ensure((size_t)m_pad * 4));
-    HIPC(ws->aeff.ensure((size_t)m_pad * K * 8));
+    const uint32_t Kp = aeff_stride((uint32_t)K);
+    HIPC(ws->aeff.ensure((size_t)m_pad * Kp * 8));
     HIPC(ws->tiles.ensure((size_t)plan.ntiles * 8));
     HIPC(ws->j0.ensure((size_t)plan.nb * 4));
     HIPC(ws->tstart.ensure((size_t)(plan.nb + 1) * 4));
@@ -1496,20 +1517,18 @@ int run_l2(const ldsc_panel* panel, const ldsc_snp* snps, uint64_t m, const ldsc
         std::vector<double>& pq = ws->host.aux;
         pq.resize(m);
         for (uint64_t j = 0; j < m; ++j) pq[j] = std::pow(maf[j] * (1.0 - maf[j]), cfg->pq_exp);
-        HIPC(ws->annot.ensure(m * 8));
-        HIPC(hipMemcpyAsync(ws->annot.p, pq.data(), m * 8, hipMemcpyHostToDevice, st));
+        HIPC(ws->pq.ensure(m * 8));
+        HIPC(hipMemcpyAsync(ws->pq.p, pq.data(), m * 8, hipMemcpyHostToDevice, st));
         HIPC(hipStreamSynchronize(st));  // `pq` is pageable and local
     }
-    if (annot)
-        HIPC(hipMemcpy2DAsync(ws->aeff.p, (size_t)m_pad * 8, annot, (size_t)m * 8, (size_t)m * 8,
-                              K, hipMemcpyHostToDevice, st));
-    {
-        const uint64_t tot = (uint64_t)m_pad * K;
-        hipLaunchKernelGGL(k_aeff, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st,
-                           ws->aeff.as<double>(), has_pq ? ws->annot.as<double>() : nullptr,
-                           (uint32_t)m, m_pad, (uint32_t)K, annot ? 0 : 1);
-        HIPC(hipGetLastError());
+    if (annot) {
+        HIPC(ws->annot.ensure((size_t)m * K * 8));
+        HIPC(hipMemcpyAsync(ws->annot.p, annot, (size_t)m * K * 8, hipMemcpyHostToDevice, st));
     }
+    hipLaunchKernelGGL(k_aeff, dim3(m_pad / 64, Kp / 16 + (Kp == 1)), dim3(256), 0, st,
+                       ws->aeff.as<double>(), annot ? ws->annot.as<double>() : nullptr,
+                       has_pq ? ws->pq.as<double>() : nullptr, (uint32_t)m, (uint32_t)K, Kp);
+    HIPC(hipGetLastError());
 
     // 3. band contraction
     BandArgs ba{};
@@ -1555,7 +1574,7 @@ int run_l2(const ldsc_panel* panel, const ldsc_snp* snps, uint64_t m, const ldsc
     const uint64_t nred = m * K;
     hipLaunchKernelGGL(k_reduce, dim3((unsigned)((nred + 255) / 256)), dim3(256), 0, st,
                        ws->rowp.as<double>(), ws->colp.as<double>(), ws->aeff.as<double>(),
-                       ws->j0.as<int>(), ws->tstart.as<int>(), plan.nb, (uint32_t)m, m_pad,
+                       ws->j0.as<int>(), ws->tstart.as<int>(), plan.nb, (uint32_t)m,
                        (uint32_t)K, ws->l2.as<double>());
     HIPC(hipGetLastError());
     HIPC(hipEventRecord(ws->ev[5], st));

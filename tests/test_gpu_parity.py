@@ -91,6 +91,32 @@ def test_l2_keep_annot_pq(oracle, use_f32):
     assert np.array_equal(maf, emaf)
 
 
+@pytest.mark.parametrize("K", [2, 16, 17, 37])
+def test_l2_many_annotations(oracle, K):
+    """Annotation counts on and across the 16-column passes of the FP64
+    contraction (one, exactly one full, one plus one column, two plus a
+    ragged third), NaN entries, pq scaling; both exact kernels and FP32."""
+    n, m = 331, 700
+    p = _panel(n, m, seed=K)
+    rng = np.random.default_rng(K)
+    annot = rng.lognormal(size=(m, K))
+    annot[rng.random((m, K)) < 0.5] = 0.0
+    annot[rng.random((m, K)) < 0.01] = np.nan
+    snps = L.make_snp_array(p.chr, p.cm, p.bp)
+    with L.Panel(p.bed_bytes(), n, m) as panel:
+        for use_f32, kern in ((False, "auto"), (False, "i8"), (True, "auto")):
+            print(f"many annotations K={K} f32={use_f32} kernel={kern}", flush=True)
+            cfg = L.L2Config(mode=L.WindowMode.Kb(150.0), chunk_size=40, use_f32=use_f32,
+                             pq_exp=0.5, exact_kernel=kern)
+            got, maf, _ = L.compute_ldscore(panel, snps, cfg, annot=annot)
+            exp, emaf = oracle.l2(p.bed_bytes(), n, p.chr, p.cm, p.bp, mode=1, window=150.0,
+                                  chunk=40, use_f32=use_f32, pq_exp=0.5,
+                                  annot=np.nan_to_num(annot))
+            assert got.shape == (m, K)
+            assert _rel(got, exp) < (TOL_F32 if use_f32 else TOL_EXACT), (use_f32, kern)
+            assert np.array_equal(maf, emaf)
+
+
 def test_l2_extract_per_chromosome(oracle):
     n, m = 400, 1200
     p = _panel(n, m, seed=31, nchr=3)
