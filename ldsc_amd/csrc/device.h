@@ -74,12 +74,20 @@ struct Workspace {
     int device = 0;
     hipStream_t stream = nullptr;
     hipStream_t copy_stream = nullptr;  // read-backs that overlap the band kernel
-    hipEvent_t ev[8] = {};
+    hipStream_t up_stream = nullptr;    // annotation upload beside the tiling
+    hipEvent_t ev[9] = {};
     DevBuf tiled, stats, bed_idx, lo, info, aeff, annot, pq, maf, mafd, tiles, j0, tstart,
         rowp, colp, l2, sk_cols, sk_norm, scratch;
     PinnedBuf pinned;       // SNP row indices (upload in flight while the host plans)
     PinnedBuf pinned_meta;  // pair bounds, band plan, stats read-back
     PinnedBuf pinned_maf;   // MAF read-back (during the band kernel)
+    // Large copies of caller memory (annotation upload, multi-annotation
+    // results) pass through kRingSlots pinned slots of kRingSlot bytes: the
+    // host's task pool fills or drains one slot while DMA moves another.
+    static constexpr size_t kRingSlot = 8u << 20;
+    static constexpr int kRingSlots = 4;
+    PinnedBuf ring;
+    hipEvent_t ring_ev[kRingSlots] = {};
     HostScratch host;
     size_t bytes() const;
     hipError_t init(int dev);

@@ -22,6 +22,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <thread>
 
 #include "band_common.h"
 
@@ -94,8 +95,14 @@ hipError_t Workspace::init(int dev) {
     if (e != hipSuccess) return e;
     e = hipStreamCreateWithFlags(&copy_stream, hipStreamNonBlocking);
     if (e != hipSuccess) return e;
+    e = hipStreamCreateWithFlags(&up_stream, hipStreamNonBlocking);
+    if (e != hipSuccess) return e;
     for (auto& x : ev) {
         e = hipEventCreate(&x);
+        if (e != hipSuccess) return e;
+    }
+    for (auto& x : ring_ev) {
+        e = hipEventCreateWithFlags(&x, hipEventDisableTiming);
         if (e != hipSuccess) return e;
     }
     return hipSuccess;
@@ -108,12 +115,73 @@ void Workspace::destroy() {
     pinned.release();
     pinned_meta.release();
     pinned_maf.release();
+    ring.release();
     for (auto& x : ev)
+        if (x) (void)hipEventDestroy(x);
+    for (auto& x : ring_ev)
         if (x) (void)hipEventDestroy(x);
     if (stream) (void)hipStreamDestroy(stream);
     if (copy_stream) (void)hipStreamDestroy(copy_stream);
+    if (up_stream) (void)hipStreamDestroy(up_stream);
     copy_stream = nullptr;
+    up_stream = nullptr;
     stream = nullptr;
+}
+
+// Host -> device copy of `bytes` of caller memory through the pinned ring,
+// on stream s, filled by the side copy pool (the caller runs it beside the
+// host's planning).  Returns once the last slot is filled; its DMA may still
+// be in flight (later work on s is ordered after it).
+static hipError_t ring_upload(Workspace* ws, void* dst, const void* src, size_t bytes,
+                              hipStream_t s) {
+    constexpr size_t SB = Workspace::kRingSlot;
+    constexpr int NS = Workspace::kRingSlots;
+    if (bytes <= SB)  // small: the runtime's own staging
+        return hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s);
+    hipError_t e = ws->ring.ensure(SB * NS);
+    if (e != hipSuccess) return e;
+    char* ring = ws->ring.as<char>();
+    size_t c = 0;
+    for (size_t off = 0; off < bytes; off += SB, ++c) {
+        const int slot = (int)(c % NS);
+        const size_t len = std::min(SB, bytes - off);
+        if (c >= (size_t)NS && (e = hipEventSynchronize(ws->ring_ev[slot])) != hipSuccess) return e;
+        parallel_copy(ring + slot * SB, static_cast<const char*>(src) + off, len, /*side=*/true);
+        e = hipMemcpyAsync(static_cast<char*>(dst) + off, ring + slot * SB, len,
+                           hipMemcpyHostToDevice, s);
+        if (e != hipSuccess) return e;
+        if ((e = hipEventRecord(ws->ring_ev[slot], s)) != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+// Device -> host copy into caller memory through the pinned ring, ordered
+// after the work already queued on s.  Returns when every byte has landed.
+static hipError_t ring_download(Workspace* ws, void* dst, const void* src, size_t bytes,
+                                hipStream_t s) {
+    constexpr size_t SB = Workspace::kRingSlot;
+    constexpr int NS = Workspace::kRingSlots;
+    hipError_t e = ws->ring.ensure(SB * NS);
+    if (e != hipSuccess) return e;
+    char* ring = ws->ring.as<char>();
+    const size_t nc = (bytes + SB - 1) / SB;
+    auto issue = [&](size_t c) {
+        const int slot = (int)(c % NS);
+        const size_t off = c * SB, len = std::min(SB, bytes - off);
+        hipError_t r = hipMemcpyAsync(ring + slot * SB, static_cast<const char*>(src) + off, len,
+                                      hipMemcpyDeviceToHost, s);
+        return r != hipSuccess ? r : hipEventRecord(ws->ring_ev[slot], s);
+    };
+    for (size_t c = 0; c < nc && c < (size_t)NS; ++c)
+        if ((e = issue(c)) != hipSuccess) return e;
+    for (size_t c = 0; c < nc; ++c) {
+        const int slot = (int)(c % NS);
+        const size_t off = c * SB, len = std::min(SB, bytes - off);
+        if ((e = hipEventSynchronize(ws->ring_ev[slot])) != hipSuccess) return e;
+        parallel_copy(static_cast<char*>(dst) + off, ring + slot * SB, len);
+        if (c + NS < nc && (e = issue(c + NS)) != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 
 static std::mutex g_ws_mu;
@@ -1363,6 +1431,26 @@ int run_l2(const ldsc_panel* panel, const ldsc_snp* snps, uint64_t m, const ldsc
                        ws->stats.as<uint32_t>(), m_pad, (uint32_t)n, d_cs, d_rs, d_lut, d_musc,
                        dev_maf ? ws->mafd.as<double>() : nullptr, use_f32 ? 1 : 0);
     HIPC(hipGetLastError());
+    // the annotation goes up on its own stream, from a thread of its own,
+    // while the device tiles and this thread plans the band
+    std::thread upload;
+    hipError_t upload_err = hipSuccess;
+    struct Joiner {
+        std::thread& t;
+        ~Joiner() {
+            if (t.joinable()) t.join();
+        }
+    } upload_joiner{upload};
+    if (annot) {
+        HIPC(ws->annot.ensure((size_t)m * K * 8));
+        const int dev = panel->device;
+        upload = std::thread([ws, annot, m, K, dev, &upload_err] {
+            hipError_t e = hipSetDevice(dev);
+            if (e == hipSuccess) e = ring_upload(ws, ws->annot.p, annot, (size_t)m * K * 8, ws->up_stream);
+            if (e == hipSuccess) e = hipEventRecord(ws->ev[8], ws->up_stream);
+            upload_err = e;
+        });
+    }
     // the MAF goes back to the host on a second stream while the band runs
     const bool has_pq = !std::isnan(cfg->pq_exp);
     const bool maf_aware = d && cfg->sketch_maf_aware;
@@ -1402,8 +1490,9 @@ int run_l2(const ldsc_panel* panel, const ldsc_snp* snps, uint64_t m, const ldsc
     const size_t off_st = off_ts + round_up((size_t)(plan.nb + 1) * 4, 16);
     const size_t off_maf = off_st + (size_t)m_pad * 16;
     const size_t off_l2 = off_maf + (size_t)m_pad * 8;
-    // results come back through pinned memory unless they are large
-    const bool stage_l2 = (size_t)m * K * 8 <= (64u << 20);
+    // results come back through pinned memory: with the call's metadata
+    // when small, else through the pinned ring
+    const bool stage_l2 = (size_t)m * K * 8 <= (8u << 20);
     const size_t meta_bytes = off_l2 + (stage_l2 ? (size_t)m * K * 8 : 0);
     // nothing in flight reads pinned_meta (the previous call ended with a
     // synchronize), so it may grow here
@@ -1522,8 +1611,9 @@ int run_l2(const ldsc_panel* panel, const ldsc_snp* snps, uint64_t m, const ldsc
         HIPC(hipStreamSynchronize(st));  // `pq` is pageable and local
     }
     if (annot) {
-        HIPC(ws->annot.ensure((size_t)m * K * 8));
-        HIPC(hipMemcpyAsync(ws->annot.p, annot, (size_t)m * K * 8, hipMemcpyHostToDevice, st));
+        upload.join();
+        HIPC(upload_err);
+        HIPC(hipStreamWaitEvent(st, ws->ev[8], 0));
     }
     hipLaunchKernelGGL(k_aeff, dim3(m_pad / 64, Kp / 16 + (Kp == 1)), dim3(256), 0, st,
                        ws->aeff.as<double>(), annot ? ws->annot.as<double>() : nullptr,
@@ -1585,10 +1675,14 @@ int run_l2(const ldsc_panel* panel, const ldsc_snp* snps, uint64_t m, const ldsc
         HIPC(hipMemcpyAsync(h_stats, ws->stats.p, (size_t)m_pad * 16, hipMemcpyDeviceToHost, st));
     }
     double* h_l2 = reinterpret_cast<double*>(hp + off_l2);
-    HIPC(hipMemcpyAsync(stage_l2 ? h_l2 : l2_out, ws->l2.p, (size_t)m * K * 8,
-                        hipMemcpyDeviceToHost, st));
-    HIPC(hipStreamSynchronize(st));
-    if (stage_l2) std::memcpy(l2_out, h_l2, (size_t)m * K * 8);
+    if (stage_l2) {
+        HIPC(hipMemcpyAsync(h_l2, ws->l2.p, (size_t)m * K * 8, hipMemcpyDeviceToHost, st));
+        HIPC(hipStreamSynchronize(st));
+        std::memcpy(l2_out, h_l2, (size_t)m * K * 8);
+    } else {
+        HIPC(ring_download(ws, l2_out, ws->l2.p, (size_t)m * K * 8, st));
+        HIPC(hipStreamSynchronize(st));
+    }
     if (maf_out && !early_maf) {
         if (!early_stats) compute_maf();  // sketch form: from the integer stats
         std::memcpy(maf_out, maf.data(), m * 8);

@@ -19,9 +19,7 @@ namespace ldsc {
 namespace {
 class TaskPool {
   public:
-    TaskPool() {
-        unsigned hw = std::thread::hardware_concurrency();
-        const int nt = (int)std::min<unsigned>(hw > 1 ? hw - 1 : 0, 7);
+    explicit TaskPool(int nt) {
         for (int i = 0; i < nt; ++i) std::thread([this] { worker(); }).detach();
         nthreads_ = nt;
     }
@@ -73,8 +71,21 @@ class TaskPool {
     uint64_t gen_ = 0;
 };
 
+// Pools are never destroyed: their workers outlive main.
 TaskPool& pool() {
-    static TaskPool* p = new TaskPool();  // never destroyed: workers outlive main
+    static TaskPool* p = [] {
+        const unsigned hw = std::thread::hardware_concurrency();
+        return new TaskPool((int)std::min<unsigned>(hw > 1 ? hw - 1 : 0, 7));
+    }();
+    return *p;
+}
+
+// Copies that run beside the host's planning (uploads) use a pool of their own.
+TaskPool& copy_pool() {
+    static TaskPool* p = [] {
+        const unsigned hw = std::thread::hardware_concurrency();
+        return new TaskPool((int)std::min<unsigned>(hw > 8 ? 3 : 1, 3));
+    }();
     return *p;
 }
 }  // namespace
@@ -84,6 +95,17 @@ void parallel_tasks(int n, const std::function<void(int)>& f) { pool().run(n, f)
 int task_count(uint64_t items) {
     if (items < 32768) return 1;
     return (int)std::min<uint64_t>((uint64_t)pool().threads() + 1, items / 16384);
+}
+
+void parallel_copy(void* dst, const void* src, size_t bytes, bool side) {
+    TaskPool& p = side ? copy_pool() : pool();
+    const int T = (int)std::max<size_t>(1, std::min<size_t>((size_t)p.threads() + 1, bytes >> 18));
+    p.run(T, [&](int t) {
+        // piece boundaries on 64-byte multiples
+        const size_t a = (bytes * t / T) & ~(size_t)63;
+        const size_t b = t + 1 == T ? bytes : (bytes * (t + 1) / T) & ~(size_t)63;
+        std::memcpy(static_cast<char*>(dst) + a, static_cast<const char*>(src) + a, b - a);
+    });
 }
 
 // ------------------------------------------------------------------------

@@ -31,6 +31,11 @@ void parallel_tasks(int n, const std::function<void(int)>& f);
 // Number of tasks worth splitting `items` independent items into.
 int task_count(uint64_t items);
 
+// memcpy of `bytes` split over the task pool (pieces of at least 256 KiB);
+// `side` uses a separate small pool, for copies that run while the calling
+// thread's own work keeps the main pool busy.
+void parallel_copy(void* dst, const void* src, size_t bytes, bool side = false);
+
 // get_block_lefts_by_chr (src/l2/window.rs:32-90).  Returns any_full_chr_window.
 bool block_lefts(const ldsc_snp* snps, uint64_t m, int mode, double value,
                  std::vector<uint64_t>& bl);

@@ -274,10 +274,13 @@ class Panel:
 # Compute
 # ----------------------------------------------------------------------------
 def compute_ldscore(panel: Panel, snps: np.ndarray, config: L2Config, annot=None,
-                    on_progress: Optional[Callable[[L2Progress], None]] = None):
+                    on_progress: Optional[Callable[[L2Progress], None]] = None, out=None):
     """compute_ldscore_global (src/l2/compute.rs:1009-2447) on the device.
 
     ``snps`` is an array of ``SNP_DTYPE``; ``annot`` an optional m x K matrix.
+    ``out`` (optional) is an m x K float64 array in column-major order that
+    receives the LD scores, e.g. the result of an earlier call reused by a
+    loop: large results then land in memory that is already mapped.
     Returns (l2 [m x K], maf [m], L2Perf).
     """
     L = N.lib()
@@ -293,7 +296,14 @@ def compute_ldscore(panel: Panel, snps: np.ndarray, config: L2Config, annot=None
             raise N.LdscError(5, f"Annotation matrix has {a.shape[0]} rows but BIM has {m} SNPs")
         K = a.shape[1]
         a = np.asfortranarray(a)
-    l2 = np.zeros((K, m), dtype=np.float64)
+    if out is None:
+        l2 = np.zeros((K, m), dtype=np.float64)
+    else:
+        if (not isinstance(out, np.ndarray) or out.dtype != np.float64 or out.shape != (m, K)
+                or not out.T.flags.c_contiguous or not out.flags.writeable):
+            raise ValueError(f"out must be a writeable column-major float64 array of shape "
+                             f"({m}, {K})")
+        l2 = out.T
     maf = np.zeros(m, dtype=np.float64)
     perf = N.Perf()
     err = N.errbuf()
@@ -317,7 +327,7 @@ def compute_ldscore(panel: Panel, snps: np.ndarray, config: L2Config, annot=None
               f"norm={perf.norm_secs:.3f}s bb_dot={perf.bb_dot_secs:.3f}s "
               f"ab_dot={perf.ab_dot_secs:.3f}s ring_store={perf.ring_store_secs:.3f}s",
               file=sys.stderr)
-    return l2.T, maf, L2Perf.from_c(perf, bool(config.sketch))
+    return (l2.T if out is None else out), maf, L2Perf.from_c(perf, bool(config.sketch))
 
 
 def compute_l2_from_bed_with_progress(panel: Panel, snps: Sequence[BimRecord], n_indiv: int,

@@ -117,6 +117,27 @@ def test_l2_many_annotations(oracle, K):
             assert np.array_equal(maf, emaf)
 
 
+def test_l2_large_annotation_matrix(oracle):
+    """A 36 MB annotation matrix and result: both cross the host copies'
+    pinned ring several times, with a ragged last piece; repeated calls
+    reuse the ring."""
+    n, m, K = 100, 100_003, 45
+    p = _panel(n, m, seed=7)
+    annot = np.random.default_rng(1).random((m, K))
+    snps = L.make_snp_array(p.chr, p.cm, p.bp)
+    exp, _ = oracle.l2(p.bed_bytes(), n, p.chr, p.cm, p.bp, mode=2, window=10, chunk=50,
+                       annot=annot)
+    cfg = L.L2Config(mode=L.WindowMode.Snp(10), chunk_size=50)
+    with L.Panel(p.bed_bytes(), n, m) as panel:
+        got, _, _ = L.compute_ldscore(panel, snps, cfg, annot=annot)
+        dest = np.full((K, m), np.nan).T
+        again, _, _ = L.compute_ldscore(panel, snps, cfg, annot=np.asfortranarray(annot), out=dest)
+        with pytest.raises(ValueError):
+            L.compute_ldscore(panel, snps, cfg, annot=annot, out=np.empty((m, K)))
+    assert _rel(got, exp) < TOL_EXACT
+    assert again is dest and np.array_equal(got, again)
+
+
 def test_l2_extract_per_chromosome(oracle):
     n, m = 400, 1200
     p = _panel(n, m, seed=31, nchr=3)
