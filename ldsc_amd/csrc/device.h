@@ -88,6 +88,10 @@ struct Workspace {
     static constexpr int kRingSlots = 4;
     PinnedBuf ring;
     hipEvent_t ring_ev[kRingSlots] = {};
+    // a large result is reduced in up to kRedGroups groups of annotations,
+    // each copied back as soon as it is done
+    static constexpr uint32_t kRedGroups = 8;
+    hipEvent_t red_ev[kRedGroups] = {};
     HostScratch host;
     size_t bytes() const;
     hipError_t init(int dev);

@@ -22,6 +22,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <functional>
 #include <thread>
 
 #include "band_common.h"
@@ -105,6 +106,10 @@ hipError_t Workspace::init(int dev) {
         e = hipEventCreateWithFlags(&x, hipEventDisableTiming);
         if (e != hipSuccess) return e;
     }
+    for (auto& x : red_ev) {
+        e = hipEventCreateWithFlags(&x, hipEventDisableTiming);
+        if (e != hipSuccess) return e;
+    }
     return hipSuccess;
 }
 
@@ -119,6 +124,8 @@ void Workspace::destroy() {
     for (auto& x : ev)
         if (x) (void)hipEventDestroy(x);
     for (auto& x : ring_ev)
+        if (x) (void)hipEventDestroy(x);
+    for (auto& x : red_ev)
         if (x) (void)hipEventDestroy(x);
     if (stream) (void)hipStreamDestroy(stream);
     if (copy_stream) (void)hipStreamDestroy(copy_stream);
@@ -155,10 +162,11 @@ static hipError_t ring_upload(Workspace* ws, void* dst, const void* src, size_t 
     return hipSuccess;
 }
 
-// Device -> host copy into caller memory through the pinned ring, ordered
-// after the work already queued on s.  Returns when every byte has landed.
+// Device -> host copy into caller memory through the pinned ring on stream
+// s; ready(end) is called before the copy of bytes [.., end) is queued (to
+// order it after their producer).  Returns when every byte has landed.
 static hipError_t ring_download(Workspace* ws, void* dst, const void* src, size_t bytes,
-                                hipStream_t s) {
+                                hipStream_t s, const std::function<hipError_t(size_t)>& ready) {
     constexpr size_t SB = Workspace::kRingSlot;
     constexpr int NS = Workspace::kRingSlots;
     hipError_t e = ws->ring.ensure(SB * NS);
@@ -168,8 +176,10 @@ static hipError_t ring_download(Workspace* ws, void* dst, const void* src, size_
     auto issue = [&](size_t c) {
         const int slot = (int)(c % NS);
         const size_t off = c * SB, len = std::min(SB, bytes - off);
-        hipError_t r = hipMemcpyAsync(ring + slot * SB, static_cast<const char*>(src) + off, len,
-                                      hipMemcpyDeviceToHost, s);
+        hipError_t r = ready(off + len);
+        if (r == hipSuccess)
+            r = hipMemcpyAsync(ring + slot * SB, static_cast<const char*>(src) + off, len,
+                               hipMemcpyDeviceToHost, s);
         return r != hipSuccess ? r : hipEventRecord(ws->ring_ev[slot], s);
     };
     for (size_t c = 0; c < nc && c < (size_t)NS; ++c)
@@ -1143,20 +1153,38 @@ __global__ __launch_bounds__(64) void k_band_f32(BandArgs a) {
 __global__ void k_reduce(const double* __restrict__ rowp, const double* __restrict__ colp,
                          const double* __restrict__ aeff, const int* __restrict__ j0,
                          const int* __restrict__ tstart, uint32_t nb, uint32_t m, uint32_t K,
-                         double* __restrict__ l2) {
+                         uint32_t k_first, uint32_t k_count, double* __restrict__ l2) {
     const uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (gid >= (uint64_t)m * K) return;
-    const uint32_t k = (uint32_t)(gid / m), j = (uint32_t)(gid % m);
+    if (gid >= (uint64_t)m * k_count) return;
+    const uint32_t k = k_first + (uint32_t)(gid / m), j = (uint32_t)(gid % m);
     const uint32_t I = j / kTile, r = j % kTile;
     double s = aeff[(size_t)j * aeff_stride(K) + k];  // self pair, r2u(j, j) = 1 (compute.rs:1658)
-    for (int J = j0[I]; J <= (int)I; ++J) {
-        const size_t t = (size_t)tstart[I] + (J - j0[I]);
-        s += rowp[(t * K + k) * kTile + r];
+    // row tiles J = j0[I] .. I, then column tiles I2 = I .. while j0[I2] <= I;
+    // loads four at a time, sums strictly in tile order
+    const double* pr = rowp + ((size_t)tstart[I] * K + k) * kTile + r;
+    const int nr = (int)I - j0[I] + 1;
+    const size_t step = (size_t)K * kTile;
+    int t = 0;
+    for (; t + 4 <= nr; t += 4) {
+        const double a0 = pr[t * step], a1 = pr[(t + 1) * step], a2 = pr[(t + 2) * step],
+                     a3 = pr[(t + 3) * step];
+        s += a0;
+        s += a1;
+        s += a2;
+        s += a3;
     }
-    for (uint32_t I2 = I; I2 < nb && (uint32_t)j0[I2] <= I; ++I2) {
-        const size_t t = (size_t)tstart[I2] + (I - j0[I2]);
-        s += colp[(t * K + k) * kTile + r];
+    for (; t < nr; ++t) s += pr[t * step];
+    uint32_t I2 = I;
+    for (; I2 + 4 <= nb && (uint32_t)j0[I2 + 3] <= I; I2 += 4) {  // j0 is nondecreasing
+        double a[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            a[u] = colp[(((size_t)tstart[I2 + u] + (I - j0[I2 + u])) * K + k) * kTile + r];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) s += a[u];
     }
+    for (; I2 < nb && (uint32_t)j0[I2] <= I; ++I2)
+        s += colp[(((size_t)tstart[I2] + (I - j0[I2])) * K + k) * kTile + r];
     l2[(size_t)k * m + j] = s;
 }
 
@@ -1660,13 +1688,19 @@ int run_l2(const ldsc_panel* panel, const ldsc_snp* snps, uint64_t m, const ldsc
     HIPC(hipGetLastError());
     HIPC(hipEventRecord(ws->ev[4], st));
 
-    // 4. deterministic reduction
-    const uint64_t nred = m * K;
-    hipLaunchKernelGGL(k_reduce, dim3((unsigned)((nred + 255) / 256)), dim3(256), 0, st,
-                       ws->rowp.as<double>(), ws->colp.as<double>(), ws->aeff.as<double>(),
-                       ws->j0.as<int>(), ws->tstart.as<int>(), plan.nb, (uint32_t)m,
-                       (uint32_t)K, ws->l2.as<double>());
-    HIPC(hipGetLastError());
+    // 4. deterministic reduction, in groups of annotations when the result
+    //    goes back through the ring, so that its copy starts with the first
+    const uint32_t ngroups = stage_l2 ? 1u : (uint32_t)std::min<uint64_t>(K, Workspace::kRedGroups);
+    for (uint32_t g = 0; g < ngroups; ++g) {
+        const uint32_t k0 = (uint32_t)(K * g / ngroups), k1 = (uint32_t)(K * (g + 1) / ngroups);
+        const uint64_t nred = m * (k1 - k0);
+        hipLaunchKernelGGL(k_reduce, dim3((unsigned)((nred + 255) / 256)), dim3(256), 0, st,
+                           ws->rowp.as<double>(), ws->colp.as<double>(), ws->aeff.as<double>(),
+                           ws->j0.as<int>(), ws->tstart.as<int>(), plan.nb, (uint32_t)m,
+                           (uint32_t)K, k0, k1 - k0, ws->l2.as<double>());
+        HIPC(hipGetLastError());
+        if (!stage_l2) HIPC(hipEventRecord(ws->red_ev[g], st));
+    }
     HIPC(hipEventRecord(ws->ev[5], st));
     if (early_maf) {  // copy it out while the band kernel runs
         HIPC(hipEventSynchronize(ws->ev[7]));
@@ -1680,7 +1714,20 @@ int run_l2(const ldsc_panel* panel, const ldsc_snp* snps, uint64_t m, const ldsc
         HIPC(hipStreamSynchronize(st));
         std::memcpy(l2_out, h_l2, (size_t)m * K * 8);
     } else {
-        HIPC(ring_download(ws, l2_out, ws->l2.p, (size_t)m * K * 8, st));
+        // each piece of the result is copied once the group of annotations
+        // holding its last byte is reduced
+        uint32_t waited = 0;
+        auto ready = [&](size_t end) -> hipError_t {
+            const uint64_t klast = (end - 1) / ((uint64_t)m * 8);
+            uint32_t g = 0;
+            while ((uint64_t)K * (g + 1) / ngroups <= klast) ++g;
+            for (; waited <= g; ++waited) {
+                hipError_t e = hipStreamWaitEvent(ws->copy_stream, ws->red_ev[waited], 0);
+                if (e != hipSuccess) return e;
+            }
+            return hipSuccess;
+        };
+        HIPC(ring_download(ws, l2_out, ws->l2.p, (size_t)m * K * 8, ws->copy_stream, ready));
         HIPC(hipStreamSynchronize(st));
     }
     if (maf_out && !early_maf) {
