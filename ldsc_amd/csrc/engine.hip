@@ -439,15 +439,35 @@ __device__ __forceinline__ void transpose_bytes4(uint32_t d0, uint32_t d1, uint3
     o[3] = __builtin_amdgcn_perm(hi23, hi01, 0x07060302u);
 }
 
+// Output forms of the gather: the gathered rows (n16 dwords each), or
+// straight into the tile layouts of k_tile_stats (PLINK-code tiles, or
+// dosage-value tiles), which skips writing and re-reading the rows.
+enum GatherForm { kGatherRows = 0, kGatherCodeTiles = 1, kGatherValueTiles = 2 };
+
+// Tile dword of gathered dword q (members 16q .. 16q+15) of SNP j, in the
+// layouts k_tile_stats writes (nkb K-blocks per SNP group).
+template <int FORM>
+__device__ __forceinline__ size_t tile_dword(uint32_t j, uint32_t q, uint32_t nkb) {
+    const uint32_t g = j / kGroup, s = j % kGroup, kb = q / 8, dq = q % 8;
+    const size_t base = ((size_t)g * nkb + kb) * 64;
+    if (FORM == kGatherCodeTiles)  // row dword 2t + h -> lane s + 32h, dword t
+        return (base + s + 32 * (dq & 1)) * 4 + (dq >> 1);
+    // row dword 2t + h -> lane s + 32(t%2), dword 2(t/2) + h
+    return (base + s + 32 * ((dq >> 1) & 1)) * 4 + 2 * (dq >> 2) + (dq & 1);
+}
+
 // PF > 0: each thread keeps its PF source dwords per row of the next pass
 // in registers, loaded while the current pass gathers (src_dw <= 512 PF).
-template <int PF>
+// FORM != kGatherRows: `out` is the tile array; n_out = nkb * 8 dwords per
+// SNP are written (members past the last read as code 01, missing), and
+// padding SNPs j in [m, m_pad) are written as all missing.
+template <int PF, int FORM>
 __global__ __launch_bounds__(512) void k_gather4(const uint8_t* __restrict__ slabs, uint64_t pitch,
                                                   uint32_t src_dw,
                                                   const uint64_t* __restrict__ bed_idx,
                                                   uint64_t base, uint32_t m, uint32_t m_pad,
                                                   const int32_t* __restrict__ map_t, uint32_t n16,
-                                                  uint32_t* __restrict__ out,
+                                                  uint32_t n_out, uint32_t* __restrict__ out,
                                                   uint32_t* __restrict__ part) {
     extern __shared__ uint32_t lds[];  // 4 * src_dw words
     __shared__ uint32_t red[8][12];
@@ -505,16 +525,15 @@ __global__ __launch_bounds__(512) void k_gather4(const uint8_t* __restrict__ sla
         for (int r = 0; r < 4; ++r) cnt[r][0] = cnt[r][1] = cnt[r][2] = 0;
         int32_t sr[16];
         uint32_t q = tid;
-        if (q < n16)
 #pragma unroll
-            for (int e = 0; e < 16; ++e) sr[e] = map_t[(size_t)e * n16 + q];
-        for (; q < n16; q += 512) {
+        for (int e = 0; e < 16; ++e) sr[e] = q < n16 ? map_t[(size_t)e * n16 + q] : -1;
+        for (; q < n_out; q += 512) {
             int32_t cur[16];
 #pragma unroll
             for (int e = 0; e < 16; ++e) cur[e] = sr[e];
-            if (q + 512 < n16)
 #pragma unroll
-                for (int e = 0; e < 16; ++e) sr[e] = map_t[(size_t)e * n16 + q + 512];
+            for (int e = 0; e < 16; ++e)
+                sr[e] = q + 512 < n16 ? map_t[(size_t)e * n16 + q + 512] : -1;
             uint32_t acc[4] = {0u, 0u, 0u, 0u};
 #pragma unroll
             for (int e = 0; e < 16; ++e) {
@@ -531,7 +550,13 @@ __global__ __launch_bounds__(512) void k_gather4(const uint8_t* __restrict__ sla
                 cnt[r][0] += __popc(~(L | H) & 0x55555555u);  // 00: dosage 2
                 cnt[r][1] += __popc(~L & H);                   // 10: dosage 1
                 cnt[r][2] += __popc(L & H);                    // 11: dosage 0
-                if (valid[r]) out[(size_t)(j0 + r) * n16 + q] = o[r];
+                if (FORM == kGatherRows) {
+                    if (valid[r]) out[(size_t)(j0 + r) * n16 + q] = o[r];
+                } else {
+                    const uint32_t w = valid[r] ? o[r] : 0x55555555u;  // padding SNP: missing
+                    out[tile_dword<FORM>(j0 + r, q, n_out / 8)] =
+                        FORM == kGatherCodeTiles ? transpose_cells(w) : value_codes(w);
+                }
             }
         }
         // block sums of the 4 x 3 counts: waves first, then across waves
@@ -1305,33 +1330,44 @@ int gather_and_tile(const ldsc_panel* panel, const uint64_t* d_bed_idx, uint64_t
     const uint64_t* rows = d_bed_idx;
     const uint32_t src_dw = (uint32_t)((panel->bps + 3) / 4);
     if (d_map && (size_t)src_dw * 16 <= 160 * 1024) {
-        // gather four rows at a time through byte-interleaved LDS
+        // gather four rows at a time through byte-interleaved LDS, into the
+        // tiles directly when tiles are wanted; stats go to slot 0
         const uint32_t n16 = (uint32_t)((n + 15) / 16);
-        HIPC(ws->scratch.ensure((size_t)m * n16 * 4));
+        const int form = !tile ? kGatherRows : vcode ? kGatherValueTiles : kGatherCodeTiles;
+        uint32_t n_out = n16;
+        uint32_t* a_out;
+        if (form == kGatherRows) {
+            HIPC(ws->scratch.ensure((size_t)m * n16 * 4));
+            a_out = ws->scratch.as<uint32_t>();
+        } else {
+            HIPC(ws->tiled.ensure((size_t)m_pad * n_pad / 4));
+            a_out = ws->tiled.as<uint32_t>();
+            n_out = nkb * 8;
+        }
         const size_t lds = (size_t)src_dw * 16;
-        const void* kfn = src_dw <= 2048 ? (const void*)k_gather4<4>
-                        : src_dw <= 4096 ? (const void*)k_gather4<8>
-                        : src_dw <= 8192 ? (const void*)k_gather4<16> : (const void*)k_gather4<0>;
+        using GatherFn = void (*)(const uint8_t*, uint64_t, uint32_t, const uint64_t*, uint64_t,
+                                  uint32_t, uint32_t, const int32_t*, uint32_t, uint32_t,
+                                  uint32_t*, uint32_t*);
+        static const GatherFn fns[4][3] = {
+            {k_gather4<4, 0>, k_gather4<4, 1>, k_gather4<4, 2>},
+            {k_gather4<8, 0>, k_gather4<8, 1>, k_gather4<8, 2>},
+            {k_gather4<16, 0>, k_gather4<16, 1>, k_gather4<16, 2>},
+            {k_gather4<0, 0>, k_gather4<0, 1>, k_gather4<0, 2>}};
+        const int pf = src_dw <= 2048 ? 0 : src_dw <= 4096 ? 1 : src_dw <= 8192 ? 2 : 3;
+        const void* kfn = (const void*)fns[pf][form];
         if (lds > 64 * 1024)
             HIPC(hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
         const uint32_t grid = (uint32_t)std::min<uint64_t>((m_pad + 3) / 4, 2048);
         const uint32_t a_m = (uint32_t)m, a_mp = (uint32_t)m_pad;
-        uint32_t* a_out = ws->scratch.as<uint32_t>();
         uint32_t* a_part = ws->stats.as<uint32_t>() + (size_t)m_pad * 4;
         const uint8_t* a_sl = panel->d_slabs;
         const uint64_t a_pitch = panel->stride;
         void* kargs[] = {(void*)&a_sl, (void*)&a_pitch, (void*)&src_dw, (void*)&d_bed_idx,
                          (void*)&base, (void*)&a_m, (void*)&a_mp, (void*)&d_map, (void*)&n16,
-                         (void*)&a_out, (void*)&a_part};
+                         (void*)&n_out, (void*)&a_out, (void*)&a_part};
         HIPC(hipLaunchKernel(kfn, dim3(grid), dim3(512), kargs, lds, st));
-        if (!tile) {  // the caller reads the gathered rows; stats are in slot 0
-            *parts = 1;
-            return LDSC_OK;
-        }
-        src = ws->scratch.as<uint8_t>();
-        pitch = 4ull * n16;
-        rows = nullptr;
-        base = 0;
+        *parts = 1;
+        return LDSC_OK;
     } else if (d_map) {
         // gather rows in the requested individual order, then tile them
         const uint32_t n16 = (uint32_t)((n + 15) / 16);

@@ -117,6 +117,8 @@ int sketch_plan(const ldsc_panel* panel_c, uint64_t d, SketchPlan** out, char* e
 // Bucket blocks per wave of k_sketch: the waves of a SNP pair-group split the
 // d/32 bucket blocks so that the whole chip works at small d too.
 constexpr uint32_t kSkChunk = 4;
+// K-steps whose operands one k_sketch wave has in flight together.
+constexpr int kSkAhead = 8;
 
 static inline uint32_t sketch_chunks(const SketchPlan* sp) {
     return (uint32_t)((sp->nblocks + kSkChunk - 1) / kSkChunk);
@@ -157,32 +159,46 @@ __global__ __launch_bounds__(64) void k_sketch(const uint32_t* __restrict__ tile
         v16i sg0, sm0, sg1, sm1;
 #pragma unroll
         for (int r = 0; r < 16; ++r) sg0[r] = sm0[r] = sg1[r] = sm1[r] = 0;
-        uint32_t w0 = 0, w1 = 0;
-        v4i P = {0, 0, 0, 0};
-        if (blk.z > 0) {
-            const uint32_t ks = (uint32_t)blk.y;
-            const size_t off = (size_t)(ks >> 2) * 256 + (ks & 3);
-            w0 = w0p[off];
-            w1 = w1p[off];
-            P = ptiles[(size_t)blk.x * 64 + lane];
-        }
-        for (int s = 0; s < blk.z; ++s) {
-            const uint32_t cw0 = w0, cw1 = w1;
-            const v4i cP = P;
-            if (s + 1 < blk.z) {  // next K-step's operands in flight
-                const uint32_t ks = (uint32_t)blk.y + s + 1;
-                const size_t off = (size_t)(ks >> 2) * 256 + (ks & 3);
-                w0 = w0p[off];
-                w1 = w1p[off];
-                P = ptiles[(size_t)(blk.x + s + 1) * 64 + lane];
+        // K-steps in groups of kSkAhead, the next group's operands in flight;
+        // steps past the block's last get P = 0 (they add nothing)
+        uint32_t w0[kSkAhead], w1[kSkAhead];
+        v4i P[kSkAhead];
+        auto fetch = [&](int s0) {
+#pragma unroll
+            for (int u = 0; u < kSkAhead; ++u) {
+                if (s0 + u < blk.z) {
+                    const uint32_t ks = (uint32_t)blk.y + s0 + u;
+                    const size_t off = (size_t)(ks >> 2) * 256 + (ks & 3);
+                    w0[u] = w0p[off];
+                    w1[u] = w1p[off];
+                    P[u] = ptiles[(size_t)(blk.x + s0 + u) * 64 + lane];
+                } else {
+                    w0[u] = w1[u] = 0u;
+                    P[u] = v4i{0, 0, 0, 0};
+                }
             }
-            v4i G0, M0, G1, M1;
-            expand_codes(cw0, G0, M0);
-            expand_codes(cw1, G1, M1);
-            sg0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(cP, G0, sg0, 0, 0, 0);
-            sm0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(cP, M0, sm0, 0, 0, 0);
-            sg1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(cP, G1, sg1, 0, 0, 0);
-            sm1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(cP, M1, sm1, 0, 0, 0);
+        };
+        fetch(0);
+        for (int s0 = 0; s0 < blk.z; s0 += kSkAhead) {
+            uint32_t cw0[kSkAhead], cw1[kSkAhead];
+            v4i cP[kSkAhead];
+#pragma unroll
+            for (int u = 0; u < kSkAhead; ++u) {
+                cw0[u] = w0[u];
+                cw1[u] = w1[u];
+                cP[u] = P[u];
+            }
+            if (s0 + kSkAhead < blk.z) fetch(s0 + kSkAhead);
+#pragma unroll
+            for (int u = 0; u < kSkAhead; ++u) {
+                v4i G0, M0, G1, M1;
+                expand_codes(cw0[u], G0, M0);
+                expand_codes(cw1[u], G1, M1);
+                sg0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(cP[u], G0, sg0, 0, 0, 0);
+                sm0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(cP[u], M0, sm0, 0, 0, 0);
+                sg1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(cP[u], G1, sg1, 0, 0, 0);
+                sm1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(cP[u], M1, sm1, 0, 0, 0);
+            }
         }
         if (RAW) {
 #pragma unroll
