@@ -105,11 +105,19 @@ void release_workspace(Workspace* ws);
 
 namespace ldsc {
 // Device-side CountSketch plan for one sketch dimension d (see sketch.hip).
+// Members are put in bucket order within chunks of `chunk` members (a
+// multiple of the 128-member K-block): without --keep, chunks of
+// kSkMembers, so that a chunk's members come from one short window of each
+// .bed row; with --keep, one chunk of all members.
+constexpr uint32_t kSkMembers = 16384;
 struct SketchPlan {
-    uint64_t d = 0, d_pad = 0, nblocks = 0, ntiles = 0;
-    int32_t* d_fam = nullptr;   // FAM row of each member (bucket order), lane-major map
-    void* d_ptiles = nullptr;   // ntiles x 1 KiB signed int8 MFMA A-operand tiles
-    int4* d_blocks = nullptr;   // per bucket block: first tile, first K-step, K-steps
+    uint64_t d = 0, d_pad = 0, nblocks = 0, ntiles = 0, chunk = 0, nchunks = 0;
+    int32_t* d_fam = nullptr;     // one chunk: FAM row of each position, lane-major map
+    uint16_t* d_local = nullptr;  // chunks: member of each position within its chunk
+                                  // (0xFFFF past the last member), nkb * 128 entries
+    void* d_ptiles = nullptr;     // ntiles x 1 KiB signed int8 MFMA A-operand tiles
+    int4* d_blocks = nullptr;     // per bucket block: first tile, number of tiles
+    int32_t* d_tile_ks = nullptr; // K-step (32 positions) of each tile
     void release();
 };
 }  // namespace ldsc
@@ -142,6 +150,13 @@ namespace ldsc {
 int gather_and_tile(const ldsc_panel* panel, const uint64_t* d_bed_idx, uint64_t base,
                     uint64_t m, const int32_t* d_map, uint64_t m_pad, bool vcode, bool tile,
                     Workspace* ws, uint32_t* parts, char* err, uint64_t err_len);
+
+// CountSketch plans with chunk-local bucket order (SketchPlan::d_local):
+// tiles the call's SNP rows in that order straight from their .bed windows
+// into ws->tiled; one integer-stat slot per chunk (*parts = chunks).
+int gather_tile_chunks(const ldsc_panel* panel, const uint64_t* d_bed_idx, uint64_t base,
+                       uint64_t m, const uint16_t* d_local, uint64_t m_pad, Workspace* ws,
+                       uint32_t* parts, char* err, uint64_t err_len);
 
 // The exact / sketch engine on workspace `ws` (which must belong to
 // panel->device).  Synchronous: outputs are written when it returns.

@@ -540,7 +540,12 @@ __global__ __launch_bounds__(512) void k_gather4(const uint8_t* __restrict__ sla
                 const int32_t s = cur[e];
                 uint32_t c4 = 0x01010101u;  // missing in all four rows
                 if (s >= 0) c4 = (lds[s >> 2] >> (2 * (s & 3))) & 0x03030303u;
-                acc[e >> 2] |= c4 << (2 * (e & 3));
+                // code tiles: member e = 4f + k straight to the tiles' cell
+                // order, bits 8k + 2f, through acc[k] bits 2f
+                if (FORM == kGatherCodeTiles)
+                    acc[e & 3] |= c4 << (2 * (e >> 2));
+                else
+                    acc[e >> 2] |= c4 << (2 * (e & 3));
             }
             uint32_t o[4];
             transpose_bytes4(acc[0], acc[1], acc[2], acc[3], o);  // o[r] = row r's dword
@@ -555,7 +560,7 @@ __global__ __launch_bounds__(512) void k_gather4(const uint8_t* __restrict__ sla
                 } else {
                     const uint32_t w = valid[r] ? o[r] : 0x55555555u;  // padding SNP: missing
                     out[tile_dword<FORM>(j0 + r, q, n_out / 8)] =
-                        FORM == kGatherCodeTiles ? transpose_cells(w) : value_codes(w);
+                        FORM == kGatherCodeTiles ? w : value_codes(w);
                 }
             }
         }
@@ -585,6 +590,140 @@ __global__ __launch_bounds__(512) void k_gather4(const uint8_t* __restrict__ sla
             const bool v = j0 + tid < m;  // padding rows: nothing
             reinterpret_cast<uint4*>(part)[j0 + tid] =
                 v ? make_uint4(2 * n2 + n1, n2 + n1 + n0, 4 * n2 + n1, n1) : make_uint4(0, 0, 0, 0);
+        }
+    }
+}
+
+// ------------------------------------------------------------------------
+// Kernel 1d: chunk-local gather into PLINK-code tiles (CountSketch order)
+// ------------------------------------------------------------------------
+// Workgroup (h, c) owns SNP rows 16h .. 16h+15 and member chunk c, whose
+// members come from FAM rows c*kSkMembers .. : one 4 KiB window of each
+// .bed row.  Wave w loads rows 16h + 4w .. +3 of the window (1 KiB of one
+// row per load), counts their integer stats (the chunk's slot) and stages
+// them byte-interleaved in LDS (word k of quad w = byte k of its four rows).
+// Then thread t builds output dwords t + 256u of the chunk (16 positions
+// each, local[] gives each position's member within the chunk) for the
+// four quads, and stores them into the tiles; positions past the last
+// member and padding SNPs read as code 01 (missing).
+constexpr int kGcRows = 16;
+constexpr uint32_t kGcWords = kSkMembers / 4;  // LDS words per quad
+constexpr uint32_t kGcStride = kGcWords + 4;   // quad stride: word kGcWords reads as
+                                               // code 01 (missing) in every byte
+
+__global__ __launch_bounds__(256) void k_gather_chunk(const uint8_t* __restrict__ slabs,
+                                                       uint64_t pitch, uint32_t src_dw, uint32_t n,
+                                                       const uint64_t* __restrict__ bed_idx,
+                                                       uint64_t base, uint32_t m, uint32_t m_pad,
+                                                       const uint16_t* __restrict__ local,
+                                                       uint32_t nkb, uint32_t* __restrict__ tiled,
+                                                       uint32_t* __restrict__ part) {
+    extern __shared__ uint32_t lds[];  // 4 quads x kGcStride
+    const int tid = threadIdx.x, wv = tid >> 6, ln = tid & 63;
+    const uint32_t j0 = blockIdx.x * kGcRows, c = blockIdx.y;
+    const uint32_t cw = kSkMembers / 16;  // window dwords per row
+    if (ln == 0) lds[wv * kGcStride + kGcWords] = 0x55555555u;
+    // load: wave wv, rows j0 + 4wv + r
+    {
+        const uint32_t* rp[4];
+        uint32_t nv[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const uint32_t j = j0 + 4 * wv + r;
+            const uint64_t row = j < m ? (bed_idx ? bed_idx[j] : base + j) : 0;
+            rp[r] = reinterpret_cast<const uint32_t*>(slabs + row * pitch);
+            nv[r] = j < m ? n : 0;
+        }
+        uint32_t cnt[4][3] = {};
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const uint32_t k0 = i * 256 + 4 * ln;   // window dword of the lane's first word
+            const uint64_t q0 = (uint64_t)c * cw + k0;
+            uint4 v[4];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                // rows are padded to 16 bytes: a 16-byte load starting below
+                // src_dw stays inside the row
+                v[r] = q0 < src_dw ? *reinterpret_cast<const uint4*>(rp[r] + q0)
+                                   : make_uint4(0u, 0u, 0u, 0u);
+            }
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                uint32_t w[4];
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const uint32_t raw = t == 0 ? v[r].x : t == 1 ? v[r].y : t == 2 ? v[r].z : v[r].w;
+                    w[r] = ts_mask(raw, q0 + t, nv[r]);
+                    const uint32_t L = w[r] & 0x55555555u, H = (w[r] >> 1) & 0x55555555u;
+                    cnt[r][0] += __popc(~(L | H) & 0x55555555u);  // 00: dosage 2
+                    cnt[r][1] += __popc(~L & H);                   // 10: dosage 1
+                    cnt[r][2] += __popc(L & H);                    // 11: dosage 0
+                }
+                uint32_t o[4];
+                transpose_bytes4(w[0], w[1], w[2], w[3], o);
+                reinterpret_cast<uint4*>(lds + wv * kGcStride)[k0 + t] =
+                    make_uint4(o[0], o[1], o[2], o[3]);
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                uint32_t x = cnt[r][k];
+#pragma unroll
+                for (int o = 32; o >= 1; o >>= 1) x += __shfl_xor(x, o);
+                cnt[r][k] = x;
+            }
+        if (ln < 4) {
+            const uint32_t j = j0 + 4 * wv + ln;
+            uint32_t n2 = 0, n1 = 0, n0 = 0;
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+                if (r == ln) {
+                    n2 = cnt[r][0];
+                    n1 = cnt[r][1];
+                    n0 = cnt[r][2];
+                }
+            if (j < m_pad)
+                reinterpret_cast<uint4*>(part)[(size_t)c * m_pad + j] =
+                    make_uint4(2 * n2 + n1, n2 + n1 + n0, 4 * n2 + n1, n1);
+        }
+    }
+    __syncthreads();
+    // gather: output dwords ql = tid + 256u of the chunk.  Member e = 4f + k
+    // of an output dword goes to bits 8k + 2f (the tiles' cell order) by
+    // placing its four-row code word in acc[k] at bits 2f before the byte
+    // transpose.
+    uint32_t* tbase = tiled + ((size_t)(j0 / kGroup) * nkb * 64 + (j0 % kGroup)) * 4;
+#pragma unroll 1
+    for (int u = 0; u < 4; ++u) {
+        const uint32_t ql = tid + 256 * u, q = c * cw + ql;
+        if (q >= nkb * 8) break;
+        const uint4* lp = reinterpret_cast<const uint4*>(local + (size_t)q * 16);
+        const uint4 e0 = lp[0], e1 = lp[1];
+        const uint32_t ev[8] = {e0.x, e0.y, e0.z, e0.w, e1.x, e1.y, e1.z, e1.w};
+        uint32_t widx[16], shv[16];
+#pragma unroll
+        for (int e = 0; e < 16; ++e) {
+            const uint32_t src = (ev[e >> 1] >> (16 * (e & 1))) & 0xFFFFu;
+            widx[e] = src == 0xFFFFu ? kGcWords : src >> 2;
+            shv[e] = 2 * (src & 3);
+        }
+        // row dword 2t + h of K-block kb -> lane s + 32h, dword t
+        const uint32_t kb = q / 8, dq = q % 8;
+        uint32_t* ob = tbase + ((size_t)kb * 64 + 32 * (dq & 1)) * 4 + (dq >> 1);
+#pragma unroll
+        for (int Q = 0; Q < 4; ++Q) {
+            const uint32_t* quad = lds + Q * kGcStride;
+            uint32_t acc[4] = {0u, 0u, 0u, 0u};
+#pragma unroll
+            for (int e = 0; e < 16; ++e)
+                acc[e & 3] |= ((quad[widx[e]] >> shv[e]) & 0x03030303u) << (2 * (e >> 2));
+            uint32_t o[4];
+            transpose_bytes4(acc[0], acc[1], acc[2], acc[3], o);  // o[r] = row 4Q + r
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+                ob[(4 * Q + r) * 4] = j0 + 4 * Q + r < m ? o[r] : 0x55555555u;
         }
     }
 }
@@ -1408,6 +1547,28 @@ int gather_and_tile(const ldsc_panel* panel, const uint64_t* d_bed_idx, uint64_t
     return LDSC_OK;
 }
 
+int gather_tile_chunks(const ldsc_panel* panel, const uint64_t* d_bed_idx, uint64_t base,
+                       uint64_t m, const uint16_t* d_local, uint64_t m_pad, Workspace* ws,
+                       uint32_t* parts, char* err, uint64_t err_len) {
+    const uint64_t n = panel->n;
+    const uint64_t n_pad = round_up(n, kKBlock);
+    const uint32_t nkb = (uint32_t)(n_pad / kKBlock);
+    const uint32_t nchunks = (uint32_t)((n + kSkMembers - 1) / kSkMembers);
+    hipStream_t st = ws->stream;
+    HIPC(ws->stats.ensure((size_t)m_pad * 16 * (1 + nchunks)));
+    HIPC(ws->tiled.ensure((size_t)m_pad * n_pad / 4));
+    *parts = nchunks;
+    const size_t lds = (size_t)4 * kGcStride * 4;
+    HIPC(hipFuncSetAttribute((const void*)k_gather_chunk,
+                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL(k_gather_chunk, dim3((unsigned)(m_pad / kGcRows), nchunks), dim3(256), lds,
+                       st, panel->d_slabs, panel->stride, (uint32_t)((panel->bps + 3) / 4),
+                       (uint32_t)n, d_bed_idx, base, (uint32_t)m, (uint32_t)m_pad, d_local, nkb,
+                       ws->tiled.as<uint32_t>(), ws->stats.as<uint32_t>() + (size_t)m_pad * 4);
+    HIPC(hipGetLastError());
+    return LDSC_OK;
+}
+
 int run_l2(const ldsc_panel* panel, const ldsc_snp* snps, uint64_t m, const ldsc_l2_config* cfg,
            const double* annot, uint64_t K, double* l2_out, double* maf_out, ldsc_l2_perf* perf,
            ldsc_progress_cb cb, void* cb_ctx, Workspace* ws, char* err, uint64_t err_len) {
@@ -1482,9 +1643,14 @@ int run_l2(const ldsc_panel* panel, const ldsc_snp* snps, uint64_t m, const ldsc
     // exact form: e2m1 MFMA while f32 holds the products exactly, int8 beyond
     const bool use_fp4 = !d && cfg->precision == LDSC_PREC_EXACT && n <= kFp4MaxIndiv;
     uint32_t parts = 1;
-    rc = gather_and_tile(panel, contiguous ? nullptr : ws->bed_idx.as<uint64_t>(),
-                         contiguous ? base : 0, m, d ? sp->d_fam : panel->d_keep32, m_pad, use_fp4,
-                         /*tile=*/true, ws, &parts, err, err_len);
+    if (d && sp->d_local)
+        rc = gather_tile_chunks(panel, contiguous ? nullptr : ws->bed_idx.as<uint64_t>(),
+                                contiguous ? base : 0, m, sp->d_local, m_pad, ws, &parts, err,
+                                err_len);
+    else
+        rc = gather_and_tile(panel, contiguous ? nullptr : ws->bed_idx.as<uint64_t>(),
+                             contiguous ? base : 0, m, d ? sp->d_fam : panel->d_keep32, m_pad,
+                             use_fp4, /*tile=*/true, ws, &parts, err, err_len);
     if (rc) return rc;
     // MAF on the device for the exact and FP32 forms; the sketch form's
     // rounding depends on chunk contiguity and is finished on the host

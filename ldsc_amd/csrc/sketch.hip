@@ -31,15 +31,19 @@ namespace ldsc {
 
 void SketchPlan::release() {
     if (d_fam) (void)hipFree(d_fam);
+    if (d_local) (void)hipFree(d_local);
     if (d_ptiles) (void)hipFree(d_ptiles);
     if (d_blocks) (void)hipFree(d_blocks);
+    if (d_tile_ks) (void)hipFree(d_tile_ks);
     d_fam = nullptr;
+    d_local = nullptr;
     d_ptiles = nullptr;
     d_blocks = nullptr;
+    d_tile_ks = nullptr;
 }
 
 // Builds (once per panel and d) the bucket order of individuals and the
-// int8 P tiles.  Host work is O(N + d).
+// int8 P tiles.  Host work is O(N + chunks * d).
 int sketch_plan(const ldsc_panel* panel_c, uint64_t d, SketchPlan** out, char* err,
                 uint64_t err_len) {
     ldsc_panel* panel = const_cast<ldsc_panel*>(panel_c);
@@ -53,51 +57,83 @@ int sketch_plan(const ldsc_panel* panel_c, uint64_t d, SketchPlan** out, char* e
     std::vector<uint32_t> bucket(n);
     std::vector<int8_t> sign(n);
     countsketch_hash(n, d, 42, bucket.data(), sign.data());  // CountSketchProj::new(n, d, 42)
-    std::vector<uint64_t> off(d + 1, 0);
-    for (uint64_t i = 0; i < n; ++i) off[bucket[i] + 1]++;
-    for (uint64_t b = 0; b < d; ++b) off[b + 1] += off[b];
-    std::vector<uint64_t> fill(off.begin(), off.end() - 1);
-    std::vector<uint32_t> perm(n);
-    for (uint64_t i = 0; i < n; ++i) perm[fill[bucket[i]]++] = (uint32_t)i;
-    std::vector<int32_t> fam_rows(n);
-    for (uint64_t k = 0; k < n; ++k)
-        fam_rows[k] = panel->keep.empty() ? (int32_t)perm[k] : (int32_t)panel->keep[perm[k]];
-    const std::vector<int32_t> fam = lane_major_map(fam_rows);
-
     auto* sp = new SketchPlan();
     sp->d = d;
     sp->d_pad = round_up(d, 32);
     sp->nblocks = sp->d_pad / 32;
-    std::vector<int4> blocks(sp->nblocks);
-    uint64_t ntiles = 0;
-    for (uint64_t B = 0; B < sp->nblocks; ++B) {
-        const uint64_t lo = off[std::min<uint64_t>(32 * B, d)];
-        const uint64_t hi = off[std::min<uint64_t>(32 * B + 32, d)];
-        const uint64_t ks0 = lo / 32, ks1 = hi > lo ? (hi + 31) / 32 : ks0;
-        blocks[B] = make_int4((int)ntiles, (int)ks0, (int)(ks1 - ks0), 0);
-        ntiles += ks1 - ks0;
+    const bool local = panel->keep.empty() && n > kSkMembers;
+    sp->chunk = local ? kSkMembers : round_up(n, kKBlock);
+    sp->nchunks = (n + sp->chunk - 1) / sp->chunk;
+    // positions c*chunk .. : the chunk's members, stable-sorted by bucket;
+    // bo[c][B] = first position of bucket block B in chunk c
+    std::vector<uint32_t> perm(n);
+    std::vector<std::vector<uint64_t>> bo(sp->nchunks, std::vector<uint64_t>(sp->nblocks + 1));
+    {
+        std::vector<uint64_t> off(d + 1);
+        for (uint64_t c = 0; c < sp->nchunks; ++c) {
+            const uint64_t c0 = c * sp->chunk, c1 = std::min<uint64_t>(n, c0 + sp->chunk);
+            std::fill(off.begin(), off.end(), 0);
+            for (uint64_t i = c0; i < c1; ++i) off[bucket[i] + 1]++;
+            off[0] = c0;
+            for (uint64_t b = 0; b < d; ++b) off[b + 1] += off[b];
+            for (uint64_t B = 0; B <= sp->nblocks; ++B) bo[c][B] = off[std::min<uint64_t>(32 * B, d)];
+            for (uint64_t i = c0; i < c1; ++i) perm[off[bucket[i]]++] = (uint32_t)i;
+        }
     }
-    sp->ntiles = ntiles;
-    // A-operand tile of (bucket block B, K-step ks): lane l is bucket row
-    // 32B + l%32; its byte p is member ks*32 + (l/32)*16 + p.
-    std::vector<int8_t> tiles(std::max<uint64_t>(ntiles, 1) * 1024, 0);
+    // P tiles, in (bucket block, chunk, K-step) order
+    std::vector<int4> blocks(sp->nblocks);
+    std::vector<int32_t> tile_ks;
     for (uint64_t B = 0; B < sp->nblocks; ++B) {
-        const int4 bl = blocks[B];
-        for (int s = 0; s < bl.z; ++s) {
-            int8_t* t = tiles.data() + (size_t)(bl.x + s) * 1024;
-            const uint64_t ks = (uint64_t)bl.y + s;
+        blocks[B] = make_int4((int)tile_ks.size(), 0, 0, 0);
+        for (uint64_t c = 0; c < sp->nchunks; ++c) {
+            const uint64_t lo = bo[c][B], hi = bo[c][B + 1];
+            if (hi > lo)
+                for (uint64_t ks = lo / 32; ks < (hi + 31) / 32; ++ks) tile_ks.push_back((int32_t)ks);
+        }
+        blocks[B].y = (int)tile_ks.size() - blocks[B].x;
+    }
+    sp->ntiles = tile_ks.size();
+    // A-operand tile of (bucket block B, K-step ks): lane l is bucket row
+    // 32B + l%32; its byte p is position ks*32 + (l/32)*16 + p.
+    std::vector<int8_t> tiles(std::max<uint64_t>(sp->ntiles, 1) * 1024, 0);
+    for (uint64_t B = 0; B < sp->nblocks; ++B)
+        for (int t = blocks[B].x; t < blocks[B].x + blocks[B].y; ++t) {
+            int8_t* tp = tiles.data() + (size_t)t * 1024;
+            const uint64_t ks = (uint64_t)tile_ks[t];
             for (int lane = 0; lane < 64; ++lane) {
                 const uint64_t b = 32 * B + (lane & 31);
                 for (int p = 0; p < 16; ++p) {
                     const uint64_t k = ks * 32 + (lane >> 5) * 16 + p;
-                    if (k < n && bucket[perm[k]] == b) t[lane * 16 + p] = sign[perm[k]];
+                    if (k < n && bucket[perm[k]] == b) tp[lane * 16 + p] = sign[perm[k]];
                 }
             }
         }
+    if (tile_ks.empty()) tile_ks.push_back(0);
+    // gather maps
+    std::vector<int32_t> fam;
+    std::vector<uint16_t> loc;
+    if (local) {
+        loc.assign(round_up(n, kKBlock), 0xFFFF);
+        for (uint64_t k = 0; k < n; ++k) loc[k] = (uint16_t)(perm[k] - (k / sp->chunk) * sp->chunk);
+    } else {
+        std::vector<int32_t> fam_rows(n);
+        for (uint64_t k = 0; k < n; ++k)
+            fam_rows[k] = panel->keep.empty() ? (int32_t)perm[k] : (int32_t)panel->keep[perm[k]];
+        fam = lane_major_map(fam_rows);
     }
     hipError_t e = hipSetDevice(panel->device);
-    if (e == hipSuccess) e = hipMalloc(&sp->d_fam, fam.size() * 4);
-    if (e == hipSuccess) e = hipMemcpy(sp->d_fam, fam.data(), fam.size() * 4, hipMemcpyHostToDevice);
+    if (local) {
+        if (e == hipSuccess) e = hipMalloc(&sp->d_local, loc.size() * 2);
+        if (e == hipSuccess)
+            e = hipMemcpy(sp->d_local, loc.data(), loc.size() * 2, hipMemcpyHostToDevice);
+    } else {
+        if (e == hipSuccess) e = hipMalloc(&sp->d_fam, fam.size() * 4);
+        if (e == hipSuccess)
+            e = hipMemcpy(sp->d_fam, fam.data(), fam.size() * 4, hipMemcpyHostToDevice);
+    }
+    if (e == hipSuccess) e = hipMalloc(&sp->d_tile_ks, tile_ks.size() * 4);
+    if (e == hipSuccess)
+        e = hipMemcpy(sp->d_tile_ks, tile_ks.data(), tile_ks.size() * 4, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMalloc(&sp->d_ptiles, tiles.size());
     if (e == hipSuccess) e = hipMemcpy(sp->d_ptiles, tiles.data(), tiles.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMalloc(&sp->d_blocks, blocks.size() * sizeof(int4));
@@ -130,7 +166,8 @@ size_t sketch_norm_bytes(const SketchPlan* sp, uint64_t m_pad) {
 
 // Wave (gp, c) handles the SNP groups 2gp, 2gp+1 (64 SNPs) and bucket
 // blocks [c * kSkChunk, (c+1) * kSkChunk).  Within a block it walks the
-// block's K-steps with the next K-step's loads in flight.  RAW writes the
+// block's tiles (each with its K-step) with the next kSkAhead tiles' loads
+// in flight.  RAW writes the
 // integer bucket sums (d x m column-major); otherwise unscaled f32 bucket
 // values into the sketch tiles sk[((g * nblocks + B) * 64 + lane) * 16 + r]
 // (bucket 32B + 8(r/4) + 4(lane/32) + r%4 of SNP 32g + lane%32) and the
@@ -139,7 +176,9 @@ size_t sketch_norm_bytes(const SketchPlan* sp, uint64_t m_pad) {
 template <bool RAW>
 __global__ __launch_bounds__(64) void k_sketch(const uint32_t* __restrict__ tiled, uint32_t nkb,
                                                 const v4i* __restrict__ ptiles,
-                                                const int4* __restrict__ blocks, uint32_t nblocks,
+                                                const int4* __restrict__ blocks,
+                                                const int32_t* __restrict__ tile_ks,
+                                                uint32_t nblocks,
                                                 const float2* __restrict__ musc, uint32_t m,
                                                 uint32_t m_pad, uint32_t d,
                                                 double* __restrict__ norm, float* __restrict__ sk,
@@ -166,8 +205,8 @@ __global__ __launch_bounds__(64) void k_sketch(const uint32_t* __restrict__ tile
         auto fetch = [&](int s0) {
 #pragma unroll
             for (int u = 0; u < kSkAhead; ++u) {
-                if (s0 + u < blk.z) {
-                    const uint32_t ks = (uint32_t)blk.y + s0 + u;
+                if (s0 + u < blk.y) {
+                    const uint32_t ks = (uint32_t)tile_ks[blk.x + s0 + u];
                     const size_t off = (size_t)(ks >> 2) * 256 + (ks & 3);
                     w0[u] = w0p[off];
                     w1[u] = w1p[off];
@@ -179,7 +218,7 @@ __global__ __launch_bounds__(64) void k_sketch(const uint32_t* __restrict__ tile
             }
         };
         fetch(0);
-        for (int s0 = 0; s0 < blk.z; s0 += kSkAhead) {
+        for (int s0 = 0; s0 < blk.y; s0 += kSkAhead) {
             uint32_t cw0[kSkAhead], cw1[kSkAhead];
             v4i cP[kSkAhead];
 #pragma unroll
@@ -188,7 +227,7 @@ __global__ __launch_bounds__(64) void k_sketch(const uint32_t* __restrict__ tile
                 cw1[u] = w1[u];
                 cP[u] = P[u];
             }
-            if (s0 + kSkAhead < blk.z) fetch(s0 + kSkAhead);
+            if (s0 + kSkAhead < blk.y) fetch(s0 + kSkAhead);
 #pragma unroll
             for (int u = 0; u < kSkAhead; ++u) {
                 v4i G0, M0, G1, M1;
@@ -271,12 +310,12 @@ hipError_t launch_sketch(const SketchPlan* sp, const uint32_t* tiled, uint32_t n
     const dim3 grid(m_pad / 64, nch), block(64);
     if (sg) {
         hipLaunchKernelGGL(k_sketch<true>, grid, block, 0, st, tiled, nkb,
-                           (const v4i*)sp->d_ptiles, sp->d_blocks, nb, musc, m, m_pad,
+                           (const v4i*)sp->d_ptiles, sp->d_blocks, sp->d_tile_ks, nb, musc, m, m_pad,
                            (uint32_t)sp->d, nullptr, nullptr, sg, sm);
         return hipGetLastError();
     }
     hipLaunchKernelGGL(k_sketch<false>, grid, block, 0, st, tiled, nkb, (const v4i*)sp->d_ptiles,
-                       sp->d_blocks, nb, musc, m, m_pad, (uint32_t)sp->d, norm, sk, nullptr,
+                       sp->d_blocks, sp->d_tile_ks, nb, musc, m, m_pad, (uint32_t)sp->d, norm, sk, nullptr,
                        nullptr);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
@@ -413,8 +452,10 @@ extern "C" int ldsc_l2_sketch_buckets(const ldsc_panel* panel, const uint64_t* b
         SCHK(ws->sk_cols.ensure(m * d * 8));
         SCHK(hipMemcpyAsync(ws->bed_idx.p, bed_idx, m * 8, hipMemcpyHostToDevice, st));
         uint32_t parts = 0;
-        rc = gather_and_tile(panel, ws->bed_idx.as<uint64_t>(), 0, m, sp->d_fam, m_pad, false,
-                             /*tile=*/true, ws, &parts, err, err_len);
+        rc = sp->d_local ? gather_tile_chunks(panel, ws->bed_idx.as<uint64_t>(), 0, m,
+                                              sp->d_local, m_pad, ws, &parts, err, err_len)
+                         : gather_and_tile(panel, ws->bed_idx.as<uint64_t>(), 0, m, sp->d_fam,
+                                           m_pad, false, /*tile=*/true, ws, &parts, err, err_len);
         if (rc) break;
         int32_t* d_sg = ws->sk_cols.as<int32_t>();
         int32_t* d_sm = d_sg + m * d;

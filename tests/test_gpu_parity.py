@@ -62,6 +62,20 @@ def test_sketch_buckets_bit_exact(oracle, n, d):
     assert np.array_equal(sg, esg) and np.array_equal(sm, esm)
 
 
+@pytest.mark.parametrize("n,d", [(16385, 7), (40000, 200), (40000, 3000)])
+def test_sketch_buckets_chunked_order(oracle, n, d):
+    """More members than one 16,384-member chunk: each chunk is put in
+    bucket order on its own (the last one ragged, down to one member); the
+    integer bucket sums do not depend on that order."""
+    print(f"sketch buckets chunked n={n} d={d}", flush=True)
+    p = _panel(n, 70, seed=n + d)
+    idx = np.arange(70, dtype=np.uint64)[::-1].copy()
+    with L.Panel(p.bed_bytes(), n, p.m) as panel:
+        sg, sm = L.sketch_buckets(panel, idx, d)
+    esg, esm = oracle.sketch_int(p.bed_bytes(), n, idx, d)
+    assert np.array_equal(sg, esg) and np.array_equal(sm, esm)
+
+
 def test_sketch_buckets_with_keep(oracle):
     n, d = 1200, 97
     p = _panel(n, 40, seed=8)
@@ -177,14 +191,16 @@ def test_l2_special_columns(oracle):
         assert np.all(np.isfinite(got))
 
 
-@pytest.mark.parametrize("d,maf_aware", [(64, False), (300, False), (300, True), (1000, False)])
-def test_l2_sketch_matches_oracle(oracle, d, maf_aware):
-    n, m = 3000, 700
+@pytest.mark.parametrize("n,d,maf_aware", [(3000, 64, False), (3000, 300, False),
+                                           (3000, 300, True), (3000, 1000, False),
+                                           (20000, 500, True)])
+def test_l2_sketch_matches_oracle(oracle, n, d, maf_aware):
+    m = 700
     p = _panel(n, m, seed=d)
     snps = L.make_snp_array(p.chr, p.cm, p.bp)
     cfg = L.L2Config(mode=L.WindowMode.Cm(0.15), chunk_size=60, sketch=d,
                      sketch_maf_aware=maf_aware, snp_level_masking=True)
-    print(f"sketch d={d} maf_aware={maf_aware}", flush=True)
+    print(f"sketch n={n} d={d} maf_aware={maf_aware}", flush=True)
     with L.Panel(p.bed_bytes(), n, m) as panel:
         got, maf, perf = L.compute_ldscore(panel, snps, cfg)
     exp, emaf = oracle.l2(p.bed_bytes(), n, p.chr, p.cm, p.bp, mode=0, window=0.15, chunk=60,
