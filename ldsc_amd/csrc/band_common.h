@@ -27,6 +27,7 @@ struct BandArgs {
     double* rowp;
     double* colp;
     const float* sk;       // sketched columns (CountSketch path) or nullptr
+    const float* sk_scale; // their per-SNP renorm scales
     const double* kappa;   // per-SNP kurtosis of the MAF-aware sketch or nullptr
     uint32_t nkb, ntiles, per_xcd, m, m_pad, K;
     double r2u_a, r2u_b, n_inv_sq_r2u_a, n_inv_sq, d_f;
@@ -256,6 +257,8 @@ hipError_t launch_sketch(const SketchPlan* sp, const uint32_t* tiled, uint32_t n
                          const float2* musc, uint32_t m, uint32_t m_pad, double n, double* norm,
                          float* sk, int32_t* sg, int32_t* sm, hipStream_t st);
 size_t sketch_norm_bytes(const SketchPlan* sp, uint64_t m_pad);
+// The per-SNP renorm scales launch_sketch leaves after the norm shares.
+const float* sketch_scales(const SketchPlan* sp, const double* norm, uint64_t m_pad);
 hipError_t launch_band_sk(const BandArgs& a, hipStream_t st);
 
 }  // namespace ldsc

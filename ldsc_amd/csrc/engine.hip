@@ -1862,6 +1862,7 @@ int run_l2(const ldsc_panel* panel, const ldsc_snp* snps, uint64_t m, const ldsc
     ba.rowp = ws->rowp.as<double>();
     ba.colp = ws->colp.as<double>();
     ba.sk = d ? ws->sk_cols.as<float>() : nullptr;
+    ba.sk_scale = d ? sketch_scales(sp, ws->sk_norm.as<double>(), m_pad) : nullptr;
     ba.kappa = maf_aware ? ws->maf.as<double>() : nullptr;
     ba.nkb = d ? (uint32_t)sp->nblocks : nkb;
     ba.ntiles = plan.ntiles;

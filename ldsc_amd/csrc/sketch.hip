@@ -16,11 +16,12 @@
 //      SG[b, j] = sum_{i in b} s_i g_ij,  SM[b, j] = sum_{i in b} s_i [present].
 //  * Bucket value y_b = inv_j (SG - mean_j SM) with the reference's f32 mean
 //    and inverse std (the same real number its f32 LUT scatter accumulates),
-//    renormalized so ||y||^2 = N, stored f32 in the lane order the
-//    contraction's MFMA reads.
+//    stored f32 in the lane order the contraction's MFMA reads.  The
+//    renormalization to ||y||^2 = N is a per-SNP scale sc_j.
 //  * The d-length contraction runs on v_mfma_f32_32x32x2f32 over 64x64 SNP
-//    tiles of the band, followed by the quadratic inversion, the unbiased
-//    r^2, masking and the annotation contraction shared with the exact path.
+//    tiles of the band; each dot product is scaled by sc_i sc_j, then the
+//    quadratic inversion, the unbiased r^2, masking and the annotation
+//    contraction shared with the exact path follow.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -161,7 +162,7 @@ static inline uint32_t sketch_chunks(const SketchPlan* sp) {
 }
 
 size_t sketch_norm_bytes(const SketchPlan* sp, uint64_t m_pad) {
-    return (size_t)sketch_chunks(sp) * m_pad * 8;
+    return (size_t)sketch_chunks(sp) * m_pad * 8 + m_pad * 4;  // norm shares, scales
 }
 
 // Wave (gp, c) handles the SNP groups 2gp, 2gp+1 (64 SNPs) and bucket
@@ -172,7 +173,7 @@ size_t sketch_norm_bytes(const SketchPlan* sp, uint64_t m_pad) {
 // values into the sketch tiles sk[((g * nblocks + B) * 64 + lane) * 16 + r]
 // (bucket 32B + 8(r/4) + 4(lane/32) + r%4 of SNP 32g + lane%32) and the
 // wave's share of each SNP's squared norm into norm[c * m_pad + snp];
-// k_sketch_scale then renormalizes.
+// k_sketch_scale then forms each SNP's renorm scale.
 template <bool RAW>
 __global__ __launch_bounds__(64) void k_sketch(const uint32_t* __restrict__ tiled, uint32_t nkb,
                                                 const v4i* __restrict__ ptiles,
@@ -279,28 +280,21 @@ __global__ __launch_bounds__(64) void k_sketch(const uint32_t* __restrict__ tile
 }
 
 // Ratio-estimator renorm ||col||^2 = N (wasm_simd.rs:1774-1783): the squared
-// norm is the chunk shares added in a fixed order; thread (tile (g, B),
-// lane) scales its 16 values.
-__global__ __launch_bounds__(64) void k_sketch_scale(float* __restrict__ sk,
-                                                      const double* __restrict__ norm,
-                                                      uint32_t nchunks, uint32_t nblocks,
-                                                      uint32_t m_pad, double n_norm) {
-    const uint32_t g = blockIdx.x / nblocks, B = blockIdx.x % nblocks;
-    const int lane = threadIdx.x;
-    const uint32_t snp = 32 * g + (lane & 31);
+// norm is the chunk shares added in a fixed order, and the column's scale
+// sqrt(N / ||y||^2) (f32) is applied by the contraction to each dot
+// product (sc_i sc_j <y_i, y_j>) rather than to the stored values.
+__global__ __launch_bounds__(256) void k_sketch_scale(const double* __restrict__ norm,
+                                                       uint32_t nchunks, uint32_t m_pad,
+                                                       double n_norm, float* __restrict__ sc) {
+    const uint32_t snp = blockIdx.x * blockDim.x + threadIdx.x;
+    if (snp >= m_pad) return;
     double nrm = 0.0;
     for (uint32_t c = 0; c < nchunks; ++c) nrm += norm[(size_t)c * m_pad + snp];
-    const float sc = nrm > 0.0 ? (float)sqrt(n_norm / nrm) : 1.0f;
-    float4* t = reinterpret_cast<float4*>(sk + (((size_t)g * nblocks + B) * 64 + lane) * 16);
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        float4 v = t[q];
-        v.x *= sc;
-        v.y *= sc;
-        v.z *= sc;
-        v.w *= sc;
-        t[q] = v;
-    }
+    sc[snp] = nrm > 0.0 ? (float)sqrt(n_norm / nrm) : 1.0f;
+}
+
+const float* sketch_scales(const SketchPlan* sp, const double* norm, uint64_t m_pad) {
+    return reinterpret_cast<const float*>(norm + (size_t)sketch_chunks(sp) * m_pad);
 }
 
 hipError_t launch_sketch(const SketchPlan* sp, const uint32_t* tiled, uint32_t nkb,
@@ -319,8 +313,8 @@ hipError_t launch_sketch(const SketchPlan* sp, const uint32_t* tiled, uint32_t n
                        nullptr);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_sketch_scale, dim3((m_pad / 32) * nb), dim3(64), 0, st, sk, norm, nch,
-                       nb, m_pad, n);
+    hipLaunchKernelGGL(k_sketch_scale, dim3((m_pad + 255) / 256), dim3(256), 0, st, norm, nch,
+                       m_pad, n, const_cast<float*>(sketch_scales(sp, norm, m_pad)));
     return hipGetLastError();
 }
 
@@ -331,7 +325,7 @@ __device__ __forceinline__ float f4c(const float4& v, int c) {
 // d-length contraction of the sketched columns over one 64x64 band tile.
 __global__ __launch_bounds__(64) void k_band_sk(BandArgs a) {
     __shared__ EpiShared sh;
-    __shared__ double kr[64], kc[64];
+    __shared__ double kr[64], kc[64], scr[64], scc[64];
     const int tile = tile_of_block(a);
     if (tile < 0) return;
     const int2 tl = a.tiles[tile];
@@ -391,6 +385,8 @@ __global__ __launch_bounds__(64) void k_band_sk(BandArgs a) {
     stage_info(a, sh, I, J);
     kr[lane] = a.kappa ? a.kappa[(size_t)I * kTile + lane] : 0.0;
     kc[lane] = a.kappa ? a.kappa[(size_t)J * kTile + lane] : 0.0;
+    scr[lane] = (double)a.sk_scale[(size_t)I * kTile + lane];
+    scc[lane] = (double)a.sk_scale[(size_t)J * kTile + lane];
     __syncthreads();
     // quadratic_sketch_correction[_maf_aware] (compute.rs:947-998)
     double r2u[2][2][16];
@@ -401,7 +397,7 @@ __global__ __launch_bounds__(64) void k_band_sk(BandArgs a) {
 #pragma unroll
             for (int r = 0; r < 16; ++r) {
                 const int row = row_of(bi, r, lane), col = 32 * bj + (lane & 31);
-                const double v = (double)acc[bi][bj][r];
+                const double v = (double)acc[bi][bj][r] * scr[row] * scc[col];
                 const double t = v * v * a.n_inv_sq;
                 const double dm3 = a.d_f - 3.0 - 2.0 * (kr[row] + kc[col]);
                 const double disc = dm3 * dm3 + 8.0 * (a.d_f * t - 1.0);
