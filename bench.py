@@ -59,6 +59,32 @@ def profiled_traffic(kernel: str):
     return None
 
 
+def measured_peaks():
+    """Sustained MFMA rates (e2m1, int8, FP32; one wave per SIMD on every CU)
+    and device-to-device copy bandwidth measured on this GPU with the
+    calibration kernels of ldsc_amd/csrc/calib.hip."""
+    import ctypes as C
+
+    lib = C.CDLL(os.path.join(ROOT, "ldsc_amd", "libldsc_calib.so"))
+    dp = C.POINTER(C.c_double)
+    lib.ldsc_calib_mfma.argtypes = [C.c_int, C.c_int, C.c_int, dp, dp]
+    lib.ldsc_calib_copy.argtypes = [C.c_int, C.c_uint64, C.c_int, dp]
+    out = {}
+    for name, form, iters in (("fp4", 0, 100_000), ("i8", 1, 100_000), ("f32", 2, 20_000)):
+        rate, ghz = C.c_double(), C.c_double()
+        rc = lib.ldsc_calib_mfma(0, form, iters, C.byref(rate), C.byref(ghz))
+        if rc:
+            raise RuntimeError(f"ldsc_calib_mfma({name}) failed: hipError {rc}")
+        out[name] = {"tflops": rate.value, "clock_ghz": ghz.value}
+    gbs = C.c_double()
+    rc = lib.ldsc_calib_copy(0, 2 << 30, 5, C.byref(gbs))
+    if rc:
+        raise RuntimeError(f"ldsc_calib_copy failed: hipError {rc}")
+    out["copy_gbs"] = gbs.value
+    log(f"measured peaks: {out}")
+    return out
+
+
 def timed_steps(L, panel, snps, cfg, steps, warmup, tag, annot=None, dest=None):
     """`dest`: an output array reused by every call (a serving loop's
     buffer), or None for a fresh result array per call."""
@@ -123,6 +149,10 @@ def leg_panel(args, L, synth):
     fp4 = p.n <= 1_800_000  # the e2m1 kernel's range (f32 sums stay exact)
     kname = "k_band_fp4" if fp4 else "k_band_i8"
     peak = FP4_PEAK_TFLOPS if fp4 else INT8_PEAK_TOPS
+    tile_s = float(np.mean([q.norm_secs for q in pf8]))
+    # tiler bytes: the call's .bed rows read, the tiled slice written
+    n_pad, m_pad = -(-p.n // 128) * 128, -(-p.m // 64) * 64
+    tile_bytes = p.m * ((p.n + 3) // 4) + m_pad * n_pad // 4
     res = {
         "value": P / t_step, "ms_per_step": t_step * 1e3, "pairs": P,
         "pairs_evaluated": pf8[-1].pairs_evaluated, "tiles": pf8[-1].tiles,
@@ -140,7 +170,26 @@ def leg_panel(args, L, synth):
                              "included pair); peak = dense " +
                              ("e2m1 (FP4)" if fp4 else "int8") + " MFMA"},
         "ld_scored_snps_per_s": p.m / t_step,
+        "decode_roofline": {"bound": "hbm", "kernel": "k_tile_stats", "bytes": tile_bytes,
+                            "achieved": tile_bytes / tile_s / 1e9, "peak": HBM_PEAK_GBS,
+                            "unit": "GB/s", "frac": tile_bytes / tile_s / 1e9 / HBM_PEAK_GBS,
+                            "note": "M*ceil(N/4) .bed bytes read + M_pad*N_pad/4 tile bytes "
+                                    "written, over the tiling phase (tiler + per-SNP stats)"},
     }
+    if args.peaks:
+        pk = args.peaks
+        mp = pk["fp4"]["tflops"] if fp4 else pk["i8"]["tflops"]
+        res["roofline"]["peak_measured"] = mp
+        res["roofline"]["frac_of_measured"] = algo / k8 / 1e12 / mp
+        res["roofline"]["clock_ghz_measured"] = pk["fp4" if fp4 else "i8"]["clock_ghz"]
+        res["decode_roofline"]["peak_measured"] = pk["copy_gbs"]
+        res["decode_roofline"]["frac_of_measured"] = tile_bytes / tile_s / 1e9 / pk["copy_gbs"]
+        if fp32:
+            fp32["peak_measured"] = pk["f32"]["tflops"]
+            fp32["frac_of_measured"] = fp32["tflops_algorithmic"] / pk["f32"]["tflops"]
+        if exact_i8:
+            exact_i8["peak_measured"] = pk["i8"]["tflops"]
+            exact_i8["frac_of_measured"] = exact_i8["achieved"] / pk["i8"]["tflops"]
     if not args.skip_cpu:
         from oracle import oracle as O
 
@@ -350,6 +399,8 @@ def main() -> int:
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-serving", action="store_true")
+    ap.add_argument("--skip-peaks", action="store_true",
+                    help="do not measure the sustained MFMA and copy peaks")
     ap.add_argument("--qps", type=float, default=200.0)
     ap.add_argument("--serve-requests", type=int, default=400)
     ap.add_argument("--serve-inflight", type=int, default=4)
@@ -380,6 +431,7 @@ def main() -> int:
     import ldsc_amd as L
     from ldsc_amd import synth
 
+    args.peaks = None if args.skip_peaks else measured_peaks()
     if args.workload == "sketch":
         print(json.dumps({"metric": "CountSketch LD scores", **leg_sketch(args, L, synth)}),
               flush=True)
@@ -416,6 +468,8 @@ def main() -> int:
                       "reduce": pan["reduce_ms"], "host_prep": pan["host_prep_ms"],
                       "library_call": pan["library_ms"]},
         "roofline": pan["roofline"],
+        "decode_roofline": pan["decode_roofline"],
+        "measured_peaks": args.peaks,
         "cpu_baseline": pan.get("cpu_baseline"),
         "sample_accuracy": pan.get("sample_accuracy"),
         "serving": serving,

@@ -2,6 +2,7 @@
 
 * libldsc_l2.so   -- the HIP engine for gfx950 behind include/ldsc_l2.h
 * libldsc_synth.so -- the seeded synthetic PLINK generator (host C, OpenMP)
+* libldsc_calib.so -- measured MFMA and HBM copy peaks for bench.py's roofline
 
 Run ``python -m ldsc_amd.build`` (or ``__graft_entry__.build()``).
 """
@@ -47,9 +48,18 @@ def build_synth() -> str:
     return out
 
 
+def build_calib() -> str:
+    out = os.path.join(HERE, "libldsc_calib.so")
+    tmp = out + ".tmp"
+    _run([HIPCC, *HIP_FLAGS, "-o", tmp, os.path.join(CSRC, "calib.hip")])
+    os.replace(tmp, out)
+    return out
+
+
 def build_all() -> None:
     build_synth()
     build_engine()
+    build_calib()
 
 
 if __name__ == "__main__":
