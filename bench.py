@@ -279,7 +279,7 @@ def leg_serving(args, L, synth):
     gaps = rng.exponential(1.0 / args.qps, size=len(reqs))
     lats, tickets = [], []
     with Server(hbm_budget_bytes=96 << 30, max_inflight=args.serve_inflight,
-                queue_capacity=4096) as srv:
+                queue_capacity=4096, batch_window_us=args.serve_window_us) as srv:
         warm = [srv.submit(panels[0], snp_sets[0][:60_000], reqs[0][2])
                 for _ in range(args.serve_inflight)]
         for t in warm:
@@ -296,6 +296,37 @@ def leg_serving(args, L, synth):
         for t in tickets:
             lats.append(srv.wait(t))
         t_end = time.perf_counter()
+        poisson_stats = srv.stats()
+    # capacity: the same requests submitted at once (a closed burst), on more
+    # workers, with band batching
+    def run_burst(window_us):
+        with Server(hbm_budget_bytes=96 << 30, max_inflight=args.serve_burst_inflight,
+                    queue_capacity=4096, batch_window_us=window_us) as srv:
+            warm = [srv.submit(panels[0], snp_sets[0][:60_000], reqs[0][2])
+                    for _ in range(args.serve_burst_inflight)]
+            for t in warm:
+                srv.wait(t)
+            s0 = srv.stats()
+            tb = time.perf_counter()
+            bt = [srv.submit(panels[w], snps, cfg, annot) for (w, snps, cfg, annot, _) in reqs]
+            blats = np.array([srv.wait(t) for t in bt])
+            tb_end = time.perf_counter()
+            s1 = srv.stats()
+        out = {"requests": len(reqs), "wall_s": tb_end - tb, "qps": len(reqs) / (tb_end - tb),
+               "snp_pairs_per_s": float(sum(r[4] for r in reqs)) / (tb_end - tb),
+               "p50_ms": float(np.percentile(blats, 50)) * 1e3,
+               "p99_ms": float(np.percentile(blats, 99)) * 1e3,
+               "max_inflight": args.serve_burst_inflight, "batch_window_us": window_us,
+               "band_launches": s1["band_launches"] - s0["band_launches"],
+               "batched_requests": s1["batched_requests"] - s0["batched_requests"],
+               "max_batch": s1["max_batch"]}
+        log(f"serving burst (window {window_us} us): {out}")
+        return out
+
+    burst = burst_unbatched = None
+    if args.serve_burst_inflight > 0:
+        burst = run_burst(args.serve_window_us)
+        burst_unbatched = run_burst(0)
     for pn in panels:
         pn.close()
     lats = np.array(lats)
@@ -308,8 +339,13 @@ def leg_serving(args, L, synth):
         "mean_ms": float(lats.mean()) * 1e3,
         "snp_pairs_per_s": total_pairs / (t_end - t_start),
         "max_inflight": args.serve_inflight,
+        "batch_window_us": args.serve_window_us,
+        "band_launches": poisson_stats["band_launches"],
+        "batched_requests": poisson_stats["batched_requests"],
         "panels": "N=50000 x M=%d and N=10000 x M=%d resident" % (args.serve_m_big,
                                                                   args.serve_m_small),
+        "burst": burst,
+        "burst_one_launch_per_request": burst_unbatched,
     }
     log(f"serving: {res}")
     return res
@@ -404,6 +440,10 @@ def main() -> int:
     ap.add_argument("--qps", type=float, default=200.0)
     ap.add_argument("--serve-requests", type=int, default=400)
     ap.add_argument("--serve-inflight", type=int, default=4)
+    ap.add_argument("--serve-window-us", type=int, default=200,
+                    help="band batching window of the request server (0: one launch each)")
+    ap.add_argument("--serve-burst-inflight", type=int, default=8,
+                    help="workers of the closed-burst capacity run (0: skip it)")
     ap.add_argument("--serve-m-big", type=int, default=1_000_000)
     ap.add_argument("--serve-m-small", type=int, default=500_000)
     ap.add_argument("--sketch-m", type=int, default=200_000)

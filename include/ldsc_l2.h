@@ -214,11 +214,26 @@ typedef struct {
     uint64_t hbm_budget_bytes; /* admission budget for request scratch     */
     uint32_t max_inflight;     /* requests resident on the device at once   */
     uint32_t queue_capacity;   /* pending requests accepted before BUSY     */
+    uint32_t batch_window_us;  /* 0: one band launch per request.  Else the */
+                               /* band tiles of requests prepared together  */
+                               /* share one launch; a request waits at most */
+                               /* this long for others still being prepared */
+    uint32_t reserved;
 } ldsc_server_config;
+
+/* Counters of a server since it was created. */
+typedef struct {
+    uint64_t requests_done;
+    uint64_t band_launches;       /* band kernel launches, batched or not        */
+    uint64_t batched_requests;    /* requests whose band shared a launch         */
+    uint32_t max_batch;           /* most requests in one band launch            */
+    uint32_t reserved;
+} ldsc_server_stats;
 
 int ldsc_server_create(const ldsc_server_config* cfg, ldsc_server** out, char* err,
                        uint64_t err_len);
 int ldsc_server_destroy(ldsc_server* server);
+int ldsc_server_stats_get(ldsc_server* server, ldsc_server_stats* out);
 
 /* Enqueue one request.  The snps/annot arrays are copied; l2_out/maf_out
  * must stay valid until the request is reported done by poll/wait. */

@@ -28,7 +28,7 @@ EXPORTED = [
     "ldsc_l2_open_panel", "ldsc_l2_close_panel", "ldsc_l2_panel_n_indiv", "ldsc_l2_compute",
     "ldsc_l2_block_left", "ldsc_l2_pair_bounds", "ldsc_l2_decode", "ldsc_l2_countsketch_hash",
     "ldsc_l2_sketch_buckets", "ldsc_server_create", "ldsc_server_destroy", "ldsc_l2_submit",
-    "ldsc_l2_poll", "ldsc_l2_wait",
+    "ldsc_l2_poll", "ldsc_l2_wait", "ldsc_server_stats_get",
 ]
 
 
@@ -70,7 +70,14 @@ class Progress(C.Structure):
 
 class ServerConfig(C.Structure):
     _fields_ = [("hbm_budget_bytes", C.c_uint64), ("max_inflight", C.c_uint32),
-                ("queue_capacity", C.c_uint32)]
+                ("queue_capacity", C.c_uint32), ("batch_window_us", C.c_uint32),
+                ("reserved", C.c_uint32)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("requests_done", C.c_uint64), ("band_launches", C.c_uint64),
+                ("batched_requests", C.c_uint64), ("max_batch", C.c_uint32),
+                ("reserved", C.c_uint32)]
 
 
 PROGRESS_CB = C.CFUNCTYPE(None, C.POINTER(Progress), C.c_void_p)
@@ -105,6 +112,7 @@ def lib():
                                  C.POINTER(u64), cp, u64]
     L.ldsc_l2_poll.argtypes = [vp, u64, C.POINTER(dbl), cp, u64]
     L.ldsc_l2_wait.argtypes = [vp, u64, C.POINTER(dbl), cp, u64]
+    L.ldsc_server_stats_get.argtypes = [vp, C.POINTER(ServerStats)]
     for name in EXPORTED:
         getattr(L, name)  # every declared symbol must resolve
     _lib = L

@@ -11,28 +11,10 @@ typedef int v4i __attribute__((ext_vector_type(4)));
 typedef int v8i __attribute__((ext_vector_type(8)));
 typedef int v16i __attribute__((ext_vector_type(16)));
 typedef float v16f __attribute__((ext_vector_type(16)));
-typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
 
 // ------------------------------------------------------------------------
-// Kernel 3: banded contraction
+// Kernel 3: banded contraction (BandArgs: device.h)
 // ------------------------------------------------------------------------
-struct BandArgs {
-    const u32x4* tiled;
-    const int2* tiles;
-    const uint32_t* lo;
-    const int2* cs;
-    const double* rs;
-    const float4* lut;
-    const double* aeff;
-    double* rowp;
-    double* colp;
-    const float* sk;       // sketched columns (CountSketch path) or nullptr
-    const float* sk_scale; // their per-SNP renorm scales
-    const double* kappa;   // per-SNP kurtosis of the MAF-aware sketch or nullptr
-    uint32_t nkb, ntiles, per_xcd, m, m_pad, K;
-    double r2u_a, r2u_b, n_inv_sq_r2u_a, n_inv_sq, d_f;
-    double n_pad;          // individuals of the tiles, padding included
-};
 
 // Tile -> XCD mapping: workgroups are dealt to the 8 XCDs round-robin, so
 // workgroup b runs on XCD b%8.  Give each XCD one contiguous stretch of the
@@ -42,6 +24,33 @@ __device__ inline int tile_of_block(const BandArgs& a) {
     const uint32_t xcd = blockIdx.x & 7u, slot = blockIdx.x >> 3;
     const uint32_t t = xcd * a.per_xcd + slot;
     return t < a.ntiles ? (int)t : -1;
+}
+
+// Several calls' bands in one launch (the request server): the tiles of
+// call r are global tiles starts[r] .. starts[r+1]-1, each with the call's
+// own arguments args[r] (device memory).  The same XCD-aware stretches as
+// tile_of_block are laid over the concatenated tile list.
+struct BandBatch {
+    const BandArgs* args;
+    const uint32_t* starts;  // nreq + 1 cumulative tile counts
+    uint32_t nreq, ntiles, per_xcd;
+};
+
+__device__ inline bool batch_tile(const BandBatch& b, uint32_t& r, int& tile) {
+    const uint32_t xcd = blockIdx.x & 7u, slot = blockIdx.x >> 3;
+    const uint32_t t = xcd * b.per_xcd + slot;
+    if (t >= b.ntiles) return false;
+    uint32_t lo = 0, hi = b.nreq;  // starts[lo] <= t < starts[hi]
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (b.starts[mid] <= t)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    r = lo;
+    tile = (int)(t - b.starts[lo]);
+    return true;
 }
 
 // Expand one dword of 16 PLINK codes (device.h bit order) into int8 dosage

@@ -31,6 +31,27 @@ constexpr uint64_t kFp4MaxIndiv = 1800000;
 
 inline uint64_t round_up(uint64_t x, uint64_t a) { return (x + a - 1) / a * a; }
 
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+
+// Arguments of one call's band contraction (kernel 3, band_common.h).
+struct BandArgs {
+    const u32x4* tiled;
+    const int2* tiles;
+    const uint32_t* lo;
+    const int2* cs;
+    const double* rs;
+    const float4* lut;
+    const double* aeff;
+    double* rowp;
+    double* colp;
+    const float* sk;       // sketched columns (CountSketch path) or nullptr
+    const float* sk_scale; // their per-SNP renorm scales
+    const double* kappa;   // per-SNP kurtosis of the MAF-aware sketch or nullptr
+    uint32_t nkb, ntiles, per_xcd, m, m_pad, K;
+    double r2u_a, r2u_b, n_inv_sq_r2u_a, n_inv_sq, d_f;
+    double n_pad;          // individuals of the tiles, padding included
+};
+
 struct DevBuf {
     void* p = nullptr;
     size_t cap = 0;
@@ -164,6 +185,59 @@ int run_l2(const ldsc_panel* panel, const ldsc_snp* snps, uint64_t m, const ldsc
            const double* annot, uint64_t K, double* l2_out, double* maf_out,
            ldsc_l2_perf* perf, ldsc_progress_cb cb, void* cb_ctx, Workspace* ws, char* err,
            uint64_t err_len);
+
+// run_l2 in three phases, so that the request server can put the band
+// contractions of several calls into one launch:
+//  * l2_begin: validation, tiling, host planning, uploads; everything the
+//    band kernel reads is queued on ws->stream before ws->ev[3].  Sets
+//    call->done when nothing is left to run (m == 0).
+//  * l2_band: the band kernel of one call on ws->stream; or
+//    band_batch_launch: the bands of several calls of one form and device
+//    in one launch on stream `bst`, after each call's ev[3]; it records each
+//    call's ws->ev[4] after the launch.
+//  * l2_end: waits for ev[4] on ws->stream, reduces, copies the results
+//    back; synchronous.
+enum BandForm { kBandFp4 = 0, kBandI8 = 1, kBandF32 = 2, kBandSketch = 3 };
+struct SketchPlan;
+struct L2Call {
+    const ldsc_panel* panel = nullptr;
+    const ldsc_snp* snps = nullptr;
+    uint64_t m = 0, K = 1, d = 0;
+    const ldsc_l2_config* cfg = nullptr;
+    double* l2_out = nullptr;
+    double* maf_out = nullptr;
+    ldsc_l2_perf* perf = nullptr;
+    ldsc_progress_cb cb = nullptr;
+    void* cb_ctx = nullptr;
+    Workspace* ws = nullptr;
+    double t_start = 0, t_prep = 0;
+    uint32_t m_pad = 0;
+    int form = kBandFp4;
+    bool use_f32 = false, early_maf = false, early_stats = false, stage_l2 = false;
+    bool any_full = false, done = true;
+    size_t off_st = 0, off_l2 = 0;  // in ws->pinned_meta
+    SketchPlan* sketch_plan = nullptr;
+    BandArgs ba{};
+};
+// Argument table of one batched band launch (device copy + pinned staging),
+// reused once the launch that read it has finished.
+struct BandBatchSlot {
+    static constexpr uint32_t kMaxCalls = 64;
+    static constexpr size_t kBytes = kMaxCalls * sizeof(BandArgs) + (kMaxCalls + 1) * 4;
+    void* d_args = nullptr;
+    void* h_args = nullptr;
+    hipEvent_t done = nullptr;
+    bool used = false;
+    void release();
+};
+int l2_begin(const ldsc_panel* panel, const ldsc_snp* snps, uint64_t m, const ldsc_l2_config* cfg,
+             const double* annot, uint64_t K, double* l2_out, double* maf_out, ldsc_l2_perf* perf,
+             ldsc_progress_cb cb, void* cb_ctx, Workspace* ws, L2Call* call, char* err,
+             uint64_t err_len);
+int l2_band(L2Call* call, char* err, uint64_t err_len);
+int band_batch_launch(L2Call* const* calls, uint32_t ncalls, BandBatchSlot* slot, hipStream_t bst,
+                      char* err, uint64_t err_len);
+int l2_end(L2Call* call, char* err, uint64_t err_len);
 
 // Device bytes a call of this shape needs (admission control).
 size_t l2_workspace_bytes(const ldsc_panel* panel, uint64_t m, const ldsc_l2_config* cfg,

@@ -61,6 +61,15 @@ void DevBuf::release() {
     cap = 0;
 }
 
+void BandBatchSlot::release() {
+    if (d_args) (void)hipFree(d_args);
+    if (h_args) (void)hipHostFree(h_args);
+    if (done) (void)hipEventDestroy(done);
+    d_args = h_args = nullptr;
+    done = nullptr;
+    used = false;
+}
+
 hipError_t PinnedBuf::ensure(size_t bytes) {
     if (bytes <= cap && p) return hipSuccess;
     release();
@@ -929,10 +938,7 @@ __device__ __forceinline__ double exact_r2u(const BandArgs& a, double gg, double
 // group 2I + w) against both column groups; 4 products x 2 blocks of
 // 32x32 int32 accumulators (128 registers), so two waves fit per SIMD and
 // one wave's code expansion overlaps the other's MFMAs.
-__global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_band_i8(BandArgs a) {
-    __shared__ EpiShared2 sh;
-    const int tile = tile_of_block(a);
-    if (tile < 0) return;
+__device__ __forceinline__ void band_i8_tile(const BandArgs& a, EpiShared2& sh, int tile) {
     const int2 tl = a.tiles[tile];
     const uint32_t I = (uint32_t)tl.x, J = (uint32_t)tl.y;
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -976,6 +982,20 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     }
 
     finish_2w(a, sh, tile, I, J, w, lane, acc);
+}
+
+__global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_band_i8(BandArgs a) {
+    __shared__ EpiShared2 sh;
+    const int tile = tile_of_block(a);
+    if (tile >= 0) band_i8_tile(a, sh, tile);
+}
+
+__global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2, 2)))
+void k_band_i8_batch(BandBatch b) {
+    __shared__ EpiShared2 sh;
+    uint32_t r;
+    int tile;
+    if (batch_tile(b, r, tile)) band_i8_tile(b.args[r], sh, tile);
 }
 
 // FP64 finish of wave w of a two-wave int8 tile from its exact products
@@ -1168,10 +1188,7 @@ __device__ __forceinline__ void vcode_kloop(const u32x4* pa0, const u32x4* pa1, 
 // skipped when the tile matches one of the frequent patterns (diagonal
 // tile, band edges); skipped accumulators stay zero and the pair mask drops
 // every pair they cover.
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1))) void k_band_fp4(BandArgs a) {
-    __shared__ EpiShared sh;
-    const int tile = tile_of_block(a);
-    if (tile < 0) return;
+__device__ __forceinline__ void band_fp4_tile(const BandArgs& a, EpiShared& sh, int tile) {
     const int2 tl = a.tiles[tile];
     const uint32_t I = (uint32_t)tl.x, J = (uint32_t)tl.y;
     const int lane = threadIdx.x;
@@ -1241,10 +1258,21 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1))) void
     band_contract(a, sh, tile, I, J, r2u);
 }
 
-__global__ __launch_bounds__(64) void k_band_f32(BandArgs a) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1))) void k_band_fp4(BandArgs a) {
     __shared__ EpiShared sh;
     const int tile = tile_of_block(a);
-    if (tile < 0) return;
+    if (tile >= 0) band_fp4_tile(a, sh, tile);
+}
+
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1)))
+void k_band_fp4_batch(BandBatch b) {
+    __shared__ EpiShared sh;
+    uint32_t r;
+    int tile;
+    if (batch_tile(b, r, tile)) band_fp4_tile(b.args[r], sh, tile);
+}
+
+__device__ __forceinline__ void band_f32_tile(const BandArgs& a, EpiShared& sh, int tile) {
     const int2 tl = a.tiles[tile];
     const uint32_t I = (uint32_t)tl.x, J = (uint32_t)tl.y;
     const int lane = threadIdx.x;
@@ -1309,6 +1337,19 @@ __global__ __launch_bounds__(64) void k_band_f32(BandArgs a) {
                 r2u[bi][bj][r] = pair_included(a, sh, I, J, row, col) ? val : 0.0;
             }
     band_contract(a, sh, tile, I, J, r2u);
+}
+
+__global__ __launch_bounds__(64) void k_band_f32(BandArgs a) {
+    __shared__ EpiShared sh;
+    const int tile = tile_of_block(a);
+    if (tile >= 0) band_f32_tile(a, sh, tile);
+}
+
+__global__ __launch_bounds__(64) void k_band_f32_batch(BandBatch b) {
+    __shared__ EpiShared sh;
+    uint32_t r;
+    int tile;
+    if (batch_tile(b, r, tile)) band_f32_tile(b.args[r], sh, tile);
 }
 
 // ------------------------------------------------------------------------
@@ -1569,10 +1610,50 @@ int gather_tile_chunks(const ldsc_panel* panel, const uint64_t* d_bed_idx, uint6
     return LDSC_OK;
 }
 
-int run_l2(const ldsc_panel* panel, const ldsc_snp* snps, uint64_t m, const ldsc_l2_config* cfg,
-           const double* annot, uint64_t K, double* l2_out, double* maf_out, ldsc_l2_perf* perf,
-           ldsc_progress_cb cb, void* cb_ctx, Workspace* ws, char* err, uint64_t err_len) {
+// MAF of the call's SNPs on the host from the exact integer stats in
+// pinned memory (the sketch form, whose rounding depends on chunk
+// contiguity).  f32 exact path: (f32)maf (normalize.rs:153).  Sketch: f64
+// on the fused path of a BED-contiguous chunk (compute.rs:639), (f32)maf
+// otherwise (compute.rs:1509-1511).
+static void compute_maf_host(const L2Call& c) {
+    const uint64_t m = c.m;
+    const ldsc_snp* snps = c.snps;
+    const uint32_t* h_stats =
+        reinterpret_cast<const uint32_t*>(c.ws->pinned_meta.as<char>() + c.off_st);
+    std::vector<double>& maf = c.ws->host.maf;
+    maf.resize(m);
+    std::vector<uint8_t>& round32 = c.ws->host.round32;
+    round32.assign(m, c.use_f32 && !c.d ? 1 : 0);
+    if (c.d) {
+        uint64_t run0 = 0;
+        while (run0 < m) {
+            uint64_t run1 = m;
+            if (c.cfg->per_chromosome) {
+                run1 = run0 + 1;
+                while (run1 < m && snps[run1].chr == snps[run0].chr) ++run1;
+            }
+            for (uint64_t s0 = run0; s0 < run1; s0 += c.cfg->chunk_size) {
+                const uint64_t e = std::min<uint64_t>(s0 + c.cfg->chunk_size, run1);
+                const bool contig = snps[e - 1].bed_idx == snps[s0].bed_idx + (e - s0) - 1;
+                for (uint64_t j = s0; j < e; ++j) round32[j] = contig ? 0 : 1;
+            }
+            run0 = run1;
+        }
+    }
+    for (uint64_t j = 0; j < m; ++j) {
+        const double v = maf_from_stats(h_stats[4 * j], h_stats[4 * j + 1]);
+        maf[j] = round32[j] ? (double)(float)v : v;
+    }
+}
+
+int l2_begin(const ldsc_panel* panel, const ldsc_snp* snps, uint64_t m, const ldsc_l2_config* cfg,
+             const double* annot, uint64_t K, double* l2_out, double* maf_out, ldsc_l2_perf* perf,
+             ldsc_progress_cb cb, void* cb_ctx, Workspace* ws, L2Call* call, char* err,
+             uint64_t err_len) {
     const double t_start = now_s();
+    L2Call& c = *call;
+    c = L2Call{};
+    c.done = true;
     int rc = check_config(cfg, err, err_len);
     if (rc) return rc;
     if (!panel) return fail(err, err_len, LDSC_ERR_INVALID_ARG, "panel is NULL");
@@ -1611,6 +1692,20 @@ int run_l2(const ldsc_panel* panel, const ldsc_snp* snps, uint64_t m, const ldsc
     const uint32_t m_pad = (uint32_t)round_up(m, kTile);
     HIPC(hipSetDevice(panel->device));
     hipStream_t st = ws->stream;
+    c.panel = panel;
+    c.snps = snps;
+    c.m = m;
+    c.cfg = cfg;
+    c.K = K;
+    c.l2_out = l2_out;
+    c.maf_out = maf_out;
+    c.perf = perf;
+    c.cb = cb;
+    c.cb_ctx = cb_ctx;
+    c.ws = ws;
+    c.t_start = t_start;
+    c.m_pad = m_pad;
+    c.d = d;
 
     SketchPlan* sp = nullptr;
     if (d) {
@@ -1642,6 +1737,8 @@ int run_l2(const ldsc_panel* panel, const ldsc_snp* snps, uint64_t m, const ldsc
     const bool use_f32 = cfg->precision == LDSC_PREC_FP32;
     // exact form: e2m1 MFMA while f32 holds the products exactly, int8 beyond
     const bool use_fp4 = !d && cfg->precision == LDSC_PREC_EXACT && n <= kFp4MaxIndiv;
+    c.use_f32 = use_f32;
+    c.form = d ? kBandSketch : use_f32 ? kBandF32 : use_fp4 ? kBandFp4 : kBandI8;
     uint32_t parts = 1;
     if (d && sp->d_local)
         rc = gather_tile_chunks(panel, contiguous ? nullptr : ws->bed_idx.as<uint64_t>(),
@@ -1686,6 +1783,8 @@ int run_l2(const ldsc_panel* panel, const ldsc_snp* snps, uint64_t m, const ldsc
     const bool maf_aware = d && cfg->sketch_maf_aware;
     const bool early_stats = has_pq || maf_aware;
     const bool early_maf = dev_maf && maf_out && !early_stats;
+    c.early_maf = early_maf;
+    c.early_stats = early_stats;
     if (early_maf) {
         HIPC(ws->pinned_maf.ensure((size_t)m * 8));
         HIPC(hipEventRecord(ws->ev[6], st));
@@ -1709,6 +1808,7 @@ int run_l2(const ldsc_panel* panel, const ldsc_snp* snps, uint64_t m, const ldsc
     std::vector<uint64_t>& lo = ws->host.lo;
     bool any_full = false;
     pair_bounds(snps, m, cfg, lo, &any_full);
+    c.any_full = any_full;
     if (!cfg->yes_really && any_full)
         printf("WARNING: LD window spans the entire chromosome. "
                "Use --yes-really to silence this warning.\n");
@@ -1723,6 +1823,9 @@ int run_l2(const ldsc_panel* panel, const ldsc_snp* snps, uint64_t m, const ldsc
     // results come back through pinned memory: with the call's metadata
     // when small, else through the pinned ring
     const bool stage_l2 = (size_t)m * K * 8 <= (8u << 20);
+    c.stage_l2 = stage_l2;
+    c.off_st = off_st;
+    c.off_l2 = off_l2;
     const size_t meta_bytes = off_l2 + (stage_l2 ? (size_t)m * K * 8 : 0);
     // nothing in flight reads pinned_meta (the previous call ended with a
     // synchronize), so it may grow here
@@ -1749,6 +1852,7 @@ int run_l2(const ldsc_panel* panel, const ldsc_snp* snps, uint64_t m, const ldsc
     std::memcpy(hp + off_j0, plan.j0.data(), (size_t)plan.nb * 4);
     std::memcpy(hp + off_ts, plan.tstart.data(), (size_t)(plan.nb + 1) * 4);
     const double t_prep = now_s();
+    c.t_prep = t_prep;
 
     HIPC(ws->lo.ensure((size_t)m_pad * 4));
     const uint32_t Kp = aeff_stride((uint32_t)K);
@@ -1772,34 +1876,6 @@ int run_l2(const ldsc_panel* panel, const ldsc_snp* snps, uint64_t m, const ldsc
     uint32_t* h_stats = reinterpret_cast<uint32_t*>(hp + off_st);
     double* h_maf = reinterpret_cast<double*>(hp + off_maf);
     std::vector<double>& maf = ws->host.maf;
-    auto compute_maf = [&]() {  // sketch form: from the integer stats on the host
-        // f32 exact path: (f32)maf (normalize.rs:153).  Sketch: f64 on the
-        // fused path of a BED-contiguous chunk (compute.rs:639), (f32)maf
-        // otherwise (compute.rs:1509-1511).
-        maf.resize(m);
-        std::vector<uint8_t>& round32 = ws->host.round32;
-        round32.assign(m, use_f32 && !d ? 1 : 0);
-        if (d) {
-            uint64_t run0 = 0;
-            while (run0 < m) {
-                uint64_t run1 = m;
-                if (cfg->per_chromosome) {
-                    run1 = run0 + 1;
-                    while (run1 < m && snps[run1].chr == snps[run0].chr) ++run1;
-                }
-                for (uint64_t s0 = run0; s0 < run1; s0 += cfg->chunk_size) {
-                    const uint64_t e = std::min<uint64_t>(s0 + cfg->chunk_size, run1);
-                    const bool contig = snps[e - 1].bed_idx == snps[s0].bed_idx + (e - s0) - 1;
-                    for (uint64_t j = s0; j < e; ++j) round32[j] = contig ? 0 : 1;
-                }
-                run0 = run1;
-            }
-        }
-        for (uint64_t j = 0; j < m; ++j) {
-            const double v = maf_from_stats(h_stats[4 * j], h_stats[4 * j + 1]);
-            maf[j] = round32[j] ? (double)(float)v : v;
-        }
-    };
     if (early_stats) {
         if (dev_maf) {
             HIPC(hipMemcpyAsync(h_maf, ws->mafd.p, m * 8, hipMemcpyDeviceToHost, st));
@@ -1808,7 +1884,7 @@ int run_l2(const ldsc_panel* panel, const ldsc_snp* snps, uint64_t m, const ldsc
         } else {
             HIPC(hipMemcpyAsync(h_stats, ws->stats.p, (size_t)m_pad * 16, hipMemcpyDeviceToHost, st));
             HIPC(hipStreamSynchronize(st));
-            compute_maf();
+            compute_maf_host(c);
         }
     }
     if (maf_aware) {
@@ -1879,17 +1955,99 @@ int run_l2(const ldsc_panel* panel, const ldsc_snp* snps, uint64_t m, const ldsc
     ba.d_f = (double)d;
     ba.n_pad = (double)round_up(n, kKBlock);
 
+    c.ba = ba;
+    c.sketch_plan = sp;
+    // everything the band reads is queued on the stream before this event
     HIPC(hipEventRecord(ws->ev[3], st));
-    if (d)
+    c.done = false;
+    return LDSC_OK;
+}
+
+int l2_band(L2Call* call, char* err, uint64_t err_len) {
+    L2Call& c = *call;
+    Workspace* ws = c.ws;
+    hipStream_t st = ws->stream;
+    const BandArgs& ba = c.ba;
+    if (c.form == kBandSketch)
         HIPC(launch_band_sk(ba, st));
-    else if (use_f32)
+    else if (c.form == kBandF32)
         hipLaunchKernelGGL(k_band_f32, dim3(8 * ba.per_xcd), dim3(64), 0, st, ba);
-    else if (use_fp4)
+    else if (c.form == kBandFp4)
         hipLaunchKernelGGL(k_band_fp4, dim3(8 * ba.per_xcd), dim3(64), 0, st, ba);
     else
         hipLaunchKernelGGL(k_band_i8, dim3(8 * ba.per_xcd), dim3(128), 0, st, ba);
     HIPC(hipGetLastError());
     HIPC(hipEventRecord(ws->ev[4], st));
+    return LDSC_OK;
+}
+
+int band_batch_launch(L2Call* const* calls, uint32_t ncalls, BandBatchSlot* slot, hipStream_t bst,
+                      char* err, uint64_t err_len) {
+    if (ncalls == 0) return LDSC_OK;
+    if (ncalls > BandBatchSlot::kMaxCalls)
+        return fail(err, err_len, LDSC_ERR_INVALID_ARG, "band batch of %u calls is too large",
+                    ncalls);
+    const int form = calls[0]->form;
+    // the slot's previous launch has read its table
+    if (slot->used) HIPC(hipEventSynchronize(slot->done));
+    if (!slot->d_args) HIPC(hipMalloc(&slot->d_args, BandBatchSlot::kBytes));
+    if (!slot->h_args) HIPC(hipHostMalloc(&slot->h_args, BandBatchSlot::kBytes, hipHostMallocDefault));
+    if (!slot->done) HIPC(hipEventCreateWithFlags(&slot->done, hipEventDisableTiming));
+    BandArgs* h_args = reinterpret_cast<BandArgs*>(slot->h_args);
+    uint32_t* h_starts = reinterpret_cast<uint32_t*>(h_args + BandBatchSlot::kMaxCalls);
+    uint32_t total = 0;
+    for (uint32_t i = 0; i < ncalls; ++i) {
+        if (calls[i]->form != form || calls[i]->panel->device != calls[0]->panel->device)
+            return fail(err, err_len, LDSC_ERR_INVALID_ARG, "mixed band batch");
+        h_args[i] = calls[i]->ba;
+        h_starts[i] = total;
+        total += calls[i]->ba.ntiles;
+        // the band reads what the call queued before its ev[3]
+        HIPC(hipStreamWaitEvent(bst, calls[i]->ws->ev[3], 0));
+    }
+    h_starts[ncalls] = total;
+    BandBatch b{};
+    b.args = reinterpret_cast<const BandArgs*>(slot->d_args);
+    b.starts = reinterpret_cast<const uint32_t*>(reinterpret_cast<const BandArgs*>(slot->d_args) +
+                                                 BandBatchSlot::kMaxCalls);
+    b.nreq = ncalls;
+    b.ntiles = total;
+    b.per_xcd = (total + 7) / 8;
+    HIPC(hipMemcpyAsync(slot->d_args, slot->h_args, BandBatchSlot::kBytes, hipMemcpyHostToDevice,
+                        bst));
+    if (total > 0) {
+        if (form == kBandF32)
+            hipLaunchKernelGGL(k_band_f32_batch, dim3(8 * b.per_xcd), dim3(64), 0, bst, b);
+        else if (form == kBandFp4)
+            hipLaunchKernelGGL(k_band_fp4_batch, dim3(8 * b.per_xcd), dim3(64), 0, bst, b);
+        else if (form == kBandI8)
+            hipLaunchKernelGGL(k_band_i8_batch, dim3(8 * b.per_xcd), dim3(128), 0, bst, b);
+        else
+            return fail(err, err_len, LDSC_ERR_INVALID_ARG, "this band form is not batched");
+        HIPC(hipGetLastError());
+    }
+    HIPC(hipEventRecord(slot->done, bst));
+    slot->used = true;
+    for (uint32_t i = 0; i < ncalls; ++i) HIPC(hipEventRecord(calls[i]->ws->ev[4], bst));
+    return LDSC_OK;
+}
+
+int l2_end(L2Call* call, char* err, uint64_t err_len) {
+    L2Call& c = *call;
+    Workspace* ws = c.ws;
+    hipStream_t st = ws->stream;
+    const uint64_t m = c.m, K = c.K;
+    const uint32_t m_pad = c.m_pad;
+    const bool stage_l2 = c.stage_l2, early_maf = c.early_maf, early_stats = c.early_stats;
+    double* l2_out = c.l2_out;
+    double* maf_out = c.maf_out;
+    ldsc_l2_perf* perf = c.perf;
+    BandPlan& plan = ws->host.plan;
+    char* hp = ws->pinned_meta.as<char>();
+    uint32_t* h_stats = reinterpret_cast<uint32_t*>(hp + c.off_st);
+    std::vector<double>& maf = ws->host.maf;
+    // the band ran on this stream or on a batch stream that recorded ev[4]
+    HIPC(hipStreamWaitEvent(st, ws->ev[4], 0));
 
     // 4. deterministic reduction, in groups of annotations when the result
     //    goes back through the ring, so that its copy starts with the first
@@ -1911,7 +2069,7 @@ int run_l2(const ldsc_panel* panel, const ldsc_snp* snps, uint64_t m, const ldsc
     } else if (maf_out && !early_stats) {
         HIPC(hipMemcpyAsync(h_stats, ws->stats.p, (size_t)m_pad * 16, hipMemcpyDeviceToHost, st));
     }
-    double* h_l2 = reinterpret_cast<double*>(hp + off_l2);
+    double* h_l2 = reinterpret_cast<double*>(hp + c.off_l2);
     if (stage_l2) {
         HIPC(hipMemcpyAsync(h_l2, ws->l2.p, (size_t)m * K * 8, hipMemcpyDeviceToHost, st));
         HIPC(hipStreamSynchronize(st));
@@ -1934,7 +2092,7 @@ int run_l2(const ldsc_panel* panel, const ldsc_snp* snps, uint64_t m, const ldsc
         HIPC(hipStreamSynchronize(st));
     }
     if (maf_out && !early_maf) {
-        if (!early_stats) compute_maf();  // sketch form: from the integer stats
+        if (!early_stats) compute_maf_host(c);  // sketch form: from the integer stats
         std::memcpy(maf_out, maf.data(), m * 8);
     }
 
@@ -1946,24 +2104,36 @@ int run_l2(const ldsc_panel* panel, const ldsc_snp* snps, uint64_t m, const ldsc
         (void)hipEventElapsedTime(&t45, ws->ev[4], ws->ev[5]);
         std::memset(perf, 0, sizeof(*perf));
         perf->m = m;
-        perf->n_indiv = n;
-        perf->bed_read_secs = t_prep - t_start;
+        perf->n_indiv = c.panel->n;
+        perf->bed_read_secs = c.t_prep - c.t_start;
         perf->norm_secs = t01 * 1e-3;
         perf->sketch_secs = t12 * 1e-3;
         perf->bb_dot_secs = t34 * 1e-3;
         perf->ring_store_secs = t45 * 1e-3;
-        perf->total_secs = now_s() - t_start;
+        perf->total_secs = now_s() - c.t_start;
         perf->pairs_included = plan.pairs_included;
         perf->pairs_evaluated = (uint64_t)plan.ntiles * kTile * kTile;
         perf->tiles = plan.ntiles;
-        perf->any_full_chr_window = any_full ? 1 : 0;
+        perf->any_full_chr_window = c.any_full ? 1 : 0;
     }
-    if (cb) {
-        const uint64_t chunks = (m + cfg->chunk_size - 1) / cfg->chunk_size;
-        ldsc_l2_progress pr{chunks, chunks, m, m, (now_s() - t_start) * 1e3};
-        cb(&pr, cb_ctx);
+    if (c.cb) {
+        const uint64_t chunks = (m + c.cfg->chunk_size - 1) / c.cfg->chunk_size;
+        ldsc_l2_progress pr{chunks, chunks, m, m, (now_s() - c.t_start) * 1e3};
+        c.cb(&pr, c.cb_ctx);
     }
+    c.done = true;
     return LDSC_OK;
+}
+
+int run_l2(const ldsc_panel* panel, const ldsc_snp* snps, uint64_t m, const ldsc_l2_config* cfg,
+           const double* annot, uint64_t K, double* l2_out, double* maf_out, ldsc_l2_perf* perf,
+           ldsc_progress_cb cb, void* cb_ctx, Workspace* ws, char* err, uint64_t err_len) {
+    L2Call c;
+    int rc = l2_begin(panel, snps, m, cfg, annot, K, l2_out, maf_out, perf, cb, cb_ctx, ws, &c,
+                      err, err_len);
+    if (rc || c.done) return rc;
+    if ((rc = l2_band(&c, err, err_len))) return rc;
+    return l2_end(&c, err, err_len);
 }
 
 }  // namespace ldsc

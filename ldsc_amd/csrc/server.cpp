@@ -11,9 +11,21 @@
 // kStarveSecs, nothing may overtake it), runs it on its own stream and
 // returns its budget.  Requests that arrive while others run are admitted as
 // soon as a slot and memory free up, and their kernels overlap on the device
-// with those already running.  Each request's result is independent of what
-// else is in flight: the computation of one request touches only its own
-// workspace, and every reduction runs in a fixed order.
+// with those already running.
+//
+// Band batching (batch_window_us > 0): a worker prepares its request
+// (tiling, planning, uploads: l2_begin) on its own stream and then hands the
+// band contraction to the batcher.  The first worker to arrive leads: it
+// waits up to the window for the other workers still preparing, then puts
+// the tiles of every request handed in so far into one launch per band form
+// (band_batch_launch, on the device's batch stream, after each request's
+// preparation); requests handed in meanwhile go into the next launch.  Each
+// worker then reduces and copies back its own result (l2_end).
+//
+// Each request's result is independent of what else is in flight or shares
+// its launch: a tile reads and writes only its own request's buffers, and
+// every reduction runs in a fixed order.
+#include <algorithm>
 #include <chrono>
 #include <condition_variable>
 #include <cstring>
@@ -53,6 +65,114 @@ struct Request {
 
 }  // namespace
 
+// Leader / follower batching of band launches (see the file comment).
+struct BandBatcher {
+    struct Entry {
+        ldsc::L2Call* call;
+        bool launched = false;
+        int rc = LDSC_OK;
+        std::string err;
+    };
+    struct Device {
+        hipStream_t stream = nullptr;
+        static constexpr int kSlots = 4;
+        ldsc::BandBatchSlot slots[kSlots];
+        int next = 0;
+    };
+    std::mutex mu;
+    std::condition_variable cv;
+    std::vector<Entry*> pending;
+    bool leading = false;
+    uint32_t preparing = 0;  // workers between taking a request and handing in its band
+    double window = 0.0;     // seconds
+    std::unordered_map<int, Device> devices;
+    uint64_t launches = 0, batched = 0;
+    uint32_t max_batch = 0;
+
+    int run(ldsc::L2Call* call, char* err, uint64_t err_len);
+    void launch_all(std::vector<Entry*>& take);
+    void destroy();
+};
+
+void BandBatcher::launch_all(std::vector<Entry*>& take) {
+    // one launch per (device, form) group, at most BandBatchSlot::kMaxCalls each
+    std::vector<bool> used(take.size(), false);
+    for (size_t i = 0; i < take.size(); ++i) {
+        if (used[i]) continue;
+        std::vector<Entry*> grp;
+        std::vector<ldsc::L2Call*> calls;
+        const int dev = take[i]->call->panel->device, form = take[i]->call->form;
+        for (size_t j = i; j < take.size() && calls.size() < ldsc::BandBatchSlot::kMaxCalls; ++j)
+            if (!used[j] && take[j]->call->panel->device == dev && take[j]->call->form == form) {
+                used[j] = true;
+                grp.push_back(take[j]);
+                calls.push_back(take[j]->call);
+            }
+        char err[512] = {0};
+        int rc = LDSC_OK;
+        Device& d = devices[dev];
+        hipError_t e = hipSetDevice(dev);
+        if (e == hipSuccess && !d.stream) e = hipStreamCreateWithFlags(&d.stream, hipStreamNonBlocking);
+        if (e != hipSuccess)
+            rc = ldsc::fail(err, sizeof(err), LDSC_ERR_DEVICE, "batch stream: %s", hipGetErrorString(e));
+        else
+            rc = ldsc::band_batch_launch(calls.data(), (uint32_t)calls.size(), &d.slots[d.next],
+                                         d.stream, err, sizeof(err));
+        d.next = (d.next + 1) % Device::kSlots;
+        for (Entry* en : grp) {
+            en->rc = rc;
+            if (rc) en->err = err;
+        }
+        if (rc == LDSC_OK) {
+            ++launches;
+            if (grp.size() > 1) batched += grp.size();
+            max_batch = std::max<uint32_t>(max_batch, (uint32_t)grp.size());
+        }
+    }
+}
+
+int BandBatcher::run(ldsc::L2Call* call, char* err, uint64_t err_len) {
+    Entry me{call};
+    std::unique_lock<std::mutex> lk(mu);
+    --preparing;
+    pending.push_back(&me);
+    cv.notify_all();
+    while (!me.launched) {
+        if (leading) {
+            cv.wait(lk);
+            continue;
+        }
+        leading = true;
+        const auto until = std::chrono::steady_clock::now() +
+                           std::chrono::duration_cast<std::chrono::steady_clock::duration>(
+                               std::chrono::duration<double>(window));
+        while (preparing > 0 && cv.wait_until(lk, until) != std::cv_status::timeout) {
+        }
+        std::vector<Entry*> take;
+        take.swap(pending);
+        lk.unlock();
+        launch_all(take);
+        lk.lock();
+        for (Entry* en : take) en->launched = true;
+        leading = false;
+        cv.notify_all();
+    }
+    if (me.rc != LDSC_OK) return ldsc::fail(err, err_len, me.rc, "%s", me.err.c_str());
+    return LDSC_OK;
+}
+
+void BandBatcher::destroy() {
+    for (auto& kv : devices) {
+        (void)hipSetDevice(kv.first);
+        if (kv.second.stream) {
+            (void)hipStreamSynchronize(kv.second.stream);
+            (void)hipStreamDestroy(kv.second.stream);
+        }
+        for (auto& sl : kv.second.slots) sl.release();
+    }
+    devices.clear();
+}
+
 struct ldsc_server {
     ldsc_server_config cfg{};
     std::mutex mu;
@@ -63,10 +183,45 @@ struct ldsc_server {
     size_t in_use = 0;
     bool stop = false;
     std::vector<std::thread> workers;
+    BandBatcher batcher;
+    uint64_t done_count = 0;
 
     void worker_main();
     Request* take_locked();
+    int run_request(Request* r, ldsc::Workspace* ws, char* err, uint64_t err_len);
 };
+
+// One request on workspace ws: prepare, band (batched or alone), finish.
+int ldsc_server::run_request(Request* r, ldsc::Workspace* ws, char* err, uint64_t err_len) {
+    const bool batch = cfg.batch_window_us > 0;
+    if (batch) {
+        std::lock_guard<std::mutex> lk(batcher.mu);
+        ++batcher.preparing;
+    }
+    ldsc::L2Call call;
+    int rc = ldsc::l2_begin(r->panel, r->snps.data(), r->snps.size(), &r->cfg,
+                            r->annot.empty() ? nullptr : r->annot.data(), r->K, r->l2_out,
+                            r->maf_out, nullptr, nullptr, nullptr, ws, &call, err, err_len);
+    const bool batched = batch && rc == LDSC_OK && !call.done && call.form != ldsc::kBandSketch;
+    if (batch && !batched) {
+        std::lock_guard<std::mutex> lk(batcher.mu);
+        --batcher.preparing;
+        batcher.cv.notify_all();
+    }
+    if (rc != LDSC_OK || call.done) return rc;
+    if (batched) {
+        rc = batcher.run(&call, err, err_len);
+    } else {
+        rc = ldsc::l2_band(&call, err, err_len);
+        if (rc == LDSC_OK) {
+            std::lock_guard<std::mutex> lk(batcher.mu);
+            ++batcher.launches;
+            batcher.max_batch = std::max<uint32_t>(batcher.max_batch, 1);
+        }
+    }
+    if (rc != LDSC_OK) return rc;
+    return ldsc::l2_end(&call, err, err_len);
+}
 
 Request* ldsc_server::take_locked() {
     if (queue.empty()) return nullptr;
@@ -105,12 +260,11 @@ void ldsc_server::worker_main() {
         if (!ws) {
             rc = ldsc::fail(err, sizeof(err), LDSC_ERR_DEVICE, "could not create a device workspace");
         } else {
-            rc = ldsc::run_l2(r->panel, r->snps.data(), r->snps.size(), &r->cfg,
-                              r->annot.empty() ? nullptr : r->annot.data(), r->K, r->l2_out,
-                              r->maf_out, nullptr, nullptr, nullptr, ws, err, sizeof(err));
+            rc = run_request(r, ws, err, sizeof(err));
         }
         {
             std::lock_guard<std::mutex> lk(mu);
+            ++done_count;
             in_use -= r->bytes;
             r->rc = rc;
             r->err = err;
@@ -130,8 +284,11 @@ extern "C" int ldsc_server_create(const ldsc_server_config* cfg, ldsc_server** o
         return ldsc::fail(err, err_len, LDSC_ERR_INVALID_ARG, "max_inflight must be in [1, 64]");
     if (cfg->queue_capacity < 1)
         return ldsc::fail(err, err_len, LDSC_ERR_INVALID_ARG, "queue_capacity must be >= 1");
+    if (cfg->batch_window_us > 1000000)
+        return ldsc::fail(err, err_len, LDSC_ERR_INVALID_ARG, "batch_window_us must be <= 1000000");
     auto* s = new ldsc_server();
     s->cfg = *cfg;
+    s->batcher.window = cfg->batch_window_us * 1e-6;
     for (uint32_t i = 0; i < cfg->max_inflight; ++i)
         s->workers.emplace_back([s] { s->worker_main(); });
     *out = s;
@@ -153,7 +310,22 @@ extern "C" int ldsc_server_destroy(ldsc_server* s) {
     }
     s->cv_work.notify_all();
     for (auto& t : s->workers) t.join();
+    s->batcher.destroy();
     delete s;
+    return LDSC_OK;
+}
+
+extern "C" int ldsc_server_stats_get(ldsc_server* s, ldsc_server_stats* out) {
+    if (!s || !out) return LDSC_ERR_INVALID_ARG;
+    std::memset(out, 0, sizeof(*out));
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        out->requests_done = s->done_count;
+    }
+    std::lock_guard<std::mutex> lk(s->batcher.mu);
+    out->band_launches = s->batcher.launches;
+    out->batched_requests = s->batcher.batched;
+    out->max_batch = s->batcher.max_batch;
     return LDSC_OK;
 }
 

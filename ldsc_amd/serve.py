@@ -1,8 +1,11 @@
 """Request server over resident panels (``ldsc_server_*`` of the C ABI).
 
 Continuous batching: up to ``max_inflight`` requests run concurrently on
-their own HIP streams, admitted first-fit against an HBM budget; results of
-a request do not depend on what else is in flight.
+their own HIP streams, admitted first-fit against an HBM budget.  With
+``batch_window_us`` > 0 the band contractions of requests prepared at the
+same time share one kernel launch (a request waits at most that long for
+others still being prepared).  Results of a request do not depend on what
+else is in flight or shares its launch.
 """
 from __future__ import annotations
 
@@ -30,9 +33,10 @@ class Ticket:
 
 class Server:
     def __init__(self, hbm_budget_bytes: int = 64 << 30, max_inflight: int = 4,
-                 queue_capacity: int = 1024):
+                 queue_capacity: int = 1024, batch_window_us: int = 200):
         L = N.lib()
-        cfg = N.ServerConfig(int(hbm_budget_bytes), int(max_inflight), int(queue_capacity))
+        cfg = N.ServerConfig(int(hbm_budget_bytes), int(max_inflight), int(queue_capacity),
+                             int(batch_window_us), 0)
         h = C.c_void_p()
         err = N.errbuf()
         N.check(L.ldsc_server_create(C.byref(cfg), C.byref(h), err, 512), err)
@@ -74,6 +78,14 @@ class Server:
         err = N.errbuf()
         N.check(N.lib().ldsc_l2_wait(self._h, t.id, C.byref(lat), err, 512), err)
         return lat.value
+
+    def stats(self) -> dict:
+        """Counters since creation: requests done, band launches, requests
+        whose band shared a launch, and the largest launch (in requests)."""
+        st = N.ServerStats()
+        N.check(N.lib().ldsc_server_stats_get(self._h, C.byref(st)), N.errbuf())
+        return {"requests_done": st.requests_done, "band_launches": st.band_launches,
+                "batched_requests": st.batched_requests, "max_batch": st.max_batch}
 
     def close(self) -> None:
         if getattr(self, "_h", None) is not None:

@@ -1,5 +1,6 @@
 """Request server: host-side errors (CPU) and continuous batching on the
-device (GPU): each result is bit-identical alone and in the mix."""
+device (GPU): each result is bit-identical alone, in the mix and in a
+shared band launch."""
 import numpy as np
 import pytest
 
@@ -14,6 +15,11 @@ def test_server_create_errors_and_unknown_ids():
         Server(max_inflight=0)
     with pytest.raises(L.LdscError):
         Server(queue_capacity=0)
+    with pytest.raises(L.LdscError):
+        Server(batch_window_us=2_000_000)
+    with Server(max_inflight=2, batch_window_us=0) as s:
+        assert s.stats() == {"requests_done": 0, "band_launches": 0, "batched_requests": 0,
+                             "max_batch": 0}
     with Server(max_inflight=2) as s:
         import ctypes as C
         err = N.errbuf()
@@ -58,6 +64,45 @@ def test_mixed_stream_equals_standalone():
             got, gmaf = t.result()
             assert np.array_equal(got, alone) and np.array_equal(gmaf, maf)
         assert all(x > 0 for x in lats)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("window_us", [0, 50_000])
+def test_batched_band_launches_equal_standalone(window_us):
+    """A burst of small requests on eight workers: with a batch window their
+    band tiles share launches (the leader waits for the workers still
+    preparing), without one every request has its own launch.  Either way
+    each result equals the standalone call bit for bit."""
+    rng = np.random.default_rng(11)
+    p = synth.make_panel(2000, 8000, seed=5, miss_rate=0.01)
+    snps_all = L.make_snp_array(p.chr, p.cm, p.bp)
+    with L.Panel(p.bed_bytes(), p.n, p.m) as panel, \
+            Server(hbm_budget_bytes=8 << 30, max_inflight=8, batch_window_us=window_us) as srv:
+        jobs = []
+        for i in range(24):
+            size = int(rng.integers(300, 2500))
+            start = int(rng.integers(0, p.m - size))
+            K = int(rng.integers(0, 3))
+            snps = snps_all[start:start + size]
+            annot = None if K == 0 else synth.make_annot(size, seed=i, n_bin=K, n_cont=0)
+            cfg = L.L2Config(mode=[L.WindowMode.Cm(0.2), L.WindowMode.Snp(120)][i % 2],
+                             chunk_size=100, snp_level_masking=bool(i % 3),
+                             exact_kernel="i8" if i % 5 == 4 else "auto")
+            jobs.append((snps, cfg, annot, srv.submit(panel, snps, cfg, annot)))
+        for (_, _, _, t) in jobs:
+            srv.wait(t)
+        st = srv.stats()
+        print(f"window {window_us} us: {st}", flush=True)
+        for snps, cfg, annot, t in jobs:
+            alone, maf, _ = L.compute_ldscore(panel, snps, cfg, annot=annot)
+            got, gmaf = t.result()
+            assert np.array_equal(got, alone) and np.array_equal(gmaf, maf)
+    assert st["requests_done"] == 24
+    if window_us:
+        assert st["batched_requests"] > 0 and st["max_batch"] > 1
+        assert st["band_launches"] < 24
+    else:
+        assert st["band_launches"] == 24 and st["batched_requests"] == 0
 
 
 @pytest.mark.gpu
