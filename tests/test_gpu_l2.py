@@ -52,3 +52,24 @@ def test_l2_matches_oracle(oracle, mode, window, masking, chunk, form):
                           chunk=chunk, use_f32=use_f32, masking=masking)
     assert _rel(got, exp) < (TOL_F32 if use_f32 else TOL_EXACT)
     assert np.array_equal(maf, emaf)
+
+
+@pytest.mark.parametrize("form", ["exact", "exact_i8", "f32"])
+def test_cpu_runnable_config(oracle, form):
+    """The reference's CPU-runnable case: founder-haplotype panel N=2,490 x
+    M=20,000, 0.5 % missing, --ld-wind-cm 1, chunk 50 (--python-compat), no
+    SNP-level masking, against the restatement of the f64 path (and of
+    --fast-f32 for the FP32 form)."""
+    p = synth.make_panel(2490, 20000, seed=20261204, miss_rate=0.005)
+    snps = L.make_snp_array(p.chr, p.cm, p.bp)
+    use_f32 = form == "f32"
+    cfg = L.L2Config(mode=L.WindowMode.Cm(1.0), chunk_size=50, use_f32=use_f32,
+                     exact_kernel="i8" if form == "exact_i8" else "auto")
+    print(f"case N=2490 M=20000 form={form}", flush=True)
+    with L.Panel(p.bed_bytes(), p.n, p.m) as panel:
+        got, maf, perf = L.compute_ldscore(panel, snps, cfg)
+    exp, emaf = oracle.l2(p.bed_bytes(), p.n, p.chr, p.cm, p.bp, mode=oracle.WINDOW_CM,
+                          window=1.0, chunk=50, use_f32=use_f32, masking=False, threads=16)
+    assert _rel(got, exp) < (TOL_F32 if use_f32 else TOL_EXACT)
+    assert np.array_equal(maf, emaf)
+    assert perf.pairs_included > 9_000_000
