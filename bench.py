@@ -241,7 +241,60 @@ def leg_panel(args, L, synth):
         }
         log(f"cpu baseline {res['cpu_baseline']['value']:.3e} pairs/s (f32 "
             f"{res['cpu_baseline']['fast_f32_value']:.3e}); {res['sample_accuracy']}")
+        res["pair_accuracy"] = pair_accuracy(args, L, O, p, snps, ms, bed_s, cfg8, cfg32)
     return res
+
+
+def pair_accuracy(args, L, O, p, snps, ms, bed_s, cfg8, cfg32, n_probe=24, d=1000):
+    """Errors of individual pair values against the FP64 restatement.
+
+    With annotation column k the indicator of probe SNP j_k, l2[i, k] is the
+    unbiased r^2 of the pair (i, j_k) for every i whose pair with j_k is
+    included (and 1 at i = j_k), so the LD-score API itself yields pair
+    values: r2u of the exact and FP32 forms against the f64 path, and the
+    sketch-corrected r2u (d buckets) against the restatement of the
+    reference's sketch path.  Also the sketch-corrected l2 against the exact
+    f64 l2 (statistical, not parity).  On the first 2,000 SNPs of the slice."""
+    ms = min(ms, 2000)
+    bed_s = bed_s[:3 + ms * ((p.n + 3) // 4)]
+    probes = np.linspace(50, ms - 50, n_probe).astype(np.int64)
+    A = np.zeros((ms, n_probe))
+    A[probes, np.arange(n_probe)] = 1.0
+    kw = dict(mode=O.WINDOW_CM, window=1.0, chunk=200, masking=True, threads=args.cpu_threads)
+    ref64, _ = O.l2(bed_s, p.n, p.chr[:ms], p.cm[:ms], p.bp[:ms], annot=A, **kw)
+    ref32, _ = O.l2(bed_s, p.n, p.chr[:ms], p.cm[:ms], p.bp[:ms], annot=A, use_f32=True, **kw)
+    refsk, _ = O.l2(bed_s, p.n, p.chr[:ms], p.cm[:ms], p.bp[:ms], annot=A, sketch=d, **kw)
+    lsk_ref, _ = O.l2(bed_s, p.n, p.chr[:ms], p.cm[:ms], p.bp[:ms], sketch=d, **kw)
+    l64, _ = O.l2(bed_s, p.n, p.chr[:ms], p.cm[:ms], p.bp[:ms], **kw)
+    cfgsk = L.L2Config(mode=L.WindowMode.Cm(1.0), chunk_size=200, snp_level_masking=True,
+                       sketch=d)
+    with L.Panel(bed_s, p.n, ms) as ps:
+        g8, _, _ = L.compute_ldscore(ps, snps[:ms], cfg8, annot=A)
+        g32, _, _ = L.compute_ldscore(ps, snps[:ms], cfg32, annot=A)
+        gsk, _, _ = L.compute_ldscore(ps, snps[:ms], cfgsk, annot=A)
+        lsk, _, _ = L.compute_ldscore(ps, snps[:ms], cfgsk)
+    incl = ref64 != 0.0  # included pairs (an exact zero r2u does not occur)
+    incl[probes, np.arange(n_probe)] = False  # drop the self terms
+
+    def stats(got, ref):
+        e = np.abs(got[incl] - ref[incl])
+        big = np.abs(ref[incl]) >= 0.01
+        return {"max_abs": float(e.max()), "rms_abs": float(np.sqrt(np.mean(e ** 2))),
+                "max_rel_where_abs_ge_0.01": float(np.max(e[big] / np.abs(ref[incl][big])))}
+
+    out = {"pairs": int(incl.sum()), "probes": n_probe,
+           "exact_r2u_vs_fp64": stats(g8, ref64),
+           "fp32_r2u_vs_fp64": stats(g32, ref64),
+           "fp32_r2u_vs_f32_restatement": stats(g32, ref32),
+           f"sketch{d}_r2u_vs_sketch_restatement": stats(gsk, refsk),
+           f"sketch{d}_l2_vs_sketch_restatement_max_rel":
+               float(np.max(np.abs(lsk - lsk_ref) / np.maximum(1.0, np.abs(lsk_ref)))),
+           f"sketch{d}_l2_vs_exact_fp64": {
+               "max_abs": float(np.max(np.abs(lsk - l64))),
+               "rms_abs": float(np.sqrt(np.mean((lsk - l64) ** 2))),
+               "pearson_r": float(np.corrcoef(lsk[:, 0], l64[:, 0])[0, 1])}}
+    log(f"pair accuracy: {out}")
+    return out
 
 
 # ---------------------------------------------------------------------------
@@ -325,8 +378,9 @@ def leg_serving(args, L, synth):
 
     burst = burst_unbatched = None
     if args.serve_burst_inflight > 0:
-        burst = run_burst(args.serve_window_us)
+        run_burst(args.serve_window_us)  # warm-up: grows the pooled workspaces
         burst_unbatched = run_burst(0)
+        burst = run_burst(args.serve_window_us)
     for pn in panels:
         pn.close()
     lats = np.array(lats)
@@ -512,6 +566,7 @@ def main() -> int:
         "measured_peaks": args.peaks,
         "cpu_baseline": pan.get("cpu_baseline"),
         "sample_accuracy": pan.get("sample_accuracy"),
+        "pair_accuracy": pan.get("pair_accuracy"),
         "serving": serving,
     }
     print(json.dumps(out), flush=True)

@@ -15,11 +15,12 @@
 //
 // Band batching (batch_window_us > 0): a worker prepares its request
 // (tiling, planning, uploads: l2_begin) on its own stream and then hands the
-// band contraction to the batcher.  The first worker to arrive leads: it
-// waits up to the window for the other workers still preparing, then puts
-// the tiles of every request handed in so far into one launch per band form
-// (band_batch_launch, on the device's batch stream, after each request's
-// preparation); requests handed in meanwhile go into the next launch.  Each
+// band contraction to the batcher.  The first worker to arrive leads: while
+// the handed-in tiles would not fill the chip it waits, up to the window,
+// for the other workers still preparing; then it puts the tiles of every
+// request handed in so far into one launch per band form
+// (band_batch_launch, after each request's preparation, on one of a few
+// streams in turn); requests handed in meanwhile go into the next launch.  Each
 // worker then reduces and copies back its own result (l2_end).
 //
 // Each request's result is independent of what else is in flight or shares
@@ -73,12 +74,18 @@ struct BandBatcher {
         int rc = LDSC_OK;
         std::string err;
     };
+    // Launches rotate over kSlots argument tables, each with its own stream,
+    // so that consecutive launches can overlap on the device as the
+    // per-request launches of separate workers do.
     struct Device {
-        hipStream_t stream = nullptr;
         static constexpr int kSlots = 4;
+        hipStream_t streams[kSlots] = {};
         ldsc::BandBatchSlot slots[kSlots];
         int next = 0;
     };
+    // A launch with this many tiles already fills the chip (one wave per
+    // SIMD on 256 CUs): the leader then stops waiting for more requests.
+    static constexpr uint64_t kFillTiles = 1024;
     std::mutex mu;
     std::condition_variable cv;
     std::vector<Entry*> pending;
@@ -111,14 +118,16 @@ void BandBatcher::launch_all(std::vector<Entry*>& take) {
         char err[512] = {0};
         int rc = LDSC_OK;
         Device& d = devices[dev];
+        const int k = d.next;
+        d.next = (d.next + 1) % Device::kSlots;
         hipError_t e = hipSetDevice(dev);
-        if (e == hipSuccess && !d.stream) e = hipStreamCreateWithFlags(&d.stream, hipStreamNonBlocking);
+        if (e == hipSuccess && !d.streams[k])
+            e = hipStreamCreateWithFlags(&d.streams[k], hipStreamNonBlocking);
         if (e != hipSuccess)
             rc = ldsc::fail(err, sizeof(err), LDSC_ERR_DEVICE, "batch stream: %s", hipGetErrorString(e));
         else
-            rc = ldsc::band_batch_launch(calls.data(), (uint32_t)calls.size(), &d.slots[d.next],
-                                         d.stream, err, sizeof(err));
-        d.next = (d.next + 1) % Device::kSlots;
+            rc = ldsc::band_batch_launch(calls.data(), (uint32_t)calls.size(), &d.slots[k],
+                                         d.streams[k], err, sizeof(err));
         for (Entry* en : grp) {
             en->rc = rc;
             if (rc) en->err = err;
@@ -146,7 +155,15 @@ int BandBatcher::run(ldsc::L2Call* call, char* err, uint64_t err_len) {
         const auto until = std::chrono::steady_clock::now() +
                            std::chrono::duration_cast<std::chrono::steady_clock::duration>(
                                std::chrono::duration<double>(window));
-        while (preparing > 0 && cv.wait_until(lk, until) != std::cv_status::timeout) {
+        auto pending_tiles = [&] {
+            uint64_t t = 0;
+            for (const Entry* en : pending) t += en->call->ba.ntiles;
+            return t;
+        };
+        // wait for the requests still being prepared while the launch would
+        // not yet fill the chip
+        while (preparing > 0 && pending_tiles() < kFillTiles &&
+               cv.wait_until(lk, until) != std::cv_status::timeout) {
         }
         std::vector<Entry*> take;
         take.swap(pending);
@@ -164,10 +181,12 @@ int BandBatcher::run(ldsc::L2Call* call, char* err, uint64_t err_len) {
 void BandBatcher::destroy() {
     for (auto& kv : devices) {
         (void)hipSetDevice(kv.first);
-        if (kv.second.stream) {
-            (void)hipStreamSynchronize(kv.second.stream);
-            (void)hipStreamDestroy(kv.second.stream);
-        }
+        for (hipStream_t& st : kv.second.streams)
+            if (st) {
+                (void)hipStreamSynchronize(st);
+                (void)hipStreamDestroy(st);
+                st = nullptr;
+            }
         for (auto& sl : kv.second.slots) sl.release();
     }
     devices.clear();
