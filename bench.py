@@ -17,9 +17,10 @@ Default run (one JSON line):
   * cpu_baseline -- the C restatement of the reference's f64 path on the
     host cores, on a bounded slice of the same panel.
 
-`--workload sketch` (CountSketch, N=100,000 x M=200,000, d in {200, 1000, 5000})
-and `--workload partitioned` (partitioned, N=20,000 x M=500,000, K=50, kb 1000) print
-their own JSON lines.  `--gpus` above 1 is refused: this build targets one GPU.
+`--workload sketch` (CountSketch, N=100,000 x M=200,000, d in {200, 1000, 5000}),
+`--workload partitioned` (partitioned, N=20,000 x M=500,000, K=50, kb 1000) and
+`--workload reference_shapes` (the panel shapes of the reference's published
+CPU wall times) print their own JSON lines.  `--gpus` above 1 is refused: this build targets one GPU.
 """
 from __future__ import annotations
 
@@ -475,12 +476,47 @@ def leg_partitioned(args, L, synth):
     return out
 
 
+# ---------------------------------------------------------------------------
+# The shapes of the reference's published CPU timings
+# ---------------------------------------------------------------------------
+def leg_reference_shapes(args, L, synth):
+    """Synthetic panels of the shapes the reference publishes l2 wall times
+    for: 1000G (N=2,490, M=1.66 M, --ld-wind-kb 1000; Rust exact f64 41.1 s,
+    --fast-f32 ~32 s on 16 vCPU of an EPYC 7R13) and the N=50,000 biobank
+    panel (M=1.66 M, kb 1000, chunk 200; exact f64 727 s, f32 424 s).  The
+    data differ (founder-haplotype generator, one chromosome), so these are
+    same-shape wall times, not a parity comparison."""
+    out = {}
+    for name, n, m, seed in (("1000G_shape", 2490, 1_660_000, 20261211),
+                             ("biobank_shape", 50_000, 1_660_000, 20261212)):
+        t0 = time.perf_counter()
+        p = synth.make_panel(n, m, seed, miss_rate=0.01)
+        snps = L.make_snp_array(p.chr, p.cm, p.bp)
+        log(f"{name}: panel N={n} M={m} generated ({time.perf_counter() - t0:.1f}s)")
+        res = {"n_indiv": n, "m_snps": m}
+        with L.Panel(p.bed_bytes(), n, m) as panel:
+            del p
+            for tag, f32 in (("exact", False), ("fp32", True)):
+                cfg = L.L2Config(mode=L.WindowMode.Kb(1000.0), chunk_size=200, use_f32=f32)
+                w, pf, _ = timed_steps(L, panel, snps, cfg, args.steps, 1, f"{name} {tag}")
+                res[tag] = {"wall_s": float(np.mean(w)), "pairs": pf[0].pairs_included,
+                            "snp_pairs_per_s": pf[0].pairs_included / float(np.mean(w)),
+                            "band_ms": float(np.mean([q.bb_dot_secs for q in pf])) * 1e3,
+                            "tile_ms": float(np.mean([q.norm_secs for q in pf])) * 1e3}
+        out[name] = res
+        log(f"{name}: {res}")
+    out["reference_published_s"] = {"1000G_shape": {"exact_f64": 41.1, "fast_f32": 32.0},
+                                     "biobank_shape": {"exact_f64": 727.0, "f32": 424.0}}
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="panel", choices=["panel", "sketch", "partitioned"])
+    ap.add_argument("--workload", default="panel",
+                    choices=["panel", "sketch", "partitioned", "reference_shapes"])
     ap.add_argument("--n", type=int, default=50_000)
     ap.add_argument("--m", type=int, default=100_000)
     ap.add_argument("--fp32-steps", type=int, default=2)
@@ -529,6 +565,10 @@ def main() -> int:
     if args.workload == "sketch":
         print(json.dumps({"metric": "CountSketch LD scores", **leg_sketch(args, L, synth)}),
               flush=True)
+        return 0
+    if args.workload == "reference_shapes":
+        print(json.dumps({"metric": "l2 wall time at the reference's published shapes",
+                          **leg_reference_shapes(args, L, synth)}), flush=True)
         return 0
     if args.workload == "partitioned":
         print(json.dumps({"metric": "partitioned LD scores (K=50)", **leg_partitioned(args, L, synth)}),
