@@ -131,6 +131,13 @@ namespace ldsc {
 // kSkMembers, so that a chunk's members come from one short window of each
 // .bed row; with --keep, one chunk of all members.
 constexpr uint32_t kSkMembers = 16384;
+
+// LDS slot of 16-byte unit u of a row quad in the chunk gather of engine.hip
+// (member-bytes 4u .. 4u+3 of its four rows).  The staging writes of a wave
+// cover units 4l + t (lane l, step t); XOR-ing the low two bits with bits
+// 3-4 puts eight consecutive lanes in eight distinct groups of four banks.
+// A bijection within aligned groups of eight units.
+__host__ __device__ inline uint32_t gc_slot(uint32_t u) { return u ^ ((u >> 3) & 3u); }
 struct SketchPlan {
     uint64_t d = 0, d_pad = 0, nblocks = 0, ntiles = 0, chunk = 0, nchunks = 0;
     int32_t* d_fam = nullptr;     // one chunk: FAM row of each position, lane-major map
@@ -176,8 +183,8 @@ int gather_and_tile(const ldsc_panel* panel, const uint64_t* d_bed_idx, uint64_t
 // tiles the call's SNP rows in that order straight from their .bed windows
 // into ws->tiled; one integer-stat slot per chunk (*parts = chunks).
 int gather_tile_chunks(const ldsc_panel* panel, const uint64_t* d_bed_idx, uint64_t base,
-                       uint64_t m, const uint16_t* d_local, uint64_t m_pad, Workspace* ws,
-                       uint32_t* parts, char* err, uint64_t err_len);
+                       uint64_t m, const uint16_t* d_local, uint32_t chunk, uint64_t m_pad,
+                       Workspace* ws, uint32_t* parts, char* err, uint64_t err_len);
 
 // The exact / sketch engine on workspace `ws` (which must belong to
 // panel->device).  Synchronous: outputs are written when it returns.

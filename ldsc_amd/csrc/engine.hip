@@ -606,20 +606,19 @@ __global__ __launch_bounds__(512) void k_gather4(const uint8_t* __restrict__ sla
 // ------------------------------------------------------------------------
 // Kernel 1d: chunk-local gather into PLINK-code tiles (CountSketch order)
 // ------------------------------------------------------------------------
-// Workgroup (h, c) owns SNP rows 16h .. 16h+15 and member chunk c, whose
-// members come from FAM rows c*kSkMembers .. : one 4 KiB window of each
-// .bed row.  Wave w loads rows 16h + 4w .. +3 of the window (1 KiB of one
-// row per load), counts their integer stats (the chunk's slot) and stages
-// them byte-interleaved in LDS (word k of quad w = byte k of its four rows).
+// Workgroup (h, c) owns SNP rows 16h .. 16h+15 and member chunk c of
+// MEMBERS members, whose members come from FAM rows c*MEMBERS .. : one
+// MEMBERS/4-byte window of each .bed row.  Wave w loads rows 16h + 4w .. +3
+// of the window (1 KiB of one row per load), counts their integer stats
+// (the chunk's slot) and stages them byte-interleaved in LDS (word k of
+// quad w = byte k of its four rows, in 16-byte slots placed by gc_slot).
 // Then thread t builds output dwords t + 256u of the chunk (16 positions
 // each, local[] gives each position's member within the chunk) for the
 // four quads, and stores them into the tiles; positions past the last
 // member and padding SNPs read as code 01 (missing).
 constexpr int kGcRows = 16;
-constexpr uint32_t kGcWords = kSkMembers / 4;  // LDS words per quad
-constexpr uint32_t kGcStride = kGcWords + 4;   // quad stride: word kGcWords reads as
-                                               // code 01 (missing) in every byte
 
+template <uint32_t MEMBERS>
 __global__ __launch_bounds__(256) void k_gather_chunk(const uint8_t* __restrict__ slabs,
                                                        uint64_t pitch, uint32_t src_dw, uint32_t n,
                                                        const uint64_t* __restrict__ bed_idx,
@@ -627,10 +626,15 @@ __global__ __launch_bounds__(256) void k_gather_chunk(const uint8_t* __restrict_
                                                        const uint16_t* __restrict__ local,
                                                        uint32_t nkb, uint32_t* __restrict__ tiled,
                                                        uint32_t* __restrict__ part) {
+    constexpr uint32_t kGcWords = MEMBERS / 4;    // LDS words per quad
+    constexpr uint32_t kGcStride = kGcWords + 4;  // quad stride; word kGcWords (its slot
+                                                  // maps to itself) reads as code 01
+                                                  // (missing) in every byte
+    constexpr uint32_t cw = MEMBERS / 16;         // window dwords per row
+    constexpr int kPasses = (int)(cw / 256);      // 256 window dwords per pass
     extern __shared__ uint32_t lds[];  // 4 quads x kGcStride
     const int tid = threadIdx.x, wv = tid >> 6, ln = tid & 63;
     const uint32_t j0 = blockIdx.x * kGcRows, c = blockIdx.y;
-    const uint32_t cw = kSkMembers / 16;  // window dwords per row
     if (ln == 0) lds[wv * kGcStride + kGcWords] = 0x55555555u;
     // load: wave wv, rows j0 + 4wv + r
     {
@@ -645,7 +649,7 @@ __global__ __launch_bounds__(256) void k_gather_chunk(const uint8_t* __restrict_
         }
         uint32_t cnt[4][3] = {};
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
+        for (int i = 0; i < kPasses; ++i) {
             const uint32_t k0 = i * 256 + 4 * ln;   // window dword of the lane's first word
             const uint64_t q0 = (uint64_t)c * cw + k0;
             uint4 v[4];
@@ -670,7 +674,7 @@ __global__ __launch_bounds__(256) void k_gather_chunk(const uint8_t* __restrict_
                 }
                 uint32_t o[4];
                 transpose_bytes4(w[0], w[1], w[2], w[3], o);
-                reinterpret_cast<uint4*>(lds + wv * kGcStride)[k0 + t] =
+                reinterpret_cast<uint4*>(lds + wv * kGcStride)[gc_slot(k0 + t)] =
                     make_uint4(o[0], o[1], o[2], o[3]);
             }
         }
@@ -705,7 +709,7 @@ __global__ __launch_bounds__(256) void k_gather_chunk(const uint8_t* __restrict_
     // transpose.
     uint32_t* tbase = tiled + ((size_t)(j0 / kGroup) * nkb * 64 + (j0 % kGroup)) * 4;
 #pragma unroll 1
-    for (int u = 0; u < 4; ++u) {
+    for (int u = 0; u < kPasses; ++u) {
         const uint32_t ql = tid + 256 * u, q = c * cw + ql;
         if (q >= nkb * 8) break;
         const uint4* lp = reinterpret_cast<const uint4*>(local + (size_t)q * 16);
@@ -715,7 +719,7 @@ __global__ __launch_bounds__(256) void k_gather_chunk(const uint8_t* __restrict_
 #pragma unroll
         for (int e = 0; e < 16; ++e) {
             const uint32_t src = (ev[e >> 1] >> (16 * (e & 1))) & 0xFFFFu;
-            widx[e] = src == 0xFFFFu ? kGcWords : src >> 2;
+            widx[e] = src == 0xFFFFu ? kGcWords : (gc_slot(src >> 4) << 2) | ((src >> 2) & 3u);
             shv[e] = 2 * (src & 3);
         }
         // row dword 2t + h of K-block kb -> lane s + 32h, dword t
@@ -1589,20 +1593,23 @@ int gather_and_tile(const ldsc_panel* panel, const uint64_t* d_bed_idx, uint64_t
 }
 
 int gather_tile_chunks(const ldsc_panel* panel, const uint64_t* d_bed_idx, uint64_t base,
-                       uint64_t m, const uint16_t* d_local, uint64_t m_pad, Workspace* ws,
-                       uint32_t* parts, char* err, uint64_t err_len) {
+                       uint64_t m, const uint16_t* d_local, uint32_t chunk, uint64_t m_pad,
+                       Workspace* ws, uint32_t* parts, char* err, uint64_t err_len) {
     const uint64_t n = panel->n;
     const uint64_t n_pad = round_up(n, kKBlock);
     const uint32_t nkb = (uint32_t)(n_pad / kKBlock);
-    const uint32_t nchunks = (uint32_t)((n + kSkMembers - 1) / kSkMembers);
+    if (chunk != kSkMembers)
+        return fail(err, err_len, LDSC_ERR_INVALID_ARG, "unsupported sketch chunk %u", chunk);
+    const uint32_t nchunks = (uint32_t)((n + chunk - 1) / chunk);
     hipStream_t st = ws->stream;
     HIPC(ws->stats.ensure((size_t)m_pad * 16 * (1 + nchunks)));
     HIPC(ws->tiled.ensure((size_t)m_pad * n_pad / 4));
     *parts = nchunks;
-    const size_t lds = (size_t)4 * kGcStride * 4;
-    HIPC(hipFuncSetAttribute((const void*)k_gather_chunk,
+    const size_t lds = (size_t)4 * (chunk / 4 + 4) * 4;  // four row quads
+    HIPC(hipFuncSetAttribute((const void*)k_gather_chunk<kSkMembers>,
                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    hipLaunchKernelGGL(k_gather_chunk, dim3((unsigned)(m_pad / kGcRows), nchunks), dim3(256), lds,
+    hipLaunchKernelGGL(k_gather_chunk<kSkMembers>, dim3((unsigned)(m_pad / kGcRows), nchunks),
+                       dim3(256), lds,
                        st, panel->d_slabs, panel->stride, (uint32_t)((panel->bps + 3) / 4),
                        (uint32_t)n, d_bed_idx, base, (uint32_t)m, (uint32_t)m_pad, d_local, nkb,
                        ws->tiled.as<uint32_t>(), ws->stats.as<uint32_t>() + (size_t)m_pad * 4);
@@ -1742,8 +1749,8 @@ int l2_begin(const ldsc_panel* panel, const ldsc_snp* snps, uint64_t m, const ld
     uint32_t parts = 1;
     if (d && sp->d_local)
         rc = gather_tile_chunks(panel, contiguous ? nullptr : ws->bed_idx.as<uint64_t>(),
-                                contiguous ? base : 0, m, sp->d_local, m_pad, ws, &parts, err,
-                                err_len);
+                                contiguous ? base : 0, m, sp->d_local, (uint32_t)sp->chunk, m_pad,
+                                ws, &parts, err, err_len);
     else
         rc = gather_and_tile(panel, contiguous ? nullptr : ws->bed_idx.as<uint64_t>(),
                              contiguous ? base : 0, m, d ? sp->d_fam : panel->d_keep32, m_pad,

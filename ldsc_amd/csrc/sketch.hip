@@ -449,7 +449,8 @@ extern "C" int ldsc_l2_sketch_buckets(const ldsc_panel* panel, const uint64_t* b
         SCHK(hipMemcpyAsync(ws->bed_idx.p, bed_idx, m * 8, hipMemcpyHostToDevice, st));
         uint32_t parts = 0;
         rc = sp->d_local ? gather_tile_chunks(panel, ws->bed_idx.as<uint64_t>(), 0, m,
-                                              sp->d_local, m_pad, ws, &parts, err, err_len)
+                                              sp->d_local, (uint32_t)sp->chunk, m_pad, ws,
+                                              &parts, err, err_len)
                          : gather_and_tile(panel, ws->bed_idx.as<uint64_t>(), 0, m, sp->d_fam,
                                            m_pad, false, /*tile=*/true, ws, &parts, err, err_len);
         if (rc) break;
