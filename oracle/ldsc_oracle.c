@@ -13,10 +13,12 @@
  * per-SNP pair bounds, banded tiles) -- is checked against an independent
  * restatement of the reference's chunk / ring / window loop.
  *
- * Dot products accumulate in 4 (f64) or 8 (f32) vector lanes; the reference
- * uses faer's blocked GEMM, whose summation order is not reproducible here,
- * so the f64 results agree with the reference to rounding (~1e-15 relative)
- * and the f32 results to the tolerance stated in the tests.
+ * The products of the chunk and window columns run as a blocked GEMM
+ * (blocks of 1024 elements, 4 (f64) or 8 (f32) vector lanes each); the
+ * reference uses faer's blocked GEMM, whose summation order is not
+ * reproducible here, so the f64 results agree with the reference to
+ * rounding (~1e-15 relative) and the f32 results to the tolerance stated in
+ * the tests.
  */
 #include <math.h>
 #include <stdint.h>
@@ -257,41 +259,113 @@ static inline double quad_corr(double val, double n_inv_sq, double d_f, double r
 }
 
 /* ------------------------------------------------------------------ */
-/* Dot products (the BB / AB GEMMs of compute.rs:1668-1810, 1949-2179)   */
+/* Blocked products of many columns (the faer GEMMs of                    */
+/* compute.rs:1701-1708, 1802-1809, 2010-2017, 2171-2178):               */
+/* G[j * na + i] = <A_i, B_j> for columns of length n given by pointers. */
+/* The length is cut into blocks of ORC_KB elements; within a block a    */
+/* 4 x 2 register tile of vector accumulators runs over it and its lane  */
+/* sums are added to G, so every product sums in the same fixed order    */
+/* whatever the threading.  Threads take (group of B, range of A) tasks. */
 /* ------------------------------------------------------------------ */
-static double dot_f64(const double* a, const double* b, uint64_t n) {
-    v4d acc0 = {0, 0, 0, 0}, acc1 = {0, 0, 0, 0};
-    uint64_t i = 0;
-    for (; i + 8 <= n; i += 8) {
-        v4d a0, a1, b0, b1;
-        memcpy(&a0, a + i, 32);
-        memcpy(&a1, a + i + 4, 32);
-        memcpy(&b0, b + i, 32);
-        memcpy(&b1, b + i + 4, 32);
-        acc0 += a0 * b0;
-        acc1 += a1 * b1;
+#define ORC_KB 1024
+#define ORC_JG 32
+#define ORC_IR 64
+
+static void gram_f64(const double* const* A, uint64_t na, const double* const* B, uint64_t nb,
+                     uint64_t n, double* G) {
+    memset(G, 0, sizeof(double) * na * nb);
+    const uint64_t ngroups = (nb + ORC_JG - 1) / ORC_JG, nranges = (na + ORC_IR - 1) / ORC_IR;
+#pragma omp parallel for schedule(dynamic, 1)
+    for (uint64_t task = 0; task < ngroups * nranges; ++task) {
+        const uint64_t g = task / nranges, h = task % nranges;
+        const uint64_t j0 = g * ORC_JG, j1 = j0 + ORC_JG < nb ? j0 + ORC_JG : nb;
+        const uint64_t i0 = h * ORC_IR, i1 = i0 + ORC_IR < na ? i0 + ORC_IR : na;
+        for (uint64_t k0 = 0; k0 < n; k0 += ORC_KB) {
+            const uint64_t k1 = k0 + ORC_KB < n ? k0 + ORC_KB : n;
+            const uint64_t kv = k0 + ((k1 - k0) / 4) * 4;  /* vector part [k0, kv) */
+            for (uint64_t i = i0; i < i1; i += 4) {
+                const uint64_t ni = i1 - i < 4 ? i1 - i : 4;
+                for (uint64_t j = j0; j < j1; j += 2) {
+                    const uint64_t nj = j1 - j < 2 ? j1 - j : 2;
+                    v4d acc[4][2];
+                    memset(acc, 0, sizeof(acc));
+                    /* edge tiles repeat a valid column; its sums are not stored */
+                    const double* pa[4];
+                    const double* pb[2];
+                    for (uint64_t u = 0; u < 4; ++u) pa[u] = A[i + (u < ni ? u : 0)];
+                    for (uint64_t v = 0; v < 2; ++v) pb[v] = B[j + (v < nj ? v : 0)];
+                    for (uint64_t k = k0; k < kv; k += 4) {
+                        v4d a0, a1, a2, a3, b0, b1;
+                        memcpy(&a0, pa[0] + k, 32);
+                        memcpy(&a1, pa[1] + k, 32);
+                        memcpy(&a2, pa[2] + k, 32);
+                        memcpy(&a3, pa[3] + k, 32);
+                        memcpy(&b0, pb[0] + k, 32);
+                        memcpy(&b1, pb[1] + k, 32);
+                        acc[0][0] += a0 * b0; acc[0][1] += a0 * b1;
+                        acc[1][0] += a1 * b0; acc[1][1] += a1 * b1;
+                        acc[2][0] += a2 * b0; acc[2][1] += a2 * b1;
+                        acc[3][0] += a3 * b0; acc[3][1] += a3 * b1;
+                    }
+                    for (uint64_t u = 0; u < ni; ++u)
+                        for (uint64_t v = 0; v < nj; ++v) {
+                            double s = (acc[u][v][0] + acc[u][v][1]) +
+                                       (acc[u][v][2] + acc[u][v][3]);
+                            for (uint64_t k = kv; k < k1; ++k) s += A[i + u][k] * B[j + v][k];
+                            G[(j + v) * na + i + u] += s;
+                        }
+                }
+            }
+        }
     }
-    v4d acc = acc0 + acc1;
-    double s = (acc[0] + acc[1]) + (acc[2] + acc[3]);
-    for (; i < n; ++i) s += a[i] * b[i];
-    return s;
 }
-static float dot_f32(const float* a, const float* b, uint64_t n) {
-    v8f acc0 = {0}, acc1 = {0};
-    uint64_t i = 0;
-    for (; i + 16 <= n; i += 16) {
-        v8f a0, a1, b0, b1;
-        memcpy(&a0, a + i, 32);
-        memcpy(&a1, a + i + 8, 32);
-        memcpy(&b0, b + i, 32);
-        memcpy(&b1, b + i + 8, 32);
-        acc0 += a0 * b0;
-        acc1 += a1 * b1;
+
+static void gram_f32(const float* const* A, uint64_t na, const float* const* B, uint64_t nb,
+                     uint64_t n, float* G) {
+    memset(G, 0, sizeof(float) * na * nb);
+    const uint64_t ngroups = (nb + ORC_JG - 1) / ORC_JG, nranges = (na + ORC_IR - 1) / ORC_IR;
+#pragma omp parallel for schedule(dynamic, 1)
+    for (uint64_t task = 0; task < ngroups * nranges; ++task) {
+        const uint64_t g = task / nranges, h = task % nranges;
+        const uint64_t j0 = g * ORC_JG, j1 = j0 + ORC_JG < nb ? j0 + ORC_JG : nb;
+        const uint64_t i0 = h * ORC_IR, i1 = i0 + ORC_IR < na ? i0 + ORC_IR : na;
+        for (uint64_t k0 = 0; k0 < n; k0 += ORC_KB) {
+            const uint64_t k1 = k0 + ORC_KB < n ? k0 + ORC_KB : n;
+            const uint64_t kv = k0 + ((k1 - k0) / 8) * 8;
+            for (uint64_t i = i0; i < i1; i += 4) {
+                const uint64_t ni = i1 - i < 4 ? i1 - i : 4;
+                for (uint64_t j = j0; j < j1; j += 2) {
+                    const uint64_t nj = j1 - j < 2 ? j1 - j : 2;
+                    v8f acc[4][2];
+                    memset(acc, 0, sizeof(acc));
+                    const float* pa[4];
+                    const float* pb[2];
+                    for (uint64_t u = 0; u < 4; ++u) pa[u] = A[i + (u < ni ? u : 0)];
+                    for (uint64_t v = 0; v < 2; ++v) pb[v] = B[j + (v < nj ? v : 0)];
+                    for (uint64_t k = k0; k < kv; k += 8) {
+                        v8f a0, a1, a2, a3, b0, b1;
+                        memcpy(&a0, pa[0] + k, 32);
+                        memcpy(&a1, pa[1] + k, 32);
+                        memcpy(&a2, pa[2] + k, 32);
+                        memcpy(&a3, pa[3] + k, 32);
+                        memcpy(&b0, pb[0] + k, 32);
+                        memcpy(&b1, pb[1] + k, 32);
+                        acc[0][0] += a0 * b0; acc[0][1] += a0 * b1;
+                        acc[1][0] += a1 * b0; acc[1][1] += a1 * b1;
+                        acc[2][0] += a2 * b0; acc[2][1] += a2 * b1;
+                        acc[3][0] += a3 * b0; acc[3][1] += a3 * b1;
+                    }
+                    for (uint64_t u = 0; u < ni; ++u)
+                        for (uint64_t v = 0; v < nj; ++v) {
+                            const v8f x = acc[u][v];
+                            float s = ((x[0] + x[1]) + (x[2] + x[3])) + ((x[4] + x[5]) + (x[6] + x[7]));
+                            for (uint64_t k = kv; k < k1; ++k) s += A[i + u][k] * B[j + v][k];
+                            G[(j + v) * na + i + u] += s;
+                        }
+                }
+            }
+        }
     }
-    v8f acc = acc0 + acc1;
-    float s = ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
-    for (; i < n; ++i) s += a[i] * b[i];
-    return s;
 }
 
 /* ------------------------------------------------------------------ */
@@ -361,6 +435,14 @@ int orc_l2(const uint8_t* slabs, uint64_t bps, uint64_t n_fam, const int64_t* ke
 
     double* r2u_bb = (double*)malloc(sizeof(double) * chunk_c * chunk_c);
     double* r2u_ab = (double*)malloc(sizeof(double) * (max_w ? max_w : 1) * chunk_c);
+    /* column pointers and products of the blocked GEMMs */
+    const uint64_t gram_len = ((max_w > chunk_c ? max_w : chunk_c) + 1) * chunk_c;
+    const double** colp64 = (const double**)malloc(sizeof(double*) * chunk_c);
+    const double** winp64 = (const double**)malloc(sizeof(double*) * (max_w + 1));
+    const float** colp32 = (const float**)malloc(sizeof(float*) * chunk_c);
+    const float** winp32 = (const float**)malloc(sizeof(float*) * (max_w + 1));
+    double* gram64 = use_f32 ? NULL : (double*)malloc(sizeof(double) * gram_len);
+    float* gram32 = use_f32 ? (float*)malloc(sizeof(float) * gram_len) : NULL;
     double* aeff_chunk = (double*)malloc(sizeof(double) * chunk_c * K);
     double* aeff_win = (double*)malloc(sizeof(double) * (max_w ? max_w : 1) * K);
     uint64_t win_lo = 0; /* window = [win_lo, chunk_start) (ring contents) */
@@ -489,16 +571,17 @@ int orc_l2(const uint8_t* slabs, uint64_t bps, uint64_t n_fam, const int64_t* ke
 
         /* ---- B x B (compute.rs:1633-1893) ---- */
         if (!chunk_all_zero) {
-#pragma omp parallel for schedule(dynamic, 1)
             for (uint64_t j = 0; j < c; ++j) {
                 const uint64_t sj = (cs0 + j) % ring;
+                if (use_f32) colp32[j] = ring32 + sj * gemm_n;
+                else colp64[j] = ring64 + sj * gemm_n;
+            }
+            if (use_f32) gram_f32(colp32, c, colp32, c, gemm_n, gram32);
+            else gram_f64(colp64, c, colp64, c, gemm_n, gram64);
+            for (uint64_t j = 0; j < c; ++j) {
                 r2u_bb[j * c + j] = 1.0;
                 for (uint64_t k = 0; k < j; ++k) {
-                    const uint64_t sk = (cs0 + k) % ring;
-                    double val = use_f32 ? (double)dot_f32(ring32 + sk * gemm_n,
-                                                           ring32 + sj * gemm_n, gemm_n)
-                                         : dot_f64(ring64 + sk * gemm_n, ring64 + sj * gemm_n,
-                                                   gemm_n);
+                    double val = use_f32 ? (double)gram32[j * c + k] : gram64[j * c + k];
                     double r;
                     if (sketch_d)
                         r = quad_corr(val, n_inv_sq, (double)sketch_d, r2u_a, r2u_b,
@@ -533,15 +616,21 @@ int orc_l2(const uint8_t* slabs, uint64_t bps, uint64_t n_fam, const int64_t* ke
 
         /* ---- A x B (compute.rs:1897-2315) ---- */
         if (w > 0 && !(chunk_all_zero && window_all_zero)) {
-#pragma omp parallel for schedule(dynamic, 1)
             for (uint64_t j = 0; j < c; ++j) {
                 const uint64_t sj = (cs0 + j) % ring;
+                if (use_f32) colp32[j] = ring32 + sj * gemm_n;
+                else colp64[j] = ring64 + sj * gemm_n;
+            }
+            for (uint64_t i = 0; i < w; ++i) {
+                const uint64_t si = (win_lo + i) % ring;
+                if (use_f32) winp32[i] = ring32 + si * gemm_n;
+                else winp64[i] = ring64 + si * gemm_n;
+            }
+            if (use_f32) gram_f32(winp32, w, colp32, c, gemm_n, gram32);
+            else gram_f64(winp64, w, colp64, c, gemm_n, gram64);
+            for (uint64_t j = 0; j < c; ++j) {
                 for (uint64_t i = 0; i < w; ++i) {
-                    const uint64_t si = (win_lo + i) % ring;
-                    double val = use_f32 ? (double)dot_f32(ring32 + si * gemm_n,
-                                                           ring32 + sj * gemm_n, gemm_n)
-                                         : dot_f64(ring64 + si * gemm_n, ring64 + sj * gemm_n,
-                                                   gemm_n);
+                    double val = use_f32 ? (double)gram32[j * w + i] : gram64[j * w + i];
                     double r;
                     if (sketch_d)
                         r = quad_corr(val, n_inv_sq, (double)sketch_d, r2u_a, r2u_b,
@@ -582,6 +671,7 @@ int orc_l2(const uint8_t* slabs, uint64_t bps, uint64_t n_fam, const int64_t* ke
 
     free(bl); free(bsz); free(ring64); free(ring32); free(kurt); free(pq);
     free(r2u_bb); free(r2u_ab); free(aeff_chunk); free(aeff_win); free(rawcol);
+    free(colp64); free(winp64); free(colp32); free(winp32); free(gram64); free(gram32);
     free(bucket); free(sgn);
     return 0;
 }
