@@ -377,11 +377,11 @@ def leg_serving(args, L, synth):
         log(f"serving burst (window {window_us} us): {out}")
         return out
 
-    burst = burst_unbatched = None
+    burst = burst_batched = None
     if args.serve_burst_inflight > 0:
         run_burst(args.serve_window_us)  # warm-up: grows the pooled workspaces
-        burst_unbatched = run_burst(0)
         burst = run_burst(args.serve_window_us)
+        burst_batched = run_burst(args.serve_batched_window_us)
     for pn in panels:
         pn.close()
     lats = np.array(lats)
@@ -400,7 +400,7 @@ def leg_serving(args, L, synth):
         "panels": "N=50000 x M=%d and N=10000 x M=%d resident" % (args.serve_m_big,
                                                                   args.serve_m_small),
         "burst": burst,
-        "burst_one_launch_per_request": burst_unbatched,
+        "burst_shared_band_launches": burst_batched,
     }
     log(f"serving: {res}")
     return res
@@ -530,8 +530,10 @@ def main() -> int:
     ap.add_argument("--qps", type=float, default=200.0)
     ap.add_argument("--serve-requests", type=int, default=400)
     ap.add_argument("--serve-inflight", type=int, default=4)
-    ap.add_argument("--serve-window-us", type=int, default=200,
+    ap.add_argument("--serve-window-us", type=int, default=0,
                     help="band batching window of the request server (0: one launch each)")
+    ap.add_argument("--serve-batched-window-us", type=int, default=200,
+                    help="batching window of the comparison burst with shared band launches")
     ap.add_argument("--serve-burst-inflight", type=int, default=8,
                     help="workers of the closed-burst capacity run (0: skip it)")
     ap.add_argument("--serve-m-big", type=int, default=1_000_000)

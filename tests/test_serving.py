@@ -17,7 +17,7 @@ def test_server_create_errors_and_unknown_ids():
         Server(queue_capacity=0)
     with pytest.raises(L.LdscError):
         Server(batch_window_us=2_000_000)
-    with Server(max_inflight=2, batch_window_us=0) as s:
+    with Server(max_inflight=2) as s:
         assert s.stats() == {"requests_done": 0, "band_launches": 0, "batched_requests": 0,
                              "max_batch": 0}
     with Server(max_inflight=2) as s:
