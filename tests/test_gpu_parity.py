@@ -290,6 +290,32 @@ def test_exact_kernels_agree():
                 assert _rel(a, b) < TOL_KERNELS, (per_chr, masking)
 
 
+@pytest.mark.parametrize("n", [1_800_000, 1_800_001])
+def test_exact_kernels_at_the_e2m1_limit(n):
+    """The largest N of the e2m1 kernel (f32 sums of products of v/2, q/2
+    exact while 9 N_pad <= 2^24) and one past it (the exact form switches to
+    the int8 kernel).  Planted rows make the sums as large as they get: two
+    rows all missing (v/2 = 1.5 in every cell, 2.25 N_pad per pair), two all
+    dosage 2, one all dosage 1.  Either way the result must equal the int8
+    kernel's (int32 sums) to a few ulps."""
+    m = 96
+    p = _panel(n, m, seed=17, miss=0.02)
+    bps = (n + 3) // 4
+    for row, byte in ((3, 0x55), (40, 0x55), (7, 0x00), (50, 0x00), (60, 0xAA)):
+        p.body[row, :] = byte
+        if n % 4:  # padding bits of the last byte stay code 00
+            p.body[row, bps - 1] &= (1 << (2 * (n % 4))) - 1
+    snps = L.make_snp_array(p.chr, p.cm, p.bp)
+    print(f"e2m1 limit n={n} m={m}", flush=True)
+    with L.Panel(p.bed_bytes(), n, m) as panel:
+        cfg = dict(mode=L.WindowMode.Snp(60), chunk_size=32)
+        a, ma, _ = L.compute_ldscore(panel, snps, L.L2Config(**cfg))
+        b, mb, _ = L.compute_ldscore(panel, snps, L.L2Config(**cfg, exact_kernel="i8"))
+    assert np.array_equal(ma, mb)
+    assert _rel(a, b) < TOL_KERNELS, _rel(a, b)
+    assert np.all(np.isfinite(a))
+
+
 def test_exact_and_fp32_at_panel_scale():
     """N=50,000 x M=8,000 slice of the panel-scale workload: the e2m1 and
     int8 exact kernels agree to a few ulps, the FP32 form agrees to its
