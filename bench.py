@@ -351,6 +351,25 @@ def leg_serving(args, L, synth):
             lats.append(srv.wait(t))
         t_end = time.perf_counter()
         poisson_stats = srv.stats()
+        # the same stream at higher offered rates (same requests, rescaled gaps)
+        sweep = []
+        for q in [float(x) for x in args.qps_sweep.split(",") if x]:
+            sl, st = [], []
+            t0 = time.perf_counter()
+            t_next = t0
+            for (which, snps, cfg, annot, _), gap in zip(reqs, gaps * (args.qps / q)):
+                t_next += gap
+                delay = t_next - time.perf_counter()
+                if delay > 0:
+                    time.sleep(delay)
+                st.append(srv.submit(panels[which], snps, cfg, annot))
+            sl = np.array([srv.wait(t) for t in st])
+            t1 = time.perf_counter()
+            sweep.append({"offered_qps": q, "achieved_qps": len(st) / (t1 - t0),
+                          "p50_ms": float(np.percentile(sl, 50)) * 1e3,
+                          "p99_ms": float(np.percentile(sl, 99)) * 1e3,
+                          "snp_pairs_per_s": float(sum(r[4] for r in reqs)) / (t1 - t0)})
+            log(f"serving at {q} QPS: {sweep[-1]}")
     # capacity: the same requests submitted at once (a closed burst), on more
     # workers, with band batching
     def run_burst(window_us):
@@ -399,6 +418,7 @@ def leg_serving(args, L, synth):
         "batched_requests": poisson_stats["batched_requests"],
         "panels": "N=50000 x M=%d and N=10000 x M=%d resident" % (args.serve_m_big,
                                                                   args.serve_m_small),
+        "qps_sweep": sweep,
         "burst": burst,
         "burst_shared_band_launches": burst_batched,
     }
@@ -528,6 +548,8 @@ def main() -> int:
     ap.add_argument("--skip-peaks", action="store_true",
                     help="do not measure the sustained MFMA and copy peaks")
     ap.add_argument("--qps", type=float, default=200.0)
+    ap.add_argument("--qps-sweep", default="500,1000,2000",
+                    help="further offered rates for the same request stream (comma list)")
     ap.add_argument("--serve-requests", type=int, default=400)
     ap.add_argument("--serve-inflight", type=int, default=4)
     ap.add_argument("--serve-window-us", type=int, default=0,
