@@ -2150,6 +2150,66 @@ using namespace ldsc;
 // ------------------------------------------------------------------------
 // C ABI: panels
 // ------------------------------------------------------------------------
+// Moves `rows` BED rows of `bps` bytes, packed back to back in `src`, to
+// rows `stride` bytes apart in `dst` (stride = bps rounded up to 16), with
+// the padding bytes zero.  One thread writes one 4-byte word of the output.
+__global__ void k_restride_rows(const uint8_t* __restrict__ src, uint64_t bps, uint64_t rows,
+                                uint8_t* __restrict__ dst, uint64_t stride) {
+    const uint64_t wpr = stride / 4;
+    const uint64_t total = rows * wpr;
+    for (uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; t < total;
+         t += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t r = t / wpr, w = t - r * wpr;
+        const uint8_t* row = src + r * bps;
+        uint32_t v = 0;
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            const uint64_t off = 4 * w + b;
+            if (off < bps) v |= (uint32_t)row[off] << (8 * b);
+        }
+        reinterpret_cast<uint32_t*>(dst + r * stride)[w] = v;
+    }
+}
+
+// Uploads the BED body (m rows of bps bytes after the 3-byte header) into
+// the panel's strided rows.  A pageable copy reaches PCIe speed only when it
+// is large and contiguous, so the body goes up in contiguous pieces of at
+// most kUploadPiece bytes into a device staging buffer, and a kernel moves
+// each piece's rows into place.  When bps is already a multiple of 16 the
+// pieces land in place directly.
+static hipError_t upload_rows(ldsc_panel* p, const uint8_t* body) {
+    constexpr uint64_t kUploadPiece = 256ull << 20;
+    const uint64_t bps = p->bps, stride = p->stride, m = p->m_bim;
+    const uint64_t piece_rows = std::max<uint64_t>(1, kUploadPiece / bps);
+    hipStream_t st = nullptr;
+    uint8_t* stage = nullptr;
+    hipError_t e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+    if (e == hipSuccess && bps != stride)
+        e = hipMalloc(&stage, std::min(piece_rows, m) * bps);
+    for (uint64_t r0 = 0; e == hipSuccess && r0 < m; r0 += piece_rows) {
+        const uint64_t rows = std::min(piece_rows, m - r0);
+        if (bps == stride) {
+            e = hipMemcpyAsync(p->d_slabs + r0 * stride, body + r0 * bps, rows * bps,
+                               hipMemcpyHostToDevice, st);
+            continue;
+        }
+        e = hipMemcpyAsync(stage, body + r0 * bps, rows * bps, hipMemcpyHostToDevice, st);
+        if (e != hipSuccess) break;
+        const uint64_t words = rows * (stride / 4);
+        const uint32_t grid = (uint32_t)std::min<uint64_t>((words + 255) / 256, 8192);
+        hipLaunchKernelGGL(k_restride_rows, dim3(grid), dim3(256), 0, st, stage, bps, rows,
+                           p->d_slabs + r0 * stride, stride);
+        e = hipGetLastError();
+    }
+    if (st) {
+        const hipError_t es = hipStreamSynchronize(st);
+        if (e == hipSuccess) e = es;
+        (void)hipStreamDestroy(st);
+    }
+    if (stage) (void)hipFree(stage);
+    return e;
+}
+
 extern "C" int ldsc_l2_open_panel(const uint8_t* bed, uint64_t bed_len, uint64_t n_fam,
                                   uint64_t m_bim, const int64_t* keep_idx, uint64_t n_keep,
                                   ldsc_panel** out, char* err, uint64_t err_len) {
@@ -2186,9 +2246,7 @@ extern "C" int ldsc_l2_open_panel(const uint8_t* bed, uint64_t bed_len, uint64_t
     // of a row are aligned vector loads that never leave the allocation
     hipError_t e = hipGetDevice(&p->device);
     if (e == hipSuccess) e = hipMalloc(&p->d_slabs, p->stride * m_bim);
-    if (e == hipSuccess) e = hipMemset(p->d_slabs, 0, p->stride * m_bim);
-    if (e == hipSuccess)
-        e = hipMemcpy2D(p->d_slabs, p->stride, bed + 3, bps, bps, m_bim, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = upload_rows(p, bed + 3);
     if (e == hipSuccess && keep_idx) {
         p->keep.assign(keep_idx, keep_idx + n_keep);
         e = hipMalloc(&p->d_keep, n_keep * 8);

@@ -51,6 +51,26 @@ def test_decode_bit_exact(oracle, n):
         assert np.array_equal(miss, exp)
 
 
+@pytest.mark.parametrize("n,m", [(2490, 450_000), (1024, 1_100_000)])
+def test_panel_upload_across_pieces(oracle, n, m):
+    """A BED body above the 256 MiB upload piece goes up in several
+    contiguous pieces; rows of 623 bytes (restrided to 624 on the device) and
+    rows of 256 bytes (already aligned) decode exactly on both sides of every
+    piece boundary."""
+    bps = (n + 3) // 4
+    piece = (256 << 20) // bps
+    print(f"panel upload n={n} m={m} piece_rows={piece}", flush=True)
+    rng = np.random.default_rng(n)
+    bed = np.empty(3 + bps * m, np.uint8)
+    bed[:3] = (0x6C, 0x1B, 0x01)
+    bed[3:] = rng.integers(0, 256, bps * m, dtype=np.uint8)
+    idx = np.array(sorted({0, 1, piece - 1, piece, piece + 1, m - 2, m - 1}), np.uint64)
+    with L.Panel(bed, n, m) as panel:
+        dos, _, st, _ = L.decode(panel, idx)
+    assert np.array_equal(st, oracle.stats(bed, n, idx))
+    assert np.array_equal(dos, oracle.decode(bed, n, idx))
+
+
 @pytest.mark.parametrize("n,d", [(1000, 50), (2049, 200), (3000, 3000), (777, 5)])
 def test_sketch_buckets_bit_exact(oracle, n, d):
     print(f"sketch buckets n={n} d={d}", flush=True)
