@@ -20,13 +20,16 @@ Default run (one JSON line):
 `--workload sketch` (CountSketch, N=100,000 x M=200,000, d in {200, 1000, 5000}),
 `--workload partitioned` (partitioned, N=20,000 x M=500,000, K=50, kb 1000) and
 `--workload reference_shapes` (the panel shapes of the reference's published
-CPU wall times) print their own JSON lines.  `--gpus` above 1 is refused: this build targets one GPU.
+CPU wall times) and `--workload cli` (the `ldsc l2` command end to end, from
+the PLINK files to the written LD-score files, at the 1000G shape) print
+their own JSON lines.  `--gpus` above 1 is refused: this build targets one GPU.
 """
 from __future__ import annotations
 
 import argparse
 import json
 import os
+import re
 import sys
 import time
 
@@ -530,13 +533,68 @@ def leg_reference_shapes(args, L, synth):
     return out
 
 
+def leg_cli(args, L, synth):
+    """The `ldsc l2` command end to end (ldsc_amd.cli): BIM/FAM/BED read, the
+    panel upload, the LD scores, the .l2.ldscore.gz / .l2.M / .l2.M_5_50 files
+    of the genome and of each chromosome, and the summary.  The panel has the
+    1000G shape (N=2,490, M=1.66 M, here over 22 chromosomes of equal SNP
+    count) with --ld-wind-kb 1000, the default chunk 200 and the default
+    per-chromosome pass.  The reference publishes 41.1 s (exact f64) and
+    about 32 s (--fast-f32) for its `l2` run on 1000G on 16 vCPU of an EPYC
+    7R13; the data differ, so this is a same-shape wall time."""
+    import contextlib
+    import io
+    import shutil
+    import tempfile
+
+    from ldsc_amd import cli
+
+    n, m = 2490, 1_660_000
+    tmp = tempfile.mkdtemp(prefix="ldsc_cli_")
+    try:
+        t0 = time.perf_counter()
+        p = synth.make_panel(n, m, 20261213, miss_rate=0.01)
+        p.chr[:] = 1 + np.arange(m) * 22 // m
+        prefix = os.path.join(tmp, "g")
+        synth.write_plink(prefix, p)
+        del p
+        log(f"cli: PLINK files N={n} M={m} written ({time.perf_counter() - t0:.1f}s)")
+        res = {"n_indiv": n, "m_snps": m, "chromosomes": 22, "window": "--ld-wind-kb 1000"}
+        for tag, extra in (("exact", []), ("fast_f32", ["--fast-f32"])):
+            walls, stages = [], []
+            for i in range(1 + args.steps):
+                err = io.StringIO()
+                argv = ["l2", "--bfile", prefix, "--out", os.path.join(tmp, "o"),
+                        "--ld-wind-kb", "1000", "--verbose-timing", *extra]
+                t1 = time.perf_counter()
+                with contextlib.redirect_stdout(io.StringIO()), contextlib.redirect_stderr(err):
+                    rc = cli.main(argv)
+                w = time.perf_counter() - t1
+                if rc != 0:
+                    raise RuntimeError(f"ldsc l2 {tag} failed: {err.getvalue()[-2000:]}")
+                st = {k: float(v) for k, v in
+                      re.findall(r"\[perf\] (\w+)=([0-9.]+)s", err.getvalue())}
+                log(f"cli {tag} run {i}: {w:.3f}s {st}")
+                if i:
+                    walls.append(w)
+                    stages.append(st)
+            keys = sorted(set().union(*stages))
+            res[tag] = {"wall_s": float(np.mean(walls)), "runs": len(walls),
+                        "stages_s": {k: float(np.mean([s_.get(k, 0.0) for s_ in stages]))
+                                     for k in keys}}
+        res["reference_published_s"] = {"exact_f64": 41.1, "fast_f32": 32.0}
+        return res
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="panel",
-                    choices=["panel", "sketch", "partitioned", "reference_shapes"])
+                    choices=["panel", "sketch", "partitioned", "reference_shapes", "cli"])
     ap.add_argument("--n", type=int, default=50_000)
     ap.add_argument("--m", type=int, default=100_000)
     ap.add_argument("--fp32-steps", type=int, default=2)
@@ -593,6 +651,10 @@ def main() -> int:
     if args.workload == "reference_shapes":
         print(json.dumps({"metric": "l2 wall time at the reference's published shapes",
                           **leg_reference_shapes(args, L, synth)}), flush=True)
+        return 0
+    if args.workload == "cli":
+        print(json.dumps({"metric": "ldsc l2 command wall time, PLINK files to LD-score files",
+                          **leg_cli(args, L, synth)}), flush=True)
         return 0
     if args.workload == "partitioned":
         print(json.dumps({"metric": "partitioned LD scores (K=50)", **leg_partitioned(args, L, synth)}),

@@ -3,6 +3,7 @@
 * libldsc_l2.so   -- the HIP engine for gfx950 behind include/ldsc_l2.h
 * libldsc_synth.so -- the seeded synthetic PLINK generator (host C, OpenMP)
 * libldsc_calib.so -- measured MFMA and HBM copy peaks for bench.py's roofline
+* libldsc_io.so    -- the host BIM reader and .l2.ldscore.gz writer (include/ldsc_io.h)
 
 Run ``python -m ldsc_amd.build`` (or ``__graft_entry__.build()``).
 """
@@ -56,8 +57,18 @@ def build_calib() -> str:
     return out
 
 
+def build_io() -> str:
+    out = os.path.join(HERE, "libldsc_io.so")
+    tmp = out + ".tmp"
+    _run(["g++", "-O3", "-fopenmp", "-fPIC", "-shared", "-std=c++17", "-Wall", "-o", tmp,
+          os.path.join(CSRC, "io.cpp"), "-lz"])
+    os.replace(tmp, out)
+    return out
+
+
 def build_all() -> None:
     build_synth()
+    build_io()
     build_engine()
     build_calib()
 

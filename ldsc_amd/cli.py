@@ -3,10 +3,10 @@
 Mirrors the reference's `l2` subcommand: flags of L2Args (src/cli.rs:169-355),
 the Python-style `--l2` injection (src/main.rs:39-78) and the orchestration
 and outputs of `run` (src/l2/mod.rs:58-795): BIM/FAM/annot/extract/keep/
-print-snps inputs, the MAF prefilter, `{out}.l2.ldscore.gz` (values `%.3f`),
+print-snps inputs, the MAF prefilter, `{out}.l2.ldscore.gz` (values `{:.3}`),
 `{out}.l2.M` / `.l2.M_5_50` (shortest round-trip decimals), per-chromosome
 files and the "Summary of LD Scores" block.  All LD-score arithmetic runs in
-libldsc_l2.so.
+libldsc_l2.so; the BIM reader and the gzip writer run in libldsc_io.so.
 
 Usage:  python -m ldsc_amd.cli l2 --bfile PREFIX --out OUT [--ld-wind-cm 1] ...
         python -m ldsc_amd.cli --bfile PREFIX --l2 --out OUT ...
@@ -25,8 +25,8 @@ from typing import List, Optional
 import numpy as np
 
 from . import _native as N
-from .l2 import L2Config, Panel, WindowMode, compute_ldscore, decode, parse_bim_str, \
-    snps_to_array
+from .formats import BimTable, write_ldscore
+from .l2 import L2Config, Panel, WindowMode, compute_ldscore, decode, make_snp_array
 
 
 # ----------------------------------------------------------------------------
@@ -173,8 +173,9 @@ def read_annot_path(path: str, thin: bool):
     return mat, names
 
 
-def read_annotations(prefix: str, thin: bool, snps) -> tuple:
-    """The three --annot forms of run (src/l2/mod.rs:147-252)."""
+def read_annotations(prefix: str, thin: bool, chr_: np.ndarray) -> tuple:
+    """The three --annot forms of run (src/l2/mod.rs:147-252); ``chr_`` is the
+    chromosome of each SNP of the LD-score list."""
     explicit = prefix.endswith((".annot", ".annot.gz", ".annot.bz2"))
     try_single = explicit
     if not explicit:
@@ -188,7 +189,7 @@ def read_annotations(prefix: str, thin: bool, snps) -> tuple:
         mat, names = read_annot_path(path, thin)
         src = path
     else:
-        chrs = sorted(set(s.chr for s in snps))
+        chrs = [int(c) for c in np.unique(chr_)]
         mats, names = [], []
         for c in chrs:
             m, nm = read_annot_path(resolve_annot_path(f"{prefix}{c}"), thin)
@@ -196,27 +197,18 @@ def read_annotations(prefix: str, thin: bool, snps) -> tuple:
             mats.append(m)
         mat = np.concatenate(mats, axis=0) if mats else np.zeros((0, 0))
         src = prefix + "*"
-    if mat.shape[0] != len(snps):
-        raise ValueError(f"Annotation file has {mat.shape[0]} rows but BIM has {len(snps)} "
+    if mat.shape[0] != len(chr_):
+        raise ValueError(f"Annotation file has {mat.shape[0]} rows but BIM has {len(chr_)} "
                          "SNPs — they must match exactly")
     print(f"Read {len(names)} annotations for {mat.shape[0]} SNPs from '{src}'")
     return mat, names
 
 
 # ----------------------------------------------------------------------------
-# outputs
+# outputs (the .l2.ldscore.gz writer is formats.write_ldscore)
 # ----------------------------------------------------------------------------
-def write_ldscore(path: str, snps, l2: np.ndarray, col_names: List[str]) -> None:
-    """write_ldscore_refs (src/l2/io.rs:94-125): gzip TSV, values to 3 places."""
-    with gzip.open(path, "wt", compresslevel=1) as f:
-        f.write("CHR\tSNP\tBP\t" + "\t".join(col_names) + "\n")
-        for i, s in enumerate(snps):
-            f.write(f"{s.chr}\t{s.snp}\t{s.bp}\t" +
-                    "\t".join("%.3f" % v for v in l2[i]) + "\n")
-
-
-def _percentile(vals: np.ndarray, q: float) -> float:
-    v = np.sort(vals)
+def _percentile(v: np.ndarray, q: float) -> float:
+    """Linear-interpolation percentile of the ascending-sorted ``v``."""
     if v.size == 1:
         return float(v[0])
     pos = q * (v.size - 1)
@@ -243,9 +235,10 @@ def print_ld_score_summary(out_path, l2_out: np.ndarray, col_names, maf_out: np.
     cols = [("MAF", maf_out)] + [(nm, l2_out[:, k]) for k, nm in enumerate(col_names)]
     print(f"\nSummary of LD Scores in {out_path}")
     print(" " * 8 + "".join(f"{nm:>8}" for nm, _ in cols))
+    srt = [np.sort(v) for _, v in cols]
     for lab in ("mean", "std", "min", "25%", "50%", "75%", "max"):
         row = f"{lab:>8}"
-        for _, v in cols:
+        for ci, (_, v) in enumerate(cols):
             if lab == "mean":
                 st = v.sum() / n
             elif lab == "std":
@@ -256,7 +249,7 @@ def print_ld_score_summary(out_path, l2_out: np.ndarray, col_names, maf_out: np.
             elif lab == "max":
                 st = v.max()
             else:
-                st = _percentile(v, {"25%": 0.25, "50%": 0.5, "75%": 0.75}[lab])
+                st = _percentile(srt[ci], {"25%": 0.25, "50%": 0.5, "75%": 0.75}[lab])
             row += f"{st:>8.4f}"
         print(row)
     print()
@@ -339,32 +332,35 @@ def run(args) -> int:
         mode = WindowMode.Cm(1.0 if args.ld_wind_cm is None else args.ld_wind_cm)
 
     bim_path, fam_path, bed_path = (args.bfile + ext for ext in (".bim", ".fam", ".bed"))
-    with open(bim_path) as f:
-        all_raw = parse_bim_str(f.read())
+    bim = BimTable.from_file(bim_path)
+    t_bim = time.perf_counter()
     n_fam = count_fam(fam_path)
-    print(f"Loaded {len(all_raw)} SNPs, {n_fam} individuals from '{args.bfile}'")
-    snps = all_raw
+    print(f"Loaded {bim.n} SNPs, {n_fam} individuals from '{args.bfile}'")
+    sel = bim.bed_idx  # BIM row of each SNP of the LD-score list, in BIM order
     if args.extract:
         ext = load_snp_set(args.extract)
-        snps = [s for s in all_raw if s.snp in ext]
-        print(f"  After --extract: {len(snps)} SNPs")
+        sel = sel[bim.name_in(ext)]
+        print(f"  After --extract: {sel.size} SNPs")
     print_set = load_print_snps(args.print_snps) if args.print_snps else None
     if args.annot and args.extract:
         print("WARNING: --annot with --extract is not supported. Ensure your annot files "
               "match the extracted SNP set.")
     annot, annot_names = (None, None)
     if args.annot:
-        annot, annot_names = read_annotations(args.annot, args.thin_annot, snps)
+        annot, annot_names = read_annotations(args.annot, args.thin_annot, bim.chr[sel])
     keep = load_individual_indices(args.keep, parse_fam_ids(fam_path)) if args.keep else None
 
+    t_in = time.perf_counter()
     bed = np.fromfile(bed_path, dtype=np.uint8)
-    panel = Panel(bed, n_fam, len(all_raw), keep=keep)
+    panel = Panel(bed, n_fam, bim.n, keep=keep)
+    del bed
     n_indiv = panel.n_indiv
+    t_panel = time.perf_counter()
 
     # MAF prefilter (src/l2/mod.rs:269-336; snp_stats.rs:150-195)
     pq_scaled = False
     if maf_pre and (args.maf is not None or pq_exp is not None):
-        idx = np.array([s.bed_idx for s in snps], np.uint64)
+        idx = sel.astype(np.uint64)
         _, _, st, _ = decode(panel, idx, dosage=False, miss=False, stats=True, std_f32=False)
         cnt = st[:, 1].astype(np.float64)
         freq = np.where(cnt > 0, st[:, 0] / np.maximum(2.0 * cnt, 1.0), 0.0)
@@ -375,11 +371,11 @@ def run(args) -> int:
         if not keep_mask.any():
             raise SystemExit("SNP prefilter removed all SNPs")
         if not keep_mask.all():
-            snps = [s for s, k in zip(snps, keep_mask) if k]
+            sel = sel[keep_mask]
             if annot is not None:
                 annot = annot[keep_mask]
         if args.maf is not None:
-            print(f"--maf-pre: kept {len(snps)} / {keep_mask.size} SNPs (MAF > {thr})")
+            print(f"--maf-pre: kept {sel.size} / {keep_mask.size} SNPs (MAF > {thr})")
         if pq_exp is not None and annot is not None:
             annot = annot * ((maf_all[keep_mask] * (1.0 - maf_all[keep_mask])) ** pq_exp)[:, None]
             pq_scaled = True
@@ -396,10 +392,16 @@ def run(args) -> int:
                    pq_exp=None if pq_scaled else pq_exp, verbose_timing=args.verbose_timing,
                    per_chromosome=not global_pass)
     t_c = time.perf_counter()
-    l2, maf, perf = compute_ldscore(panel, snps_to_array(snps), cfg, annot=annot)
+    snps = make_snp_array(bim.chr[sel], bim.cm[sel], bim.bp[sel], bed_idx=sel)
+    l2, maf, perf = compute_ldscore(panel, snps, cfg, annot=annot)
     panel.close()
+    t_done = time.perf_counter()
     if args.verbose_timing:
-        print(f"[perf] compute_ldscore_total={time.perf_counter() - t_c:.3f}s", file=sys.stderr)
+        print(f"[perf] bim_parse={t_bim - t_start:.3f}s", file=sys.stderr)
+        print(f"[perf] other_inputs={t_in - t_bim:.3f}s", file=sys.stderr)
+        print(f"[perf] bed_read_and_panel={t_panel - t_in:.3f}s", file=sys.stderr)
+        print(f"[perf] maf_prefilter={t_c - t_panel:.3f}s", file=sys.stderr)
+        print(f"[perf] compute_ldscore_total={t_done - t_c:.3f}s", file=sys.stderr)
 
     # outputs (src/l2/mod.rs:584-795)
     suffix = "" if pq_exp is None else f"_S{rust_display(pq_exp)}"
@@ -407,12 +409,13 @@ def run(args) -> int:
     pq = None if pq_exp is None else (maf * (1.0 - maf)) ** pq_exp
     pq_for_m = None if pq_scaled else pq
 
-    def out_ok(i):
-        return (args.maf is None or maf[i] > args.maf) and \
-            (print_set is None or snps[i].snp in print_set)
-
-    out_pos = [i for i in range(len(snps)) if out_ok(i)]
-    if print_set is not None and not out_pos:
+    ok = np.ones(sel.size, dtype=bool)
+    if args.maf is not None:
+        ok &= maf > args.maf
+    if print_set is not None:
+        ok &= bim.name_in(print_set, sel)
+    out_pos = np.flatnonzero(ok)
+    if print_set is not None and not out_pos.size:
         raise SystemExit("After merging with --print-snps, no SNPs remain.")
 
     def seq_sum(x):  # sequential f64 sums, as the reference accumulates M
@@ -430,17 +433,18 @@ def run(args) -> int:
         return [float(pos.size)], [float(big.sum())]
 
     out_path = f"{args.out}.l2.ldscore.gz"
-    write_ldscore(out_path, [snps[i] for i in out_pos], l2[out_pos], col_names)
-    ma, m5 = m_vals(range(len(snps)))
+    write_ldscore(out_path, bim, sel, l2, out_pos, col_names)
+    ma, m5 = m_vals(np.arange(sel.size))
     with open(f"{args.out}.l2.M", "w") as f:
         f.write(format_m_vals(ma))
     with open(f"{args.out}.l2.M_5_50", "w") as f:
         f.write(format_m_vals(m5))
-    for c in sorted(set(s.chr for s in snps)):
-        pos_all = [i for i, s in enumerate(snps) if s.chr == c]
-        pos_out = [i for i in pos_all if out_ok(i)]
+    chr_sel = bim.chr[sel]
+    for c in (int(x) for x in np.unique(chr_sel)):
+        pos_all = np.flatnonzero(chr_sel == c)
+        pos_out = pos_all[ok[pos_all]]
         path = f"{args.out}{c}.l2.ldscore.gz"
-        write_ldscore(path, [snps[i] for i in pos_out], l2[pos_out], col_names)
+        write_ldscore(path, bim, sel, l2, pos_out, col_names)
         ma_c, m5_c = m_vals(pos_all)
         with open(f"{args.out}{c}.l2.M", "w") as f:
             f.write(format_m_vals(ma_c))
@@ -450,7 +454,9 @@ def run(args) -> int:
               f"M_5_50={','.join(rust_display(v) for v in m5_c)})")
     print_ld_score_summary(out_path, l2[out_pos], col_names, maf[out_pos])
     if args.verbose_timing:
-        print(f"[perf] total={time.perf_counter() - t_start:.3f}s", file=sys.stderr)
+        t_end = time.perf_counter()
+        print(f"[perf] outputs={t_end - t_done:.3f}s", file=sys.stderr)
+        print(f"[perf] total={t_end - t_start:.3f}s", file=sys.stderr)
     return 0
 
 

@@ -1,0 +1,489 @@
+// io.cpp -- host-side BIM reader and .l2.ldscore.gz writer (include/ldsc_io.h).
+//
+// BIM: the text is cut into one segment per thread at line starts, each
+// segment counts its lines, and a prefix sum gives every segment the line
+// index it starts at; the segments are then parsed independently.  The
+// first failing line over all segments is the error reported, as a
+// sequential reader would report it.
+//
+// ldscore: rows are formatted in parallel blocks into one text buffer per
+// batch of rows, and the buffer is deflated in parallel 1 MiB pieces.
+// Each piece is a raw deflate stream primed with the 32 KiB of text before
+// it; all but the last end on a byte-aligned sync flush, so their
+// concatenation is one valid deflate stream and the file is a single gzip
+// member (readable by any gzip decoder, including single-member ones).
+#include "../../include/ldsc_io.h"
+
+#include <locale.h>
+#include <omp.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <zlib.h>
+
+#include <algorithm>
+#include <cmath>
+#include <string>
+#include <vector>
+
+namespace {
+
+constexpr int kErrInvalid = 1;
+
+void set_err(char* err, uint64_t err_len, const std::string& msg) {
+    if (err && err_len) {
+        size_t k = std::min<size_t>(msg.size(), err_len - 1);
+        memcpy(err, msg.data(), k);
+        err[k] = 0;
+    }
+}
+
+int threads_or_default(int t) { return t > 0 ? t : omp_get_max_threads(); }
+
+// ---------------------------------------------------------------------------
+// Rust `str::parse` grammars
+// ---------------------------------------------------------------------------
+// u8 / u32: optional '+', then one or more ASCII digits, value in range.
+template <typename T>
+bool parse_unsigned(const char* s, size_t n, T* out) {
+    if (n && s[0] == '+') { ++s; --n; }
+    if (!n) return false;
+    uint64_t v = 0;
+    const uint64_t lim = (uint64_t)(T)~(T)0;
+    for (size_t i = 0; i < n; ++i) {
+        unsigned d = (unsigned char)s[i] - '0';
+        if (d > 9) return false;
+        v = v * 10 + d;
+        if (v > lim) return false;
+    }
+    *out = (T)v;
+    return true;
+}
+
+bool ieq(const char* s, size_t n, const char* lit) {
+    size_t k = strlen(lit);
+    if (n != k) return false;
+    for (size_t i = 0; i < n; ++i)
+        if ((s[i] | 0x20) != lit[i]) return false;
+    return true;
+}
+
+locale_t c_locale() {
+    static locale_t loc = newlocale(LC_ALL_MASK, "C", (locale_t)0);
+    return loc;
+}
+
+// f64: [+-]? ( inf | infinity | nan | digits [. digits?] | . digits ) ([eE][+-]?digits)?
+// (inf/infinity/nan without exponent, any case).  The decimal value is
+// rounded correctly by strtod_l in the C locale.
+bool parse_f64(const char* s, size_t n, double* out) {
+    size_t i = 0;
+    bool neg = false;
+    if (i < n && (s[i] == '+' || s[i] == '-')) { neg = s[i] == '-'; ++i; }
+    const char* b = s + i;
+    size_t m = n - i;
+    if (ieq(b, m, "inf") || ieq(b, m, "infinity")) { *out = neg ? -HUGE_VAL : HUGE_VAL; return true; }
+    if (ieq(b, m, "nan")) { *out = neg ? -NAN : NAN; return true; }
+    size_t int_d = 0, frac_d = 0;
+    while (i < n && (unsigned)(s[i] - '0') <= 9) { ++i; ++int_d; }
+    if (i < n && s[i] == '.') {
+        ++i;
+        while (i < n && (unsigned)(s[i] - '0') <= 9) { ++i; ++frac_d; }
+    }
+    if (int_d + frac_d == 0) return false;
+    if (i < n && (s[i] == 'e' || s[i] == 'E')) {
+        ++i;
+        if (i < n && (s[i] == '+' || s[i] == '-')) ++i;
+        size_t ed = 0;
+        while (i < n && (unsigned)(s[i] - '0') <= 9) { ++i; ++ed; }
+        if (!ed) return false;
+    }
+    if (i != n) return false;
+    char small[64];
+    std::string big;
+    const char* z;
+    if (n < sizeof(small)) {
+        memcpy(small, s, n);
+        small[n] = 0;
+        z = small;
+    } else {
+        big.assign(s, n);
+        z = big.c_str();
+    }
+    *out = strtod_l(z, nullptr, c_locale());
+    return true;
+}
+
+// Strict UTF-8 (no overlongs, no surrogates, at most U+10FFFF).
+bool valid_utf8(const unsigned char* s, size_t n) {
+    size_t i = 0;
+    while (i < n) {
+        unsigned c = s[i];
+        if (c < 0x80) { ++i; continue; }
+        size_t k;
+        unsigned lo = 0x80, hi = 0xBF;
+        if (c >= 0xC2 && c <= 0xDF) k = 1;
+        else if (c == 0xE0) { k = 2; lo = 0xA0; }
+        else if (c >= 0xE1 && c <= 0xEC) k = 2;
+        else if (c == 0xED) { k = 2; hi = 0x9F; }
+        else if (c >= 0xEE && c <= 0xEF) k = 2;
+        else if (c == 0xF0) { k = 3; lo = 0x90; }
+        else if (c >= 0xF1 && c <= 0xF3) k = 3;
+        else if (c == 0xF4) { k = 3; hi = 0x8F; }
+        else return false;
+        if (i + k >= n) return false;
+        unsigned c1 = s[i + 1];
+        if (c1 < lo || c1 > hi) return false;
+        for (size_t j = 2; j <= k; ++j)
+            if ((s[i + j] & 0xC0) != 0x80) return false;
+        i += k + 1;
+    }
+    return true;
+}
+
+// One BIM line (without its terminator).  Returns 0, or the error kind:
+// 1 not UTF-8, 2 no SNP column, 3 no A1 column, 4 no A2 column.
+int parse_bim_line(const char* s, size_t n, uint64_t base_off, uint8_t* chr, double* cm,
+                   uint32_t* bp, uint64_t* name_off, uint32_t* name_len) {
+    bool ascii = true;
+    for (size_t i = 0; i < n; ++i) ascii &= (unsigned char)s[i] < 0x80;
+    if (!ascii && !valid_utf8((const unsigned char*)s, n)) return 1;
+    const char* f[6];
+    size_t fl[6];
+    int nf = 0;
+    size_t st = 0;
+    for (size_t i = 0; i <= n && nf < 6; ++i) {
+        if (i == n || s[i] == '\t') {
+            f[nf] = s + st;
+            fl[nf] = i - st;
+            ++nf;
+            st = i + 1;
+        }
+    }
+    uint8_t c8 = 0;
+    if (!parse_unsigned<uint8_t>(f[0], fl[0], &c8)) c8 = 0;
+    if (nf < 2) return 2;
+    double c = 0.0;
+    if (nf < 3 || !parse_f64(f[2], fl[2], &c)) c = 0.0;
+    uint32_t b = 0;
+    if (nf < 4 || !parse_unsigned<uint32_t>(f[3], fl[3], &b)) b = 0;
+    if (nf < 5) return 3;
+    if (nf < 6) return 4;
+    *chr = c8;
+    *cm = c;
+    *bp = b;
+    *name_off = base_off + (uint64_t)(f[1] - s);
+    *name_len = (uint32_t)fl[1];
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Rust `{:.3}` for f64
+// ---------------------------------------------------------------------------
+char* put_u64(char* p, uint64_t v) {
+    char tmp[24];
+    int k = 0;
+    do { tmp[k++] = (char)('0' + v % 10); v /= 10; } while (v);
+    while (k) *p++ = tmp[--k];
+    return p;
+}
+
+// av = m * 2^sh exactly, m < 2^53 an integer, so av * 1000 = (m * 1000) * 2^sh
+// with m * 1000 < 2^63, and q = round_half_even(av * 1000) follows from
+// 128-bit integer shifts.  Returns false when q needs more than 63 bits
+// (av above about 9.2e15); the caller then uses snprintf, which glibc also
+// rounds exactly.
+bool scaled_round(double av, uint64_t* q) {
+    int e;
+    double fr = frexp(av, &e);             // av = fr * 2^e, fr in [0.5, 1)
+    if (av == 0.0) { *q = 0; return true; }
+    uint64_t m = (uint64_t)ldexp(fr, 53);  // exact: av = m * 2^(e-53)
+    int sh = e - 53;
+    unsigned __int128 p = (unsigned __int128)m * 1000u;
+    if (sh >= 0) {
+        if (sh > 10) return false;
+        unsigned __int128 r = p << sh;
+        if (r >> 63) return false;
+        *q = (uint64_t)r;
+        return true;
+    }
+    int s = -sh;
+    if (s >= 127) { *q = 0; return true; }  // p < 2^63 < 2^(s-1): below one half
+    unsigned __int128 qq = p >> s;
+    unsigned __int128 rem = p - (qq << s);
+    unsigned __int128 half = (unsigned __int128)1 << (s - 1);
+    if (rem > half || (rem == half && (qq & 1))) ++qq;
+    if (qq >> 63) return false;
+    *q = (uint64_t)qq;
+    return true;
+}
+
+size_t fmt3(double v, char* out) {
+    if (std::isnan(v)) { memcpy(out, "NaN", 3); return 3; }
+    if (std::isinf(v)) {
+        if (v < 0) { memcpy(out, "-inf", 4); return 4; }
+        memcpy(out, "inf", 3);
+        return 3;
+    }
+    uint64_t q;
+    if (!scaled_round(std::fabs(v), &q)) return (size_t)snprintf(out, 330, "%.3f", v);
+    char* p = out;
+    if (std::signbit(v)) *p++ = '-';
+    p = put_u64(p, q / 1000);
+    unsigned f = (unsigned)(q % 1000);
+    *p++ = '.';
+    *p++ = (char)('0' + f / 100);
+    *p++ = (char)('0' + f / 10 % 10);
+    *p++ = (char)('0' + f % 10);
+    return (size_t)(p - out);
+}
+
+// ---------------------------------------------------------------------------
+// gzip writer: parallel pieces that form one deflate stream
+// ---------------------------------------------------------------------------
+struct GzOut {
+    FILE* f = nullptr;
+    uLong crc = 0;
+    uint64_t total = 0;
+    std::string window;  // the last <= 32 KiB of text written so far
+};
+
+constexpr size_t kPiece = 256u << 10;
+constexpr size_t kDict = 32768;
+
+bool write_gz_header(GzOut& g) {
+    const unsigned char h[10] = {0x1f, 0x8b, 8, 0, 0, 0, 0, 0, 4, 255};
+    return fwrite(h, 1, 10, g.f) == 10;
+}
+
+// Deflate `text` (continuing the stream in g); `last` closes it.
+bool gz_append(GzOut& g, const std::string& text, bool last, int threads, std::string& why) {
+    size_t n = text.size();
+    size_t np = std::max<size_t>(1, (n + kPiece - 1) / kPiece);
+    std::vector<std::string> outs(np);
+    std::vector<uLong> crcs(np);
+    std::vector<int> ok(np, 1);
+#pragma omp parallel for schedule(dynamic, 1) num_threads(threads)
+    for (size_t i = 0; i < np; ++i) {
+        size_t a = i * kPiece, b = std::min(n, a + kPiece);
+        z_stream zs;
+        memset(&zs, 0, sizeof(zs));
+        if (deflateInit2(&zs, 1, Z_DEFLATED, -15, 8, Z_DEFAULT_STRATEGY) != Z_OK) { ok[i] = 0; continue; }
+        // prime with the 32 KiB before this piece (from this text, else the carried window)
+        std::string dict;
+        if (a >= kDict) {
+            dict.assign(text.data() + a - kDict, kDict);
+        } else {
+            size_t from_prev = std::min(g.window.size(), kDict - a);
+            dict.assign(g.window.data() + g.window.size() - from_prev, from_prev);
+            dict.append(text.data(), a);
+        }
+        if (!dict.empty() &&
+            deflateSetDictionary(&zs, (const Bytef*)dict.data(), (uInt)dict.size()) != Z_OK) {
+            ok[i] = 0;
+            deflateEnd(&zs);
+            continue;
+        }
+        std::string& o = outs[i];
+        o.resize(deflateBound(&zs, b - a) + 64);
+        zs.next_in = (Bytef*)(text.data() + a);
+        zs.avail_in = (uInt)(b - a);
+        zs.next_out = (Bytef*)&o[0];
+        zs.avail_out = (uInt)o.size();
+        int flush = (last && i + 1 == np) ? Z_FINISH : Z_SYNC_FLUSH;
+        int rc = deflate(&zs, flush);
+        if ((flush == Z_FINISH && rc != Z_STREAM_END) || (flush != Z_FINISH && rc != Z_OK) ||
+            zs.avail_in != 0)
+            ok[i] = 0;
+        o.resize(o.size() - zs.avail_out);
+        deflateEnd(&zs);
+        crcs[i] = crc32(0L, (const Bytef*)(text.data() + a), (uInt)(b - a));
+    }
+    for (size_t i = 0; i < np; ++i) {
+        if (!ok[i]) { why = "deflate failed"; return false; }
+        size_t len = std::min(n, (i + 1) * kPiece) - std::min(n, i * kPiece);
+        g.crc = crc32_combine(g.crc, crcs[i], (z_off_t)len);
+        if (!outs[i].empty() && fwrite(outs[i].data(), 1, outs[i].size(), g.f) != outs[i].size()) {
+            why = "write failed";
+            return false;
+        }
+    }
+    g.total += n;
+    if (n >= kDict) {
+        g.window.assign(text.data() + n - kDict, kDict);
+    } else {
+        g.window.append(text);
+        if (g.window.size() > kDict) g.window.erase(0, g.window.size() - kDict);
+    }
+    return true;
+}
+
+bool write_gz_trailer(GzOut& g) {
+    unsigned char t[8];
+    for (int i = 0; i < 4; ++i) t[i] = (unsigned char)(g.crc >> (8 * i));
+    for (int i = 0; i < 4; ++i) t[4 + i] = (unsigned char)(g.total >> (8 * i));
+    return fwrite(t, 1, 8, g.f) == 8;
+}
+
+}  // namespace
+
+extern "C" uint64_t ldsc_io_count_lines(const char* text, uint64_t len) {
+    if (!text || !len) return 0;
+    uint64_t n = 0;
+    const char* p = text;
+    const char* end = text + len;
+    while (p < end) {
+        const char* q = (const char*)memchr(p, '\n', (size_t)(end - p));
+        if (!q) return n + 1;
+        ++n;
+        p = q + 1;
+    }
+    return n;
+}
+
+extern "C" int ldsc_io_parse_bim(const char* text, uint64_t len, uint64_t n, uint8_t* chr,
+                                 double* cm, uint32_t* bp, uint64_t* name_off,
+                                 uint32_t* name_len, int threads, char* err, uint64_t err_len) {
+    if ((len && !text) || (n && (!chr || !cm || !bp || !name_off || !name_len))) {
+        set_err(err, err_len, "ldsc_io_parse_bim: null argument");
+        return kErrInvalid;
+    }
+    if (ldsc_io_count_lines(text, len) != n) {
+        set_err(err, err_len, "ldsc_io_parse_bim: n differs from the text's line count");
+        return kErrInvalid;
+    }
+    if (!n) return 0;
+    int T = (int)std::min<uint64_t>(threads_or_default(threads), std::max<uint64_t>(1, len >> 16));
+    // segment starts: the first line start at or after t*len/T
+    std::vector<uint64_t> seg(T + 1, len);
+    seg[0] = 0;
+    for (int t = 1; t < T; ++t) {
+        uint64_t a = len * (uint64_t)t / (uint64_t)T;
+        const char* q = (const char*)memchr(text + a - 1, '\n', (size_t)(len - a + 1));
+        seg[t] = q ? (uint64_t)(q - text) + 1 : len;
+        if (seg[t] < seg[t - 1]) seg[t] = seg[t - 1];
+    }
+    std::vector<uint64_t> lines(T + 1, 0);
+#pragma omp parallel for num_threads(T)
+    for (int t = 0; t < T; ++t)
+        lines[t + 1] = ldsc_io_count_lines(text + seg[t], seg[t + 1] - seg[t]);
+    for (int t = 0; t < T; ++t) lines[t + 1] += lines[t];
+    std::vector<uint64_t> bad_line(T, UINT64_MAX);
+    std::vector<int> bad_kind(T, 0);
+#pragma omp parallel for num_threads(T)
+    for (int t = 0; t < T; ++t) {
+        uint64_t li = lines[t];
+        const char* p = text + seg[t];
+        const char* end = text + seg[t + 1];
+        while (p < end) {
+            const char* q = (const char*)memchr(p, '\n', (size_t)(end - p));
+            size_t l = q ? (size_t)(q - p) : (size_t)(end - p);
+            if (q && l && p[l - 1] == '\r') --l;  // "\r\n" ends a line; a bare final '\r' stays
+            int k = parse_bim_line(p, l, (uint64_t)(p - text), chr + li, cm + li, bp + li,
+                                   name_off + li, name_len + li);
+            if (k) {
+                bad_line[t] = li;
+                bad_kind[t] = k;
+                break;
+            }
+            ++li;
+            p = q ? q + 1 : end;
+        }
+    }
+    for (int t = 0; t < T; ++t) {
+        if (bad_line[t] == UINT64_MAX) continue;
+        uint64_t ln = bad_line[t] + 1;
+        std::string msg;
+        switch (bad_kind[t]) {
+            case 1: msg = "reading BIM line " + std::to_string(ln) +
+                          ": stream did not contain valid UTF-8"; break;
+            case 2: msg = "BIM line " + std::to_string(ln) + ": expected SNP column"; break;
+            case 3: msg = "BIM line " + std::to_string(ln) + ": missing A1 col"; break;
+            default: msg = "BIM line " + std::to_string(ln) + ": missing A2 col"; break;
+        }
+        set_err(err, err_len, msg);
+        return kErrInvalid;
+    }
+    return 0;
+}
+
+extern "C" uint64_t ldsc_io_format_fixed3(double v, char* out, uint64_t cap) {
+    char buf[400];
+    size_t k = fmt3(v, buf);
+    if (out && cap) {
+        size_t c = std::min<size_t>(k, cap - 1);
+        memcpy(out, buf, c);
+        out[c] = 0;
+    }
+    return k;
+}
+
+extern "C" int ldsc_io_write_ldscore(const char* path, const char* header_tail,
+                                     const uint8_t* chr, const uint32_t* bp, const char* names,
+                                     const uint64_t* name_off, const uint32_t* name_len,
+                                     const double* l2, uint64_t K, const int64_t* rows,
+                                     uint64_t n_rows, int threads, char* err, uint64_t err_len) {
+    if (!path || !header_tail || (n_rows && (!chr || !bp || !names || !name_off || !name_len ||
+                                             !rows || (K && !l2)))) {
+        set_err(err, err_len, "ldsc_io_write_ldscore: null argument");
+        return kErrInvalid;
+    }
+    int T = threads_or_default(threads);
+    GzOut g;
+    g.f = fopen(path, "wb");
+    if (!g.f) {
+        set_err(err, err_len, std::string("creating output '") + path + "'");
+        return kErrInvalid;
+    }
+    std::string why;
+    bool ok = write_gz_header(g);
+    std::string head = std::string("CHR\tSNP\tBP") + header_tail + "\n";
+    // rows per formatting block and per deflate batch
+    const uint64_t kBlock = 4096;
+    const uint64_t kBatch = kBlock * 256;
+    std::string text;
+    ok = ok && gz_append(g, head, n_rows == 0, T, why);
+    std::vector<std::string> blocks(kBatch / kBlock);
+    for (uint64_t b0 = 0; ok && b0 < n_rows; b0 += kBatch) {
+        uint64_t b1 = std::min(n_rows, b0 + kBatch);
+        uint64_t nb = (b1 - b0 + kBlock - 1) / kBlock;
+#pragma omp parallel for schedule(dynamic, 1) num_threads(T)
+        for (uint64_t bi = 0; bi < nb; ++bi) {
+            std::string& s = blocks[bi];
+            s.clear();
+            uint64_t r0 = b0 + bi * kBlock, r1 = std::min(b1, r0 + kBlock);
+            char num[400];
+            for (uint64_t i = r0; i < r1; ++i) {
+                int64_t r = rows[i];
+                char* p = put_u64(num, chr[r]);
+                *p++ = '\t';
+                s.append(num, (size_t)(p - num));
+                s.append(names + name_off[r], name_len[r]);
+                p = num;
+                *p++ = '\t';
+                p = put_u64(p, bp[r]);
+                s.append(num, (size_t)(p - num));
+                for (uint64_t k = 0; k < K; ++k) {
+                    num[0] = '\t';
+                    size_t w = fmt3(l2[(uint64_t)r * K + k], num + 1);
+                    s.append(num, w + 1);
+                }
+                s.push_back('\n');
+            }
+        }
+        size_t tot = 0;
+        for (uint64_t bi = 0; bi < nb; ++bi) tot += blocks[bi].size();
+        text.clear();
+        text.reserve(tot);
+        for (uint64_t bi = 0; bi < nb; ++bi) text.append(blocks[bi]);
+        ok = gz_append(g, text, b1 == n_rows, T, why);
+    }
+    ok = ok && write_gz_trailer(g);
+    if (fclose(g.f) != 0) ok = false;
+    if (!ok) {
+        set_err(err, err_len, std::string("writing '") + path + "': " +
+                                  (why.empty() ? "I/O error" : why));
+        return kErrInvalid;
+    }
+    return 0;
+}

@@ -141,14 +141,16 @@ class L2Output:
 # Text inputs
 # ----------------------------------------------------------------------------
 def _lines(s: str):
-    """``str::lines`` semantics: split on \\n, strip one trailing \\r, no
-    trailing empty record."""
+    """``BufRead::lines`` semantics: split on \\n, strip one \\r before a
+    \\n, no trailing empty record (a final line without \\n keeps its \\r)."""
     if not s:
         return []
     parts = s.split("\n")
-    if parts and parts[-1] == "":
-        parts.pop()
-    return [p[:-1] if p.endswith("\r") else p for p in parts]
+    last = parts.pop()
+    out = [p[:-1] if p.endswith("\r") else p for p in parts]
+    if last:
+        out.append(last)
+    return out
 
 
 def _parse_u(s: str, bits: int) -> Optional[int]:
@@ -160,7 +162,8 @@ def _parse_u(s: str, bits: int) -> Optional[int]:
 
 
 def _parse_f64(s: str) -> Optional[float]:
-    if not s or s != s.strip() or "_" in s:
+    """Rust ``f64::from_str``: ASCII only, no surrounding space, no ``_``."""
+    if not s or not s.isascii() or s != s.strip() or "_" in s:
         return None
     try:
         return float(s)
