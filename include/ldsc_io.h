@@ -2,7 +2,7 @@
  * ldsc_io.h — C ABI of the host-side file formats around the LD-score path.
  *
  * libldsc_io.so is plain C++ (no device code).  It reads the PLINK `.bim`
- * text and writes the `.l2.ldscore.gz` tables of the `l2` subcommand on
+ * text and the `.annot` tables, and writes the `.l2.ldscore.gz` tables of the `l2` subcommand on
  * all host cores: a 1.66 M-SNP BIM parses in about 0.1 s on 8 cores, and
  * the gzip writer formats and deflates in parallel pieces that form one
  * standard gzip member.
@@ -58,6 +58,27 @@ int ldsc_io_write_ldscore(const char* path, const char* header_tail,
                           const uint32_t* name_len, const double* l2, uint64_t K,
                           const int64_t* rows, uint64_t n_rows, int threads,
                           char* err, uint64_t err_len);
+
+/*
+ * Lines of `text` that are not empty once every trailing '\r' / '\n' is
+ * removed: the header plus body rows of read_tsv (src/frame.rs:319-344).
+ */
+uint64_t ldsc_io_count_nonempty_lines(const char* text, uint64_t len);
+
+/*
+ * The body pass of read_tsv (src/frame.rs:319-413) and read_annot_path
+ * (src/parse.rs:191-229) over a decompressed `.annot` text: the first
+ * non-empty line is the header (its `ncol` columns and the separator --
+ * `use_tab` when the header holds a tab, else runs of whitespace -- are
+ * found by the caller); every later non-empty line is a row, padded with
+ * empty cells to ncol and cut at ncol.  Cell (i, j) for j >= skip goes to
+ * out[i * (ncol - skip) + (j - skip)]: trimmed, NA / NaN / "." / empty or
+ * not a Rust f64 -> 0, else the value.  n_rows = count_nonempty - 1.
+ * Invalid UTF-8 is an error naming the line.
+ */
+int ldsc_io_parse_annot(const char* text, uint64_t len, int use_tab, uint64_t ncol,
+                        uint64_t skip, uint64_t n_rows, double* out, int threads,
+                        char* err, uint64_t err_len);
 
 /*
  * Rust `{:.3}` of one f64 into `out` (at least 32 bytes for |v| < 1e25,

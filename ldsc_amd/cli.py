@@ -25,7 +25,7 @@ from typing import List, Optional
 import numpy as np
 
 from . import _native as N
-from .formats import BimTable, write_ldscore
+from .formats import BimTable, parse_annot_text, read_text_bytes, write_ldscore
 from .l2 import L2Config, Panel, WindowMode, compute_ldscore, decode, make_snp_array
 
 
@@ -123,19 +123,6 @@ def load_individual_indices(keep_path: str, fam_ids) -> np.ndarray:
     return np.array(idx, np.int64)
 
 
-NA_TOKENS = {"", "NA", "na", "NaN", "nan", "."}
-
-
-def _to_f64(s: str) -> float:
-    s = s.strip()
-    if s in NA_TOKENS:
-        return float("nan")
-    try:
-        return float(s)
-    except ValueError:
-        return float("nan")
-
-
 def resolve_annot_path(prefix: str) -> str:
     """resolve_annot_path (src/parse.rs:166-187)."""
     if prefix.endswith((".annot", ".annot.gz", ".annot.bz2")):
@@ -149,28 +136,10 @@ def resolve_annot_path(prefix: str) -> str:
 
 
 def read_annot_path(path: str, thin: bool):
-    """read_annot_path (src/parse.rs:191-229): header row; tab-separated if the
-    header has a tab, else whitespace; full format skips CHR SNP BP CM;
-    NaN -> 0."""
-    with open_text(path) as f:
-        lines = [ln.rstrip("\n").rstrip("\r") for ln in f]
-    lines = [ln for ln in lines if ln]
-    if not lines:
-        raise ValueError(f"'{path}' is empty (no header row)")
-    use_tab = "\t" in lines[0]
-    split = (lambda s: s.split("\t")) if use_tab else (lambda s: s.split())
-    header = split(lines[0])
-    skip = 0 if thin else 4
-    if len(header) <= skip:
-        raise ValueError(f"Annot file '{path}' has {len(header)} columns; expected > {skip}")
-    names = header[skip:]
-    mat = np.zeros((len(lines) - 1, len(names)))
-    for i, ln in enumerate(lines[1:]):
-        cols = split(ln)
-        cols += [""] * (len(header) - len(cols))
-        mat[i] = [_to_f64(c) for c in cols[skip:len(header)]]
-    mat[np.isnan(mat)] = 0.0
-    return mat, names
+    """read_annot_path (src/parse.rs:191-229) through libldsc_io.so: header
+    row; tab-separated if the header has a tab, else whitespace; the full
+    format skips CHR SNP BP CM; NA / NaN / non-numbers -> 0."""
+    return parse_annot_text(read_text_bytes(path), thin, path)
 
 
 def read_annotations(prefix: str, thin: bool, chr_: np.ndarray) -> tuple:

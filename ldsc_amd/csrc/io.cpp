@@ -325,6 +325,116 @@ bool write_gz_trailer(GzOut& g) {
     return fwrite(t, 1, 8, g.f) == 8;
 }
 
+
+// ---------------------------------------------------------------------------
+// annotation tables (read_tsv + read_annot_path)
+// ---------------------------------------------------------------------------
+// Length of the `char::is_whitespace` character starting at p (0 if none):
+// U+0009-000D, U+0020, U+0085, U+00A0, U+1680, U+2000-200A, U+2028, U+2029,
+// U+202F, U+205F, U+3000.
+size_t ws_fwd(const unsigned char* p, const unsigned char* end) {
+    const size_t n = (size_t)(end - p);
+    if (!n) return 0;
+    const unsigned c = p[0];
+    if (c == ' ' || (c >= 0x09 && c <= 0x0D)) return 1;
+    if (c < 0x80) return 0;
+    if (n >= 2 && c == 0xC2 && (p[1] == 0x85 || p[1] == 0xA0)) return 2;
+    if (n < 3) return 0;
+    if (c == 0xE1 && p[1] == 0x9A && p[2] == 0x80) return 3;
+    if (c == 0xE2 && p[1] == 0x80 &&
+        ((p[2] >= 0x80 && p[2] <= 0x8A) || p[2] == 0xA8 || p[2] == 0xA9 || p[2] == 0xAF))
+        return 3;
+    if (c == 0xE2 && p[1] == 0x81 && p[2] == 0x9F) return 3;
+    if (c == 0xE3 && p[1] == 0x80 && p[2] == 0x80) return 3;
+    return 0;
+}
+
+// Length of the whitespace character that ends just before `end` (0 if none).
+size_t ws_back(const unsigned char* begin, const unsigned char* end) {
+    const size_t n = (size_t)(end - begin);
+    if (n >= 1 && ws_fwd(end - 1, end) == 1) return 1;
+    if (n >= 2 && ws_fwd(end - 2, end) == 2) return 2;
+    if (n >= 3 && ws_fwd(end - 3, end) == 3) return 3;
+    return 0;
+}
+
+// parse_f64_or_nan (src/frame.rs:68-74) after `trim()`, then NaN -> 0 as
+// read_annot_path stores it (src/parse.rs:219-223).
+double annot_cell(const char* s, size_t n) {
+    const unsigned char* b = (const unsigned char*)s;
+    const unsigned char* e = b + n;
+    for (size_t k; b < e && (k = ws_fwd(b, e)) != 0;) b += k;
+    for (size_t k; e > b && (k = ws_back(b, e)) != 0;) e -= k;
+    const char* t = (const char*)b;
+    const size_t m = (size_t)(e - b);
+    if (m == 0 || ieq(t, m, "na") || ieq(t, m, "nan") || (m == 1 && t[0] == '.')) return 0.0;
+    double v;
+    if (!parse_f64(t, m, &v) || std::isnan(v)) return 0.0;
+    return v;
+}
+
+// One body row: the first ncol cells (missing ones are empty), columns
+// skip.. stored.  Tab mode splits on every '\t'; otherwise on runs of
+// whitespace with none leading or trailing (`split_whitespace`).
+void annot_row(const char* s, size_t n, bool use_tab, uint64_t ncol, uint64_t skip, double* out) {
+    const unsigned char* p = (const unsigned char*)s;
+    const unsigned char* end = p + n;
+    uint64_t j = 0;
+    if (use_tab) {
+        const unsigned char* st = p;
+        for (;; ++p) {
+            if (p == end || *p == '\t') {
+                if (j >= skip) out[j - skip] = annot_cell((const char*)st, (size_t)(p - st));
+                if (++j == ncol || p == end) break;
+                st = p + 1;
+            }
+        }
+    } else {
+        while (j < ncol) {
+            for (size_t k; p < end && (k = ws_fwd(p, end)) != 0;) p += k;
+            if (p == end) break;
+            const unsigned char* st = p;
+            while (p < end && ws_fwd(p, end) == 0) ++p;
+            if (j >= skip) out[j - skip] = annot_cell((const char*)st, (size_t)(p - st));
+            ++j;
+        }
+    }
+    for (; j < ncol; ++j)
+        if (j >= skip) out[j - skip] = 0.0;
+}
+
+// A line of read_tsv: its bytes up to '\n' with every trailing '\r' / '\n'
+// removed; empty lines are skipped.
+inline size_t trimmed_len(const char* p, size_t l) {
+    while (l && (p[l - 1] == '\r' || p[l - 1] == '\n')) --l;
+    return l;
+}
+
+// Segment starts at line starts, one segment per thread.
+std::vector<uint64_t> line_segments(const char* text, uint64_t len, int T) {
+    std::vector<uint64_t> seg(T + 1, len);
+    seg[0] = 0;
+    for (int t = 1; t < T; ++t) {
+        const uint64_t a = len * (uint64_t)t / (uint64_t)T;
+        const char* q = (const char*)memchr(text + a - 1, '\n', (size_t)(len - a + 1));
+        seg[t] = q ? (uint64_t)(q - text) + 1 : len;
+        if (seg[t] < seg[t - 1]) seg[t] = seg[t - 1];
+    }
+    return seg;
+}
+
+uint64_t count_nonempty(const char* text, uint64_t len) {
+    uint64_t n = 0;
+    const char* p = text;
+    const char* end = text + len;
+    while (p < end) {
+        const char* q = (const char*)memchr(p, '\n', (size_t)(end - p));
+        const size_t l = q ? (size_t)(q - p) : (size_t)(end - p);
+        n += trimmed_len(p, l) != 0;
+        p = q ? q + 1 : end;
+    }
+    return n;
+}
 }  // namespace
 
 extern "C" uint64_t ldsc_io_count_lines(const char* text, uint64_t len) {
@@ -354,15 +464,7 @@ extern "C" int ldsc_io_parse_bim(const char* text, uint64_t len, uint64_t n, uin
     }
     if (!n) return 0;
     int T = (int)std::min<uint64_t>(threads_or_default(threads), std::max<uint64_t>(1, len >> 16));
-    // segment starts: the first line start at or after t*len/T
-    std::vector<uint64_t> seg(T + 1, len);
-    seg[0] = 0;
-    for (int t = 1; t < T; ++t) {
-        uint64_t a = len * (uint64_t)t / (uint64_t)T;
-        const char* q = (const char*)memchr(text + a - 1, '\n', (size_t)(len - a + 1));
-        seg[t] = q ? (uint64_t)(q - text) + 1 : len;
-        if (seg[t] < seg[t - 1]) seg[t] = seg[t - 1];
-    }
+    const std::vector<uint64_t> seg = line_segments(text, len, T);
     std::vector<uint64_t> lines(T + 1, 0);
 #pragma omp parallel for num_threads(T)
     for (int t = 0; t < T; ++t)
@@ -404,6 +506,62 @@ extern "C" int ldsc_io_parse_bim(const char* text, uint64_t len, uint64_t n, uin
         set_err(err, err_len, msg);
         return kErrInvalid;
     }
+    return 0;
+}
+
+extern "C" uint64_t ldsc_io_count_nonempty_lines(const char* text, uint64_t len) {
+    return (text && len) ? count_nonempty(text, len) : 0;
+}
+
+extern "C" int ldsc_io_parse_annot(const char* text, uint64_t len, int use_tab, uint64_t ncol,
+                                   uint64_t skip, uint64_t n_rows, double* out, int threads,
+                                   char* err, uint64_t err_len) {
+    if ((len && !text) || skip >= ncol || (n_rows && !out)) {
+        set_err(err, err_len, "ldsc_io_parse_annot: bad arguments");
+        return kErrInvalid;
+    }
+    const uint64_t ne = ldsc_io_count_nonempty_lines(text, len);
+    if (ne == 0 || ne - 1 != n_rows) {
+        set_err(err, err_len, "ldsc_io_parse_annot: n_rows differs from the text's body rows");
+        return kErrInvalid;
+    }
+    const uint64_t width = ncol - skip;
+    const int T = (int)std::min<uint64_t>(threads_or_default(threads),
+                                          std::max<uint64_t>(1, len >> 16));
+    const std::vector<uint64_t> seg = line_segments(text, len, T);
+    std::vector<uint64_t> before(T + 1, 0);
+#pragma omp parallel for num_threads(T)
+    for (int t = 0; t < T; ++t) before[t + 1] = count_nonempty(text + seg[t], seg[t + 1] - seg[t]);
+    for (int t = 0; t < T; ++t) before[t + 1] += before[t];
+    std::vector<uint64_t> bad(T, UINT64_MAX);
+#pragma omp parallel for num_threads(T)
+    for (int t = 0; t < T; ++t) {
+        uint64_t k = before[t];  // non-empty lines before this segment (the header is k = 0)
+        const char* p = text + seg[t];
+        const char* end = text + seg[t + 1];
+        while (p < end) {
+            const char* q = (const char*)memchr(p, '\n', (size_t)(end - p));
+            const size_t l0 = q ? (size_t)(q - p) : (size_t)(end - p);
+            const size_t l = trimmed_len(p, l0);
+            bool ascii = true;
+            for (size_t i = 0; i < l0; ++i) ascii &= (unsigned char)p[i] < 0x80;
+            if (!ascii && !valid_utf8((const unsigned char*)p, l0)) {
+                bad[t] = (uint64_t)(p - text);
+                break;
+            }
+            if (l) {
+                if (k > 0) annot_row(p, l, use_tab != 0, ncol, skip, out + (k - 1) * width);
+                ++k;
+            }
+            p = q ? q + 1 : end;
+        }
+    }
+    for (int t = 0; t < T; ++t)
+        if (bad[t] != UINT64_MAX) {
+            set_err(err, err_len, "stream did not contain valid UTF-8 (line " +
+                                      std::to_string(ldsc_io_count_lines(text, bad[t]) + 1) + ")");
+            return kErrInvalid;
+        }
     return 0;
 }
 

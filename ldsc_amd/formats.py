@@ -1,5 +1,6 @@
-"""ctypes binding of ``include/ldsc_io.h`` (libldsc_io.so): the BIM table and
-the ``.l2.ldscore.gz`` writer of the ``l2`` command line.
+"""ctypes binding of ``include/ldsc_io.h`` (libldsc_io.so): the BIM table,
+the ``.annot`` reader and the ``.l2.ldscore.gz`` writer of the ``l2``
+command line.
 
 ``BimTable`` holds the parsed columns of a ``.bim`` as numpy arrays, with the
 SNP IDs left as byte spans of the file's text; ``str`` objects are made only
@@ -8,8 +9,11 @@ loaded from the package directory; when it is missing the first use raises.
 """
 from __future__ import annotations
 
+import bz2
 import ctypes as C
+import gzip
 import os
+import re
 from typing import Iterable, List, Optional, Sequence
 
 import numpy as np
@@ -17,7 +21,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libldsc_io.so")
 EXPORTED = ["ldsc_io_count_lines", "ldsc_io_parse_bim", "ldsc_io_write_ldscore",
-            "ldsc_io_format_fixed3"]
+            "ldsc_io_format_fixed3", "ldsc_io_count_nonempty_lines", "ldsc_io_parse_annot"]
 
 _lib = None
 
@@ -38,6 +42,10 @@ def lib() -> C.CDLL:
         L.ldsc_io_write_ldscore.restype = I
         L.ldsc_io_format_fixed3.argtypes = [C.c_double, C.c_char_p, U64]
         L.ldsc_io_format_fixed3.restype = U64
+        L.ldsc_io_count_nonempty_lines.argtypes = [P, U64]
+        L.ldsc_io_count_nonempty_lines.restype = U64
+        L.ldsc_io_parse_annot.argtypes = [P, U64, I, U64, U64, U64, P, I, C.c_char_p, U64]
+        L.ldsc_io_parse_annot.restype = I
         _lib = L
     return _lib
 
@@ -133,3 +141,59 @@ def write_ldscore(path: str, table: BimTable, bed_idx: np.ndarray, l2: np.ndarra
                                      l2.shape[1], _ptr(rows), rows.size, int(threads), err, 512)
     if rc:
         raise OSError(err.value.decode(errors="replace"))
+
+
+# `char::is_whitespace`, the separator set of `split_whitespace` and `trim`
+_WS = "\t\n\x0b\x0c\r \x85\xa0\u1680\u2000-\u200a\u2028\u2029\u202f\u205f\u3000"
+_WS_RUN = re.compile(f"[{_WS}]+")
+
+
+def read_text_bytes(path: str) -> bytes:
+    """open_text_reader (src/frame.rs:420-432): gzip (every member) / bzip2
+    by suffix, else the plain file."""
+    with open(path, "rb") as f:
+        raw = f.read()
+    if path.endswith(".gz"):
+        return gzip.decompress(raw)
+    if path.endswith(".bz2"):
+        return bz2.decompress(raw)
+    return raw
+
+
+def parse_annot_text(data: bytes, thin: bool, path: str = "<annot>"):
+    """read_tsv (src/frame.rs:319-413) + read_annot_path (src/parse.rs:191-229)
+    over decompressed text: the first non-empty line is the header, split on
+    tabs when it holds one, else on whitespace runs; the full format skips
+    CHR SNP BP CM.  Returns (matrix rows x annotations, names); NA, NaN and
+    cells that are not numbers are 0."""
+    pos, header = 0, None
+    while pos < len(data):
+        nl = data.find(b"\n", pos)
+        end = len(data) if nl < 0 else nl
+        line = data[pos:end].rstrip(b"\r\n")
+        pos = end + 1
+        if line:
+            header = line
+            break
+    if header is None:
+        raise ValueError(f"'{path}' is empty (no header row)")
+    try:
+        htext = header.decode()
+    except UnicodeDecodeError:
+        raise ValueError(f"reading line of '{path}': stream did not contain valid UTF-8")
+    use_tab = "\t" in htext
+    cols = htext.split("\t") if use_tab else [c for c in _WS_RUN.split(htext) if c]
+    skip = 0 if thin else 4
+    if len(cols) <= skip:
+        kind = "thin format" if thin else "full format: CHR SNP BP CM + annotations"
+        raise ValueError(f"Annot file '{path}' has {len(cols)} columns; expected > {skip} "
+                         f"({kind})")
+    L = lib()
+    n_rows = int(L.ldsc_io_count_nonempty_lines(data, len(data))) - 1
+    out = np.zeros((n_rows, len(cols) - skip), np.float64)
+    err = C.create_string_buffer(512)
+    rc = L.ldsc_io_parse_annot(data, len(data), int(use_tab), len(cols), skip, n_rows,
+                               _ptr(out), 0, err, 512)
+    if rc:
+        raise ValueError(f"reading line of '{path}': {err.value.decode(errors='replace')}")
+    return out, cols[skip:]

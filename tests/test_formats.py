@@ -165,3 +165,111 @@ def test_write_ldscore_empty_and_errors(tmp_path):
     with pytest.raises(OSError, match="creating output"):
         F.write_ldscore(str(tmp_path / "no" / "x.gz"), t, np.array([0]), np.array([[1.0]]),
                         np.array([0]), ["L2"])
+
+
+_RUST_WS = ("\t\n\x0b\x0c\r \x85\xa0\u1680" + "".join(chr(c) for c in range(0x2000, 0x200B)) +
+            "\u2028\u2029\u202f\u205f\u3000")
+
+
+def _rust_f64(s):
+    """Rust f64::from_str on an already-trimmed ASCII token, else None."""
+    if not s or not s.isascii() or "_" in s or s != s.strip():
+        return None
+    try:
+        return float(s)
+    except ValueError:
+        return None
+
+
+def _annot_by_lines(text, thin):
+    """read_tsv (src/frame.rs:319-413) + read_annot_path (src/parse.rs:191-229)
+    line by line: non-empty lines after removing trailing CR/LF, tab split
+    when the header holds a tab, else runs of Rust whitespace; pad / cut to
+    the header width; trimmed cells; NA/NaN/./empty/non-numbers -> 0."""
+    lines = [ln.rstrip("\r\n") for ln in text.split("\n")]
+    lines = [ln for ln in lines if ln]
+    use_tab = "\t" in lines[0]
+
+    def split(s):
+        if use_tab:
+            return s.split("\t")
+        return [c for c in re.split(f"[{_RUST_WS}]+", s) if c]
+
+    header = split(lines[0])
+    skip = 0 if thin else 4
+    rows = []
+    for ln in lines[1:]:
+        c = split(ln)[:len(header)]
+        c += [""] * (len(header) - len(c))
+        vals = []
+        for cell in c[skip:]:
+            t = cell.strip(_RUST_WS)
+            if t.lower() in ("", "na", "nan", "."):
+                vals.append(0.0)
+                continue
+            v = _rust_f64(t)
+            vals.append(0.0 if v is None or v != v else v)
+        rows.append(vals)
+    return np.array(rows, dtype=np.float64).reshape(len(rows), len(header) - skip), header[skip:]
+
+
+def _annot_fuzz(rng, n, use_tab, width):
+    toks = ["0", "1", "0.25", "-3e-2", "NA", "na", "nA", "NaN", "nan", "NAN", ".", "", "1_0",
+            "inf", "-Infinity", " 2.5 ", "\xa03\u3000", "x", "1e400", "+.5", "5.", "١"]
+    sep = "\t" if use_tab else " "
+    head = sep.join((["CHR", "SNP", "BP", "CM"] if width > 4 else []) +
+                    [f"A{k}" for k in range(width - (4 if width > 4 else 0))])
+    out = [head]
+    for i in range(n):
+        k = width + int(rng.integers(-2, 3))
+        cells = [toks[rng.integers(len(toks))] for _ in range(max(k, 0))]
+        if not use_tab:
+            cells = [c.strip(_RUST_WS) or "NA" for c in cells]
+            seps = [[" ", "  ", "\t", "\xa0", "  "][rng.integers(5)] for _ in cells]
+            out.append("".join(c + s for c, s in zip(cells, seps)).rstrip(_RUST_WS))
+        else:
+            out.append(sep.join(cells))
+        if i % 37 == 0:
+            out.append("")
+    return "\r\n".join(out) + "\n\n"
+
+
+@pytest.mark.parametrize("use_tab,thin", [(True, True), (True, False), (False, True),
+                                          (False, False)])
+def test_annot_parser_matches_line_reader(use_tab, thin):
+    rng = np.random.default_rng(7 + 2 * use_tab + thin)
+    text = _annot_fuzz(rng, 4000, use_tab, 7)
+    got, names = F.parse_annot_text(text.encode(), thin)
+    exp, enames = _annot_by_lines(text, thin)
+    assert names == enames
+    assert got.shape == exp.shape
+    assert np.array_equal(got, exp)
+
+
+def test_annot_known_answers(tmp_path):
+    text = ("CHR\tSNP\tBP\tCM\tA\tB\n"
+            "1\trs1\t1\t0\t1\tNA\n"
+            "\r\n"
+            "1\trs2\t2\t0\t 0.5 \t1_0\n"
+            "1\trs3\t3\t0\tinf\n"
+            "1\trs4\t4\t0\tnan\t2\textra\n")
+    m, names = F.parse_annot_text(text.encode(), thin=False)
+    assert names == ["A", "B"]
+    assert m.tolist() == [[1.0, 0.0], [0.5, 0.0], [float("inf"), 0.0], [0.0, 2.0]]
+    m, names = F.parse_annot_text(b"X Y\n1\xc2\xa0 2\n", thin=True)
+    assert names == ["X", "Y"] and m.tolist() == [[1.0, 2.0]]
+    m, names = F.parse_annot_text(b"\n\nX\tY\n", thin=True)
+    assert m.shape == (0, 2)
+    with pytest.raises(ValueError, match="empty"):
+        F.parse_annot_text(b"\r\n\n", thin=True)
+    with pytest.raises(ValueError, match="has 2 columns"):
+        F.parse_annot_text(b"A\tB\n1\t2\n", thin=False)
+    with pytest.raises(ValueError, match="UTF-8"):
+        F.parse_annot_text(b"A\tB\n1\t\xff\n", thin=True)
+    # the compressed forms read through the same path
+    import bz2 as _bz2
+    (tmp_path / "a.annot.gz").write_bytes(gzip.compress(text.encode()))
+    (tmp_path / "a.annot.bz2").write_bytes(_bz2.compress(text.encode()))
+    for f in ("a.annot.gz", "a.annot.bz2"):
+        raw = F.read_text_bytes(str(tmp_path / f))
+        assert raw == text.encode()
