@@ -50,11 +50,14 @@ int ldsc_io_parse_bim(const char* text, uint64_t len, uint64_t n,
  *   chr[r] \t names[name_off[r]..+name_len[r]] \t bp[r] \t l2[r*K+k] (k<K)
  * with values written as Rust's `{:.3}` (exact decimal value, rounded
  * half to even; "NaN", "inf", "-inf"; a negative value that rounds to zero
- * keeps its sign).  `l2` is row-major with K columns.
+ * keeps its sign).  chr, bp, name_off, name_len and the rows of `l2`
+ * (row-major, K columns) describe n_snps SNPs; `names` holds names_len
+ * bytes.  A row index or name span outside them is an error, and no file
+ * is created then.
  */
-int ldsc_io_write_ldscore(const char* path, const char* header_tail,
+int ldsc_io_write_ldscore(const char* path, const char* header_tail, uint64_t n_snps,
                           const uint8_t* chr, const uint32_t* bp,
-                          const char* names, const uint64_t* name_off,
+                          const char* names, uint64_t names_len, const uint64_t* name_off,
                           const uint32_t* name_len, const double* l2, uint64_t K,
                           const int64_t* rows, uint64_t n_rows, int threads,
                           char* err, uint64_t err_len);

@@ -576,17 +576,35 @@ extern "C" uint64_t ldsc_io_format_fixed3(double v, char* out, uint64_t cap) {
     return k;
 }
 
-extern "C" int ldsc_io_write_ldscore(const char* path, const char* header_tail,
+extern "C" int ldsc_io_write_ldscore(const char* path, const char* header_tail, uint64_t n_snps,
                                      const uint8_t* chr, const uint32_t* bp, const char* names,
-                                     const uint64_t* name_off, const uint32_t* name_len,
-                                     const double* l2, uint64_t K, const int64_t* rows,
-                                     uint64_t n_rows, int threads, char* err, uint64_t err_len) {
+                                     uint64_t names_len, const uint64_t* name_off,
+                                     const uint32_t* name_len, const double* l2, uint64_t K,
+                                     const int64_t* rows, uint64_t n_rows, int threads,
+                                     char* err, uint64_t err_len) {
     if (!path || !header_tail || (n_rows && (!chr || !bp || !names || !name_off || !name_len ||
                                              !rows || (K && !l2)))) {
         set_err(err, err_len, "ldsc_io_write_ldscore: null argument");
         return kErrInvalid;
     }
     int T = threads_or_default(threads);
+    // every row index and name span must lie inside the arrays described
+    int bad = 0;
+#pragma omp parallel for reduction(| : bad) num_threads(T)
+    for (uint64_t i = 0; i < n_rows; ++i) {
+        const int64_t r = rows[i];
+        if (r < 0 || (uint64_t)r >= n_snps) {
+            bad |= 1;
+        } else {
+            const uint64_t o = name_off[r];
+            bad |= (o > names_len || names_len - o < name_len[r]) ? 2 : 0;
+        }
+    }
+    if (bad) {
+        set_err(err, err_len, (bad & 1) ? "ldsc_io_write_ldscore: row index out of range"
+                                        : "ldsc_io_write_ldscore: SNP ID span outside the names");
+        return kErrInvalid;
+    }
     GzOut g;
     g.f = fopen(path, "wb");
     if (!g.f) {

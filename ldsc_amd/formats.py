@@ -37,8 +37,8 @@ def lib() -> C.CDLL:
         L.ldsc_io_count_lines.restype = U64
         L.ldsc_io_parse_bim.argtypes = [P, U64, U64, P, P, P, P, P, I, C.c_char_p, U64]
         L.ldsc_io_parse_bim.restype = I
-        L.ldsc_io_write_ldscore.argtypes = [C.c_char_p, C.c_char_p, P, P, P, P, P, P, U64, P,
-                                            U64, I, C.c_char_p, U64]
+        L.ldsc_io_write_ldscore.argtypes = [C.c_char_p, C.c_char_p, U64, P, P, P, U64, P, P, P,
+                                            U64, P, U64, I, C.c_char_p, U64]
         L.ldsc_io_write_ldscore.restype = I
         L.ldsc_io_format_fixed3.argtypes = [C.c_double, C.c_char_p, U64]
         L.ldsc_io_format_fixed3.restype = U64
@@ -136,9 +136,10 @@ def write_ldscore(path: str, table: BimTable, bed_idx: np.ndarray, l2: np.ndarra
     ln = np.ascontiguousarray(table.name_len[bed_idx])
     tail = "".join("\t" + c for c in col_names).encode()
     err = C.create_string_buffer(512)
-    rc = lib().ldsc_io_write_ldscore(os.fsencode(path), tail, _ptr(chr_), _ptr(bp),
-                                     C.c_char_p(table.data), _ptr(off), _ptr(ln), _ptr(l2),
-                                     l2.shape[1], _ptr(rows), rows.size, int(threads), err, 512)
+    rc = lib().ldsc_io_write_ldscore(os.fsencode(path), tail, bed_idx.size, _ptr(chr_),
+                                     _ptr(bp), C.c_char_p(table.data), len(table.data),
+                                     _ptr(off), _ptr(ln), _ptr(l2), l2.shape[1], _ptr(rows),
+                                     rows.size, int(threads), err, 512)
     if rc:
         raise OSError(err.value.decode(errors="replace"))
 

@@ -273,3 +273,31 @@ def test_annot_known_answers(tmp_path):
     for f in ("a.annot.gz", "a.annot.bz2"):
         raw = F.read_text_bytes(str(tmp_path / f))
         assert raw == text.encode()
+
+
+def test_write_ldscore_c_abi_checks_its_arrays(tmp_path):
+    """The C entry point rejects row indices and SNP ID spans outside the
+    arrays it is given, before it creates the file."""
+    import ctypes as C
+
+    chr_ = np.array([1, 2], np.uint8)
+    bp = np.array([10, 20], np.uint32)
+    names = b"rs1rs2"
+    off = np.array([0, 3], np.uint64)
+    ln = np.array([3, 3], np.uint32)
+    l2 = np.array([1.0, 2.0])
+    err = C.create_string_buffer(256)
+    path = tmp_path / "x.gz"
+
+    def call(rows, names_len=len(names)):
+        rows = np.array(rows, np.int64)
+        return F.lib().ldsc_io_write_ldscore(
+            os.fsencode(str(path)), b"\tL2", 2, chr_.ctypes.data, bp.ctypes.data, names,
+            names_len, off.ctypes.data, ln.ctypes.data, l2.ctypes.data, 1, rows.ctypes.data,
+            rows.size, 1, err, 256)
+
+    assert call([2]) == 1 and b"row index out of range" in err.value and not path.exists()
+    assert call([-1]) == 1 and not path.exists()
+    assert call([1], names_len=5) == 1 and b"span" in err.value and not path.exists()
+    assert call([1, 0]) == 0
+    assert _one_member(str(path)) == "CHR\tSNP\tBP\tL2\n2\trs2\t20\t2.000\n1\trs1\t10\t1.000\n"
