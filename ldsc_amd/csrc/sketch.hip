@@ -318,11 +318,38 @@ hipError_t launch_sketch(const SketchPlan* sp, const uint32_t* tiled, uint32_t n
     return hipGetLastError();
 }
 
-__device__ __forceinline__ float f4c(const float4& v, int c) {
-    return c == 0 ? v.x : (c == 1 ? v.y : (c == 2 ? v.z : v.w));
+// Three bf16 parts of an f32: y = hi + mid + lo + e with each part the
+// round-to-nearest-even bf16 of what the previous ones leave (the
+// subtractions are exact in f32), |mid| <= 2^-8 |y|, |lo| <= 2^-16 |y| and
+// |e| <= 2^-25 |y|.
+struct Bf16x3 {
+    bf16x8 hi, mid, lo;
+};
+
+__device__ __forceinline__ Bf16x3 split3(const float4& p, const float4& q) {
+    const float v[8] = {p.x, p.y, p.z, p.w, q.x, q.y, q.z, q.w};
+    Bf16x3 o;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const __bf16 h = (__bf16)v[i];
+        const float r1 = v[i] - (float)h;
+        const __bf16 m = (__bf16)r1;
+        o.hi[i] = h;
+        o.mid[i] = m;
+        o.lo[i] = (__bf16)(r1 - (float)m);
+    }
+    return o;
 }
 
-// d-length contraction of the sketched columns over one 64x64 band tile.
+// d-length contraction of the sketched columns over one 64x64 band tile on
+// the bf16 MFMA.  Each f32 value is split into three bf16 parts in
+// registers and a product is formed from the six part products of order
+// <= 2 (hi*hi, hi*mid, mid*hi, hi*lo, mid*mid, lo*hi), smallest first; the
+// dropped ones are below 2^-23 of it, so each product is within about
+// 2^-22 relative, next to the f32 rounding of the FP32 MFMA it replaces,
+// at six 32x32x16 bf16 steps per 16 buckets instead of eight 32x32x2 f32
+// steps.  Value r of a lane is slot r % 8 of step r / 8 for both operands,
+// so each step adds the products of the same 16 buckets.
 __global__ __launch_bounds__(64) void k_band_sk(BandArgs a) {
     __shared__ EpiShared sh;
     __shared__ double kr[64], kc[64], scr[64], scc[64];
@@ -345,41 +372,50 @@ __global__ __launch_bounds__(64) void k_band_sk(BandArgs a) {
 #pragma unroll
             for (int r = 0; r < 16; ++r) acc[bi][bj][r] = 0.0f;
 
-    float4 xa0[4], xa1[4], xb0[4], xb1[4];
+    // x[operand][q]: values 4q .. 4q+3 of the lane; operands A0, A1, B0, B1
+    float4 x[4][4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-        xa0[q] = pa0[q];
-        xa1[q] = pa1[q];
-        xb0[q] = pb0[q];
-        xb1[q] = pb1[q];
+        x[0][q] = pa0[q];
+        x[1][q] = pa1[q];
+        x[2][q] = pb0[q];
+        x[3][q] = pb1[q];
     }
     for (uint32_t B = 0; B < a.nkb; ++B) {
         const size_t o = (size_t)(B + 1 < a.nkb ? B + 1 : B) * 256;
-        float4 ya0[4], ya1[4], yb0[4], yb1[4];
+        float4 y[4][4];
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            ya0[q] = pa0[o + q];
-            ya1[q] = pa1[o + q];
-            yb0[q] = pb0[o + q];
-            yb1[q] = pb1[o + q];
+            y[0][q] = pa0[o + q];
+            y[1][q] = pa1[o + q];
+            y[2][q] = pb0[o + q];
+            y[3][q] = pb1[o + q];
         }
         __builtin_amdgcn_sched_barrier(0);  // keep the prefetch a block ahead
 #pragma unroll
-        for (int s = 0; s < 16; ++s) {
-            const float fa0 = f4c(xa0[s >> 2], s & 3), fa1 = f4c(xa1[s >> 2], s & 3);
-            const float fb0 = f4c(xb0[s >> 2], s & 3), fb1 = f4c(xb1[s >> 2], s & 3);
-            acc[0][0] = __builtin_amdgcn_mfma_f32_32x32x2f32(fa0, fb0, acc[0][0], 0, 0, 0);
-            acc[0][1] = __builtin_amdgcn_mfma_f32_32x32x2f32(fa0, fb1, acc[0][1], 0, 0, 0);
-            acc[1][0] = __builtin_amdgcn_mfma_f32_32x32x2f32(fa1, fb0, acc[1][0], 0, 0, 0);
-            acc[1][1] = __builtin_amdgcn_mfma_f32_32x32x2f32(fa1, fb1, acc[1][1], 0, 0, 0);
+        for (int u = 0; u < 2; ++u) {
+            Bf16x3 op[4];
+#pragma unroll
+            for (int t = 0; t < 4; ++t) op[t] = split3(x[t][2 * u], x[t][2 * u + 1]);
+#pragma unroll
+            for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+                for (int bj = 0; bj < 2; ++bj) {
+                    const Bf16x3& A = op[bi];
+                    const Bf16x3& Bo = op[2 + bj];
+                    v16f c = acc[bi][bj];
+                    c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A.lo, Bo.hi, c, 0, 0, 0);
+                    c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A.mid, Bo.mid, c, 0, 0, 0);
+                    c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A.hi, Bo.lo, c, 0, 0, 0);
+                    c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A.mid, Bo.hi, c, 0, 0, 0);
+                    c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A.hi, Bo.mid, c, 0, 0, 0);
+                    acc[bi][bj] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A.hi, Bo.hi, c, 0, 0, 0);
+                }
         }
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            xa0[q] = ya0[q];
-            xa1[q] = ya1[q];
-            xb0[q] = yb0[q];
-            xb1[q] = yb1[q];
-        }
+        for (int t = 0; t < 4; ++t)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) x[t][q] = y[t][q];
     }
 
     stage_info(a, sh, I, J);
