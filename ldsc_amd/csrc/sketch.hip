@@ -18,10 +18,11 @@
 //    and inverse std (the same real number its f32 LUT scatter accumulates),
 //    stored f32 in the lane order the contraction's MFMA reads.  The
 //    renormalization to ||y||^2 = N is a per-SNP scale sc_j.
-//  * The d-length contraction runs on v_mfma_f32_32x32x2f32 over 64x64 SNP
-//    tiles of the band; each dot product is scaled by sc_i sc_j, then the
-//    quadratic inversion, the unbiased r^2, masking and the annotation
-//    contraction shared with the exact path follow.
+//  * The d-length contraction runs over 64x64 SNP tiles of the band on the
+//    bf16 MFMA, each f32 value split into three bf16 parts (f32-level
+//    products); each dot product is scaled by sc_i sc_j, then the quadratic
+//    inversion, the unbiased r^2, masking and the annotation contraction
+//    shared with the exact path follow.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
