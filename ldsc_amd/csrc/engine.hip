@@ -1682,15 +1682,35 @@ int l2_begin(const ldsc_panel* panel, const ldsc_snp* snps, uint64_t m, const ld
     }
     if (!snps || !l2_out) return fail(err, err_len, LDSC_ERR_INVALID_ARG, "NULL SNP or output array");
     if (m >= (1ull << 31)) return fail(err, err_len, LDSC_ERR_INVALID_ARG, "too many SNPs");
-    bool contiguous = true;  // rows base .. base+m-1 of the panel need no index upload
+    // rows base .. base+m-1 of the panel need no index upload; the check
+    // runs in parallel pieces and reports the first SNP out of range
     const uint64_t base = snps[0].bed_idx;
-    for (uint64_t j = 0; j < m; ++j) {
-        if (snps[j].bed_idx >= panel->m_bim)
-            return fail(err, err_len, LDSC_ERR_INVALID_ARG,
-                        "SNP %llu has bed_idx %llu beyond the panel's %llu SNPs",
-                        (unsigned long long)j, (unsigned long long)snps[j].bed_idx,
-                        (unsigned long long)panel->m_bim);
-        contiguous = contiguous && snps[j].bed_idx == base + j;
+    bool contiguous = true;
+    {
+        const int T = task_count(m);
+        std::vector<uint64_t> bad(T, UINT64_MAX);
+        std::vector<uint8_t> contig(T, 1);
+        parallel_tasks(T, [&](int t) {
+            const uint64_t a = m * t / T, b = m * (t + 1) / T;
+            bool ct = true;
+            for (uint64_t j = a; j < b; ++j) {
+                const uint64_t bi = snps[j].bed_idx;
+                if (bi >= panel->m_bim) {
+                    bad[t] = j;
+                    break;
+                }
+                ct = ct && bi == base + j;
+            }
+            contig[t] = ct ? 1 : 0;
+        });
+        for (int t = 0; t < T; ++t) {
+            if (bad[t] != UINT64_MAX)
+                return fail(err, err_len, LDSC_ERR_INVALID_ARG,
+                            "SNP %llu has bed_idx %llu beyond the panel's %llu SNPs",
+                            (unsigned long long)bad[t], (unsigned long long)snps[bad[t]].bed_idx,
+                            (unsigned long long)panel->m_bim);
+            contiguous = contiguous && contig[t];
+        }
     }
     if (d && d > 1000000)
         return fail(err, err_len, LDSC_ERR_SKETCH_RANGE, "--sketch %llu too large",
@@ -1726,7 +1746,11 @@ int l2_begin(const ldsc_panel* panel, const ldsc_snp* snps, uint64_t m, const ld
     if (!contiguous) {
         HIPC(ws->pinned.ensure((size_t)m * 8));
         h_bidx = ws->pinned.as<uint64_t>();
-        for (uint64_t j = 0; j < m; ++j) h_bidx[j] = snps[j].bed_idx;
+        const int T = task_count(m);
+        parallel_tasks(T, [&](int t) {
+            const uint64_t a = m * t / T, b = m * (t + 1) / T;
+            for (uint64_t j = a; j < b; ++j) h_bidx[j] = snps[j].bed_idx;
+        });
         HIPC(ws->bed_idx.ensure((size_t)m * 8));
     }
     HIPC(ws->info.ensure((size_t)m_pad * 48));

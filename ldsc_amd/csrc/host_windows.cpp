@@ -9,6 +9,8 @@
 #include <mutex>
 #include <thread>
 
+#include <sched.h>
+
 #include "internal.h"
 
 namespace ldsc {
@@ -71,12 +73,19 @@ class TaskPool {
     uint64_t gen_ = 0;
 };
 
-// Pools are never destroyed: their workers outlive main.
+// CPUs this process may run on (its affinity mask, which a container or a
+// batch slot narrows; hardware_concurrency counts every CPU of the node).
+int usable_cpus() {
+    cpu_set_t set;
+    CPU_ZERO(&set);
+    if (sched_getaffinity(0, sizeof(set), &set) == 0) return CPU_COUNT(&set);
+    return (int)std::thread::hardware_concurrency();
+}
+
+// Pools are never destroyed: their workers outlive main.  The planning pool
+// takes all usable CPUs but one, at most 15 workers.
 TaskPool& pool() {
-    static TaskPool* p = [] {
-        const unsigned hw = std::thread::hardware_concurrency();
-        return new TaskPool((int)std::min<unsigned>(hw > 1 ? hw - 1 : 0, 7));
-    }();
+    static TaskPool* p = [] { return new TaskPool(std::min(std::max(usable_cpus() - 1, 0), 15)); }();
     return *p;
 }
 
@@ -145,20 +154,33 @@ struct Run {
     uint64_t start, end;
 };
 
+// Contiguous runs of one chromosome code (window.rs:41-49) of m >= 1 SNPs,
+// found in parallel pieces: each task lists the run starts in its share.
+static std::vector<Run> chromosome_runs(const ldsc_snp* snps, uint64_t m) {
+    const int T = task_count(m);
+    std::vector<std::vector<uint64_t>> starts(T);
+    parallel_tasks(T, [&](int t) {
+        const uint64_t a = std::max<uint64_t>(1, m * t / T), b = m * (t + 1) / T;
+        for (uint64_t k = a; k < b; ++k)
+            if (snps[k].chr != snps[k - 1].chr) starts[t].push_back(k);
+    });
+    std::vector<Run> runs;
+    uint64_t s0 = 0;
+    for (const auto& v : starts)
+        for (uint64_t k : v) {
+            runs.push_back({s0, k});
+            s0 = k;
+        }
+    runs.push_back({s0, m});
+    return runs;
+}
+
 bool block_lefts(const ldsc_snp* snps, uint64_t m, int mode, double value,
                  std::vector<uint64_t>& bl) {
     bl.resize(m);
     if (m == 0) return false;
     const bool cm = mode == LDSC_WINDOW_CM;
-    // contiguous runs of one chromosome code (window.rs:41-49)
-    std::vector<Run> runs;
-    uint64_t s0 = 0;
-    for (uint64_t k = 1; k < m; ++k)
-        if (snps[k].chr != snps[k - 1].chr) {
-            runs.push_back({s0, k});
-            s0 = k;
-        }
-    runs.push_back({s0, m});
+    const std::vector<Run> runs = chromosome_runs(snps, m);
     const int T = task_count(m);
     bool any_full = false;
     if (mode == LDSC_WINDOW_SNP) {
@@ -241,32 +263,24 @@ void pair_bounds(const ldsc_snp* snps, uint64_t m, const ldsc_l2_config* cfg,
     if (any_full) *any_full = full;
     if (cfg->snp_level_masking || m == 0) return;
     const uint64_t c = cfg->chunk_size;
-    std::vector<std::pair<uint64_t, uint64_t>> runs;  // chunk grids
-    uint64_t run0 = 0;
-    while (run0 < m) {
-        uint64_t run1 = m;
-        if (cfg->per_chromosome) {
-            run1 = run0 + 1;
-            while (run1 < m && snps[run1].chr == snps[run0].chr) ++run1;
-        }
-        runs.push_back({run0, run1});
-        run0 = run1;
-    }
+    // chunk grids: one per chromosome run, or one over all SNPs
+    const std::vector<Run> runs =
+        cfg->per_chromosome ? chromosome_runs(snps, m) : std::vector<Run>{{0, m}};
     const int T = task_count(m);
     parallel_tasks(T, [&](int t) {
         const uint64_t a = m * t / T, b = m * (t + 1) / T;
-        for (const auto& rr : runs) {
-            if (rr.second <= a || rr.first >= b) continue;
+        for (const Run& rr : runs) {
+            if (rr.end <= a || rr.start >= b) continue;
             // first chunk start of the run at or after a
-            uint64_t s = rr.first;
+            uint64_t s = rr.start;
             if (a > s) s += (a - s + c - 1) / c * c;
-            for (; s < rr.second && s < b; s += c) {
-                const uint64_t sl = s - rr.first;  // chunk start, local to the run
+            for (; s < rr.end && s < b; s += c) {
+                const uint64_t sl = s - rr.start;  // chunk start, local to the run
                 const uint64_t dist = s - lo[s];
                 uint64_t bsz = dist == 0 ? 0 : ((dist + c - 1) / c) * c;
                 if (bsz > sl) bsz = sl;
                 const uint64_t a_left = s - bsz;
-                const uint64_t e = s + c < rr.second ? s + c : rr.second;
+                const uint64_t e = s + c < rr.end ? s + c : rr.end;
                 for (uint64_t j = s; j < e; ++j) lo[j] = a_left;
             }
         }
