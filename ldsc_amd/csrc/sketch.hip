@@ -342,6 +342,35 @@ __device__ __forceinline__ Bf16x3 split3(const float4& p, const float4& q) {
     return o;
 }
 
+// One 16-bucket step of the 64x64 tile: the six part products of order
+// <= 2, smallest first, the four accumulators taking turns so that
+// consecutive MFMAs are independent.
+__device__ __forceinline__ void sk_mma(const Bf16x3 (&op)[4], v16f (&acc)[2][2]) {
+#pragma unroll
+    for (int pp = 0; pp < 6; ++pp)
+#pragma unroll
+        for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+            for (int bj = 0; bj < 2; ++bj) {
+                const Bf16x3& A = op[bi];
+                const Bf16x3& Bo = op[2 + bj];
+                const bf16x8& pa = pp == 0 ? A.lo : pp == 1 || pp == 3 ? A.mid : A.hi;
+                const bf16x8& pb = pp == 2 ? Bo.lo : pp == 1 || pp == 4 ? Bo.mid : Bo.hi;
+                acc[bi][bj] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(pa, pb, acc[bi][bj], 0, 0, 0);
+            }
+}
+
+// Asks the scheduler to issue the split's VALU work between the 24 MFMAs
+// of a step rather than before them.
+__device__ __forceinline__ void sk_interleave() {
+#pragma unroll
+    for (int i = 0; i < 24; ++i) {
+        __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // one MFMA
+        __builtin_amdgcn_sched_group_barrier(0x002, 10, 0); // then VALU
+    }
+    __builtin_amdgcn_sched_barrier(0);
+}
+
 // d-length contraction of the sketched columns over one 64x64 band tile on
 // the bf16 MFMA.  Each f32 value is split into three bf16 parts in
 // registers and a product is formed from the six part products of order
@@ -373,7 +402,10 @@ __global__ __launch_bounds__(64) void k_band_sk(BandArgs a) {
 #pragma unroll
             for (int r = 0; r < 16; ++r) acc[bi][bj][r] = 0.0f;
 
-    // x[operand][q]: values 4q .. 4q+3 of the lane; operands A0, A1, B0, B1
+    // x[operand][q]: values 4q .. 4q+3 of the lane; operands A0, A1, B0, B1.
+    // Software pipeline over the two 16-bucket steps of each block: the
+    // split of the next step runs on the VALU while the MFMAs of the current
+    // one run (e0 holds step 0 of a block, e1 step 1).
     float4 x[4][4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
@@ -382,6 +414,9 @@ __global__ __launch_bounds__(64) void k_band_sk(BandArgs a) {
         x[2][q] = pb0[q];
         x[3][q] = pb1[q];
     }
+    Bf16x3 e0[4], e1[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) e0[t] = split3(x[t][0], x[t][1]);
     for (uint32_t B = 0; B < a.nkb; ++B) {
         const size_t o = (size_t)(B + 1 < a.nkb ? B + 1 : B) * 256;
         float4 y[4][4];
@@ -394,25 +429,13 @@ __global__ __launch_bounds__(64) void k_band_sk(BandArgs a) {
         }
         __builtin_amdgcn_sched_barrier(0);  // keep the prefetch a block ahead
 #pragma unroll
-        for (int u = 0; u < 2; ++u) {
-            Bf16x3 op[4];
+        for (int t = 0; t < 4; ++t) e1[t] = split3(x[t][2], x[t][3]);
+        sk_mma(e0, acc);
+        sk_interleave();
 #pragma unroll
-            for (int t = 0; t < 4; ++t) op[t] = split3(x[t][2 * u], x[t][2 * u + 1]);
-#pragma unroll
-            for (int bi = 0; bi < 2; ++bi)
-#pragma unroll
-                for (int bj = 0; bj < 2; ++bj) {
-                    const Bf16x3& A = op[bi];
-                    const Bf16x3& Bo = op[2 + bj];
-                    v16f c = acc[bi][bj];
-                    c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A.lo, Bo.hi, c, 0, 0, 0);
-                    c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A.mid, Bo.mid, c, 0, 0, 0);
-                    c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A.hi, Bo.lo, c, 0, 0, 0);
-                    c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A.mid, Bo.hi, c, 0, 0, 0);
-                    c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A.hi, Bo.mid, c, 0, 0, 0);
-                    acc[bi][bj] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A.hi, Bo.hi, c, 0, 0, 0);
-                }
-        }
+        for (int t = 0; t < 4; ++t) e0[t] = split3(y[t][0], y[t][1]);
+        sk_mma(e1, acc);
+        sk_interleave();
 #pragma unroll
         for (int t = 0; t < 4; ++t)
 #pragma unroll
