@@ -232,6 +232,8 @@ typedef struct {
 
 int ldsc_server_create(const ldsc_server_config* cfg, ldsc_server** out, char* err,
                        uint64_t err_len);
+/* Finishes the running requests, abandons the queued ones and frees the
+ * server.  No other call on the server may run during or after it. */
 int ldsc_server_destroy(ldsc_server* server);
 int ldsc_server_stats_get(ldsc_server* server, ldsc_server_stats* out);
 

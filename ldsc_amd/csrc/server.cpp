@@ -96,13 +96,21 @@ struct BandBatcher {
     uint64_t launches = 0, batched = 0;
     uint32_t max_batch = 0;
 
+    // what one round of launches did, added to the counters under mu
+    struct Tally {
+        uint64_t launches = 0, batched = 0;
+        uint32_t max_batch = 0;
+    };
+
     int run(ldsc::L2Call* call, char* err, uint64_t err_len);
-    void launch_all(std::vector<Entry*>& take);
+    Tally launch_all(std::vector<Entry*>& take);
     void destroy();
 };
 
-void BandBatcher::launch_all(std::vector<Entry*>& take) {
+// Runs on the leader alone (mu released): it owns `devices` meanwhile.
+BandBatcher::Tally BandBatcher::launch_all(std::vector<Entry*>& take) {
     // one launch per (device, form) group, at most BandBatchSlot::kMaxCalls each
+    Tally tally;
     std::vector<bool> used(take.size(), false);
     for (size_t i = 0; i < take.size(); ++i) {
         if (used[i]) continue;
@@ -133,11 +141,12 @@ void BandBatcher::launch_all(std::vector<Entry*>& take) {
             if (rc) en->err = err;
         }
         if (rc == LDSC_OK) {
-            ++launches;
-            if (grp.size() > 1) batched += grp.size();
-            max_batch = std::max<uint32_t>(max_batch, (uint32_t)grp.size());
+            ++tally.launches;
+            if (grp.size() > 1) tally.batched += grp.size();
+            tally.max_batch = std::max<uint32_t>(tally.max_batch, (uint32_t)grp.size());
         }
     }
+    return tally;
 }
 
 int BandBatcher::run(ldsc::L2Call* call, char* err, uint64_t err_len) {
@@ -168,8 +177,11 @@ int BandBatcher::run(ldsc::L2Call* call, char* err, uint64_t err_len) {
         std::vector<Entry*> take;
         take.swap(pending);
         lk.unlock();
-        launch_all(take);
+        const Tally t = launch_all(take);
         lk.lock();
+        launches += t.launches;
+        batched += t.batched;
+        max_batch = std::max(max_batch, t.max_batch);
         for (Entry* en : take) en->launched = true;
         leading = false;
         cv.notify_all();
