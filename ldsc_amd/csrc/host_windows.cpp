@@ -26,24 +26,31 @@ class TaskPool {
         nthreads_ = nt;
     }
     int threads() const { return nthreads_; }
+    // Runs f(0) .. f(n-1) on the caller and up to n-1 workers.  Only as many
+    // workers as can help are woken, and the caller waits only for those
+    // that joined: it closes the job once its own loop ends, and a worker
+    // that wakes later stays out.
     void run(int n, const std::function<void(int)>& f) {
         std::unique_lock<std::mutex> busy(run_mu_, std::try_to_lock);
         if (!busy.owns_lock() || nthreads_ == 0 || n <= 1) {
             for (int i = 0; i < n; ++i) f(i);
             return;
         }
+        const int want = std::min(n - 1, nthreads_);
         {
             std::lock_guard<std::mutex> lk(mu_);
             job_ = &f;
             ntasks_ = n;
             next_.store(0);
-            active_ = nthreads_;
+            slots_ = want;
+            running_ = 0;
             ++gen_;
         }
-        cv_.notify_all();
+        for (int i = 0; i < want; ++i) cv_.notify_one();
         for (int i; (i = next_.fetch_add(1)) < n;) f(i);
         std::unique_lock<std::mutex> lk(mu_);
-        done_cv_.wait(lk, [this] { return active_ == 0; });
+        slots_ = 0;
+        done_cv_.wait(lk, [this] { return running_ == 0; });
         job_ = nullptr;
     }
 
@@ -55,20 +62,22 @@ class TaskPool {
             int n;
             {
                 std::unique_lock<std::mutex> lk(mu_);
-                cv_.wait(lk, [&] { return gen_ != seen; });
+                cv_.wait(lk, [&] { return gen_ != seen && slots_ > 0; });
                 seen = gen_;
+                --slots_;
+                ++running_;
                 f = job_;
                 n = ntasks_;
             }
             for (int i; (i = next_.fetch_add(1)) < n;) (*f)(i);
             std::lock_guard<std::mutex> lk(mu_);
-            if (--active_ == 0) done_cv_.notify_one();
+            if (--running_ == 0) done_cv_.notify_one();
         }
     }
     std::mutex run_mu_, mu_;
     std::condition_variable cv_, done_cv_;
     const std::function<void(int)>* job_ = nullptr;
-    int ntasks_ = 0, active_ = 0, nthreads_ = 0;
+    int ntasks_ = 0, slots_ = 0, running_ = 0, nthreads_ = 0;
     std::atomic<int> next_{0};
     uint64_t gen_ = 0;
 };
