@@ -320,7 +320,10 @@ def run(args) -> int:
     keep = load_individual_indices(args.keep, parse_fam_ids(fam_path)) if args.keep else None
 
     t_in = time.perf_counter()
-    bed = np.fromfile(bed_path, dtype=np.uint8)
+    # mapped, not read: the upload copies straight from the page cache (an
+    # empty file cannot be mapped; the panel then reports it too short)
+    bed = (np.memmap(bed_path, dtype=np.uint8, mode="r") if os.path.getsize(bed_path) > 0
+           else np.zeros(0, np.uint8))
     panel = Panel(bed, n_fam, bim.n, keep=keep)
     del bed
     n_indiv = panel.n_indiv
