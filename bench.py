@@ -22,7 +22,7 @@ Default run (one JSON line):
 `--workload reference_shapes` (the panel shapes of the reference's published
 CPU wall times) and `--workload cli` (the `ldsc l2` command end to end, from
 the PLINK files to the written LD-score files, at the 1000G shape) print
-their own JSON lines.  `--gpus` above 1 is refused: this build targets one GPU.
+their own JSON lines.  `--gpus` above 1 is refused: the engine runs on one GPU.
 """
 from __future__ import annotations
 
@@ -637,7 +637,7 @@ def main() -> int:
         return 0
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the LD-score "
-              "engine has no multi-GPU path in this build.", file=sys.stderr)
+              "engine runs on one GPU and has no multi-GPU path.", file=sys.stderr)
         return 2
 
     import ldsc_amd as L
