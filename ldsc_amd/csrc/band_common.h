@@ -24,7 +24,7 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 __device__ inline int tile_of_block(const BandArgs& a) {
     const uint32_t xcd = blockIdx.x & 7u, slot = blockIdx.x >> 3;
     const uint32_t t = xcd * a.per_xcd + slot;
-    return t < a.ntiles ? (int)t : -1;
+    return t < a.ntiles ? (int)(a.tile0 + t) : -1;
 }
 
 // Several calls' bands in one launch (the request server): the tiles of

@@ -48,6 +48,7 @@ struct BandArgs {
     const float* sk_scale; // their per-SNP renorm scales
     const double* kappa;   // per-SNP kurtosis of the MAF-aware sketch or nullptr
     uint32_t nkb, ntiles, per_xcd, m, m_pad, K;
+    uint32_t tile0;        // first tile of this launch in the call's tile list
     double r2u_a, r2u_b, n_inv_sq_r2u_a, n_inv_sq, d_f;
     double n_pad;          // individuals of the tiles, padding included
 };
@@ -225,6 +226,15 @@ struct L2Call {
     size_t off_st = 0, off_l2 = 0;  // in ws->pinned_meta
     SketchPlan* sketch_plan = nullptr;
     BandArgs ba{};
+    // A large annotation goes up in row segments; the band of segment s
+    // (row blocks seg_I[s] .. seg_I[s+1]-1) starts once rows up to
+    // 64 seg_I[s+1] are on the device, while the next segment uploads.
+    static constexpr uint32_t kMaxSeg = 4;
+    const double* annot = nullptr;
+    uint32_t nseg = 1, Kp = 1;
+    uint32_t seg_I[kMaxSeg + 1] = {};
+    uint32_t seg_dl_a = 0, seg_dl_b = 0;  // reduced rows l2_end brings back
+    bool has_pq = false;
 };
 // Argument table of one batched band launch (device copy + pinned staging),
 // reused once the launch that read it has finished.
@@ -237,10 +247,13 @@ struct BandBatchSlot {
     bool used = false;
     void release();
 };
+// segment_annot: a large annotation may go up in row segments that l2_band
+// interleaves with the band (one-call path only; a batched band needs the
+// whole annotation at l2_begin's end).
 int l2_begin(const ldsc_panel* panel, const ldsc_snp* snps, uint64_t m, const ldsc_l2_config* cfg,
              const double* annot, uint64_t K, double* l2_out, double* maf_out, ldsc_l2_perf* perf,
              ldsc_progress_cb cb, void* cb_ctx, Workspace* ws, L2Call* call, char* err,
-             uint64_t err_len);
+             uint64_t err_len, bool segment_annot = false);
 int l2_band(L2Call* call, char* err, uint64_t err_len);
 int band_batch_launch(L2Call* const* calls, uint32_t ncalls, BandBatchSlot* slot, hipStream_t bst,
                       char* err, uint64_t err_len);

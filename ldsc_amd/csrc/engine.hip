@@ -146,13 +146,17 @@ void Workspace::destroy() {
 
 // Host -> device copy of `bytes` of caller memory through the pinned ring,
 // on stream s, filled by the side copy pool (the caller runs it beside the
-// host's planning).  Returns once the last slot is filled; its DMA may still
-// be in flight (later work on s is ordered after it).
+// host's planning).  With `fill`, slot bytes [off, off+len) of the
+// destination come from fill(slot, off, len) instead of src + off.  A slot
+// is refilled only once its previous DMA (of any earlier transfer) is done.
+// Returns once the last slot is filled; its DMA may still be in flight
+// (later work on s is ordered after it).
+using RingFill = std::function<void(char* slot, size_t off, size_t len)>;
 static hipError_t ring_upload(Workspace* ws, void* dst, const void* src, size_t bytes,
-                              hipStream_t s) {
+                              hipStream_t s, const RingFill* fill = nullptr) {
     constexpr size_t SB = Workspace::kRingSlot;
     constexpr int NS = Workspace::kRingSlots;
-    if (bytes <= SB)  // small: the runtime's own staging
+    if (bytes <= SB && !fill)  // small: the runtime's own staging
         return hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s);
     hipError_t e = ws->ring.ensure(SB * NS);
     if (e != hipSuccess) return e;
@@ -161,8 +165,11 @@ static hipError_t ring_upload(Workspace* ws, void* dst, const void* src, size_t 
     for (size_t off = 0; off < bytes; off += SB, ++c) {
         const int slot = (int)(c % NS);
         const size_t len = std::min(SB, bytes - off);
-        if (c >= (size_t)NS && (e = hipEventSynchronize(ws->ring_ev[slot])) != hipSuccess) return e;
-        parallel_copy(ring + slot * SB, static_cast<const char*>(src) + off, len, /*side=*/true);
+        if ((e = hipEventSynchronize(ws->ring_ev[slot])) != hipSuccess) return e;
+        if (fill)
+            (*fill)(ring + slot * SB, off, len);
+        else
+            parallel_copy(ring + slot * SB, static_cast<const char*>(src) + off, len, /*side=*/true);
         e = hipMemcpyAsync(static_cast<char*>(dst) + off, ring + slot * SB, len,
                            hipMemcpyHostToDevice, s);
         if (e != hipSuccess) return e;
@@ -171,11 +178,56 @@ static hipError_t ring_upload(Workspace* ws, void* dst, const void* src, size_t 
     return hipSuccess;
 }
 
+// Elements [e0, e1) of a block holding rows [a, a + rows) of every column
+// of a column-major matrix with m rows (the block column-major with column
+// stride `rows`), copied from the matrix into the block or back, in column
+// runs split over the main or side pool.
+static void copy_block_rows(char* block, size_t e_base, double* mat, uint64_t m, uint64_t a,
+                            uint64_t rows, size_t e0, size_t e1, bool to_block, bool side) {
+    const int T = copy_task_count((e1 - e0) * 8, side);
+    auto task = [&](int t) {
+        size_t e = e0 + (e1 - e0) * t / T;
+        const size_t end = e0 + (e1 - e0) * (t + 1) / T;
+        while (e < end) {
+            const uint64_t k = e / rows, row = e % rows;
+            const size_t run = std::min<size_t>(end - e, rows - row);
+            char* b = block + (e - e_base) * 8;
+            double* x = mat + k * m + a + row;
+            if (to_block)
+                std::memcpy(b, x, run * 8);
+            else
+                std::memcpy(x, b, run * 8);
+            e += run;
+        }
+    };
+    if (side)
+        parallel_tasks_side(T, task);
+    else
+        parallel_tasks(T, task);
+}
+
+// Rows [r0, r1) of every column of the column-major m x K annotation go up
+// as one block, column-major with column stride r1 - r0, at row offset r0
+// of the device copy (segments lie back to back in row order).
+static hipError_t upload_annot_rows(Workspace* ws, const double* annot, uint64_t m, uint64_t K,
+                                    uint64_t r0, uint64_t r1, hipStream_t s) {
+    const uint64_t rows = r1 - r0;
+    const RingFill fill = [=](char* slot, size_t off, size_t len) {
+        copy_block_rows(slot, off / 8, const_cast<double*>(annot), m, r0, rows, off / 8,
+                        (off + len) / 8, /*to_block=*/true, /*side=*/true);
+    };
+    return ring_upload(ws, ws->annot.as<double>() + r0 * K, nullptr, rows * K * 8, s, &fill);
+}
+
 // Device -> host copy into caller memory through the pinned ring on stream
 // s; ready(end) is called before the copy of bytes [.., end) is queued (to
-// order it after their producer).  Returns when every byte has landed.
+// order it after their producer).  With `drain`, slot bytes [off, off+len)
+// go out through drain(slot, off, len) instead of to dst + off.  Returns
+// when every byte has landed.
+using RingDrain = std::function<void(const char* slot, size_t off, size_t len)>;
 static hipError_t ring_download(Workspace* ws, void* dst, const void* src, size_t bytes,
-                                hipStream_t s, const std::function<hipError_t(size_t)>& ready) {
+                                hipStream_t s, const std::function<hipError_t(size_t)>& ready,
+                                const RingDrain* drain = nullptr) {
     constexpr size_t SB = Workspace::kRingSlot;
     constexpr int NS = Workspace::kRingSlots;
     hipError_t e = ws->ring.ensure(SB * NS);
@@ -186,6 +238,8 @@ static hipError_t ring_download(Workspace* ws, void* dst, const void* src, size_
         const int slot = (int)(c % NS);
         const size_t off = c * SB, len = std::min(SB, bytes - off);
         hipError_t r = ready(off + len);
+        // after any upload DMA still reading the slot
+        if (r == hipSuccess) r = hipStreamWaitEvent(s, ws->ring_ev[slot], 0);
         if (r == hipSuccess)
             r = hipMemcpyAsync(ring + slot * SB, static_cast<const char*>(src) + off, len,
                                hipMemcpyDeviceToHost, s);
@@ -197,10 +251,26 @@ static hipError_t ring_download(Workspace* ws, void* dst, const void* src, size_
         const int slot = (int)(c % NS);
         const size_t off = c * SB, len = std::min(SB, bytes - off);
         if ((e = hipEventSynchronize(ws->ring_ev[slot])) != hipSuccess) return e;
-        parallel_copy(static_cast<char*>(dst) + off, ring + slot * SB, len);
+        if (drain)
+            (*drain)(ring + slot * SB, off, len);
+        else
+            parallel_copy(static_cast<char*>(dst) + off, ring + slot * SB, len);
         if (c + NS < nc && (e = issue(c + NS)) != hipSuccess) return e;
     }
     return hipSuccess;
+}
+
+// LD scores of rows [a, b), reduced as one block (column-major within it,
+// at row offset a of the device result), into the column-major m x K `out`.
+static hipError_t download_l2_rows(Workspace* ws, double* out, uint64_t m, uint64_t K, uint64_t a,
+                                   uint64_t b, hipStream_t s) {
+    const uint64_t rows = b - a;
+    const RingDrain drain = [=](const char* slot, size_t off, size_t len) {
+        copy_block_rows(const_cast<char*>(slot), off / 8, out, m, a, rows, off / 8,
+                        (off + len) / 8, /*to_block=*/false, /*side=*/false);
+    };
+    return ring_download(ws, nullptr, ws->l2.as<double>() + a * K, rows * K * 8, s,
+                         [](size_t) { return hipSuccess; }, &drain);
 }
 
 static std::mutex g_ws_mu;
@@ -793,22 +863,25 @@ __global__ void k_snp_info(const uint32_t* __restrict__ part, uint32_t nparts,
 }
 
 // A_eff (m_pad rows of Kp = aeff_stride(K) doubles, annotations contiguous)
-// from the uploaded annotation `src` (column-major m x K, column stride m;
-// ones when null): NaN -> 0 (read_annot, parse.rs:224), times pq_j when
-// given (compute.rs:1595-1616); padding rows and columns are zero.  A
-// 64-SNP x 16-annotation block goes through LDS, so that both the reads
-// (along SNPs) and the writes (along annotations) are coalesced.
+// from the uploaded annotation `src` (ones when null): NaN -> 0 (read_annot,
+// parse.rs:224), times pq_j when given (compute.rs:1595-1616); padding rows
+// and columns are zero.  The launch covers rows j_first .. (a multiple of
+// 64) of which `src` holds, column-major with column stride `stride`, the
+// rows from j_first on.  A 64-SNP x 16-annotation block goes through LDS,
+// so that both the reads (along SNPs) and the writes (along annotations)
+// are coalesced.
 __global__ __launch_bounds__(256) void k_aeff(double* __restrict__ out, const double* __restrict__ src,
+                                              uint64_t stride, uint32_t j_first,
                                               const double* __restrict__ pq, uint32_t m, uint32_t K,
                                               uint32_t Kp) {
     __shared__ double blk[16][65];
-    const uint32_t j0 = blockIdx.x * 64, k0 = blockIdx.y * 16, t = threadIdx.x;
+    const uint32_t j0 = j_first + blockIdx.x * 64, k0 = blockIdx.y * 16, t = threadIdx.x;
     if (Kp == 1) {  // one annotation: 64 SNPs, no transposition
         if (t < 64) {
             const uint32_t j = j0 + t;
             double v = 0.0;
             if (j < m) {
-                v = src ? src[j] : 1.0;
+                v = src ? src[j - j_first] : 1.0;
                 if (v != v) v = 0.0;
                 if (pq) v *= pq[j];
             }
@@ -821,7 +894,7 @@ __global__ __launch_bounds__(256) void k_aeff(double* __restrict__ out, const do
         const uint32_t kk = t / 64 + 4 * i, jj = t % 64, j = j0 + jj, k = k0 + kk;
         double v = 0.0;
         if (j < m && k < K) {
-            v = src ? src[(size_t)k * m + j] : 1.0;
+            v = src ? src[(size_t)k * stride + (j - j_first)] : 1.0;
             if (v != v) v = 0.0;
             if (pq) v *= pq[j];
         }
@@ -1359,13 +1432,19 @@ __global__ __launch_bounds__(64) void k_band_f32_batch(BandBatch b) {
 // ------------------------------------------------------------------------
 // Kernel 4: deterministic per-SNP reduction
 // ------------------------------------------------------------------------
+// SNPs j_begin .. j_begin + j_count - 1 and annotations k_first ..
+// k_first + k_count - 1; the result of (j, k) goes to
+// l2[j_begin K + k j_count + (j - j_begin)]: column-major over the whole
+// matrix when the range is every SNP, else a block of its own (the rows of
+// a segment, column-major within it).
 __global__ void k_reduce(const double* __restrict__ rowp, const double* __restrict__ colp,
                          const double* __restrict__ aeff, const int* __restrict__ j0,
-                         const int* __restrict__ tstart, uint32_t nb, uint32_t m, uint32_t K,
-                         uint32_t k_first, uint32_t k_count, double* __restrict__ l2) {
+                         const int* __restrict__ tstart, uint32_t nb, uint32_t j_begin,
+                         uint32_t j_count, uint32_t K, uint32_t k_first, uint32_t k_count,
+                         double* __restrict__ l2) {
     const uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (gid >= (uint64_t)m * k_count) return;
-    const uint32_t k = k_first + (uint32_t)(gid / m), j = (uint32_t)(gid % m);
+    if (gid >= (uint64_t)j_count * k_count) return;
+    const uint32_t k = k_first + (uint32_t)(gid / j_count), j = j_begin + (uint32_t)(gid % j_count);
     const uint32_t I = j / kTile, r = j % kTile;
     double s = aeff[(size_t)j * aeff_stride(K) + k];  // self pair, r2u(j, j) = 1 (compute.rs:1658)
     // row tiles J = j0[I] .. I, then column tiles I2 = I .. while j0[I2] <= I;
@@ -1394,7 +1473,7 @@ __global__ void k_reduce(const double* __restrict__ rowp, const double* __restri
     }
     for (; I2 < nb && (uint32_t)j0[I2] <= I; ++I2)
         s += colp[(((size_t)tstart[I2] + (I - j0[I2])) * K + k) * kTile + r];
-    l2[(size_t)k * m + j] = s;
+    l2[(size_t)j_begin * K + (size_t)k * j_count + (j - j_begin)] = s;
 }
 
 // ------------------------------------------------------------------------
@@ -1656,7 +1735,7 @@ static void compute_maf_host(const L2Call& c) {
 int l2_begin(const ldsc_panel* panel, const ldsc_snp* snps, uint64_t m, const ldsc_l2_config* cfg,
              const double* annot, uint64_t K, double* l2_out, double* maf_out, ldsc_l2_perf* perf,
              ldsc_progress_cb cb, void* cb_ctx, Workspace* ws, L2Call* call, char* err,
-             uint64_t err_len) {
+             uint64_t err_len, bool segment_annot) {
     const double t_start = now_s();
     L2Call& c = *call;
     c = L2Call{};
@@ -1799,12 +1878,26 @@ int l2_begin(const ldsc_panel* panel, const ldsc_snp* snps, uint64_t m, const ld
             if (t.joinable()) t.join();
         }
     } upload_joiner{upload};
+    // A large annotation goes up in row segments: the first here, the others
+    // in l2_band, each while the band of the segment before it runs.
+    uint32_t nseg = 1;
+    if (segment_annot && annot) {
+        const uint64_t bytes = m * K * 8;
+        nseg = bytes >= (128ull << 20) ? 4u : bytes >= (32ull << 20) ? 2u : 1u;
+        if (m < (uint64_t)nseg * 1024) nseg = 1;
+    }
+    uint32_t seg_r[L2Call::kMaxSeg + 1];
+    for (uint32_t sg = 0; sg <= nseg; ++sg)
+        seg_r[sg] = sg == nseg ? m_pad : (uint32_t)round_up(m * sg / nseg, kTile);
     if (annot) {
         HIPC(ws->annot.ensure((size_t)m * K * 8));
         const int dev = panel->device;
-        upload = std::thread([ws, annot, m, K, dev, &upload_err] {
+        const uint64_t r1 = seg_r[1];
+        upload = std::thread([ws, annot, m, K, dev, nseg, r1, &upload_err] {
             hipError_t e = hipSetDevice(dev);
-            if (e == hipSuccess) e = ring_upload(ws, ws->annot.p, annot, (size_t)m * K * 8, ws->up_stream);
+            if (e == hipSuccess)
+                e = nseg > 1 ? upload_annot_rows(ws, annot, m, K, 0, r1, ws->up_stream)
+                             : ring_upload(ws, ws->annot.p, annot, (size_t)m * K * 8, ws->up_stream);
             if (e == hipSuccess) e = hipEventRecord(ws->ev[8], ws->up_stream);
             upload_err = e;
         });
@@ -1952,10 +2045,17 @@ int l2_begin(const ldsc_panel* panel, const ldsc_snp* snps, uint64_t m, const ld
         HIPC(upload_err);
         HIPC(hipStreamWaitEvent(st, ws->ev[8], 0));
     }
-    hipLaunchKernelGGL(k_aeff, dim3(m_pad / 64, Kp / 16 + (Kp == 1)), dim3(256), 0, st,
+    // A_eff of every row, or of the first segment's rows
+    hipLaunchKernelGGL(k_aeff, dim3(seg_r[1] / 64, Kp / 16 + (Kp == 1)), dim3(256), 0, st,
                        ws->aeff.as<double>(), annot ? ws->annot.as<double>() : nullptr,
+                       (uint64_t)(nseg > 1 ? seg_r[1] : m), 0u,
                        has_pq ? ws->pq.as<double>() : nullptr, (uint32_t)m, (uint32_t)K, Kp);
     HIPC(hipGetLastError());
+    c.annot = annot;
+    c.nseg = nseg;
+    c.Kp = Kp;
+    c.has_pq = has_pq;
+    for (uint32_t sg = 0; sg <= nseg; ++sg) c.seg_I[sg] = seg_r[sg] / kTile;
 
     // 3. band contraction
     BandArgs ba{};
@@ -1994,20 +2094,87 @@ int l2_begin(const ldsc_panel* panel, const ldsc_snp* snps, uint64_t m, const ld
     return LDSC_OK;
 }
 
+static hipError_t launch_band_form(int form, const BandArgs& ba, hipStream_t st) {
+    if (form == kBandSketch) return launch_band_sk(ba, st);
+    if (form == kBandF32)
+        hipLaunchKernelGGL(k_band_f32, dim3(8 * ba.per_xcd), dim3(64), 0, st, ba);
+    else if (form == kBandFp4)
+        hipLaunchKernelGGL(k_band_fp4, dim3(8 * ba.per_xcd), dim3(64), 0, st, ba);
+    else
+        hipLaunchKernelGGL(k_band_i8, dim3(8 * ba.per_xcd), dim3(128), 0, st, ba);
+    return hipGetLastError();
+}
+
 int l2_band(L2Call* call, char* err, uint64_t err_len) {
     L2Call& c = *call;
     Workspace* ws = c.ws;
     hipStream_t st = ws->stream;
-    const BandArgs& ba = c.ba;
-    if (c.form == kBandSketch)
-        HIPC(launch_band_sk(ba, st));
-    else if (c.form == kBandF32)
-        hipLaunchKernelGGL(k_band_f32, dim3(8 * ba.per_xcd), dim3(64), 0, st, ba);
-    else if (c.form == kBandFp4)
-        hipLaunchKernelGGL(k_band_fp4, dim3(8 * ba.per_xcd), dim3(64), 0, st, ba);
-    else
-        hipLaunchKernelGGL(k_band_i8, dim3(8 * ba.per_xcd), dim3(128), 0, st, ba);
-    HIPC(hipGetLastError());
+    if (c.nseg <= 1) {
+        HIPC(launch_band_form(c.form, c.ba, st));
+    } else {
+        // segment sg: its annotation rows go up (the first already did) while
+        // the band of the segments before it runs, then its A_eff rows, then
+        // the band of its row blocks, whose columns lie in this segment or
+        // the ones before
+        const std::vector<int>& ts = ws->host.plan.tstart;
+        const std::vector<int>& j0v = ws->host.plan.j0;
+        uint32_t red_to = 0;            // rows reduced so far
+        uint32_t dl_a = 0, dl_b = 0;    // rows reduced after the last band, not yet back
+        for (uint32_t sg = 0; sg < c.nseg; ++sg) {
+            const uint32_t r0 = c.seg_I[sg] * kTile, r1 = c.seg_I[sg + 1] * kTile;
+            if (sg > 0) {
+                const uint64_t u1 = std::min<uint64_t>(r1, c.m);
+                HIPC(upload_annot_rows(ws, c.annot, c.m, c.K, r0, u1, ws->up_stream));
+                HIPC(hipEventRecord(ws->ev[8], ws->up_stream));
+                HIPC(hipStreamWaitEvent(st, ws->ev[8], 0));
+                hipLaunchKernelGGL(k_aeff, dim3((r1 - r0) / 64, c.Kp / 16 + (c.Kp == 1)), dim3(256),
+                                   0, st, ws->aeff.as<double>(),
+                                   ws->annot.as<double>() + (size_t)r0 * c.K, (uint64_t)(u1 - r0), r0,
+                                   c.has_pq ? ws->pq.as<double>() : nullptr, (uint32_t)c.m,
+                                   (uint32_t)c.K, c.Kp);
+                HIPC(hipGetLastError());
+            }
+            BandArgs b = c.ba;
+            b.tile0 = (uint32_t)ts[c.seg_I[sg]];
+            b.ntiles = (uint32_t)ts[c.seg_I[sg + 1]] - b.tile0;
+            b.per_xcd = (b.ntiles + 7) / 8;
+            if (b.ntiles) HIPC(launch_band_form(c.form, b, st));
+            // rows this band completes (their row tiles are in this or earlier
+            // segments, and no later row block reaches their column block) are
+            // reduced right after it, on the band's stream
+            const uint32_t done_to =
+                sg + 1 < c.nseg
+                    ? std::min<uint32_t>((uint32_t)c.m, (uint32_t)j0v[c.seg_I[sg + 1]] * kTile)
+                    : (uint32_t)c.m;
+            const uint32_t prev_a = dl_a, prev_b = dl_b;
+            dl_a = dl_b = red_to;
+            if (sg + 1 == c.nseg) HIPC(hipEventRecord(ws->ev[4], st));
+            if (done_to > red_to) {
+                const uint64_t nred = (uint64_t)(done_to - red_to) * c.K;
+                hipLaunchKernelGGL(k_reduce, dim3((unsigned)((nred + 255) / 256)), dim3(256), 0, st,
+                                   ws->rowp.as<double>(), ws->colp.as<double>(),
+                                   ws->aeff.as<double>(), ws->j0.as<int>(), ws->tstart.as<int>(),
+                                   ws->host.plan.nb, red_to, done_to - red_to, (uint32_t)c.K, 0u,
+                                   (uint32_t)c.K, ws->l2.as<double>());
+                HIPC(hipGetLastError());
+                dl_b = done_to;
+                red_to = done_to;
+            }
+            HIPC(hipEventRecord(ws->red_ev[sg], st));
+            // the rows reduced after the previous segment's band go back while
+            // this band runs
+            if (prev_b > prev_a) {
+                HIPC(hipStreamWaitEvent(ws->copy_stream, ws->red_ev[sg - 1], 0));
+                HIPC(download_l2_rows(ws, c.l2_out, c.m, c.K, prev_a, prev_b, ws->copy_stream));
+            }
+        }
+        // the last reduced rows go back in l2_end, after the last reduce
+        c.seg_dl_a = dl_a;
+        c.seg_dl_b = dl_b;
+        HIPC(hipEventRecord(ws->ev[5], st));
+        HIPC(hipStreamWaitEvent(ws->copy_stream, ws->ev[5], 0));
+        return LDSC_OK;
+    }
     HIPC(hipEventRecord(ws->ev[4], st));
     return LDSC_OK;
 }
@@ -2082,18 +2249,21 @@ int l2_end(L2Call* call, char* err, uint64_t err_len) {
 
     // 4. deterministic reduction, in groups of annotations when the result
     //    goes back through the ring, so that its copy starts with the first
-    const uint32_t ngroups = stage_l2 ? 1u : (uint32_t)std::min<uint64_t>(K, Workspace::kRedGroups);
+    //    (a segmented band reduced its rows segment by segment already)
+    const bool seg = c.nseg > 1;
+    const uint32_t ngroups =
+        seg ? 0u : stage_l2 ? 1u : (uint32_t)std::min<uint64_t>(K, Workspace::kRedGroups);
     for (uint32_t g = 0; g < ngroups; ++g) {
         const uint32_t k0 = (uint32_t)(K * g / ngroups), k1 = (uint32_t)(K * (g + 1) / ngroups);
         const uint64_t nred = m * (k1 - k0);
         hipLaunchKernelGGL(k_reduce, dim3((unsigned)((nred + 255) / 256)), dim3(256), 0, st,
                            ws->rowp.as<double>(), ws->colp.as<double>(), ws->aeff.as<double>(),
-                           ws->j0.as<int>(), ws->tstart.as<int>(), plan.nb, (uint32_t)m,
+                           ws->j0.as<int>(), ws->tstart.as<int>(), plan.nb, 0u, (uint32_t)m,
                            (uint32_t)K, k0, k1 - k0, ws->l2.as<double>());
         HIPC(hipGetLastError());
         if (!stage_l2) HIPC(hipEventRecord(ws->red_ev[g], st));
     }
-    HIPC(hipEventRecord(ws->ev[5], st));
+    if (!seg) HIPC(hipEventRecord(ws->ev[5], st));
     if (early_maf) {  // copy it out while the band kernel runs
         HIPC(hipEventSynchronize(ws->ev[7]));
         std::memcpy(maf_out, ws->pinned_maf.p, m * 8);
@@ -2105,6 +2275,11 @@ int l2_end(L2Call* call, char* err, uint64_t err_len) {
         HIPC(hipMemcpyAsync(h_l2, ws->l2.p, (size_t)m * K * 8, hipMemcpyDeviceToHost, st));
         HIPC(hipStreamSynchronize(st));
         std::memcpy(l2_out, h_l2, (size_t)m * K * 8);
+    } else if (seg) {
+        // the rows of the last segment's reduce (queued after it on the copy stream)
+        if (c.seg_dl_b > c.seg_dl_a)
+            HIPC(download_l2_rows(ws, l2_out, m, K, c.seg_dl_a, c.seg_dl_b, ws->copy_stream));
+        HIPC(hipStreamSynchronize(st));
     } else {
         // each piece of the result is copied once the group of annotations
         // holding its last byte is reduced
@@ -2161,7 +2336,7 @@ int run_l2(const ldsc_panel* panel, const ldsc_snp* snps, uint64_t m, const ldsc
            ldsc_progress_cb cb, void* cb_ctx, Workspace* ws, char* err, uint64_t err_len) {
     L2Call c;
     int rc = l2_begin(panel, snps, m, cfg, annot, K, l2_out, maf_out, perf, cb, cb_ctx, ws, &c,
-                      err, err_len);
+                      err, err_len, /*segment_annot=*/true);
     if (rc || c.done) return rc;
     if ((rc = l2_band(&c, err, err_len))) return rc;
     return l2_end(&c, err, err_len);

@@ -115,9 +115,16 @@ int task_count(uint64_t items) {
     return (int)std::min<uint64_t>((uint64_t)pool().threads() + 1, items / 16384);
 }
 
+void parallel_tasks_side(int n, const std::function<void(int)>& f) { copy_pool().run(n, f); }
+
+int copy_task_count(size_t bytes, bool side) {
+    TaskPool& p = side ? copy_pool() : pool();
+    return (int)std::max<size_t>(1, std::min<size_t>((size_t)p.threads() + 1, bytes >> 18));
+}
+
 void parallel_copy(void* dst, const void* src, size_t bytes, bool side) {
     TaskPool& p = side ? copy_pool() : pool();
-    const int T = (int)std::max<size_t>(1, std::min<size_t>((size_t)p.threads() + 1, bytes >> 18));
+    const int T = copy_task_count(bytes, side);
     p.run(T, [&](int t) {
         // piece boundaries on 64-byte multiples
         const size_t a = (bytes * t / T) & ~(size_t)63;

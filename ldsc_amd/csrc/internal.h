@@ -35,6 +35,10 @@ int task_count(uint64_t items);
 // `side` uses a separate small pool, for copies that run while the calling
 // thread's own work keeps the main pool busy.
 void parallel_copy(void* dst, const void* src, size_t bytes, bool side = false);
+// n tasks on the side pool (the one parallel_copy(.., side = true) uses).
+void parallel_tasks_side(int n, const std::function<void(int)>& f);
+// Tasks worth running for a copy of `bytes` on the main or the side pool.
+int copy_task_count(size_t bytes, bool side);
 
 // get_block_lefts_by_chr (src/l2/window.rs:32-90).  Returns any_full_chr_window.
 bool block_lefts(const ldsc_snp* snps, uint64_t m, int mode, double value,

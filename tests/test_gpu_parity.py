@@ -172,6 +172,32 @@ def test_l2_large_annotation_matrix(oracle):
     assert again is dest and np.array_equal(got, again)
 
 
+@pytest.mark.parametrize("m,K", [(100_000, 48), (200_000, 96)])
+def test_segmented_annotation_upload_matches_one_piece(m, K):
+    """A large annotation goes up in row segments interleaved with the band
+    (two segments at 38 MB, four at 154 MB) on the one-call path; the
+    request server uploads it in one piece.  The tiles and every reduction
+    are the same, so the results are bit-identical."""
+    from ldsc_amd.serve import Server
+
+    n = 600
+    print(f"segmented annotation m={m} K={K}", flush=True)
+    p = _panel(n, m, seed=K, miss=0.01, nchr=2)
+    snps = L.make_snp_array(p.chr, p.cm, p.bp)
+    annot = np.random.default_rng(K).random((m, K))
+    annot[::97, 3] = np.nan
+    cfg = L.L2Config(mode=L.WindowMode.Kb(300.0), chunk_size=200)
+    with L.Panel(p.bed_bytes(), n, m) as panel:
+        got, maf, _ = L.compute_ldscore(panel, snps, cfg, annot=annot)
+        with Server(hbm_budget_bytes=16 << 30, max_inflight=1) as srv:
+            t = srv.submit(panel, snps, cfg, annot)
+            srv.wait(t)
+            ref, rmaf = t.result()
+    assert np.array_equal(got, ref)
+    assert np.array_equal(maf, rmaf)
+    assert np.all(np.isfinite(got))
+
+
 def test_l2_extract_per_chromosome(oracle):
     n, m = 400, 1200
     p = _panel(n, m, seed=31, nchr=3)
