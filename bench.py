@@ -608,7 +608,7 @@ def main() -> int:
     ap.add_argument("--qps", type=float, default=200.0)
     ap.add_argument("--qps-sweep", default="500,1000,2000",
                     help="further offered rates for the same request stream (comma list)")
-    ap.add_argument("--serve-requests", type=int, default=400)
+    ap.add_argument("--serve-requests", type=int, default=1000)
     ap.add_argument("--serve-inflight", type=int, default=4)
     ap.add_argument("--serve-window-us", type=int, default=0,
                     help="band batching window of the request server (0: one launch each)")
