@@ -121,24 +121,29 @@ def write_ldscore(path: str, table: BimTable, bed_idx: np.ndarray, l2: np.ndarra
 
     ``bed_idx[i]`` is the BIM row of LD-score row ``i`` of ``l2`` (shape
     m x K); ``rows`` lists the LD-score rows to write, in order."""
-    l2 = np.ascontiguousarray(l2, dtype=np.float64)
+    l2 = np.asarray(l2, dtype=np.float64)
     if l2.ndim == 1:
         l2 = l2[:, None]
     bed_idx = np.asarray(bed_idx, dtype=np.int64)
     if l2.shape[0] != bed_idx.size:
         raise ValueError(f"write_ldscore: {l2.shape[0]} LD-score rows for {bed_idx.size} SNPs")
-    rows = np.ascontiguousarray(rows, dtype=np.int64)
+    rows = np.asarray(rows, dtype=np.int64)
     if rows.size and (rows.min() < 0 or rows.max() >= bed_idx.size):
         raise ValueError("write_ldscore: row index out of range")
-    chr_ = np.ascontiguousarray(table.chr[bed_idx])
-    bp = np.ascontiguousarray(table.bp[bed_idx])
-    off = np.ascontiguousarray(table.name_off[bed_idx])
-    ln = np.ascontiguousarray(table.name_len[bed_idx])
+    # only the written rows are gathered (row-major), so a file of one
+    # chromosome costs its own rows, not the whole table
+    sub = bed_idx[rows]
+    vals = np.ascontiguousarray(l2[rows])
+    chr_ = np.ascontiguousarray(table.chr[sub])
+    bp = np.ascontiguousarray(table.bp[sub])
+    off = np.ascontiguousarray(table.name_off[sub])
+    ln = np.ascontiguousarray(table.name_len[sub])
+    local = np.arange(rows.size, dtype=np.int64)
     tail = "".join("\t" + c for c in col_names).encode()
     err = C.create_string_buffer(512)
-    rc = lib().ldsc_io_write_ldscore(os.fsencode(path), tail, bed_idx.size, _ptr(chr_),
+    rc = lib().ldsc_io_write_ldscore(os.fsencode(path), tail, rows.size, _ptr(chr_),
                                      _ptr(bp), C.c_char_p(table.data), len(table.data),
-                                     _ptr(off), _ptr(ln), _ptr(l2), l2.shape[1], _ptr(rows),
+                                     _ptr(off), _ptr(ln), _ptr(vals), l2.shape[1], _ptr(local),
                                      rows.size, int(threads), err, 512)
     if rc:
         raise OSError(err.value.decode(errors="replace"))
