@@ -172,21 +172,23 @@ def test_l2_large_annotation_matrix(oracle):
     assert again is dest and np.array_equal(got, again)
 
 
-@pytest.mark.parametrize("m,K", [(100_000, 48), (200_000, 96)])
-def test_segmented_annotation_upload_matches_one_piece(m, K):
+@pytest.mark.parametrize("m,K,f32,pq", [(100_000, 48, False, None), (200_000, 96, False, None),
+                                        (100_000, 48, True, None), (100_000, 48, False, 0.75)])
+def test_segmented_annotation_upload_matches_one_piece(m, K, f32, pq):
     """A large annotation goes up in row segments interleaved with the band
     (two segments at 38 MB, four at 154 MB) on the one-call path; the
     request server uploads it in one piece.  The tiles and every reduction
-    are the same, so the results are bit-identical."""
+    are the same, so the results are bit-identical, for the FP32 form and
+    with pq scaling too."""
     from ldsc_amd.serve import Server
 
     n = 600
-    print(f"segmented annotation m={m} K={K}", flush=True)
+    print(f"segmented annotation m={m} K={K} f32={f32} pq={pq}", flush=True)
     p = _panel(n, m, seed=K, miss=0.01, nchr=2)
     snps = L.make_snp_array(p.chr, p.cm, p.bp)
     annot = np.random.default_rng(K).random((m, K))
     annot[::97, 3] = np.nan
-    cfg = L.L2Config(mode=L.WindowMode.Kb(300.0), chunk_size=200)
+    cfg = L.L2Config(mode=L.WindowMode.Kb(300.0), chunk_size=200, use_f32=f32, pq_exp=pq)
     with L.Panel(p.bed_bytes(), n, m) as panel:
         got, maf, _ = L.compute_ldscore(panel, snps, cfg, annot=annot)
         with Server(hbm_budget_bytes=16 << 30, max_inflight=1) as srv:
