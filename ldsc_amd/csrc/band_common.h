@@ -246,26 +246,30 @@ __device__ __forceinline__ float pick_code(const float4& L, uint32_t code) {
 
 // The band's tiles: row block I pairs with column blocks J0(I) .. I, where
 // J0 is the block of the pair bound of the block's first SNP (the bounds do
-// not decrease).  Counts and starts first, then the list in parallel pieces.
+// not decrease).  plan_band finds the counts and starts; fill_band_tiles
+// writes the list (ntiles entries) in parallel pieces.
 inline void plan_band(const std::vector<uint64_t>& lo, uint64_t m, BandPlan& p) {
     p.m_pad = (uint32_t)round_up(m, kTile);
     p.nb = p.m_pad / kTile;
-    p.j0.assign(p.nb, 0);
-    p.tstart.assign(p.nb + 1, 0);
+    p.j0.resize(p.nb);
+    p.tstart.resize(p.nb + 1);
+    p.tstart[0] = 0;
     for (uint32_t I = 0; I < p.nb; ++I) {
         const int J0 = (int)(lo[(uint64_t)I * kTile] / kTile);
         p.j0[I] = J0;
         p.tstart[I + 1] = p.tstart[I] + ((int)I - J0 + 1);
     }
-    p.tiles.resize((size_t)p.tstart[p.nb]);
-    const int T = task_count((uint64_t)p.tiles.size());
+    p.ntiles = (uint32_t)p.tstart[p.nb];
+}
+
+inline void fill_band_tiles(const BandPlan& p, int2* tiles) {
+    const int T = task_count((uint64_t)p.ntiles);
     parallel_tasks(T, [&](int t) {
         const uint32_t a = (uint32_t)((uint64_t)p.nb * t / T), b = (uint32_t)((uint64_t)p.nb * (t + 1) / T);
         for (uint32_t I = a; I < b; ++I)
             for (int J = p.j0[I]; J <= (int)I; ++J)
-                p.tiles[(size_t)p.tstart[I] + (J - p.j0[I])] = make_int2((int)I, J);
+                tiles[(size_t)p.tstart[I] + (J - p.j0[I])] = make_int2((int)I, J);
     });
-    p.ntiles = (uint32_t)p.tiles.size();
 }
 
 // CountSketch (sketch.hip)

@@ -73,11 +73,10 @@ struct PinnedBuf {
     T* as() const { return static_cast<T*>(p); }
 };
 
-// Host-side band plan: tiles (I, J) of the 64x64 band, per block row I its
-// first column block j0[I] and its first tile tstart[I].
+// Host-side band plan of the 64x64 tiles (I, J): per block row I its first
+// column block j0[I] and its first tile tstart[I].
 struct BandPlan {
     uint32_t m_pad = 0, nb = 0, ntiles = 0;
-    std::vector<int2> tiles;
     std::vector<int> j0, tstart;
     uint64_t pairs_included = 0;
 };

@@ -1972,7 +1972,7 @@ int l2_begin(const ldsc_panel* panel, const ldsc_snp* snps, uint64_t m, const ld
         plan.pairs_included = 0;
         for (uint64_t v : part) plan.pairs_included += v;
     }
-    std::memcpy(hp + off_tiles, plan.tiles.data(), (size_t)plan.ntiles * 8);
+    fill_band_tiles(plan, reinterpret_cast<int2*>(hp + off_tiles));
     std::memcpy(hp + off_j0, plan.j0.data(), (size_t)plan.nb * 4);
     std::memcpy(hp + off_ts, plan.tstart.data(), (size_t)(plan.nb + 1) * 4);
     const double t_prep = now_s();

@@ -140,45 +140,115 @@ static inline double coord_of(const ldsc_snp& s, bool cm) {
     return cm ? s.cm : (double)s.bp / 1000.0;
 }
 
-// get_block_lefts_f64 (src/l2/window.rs:15-28) over i in [i0, i1) of the
-// chromosome run [start, end): first j with |c_i - c_j| <= W, advancing
-// while the distance is strictly greater.  A run whose coordinates are
-// non-decreasing makes "distance > W" a prefix property of j <= i, so the
-// scan may begin anywhere with a binary search for its first j and agrees
-// with the scan from the run's start; other runs must start at i0 = start.
-static void block_lefts_range(const ldsc_snp* snps, uint64_t start, uint64_t i0, uint64_t i1,
-                              uint64_t end, bool cm, double W, bool sorted, uint64_t* out) {
-    uint64_t j = start;
-    if (sorted && i0 > start) {
-        const double ci = coord_of(snps[i0], cm);
-        uint64_t lo = start, hi = i0;  // first j in [start, i0] with !(|ci - cj| > W)
-        while (lo < hi) {
-            const uint64_t mid = lo + (hi - lo) / 2;
-            if (std::fabs(ci - coord_of(snps[mid], cm)) > W) lo = mid + 1;
-            else hi = mid;
-        }
-        j = lo;
-    }
-    for (uint64_t i = i0; i < i1; ++i) {
-        const double ci = coord_of(snps[i], cm);
-        while (j < end && std::fabs(ci - coord_of(snps[j], cm)) > W) ++j;
-        out[i] = j;
-    }
-}
-
 struct Run {
     uint64_t start, end;
 };
 
+// get_block_lefts_f64 (src/l2/window.rs:15-28) over i in [i0, i1) of the
+// chromosome run [start, end) of the coordinates c: first j with
+// |c_i - c_j| <= W, advancing while the distance is strictly greater.
+static void block_lefts_scan(const double* c, uint64_t start, uint64_t i0, uint64_t i1,
+                             uint64_t end, double W, uint64_t* out) {
+    uint64_t j = start;
+    for (uint64_t i = i0; i < i1; ++i) {
+        const double ci = c[i];
+        while (j < end && std::fabs(ci - c[j]) > W) ++j;
+        out[i] = j;
+    }
+}
+
+// First j in [start, i] with !(|c_i - c_j| > W), by binary search: valid
+// where "distance > W" is a prefix property of j <= i.
+static inline uint64_t first_in_window(const double* c, uint64_t start, uint64_t i, double W) {
+    const double ci = c[i];
+    uint64_t lo = start, hi = i;
+    while (lo < hi) {
+        const uint64_t mid = lo + (hi - lo) / 2;
+        if (std::fabs(ci - c[mid]) > W) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// One step of the scan from j for SNP i, testing four j at once: the advance
+// is the length of the prefix of true tests.
+static inline uint64_t advance_window(const double* c, double ci, uint64_t j, uint64_t end,
+                                      double W) {
+    for (;;) {
+        if (j + 4 > end) {
+            while (j < end && std::fabs(ci - c[j]) > W) ++j;
+            return j;
+        }
+        const unsigned p0 = std::fabs(ci - c[j]) > W, p1 = std::fabs(ci - c[j + 1]) > W,
+                       p2 = std::fabs(ci - c[j + 2]) > W, p3 = std::fabs(ci - c[j + 3]) > W;
+        const unsigned q1 = p0 & p1, q2 = q1 & p2;
+        const unsigned adv = p0 + q1 + q2 + (q2 & p3);
+        j += adv;
+        if (adv < 4) return j;
+    }
+}
+
+// The same over a run whose coordinates never decrease (no NaN).  Then
+// "distance > W" is a prefix property of j <= i (rounding is monotone) and
+// false at j = i (W >= 0), so a scan may begin anywhere with a binary search
+// for its first j.  Four such scans over consecutive quarters of [i0, i1)
+// run interleaved, so that their dependent chains (each j waits for the one
+// before) overlap in the core.  The result is the single scan's.
+static void block_lefts_sorted(const double* c, uint64_t start, uint64_t i0, uint64_t i1,
+                               uint64_t end, double W, uint64_t* out) {
+    constexpr int S = 4;
+    uint64_t is[S + 1], js[S];
+    for (int q = 0; q <= S; ++q) is[q] = i0 + (i1 - i0) * q / S;
+    for (int q = 0; q < S; ++q) js[q] = first_in_window(c, start, is[q], W);
+    const uint64_t len = is[1] - is[0];  // the shortest quarter
+    for (uint64_t t = 0; t < len; ++t) {
+#pragma unroll
+        for (int q = 0; q < S; ++q) {
+            const uint64_t i = is[q] + t;
+            js[q] = advance_window(c, c[i], js[q], end, W);
+            out[i] = js[q];
+        }
+    }
+    for (int q = 0; q < S; ++q)
+        for (uint64_t i = is[q] + len; i < is[q + 1]; ++i) {
+            js[q] = advance_window(c, c[i], js[q], end, W);
+            out[i] = js[q];
+        }
+}
+
 // Contiguous runs of one chromosome code (window.rs:41-49) of m >= 1 SNPs,
 // found in parallel pieces: each task lists the run starts in its share.
-static std::vector<Run> chromosome_runs(const ldsc_snp* snps, uint64_t m) {
+// With `c`, the same pass stores each SNP's window coordinate there and sets
+// `unsorted` for every run holding a coordinate that is not >= the one
+// before it (a NaN counts).
+static std::vector<Run> chromosome_runs(const ldsc_snp* snps, uint64_t m, bool cm = false,
+                                        double* c = nullptr,
+                                        std::vector<uint8_t>* unsorted = nullptr) {
     const int T = task_count(m);
-    std::vector<std::vector<uint64_t>> starts(T);
+    std::vector<std::vector<uint64_t>> starts(T), drops(T);
     parallel_tasks(T, [&](int t) {
-        const uint64_t a = std::max<uint64_t>(1, m * t / T), b = m * (t + 1) / T;
-        for (uint64_t k = a; k < b; ++k)
-            if (snps[k].chr != snps[k - 1].chr) starts[t].push_back(k);
+        const uint64_t a = m * t / T, b = m * (t + 1) / T;
+        const uint64_t k0 = std::max<uint64_t>(1, a);
+        if (c) {
+            if (a == 0) c[0] = coord_of(snps[0], cm);
+            double prev = k0 - 1 >= a ? c[k0 - 1] : coord_of(snps[k0 - 1], cm);
+            bool dropped = false;  // one drop per run piece is enough
+            for (uint64_t k = k0; k < b; ++k) {
+                const double ck = coord_of(snps[k], cm);
+                c[k] = ck;
+                if (snps[k].chr != snps[k - 1].chr) {
+                    starts[t].push_back(k);
+                    dropped = false;
+                } else if (!(ck >= prev) && !dropped) {
+                    drops[t].push_back(k);
+                    dropped = true;
+                }
+                prev = ck;
+            }
+        } else {
+            for (uint64_t k = k0; k < b; ++k)
+                if (snps[k].chr != snps[k - 1].chr) starts[t].push_back(k);
+        }
     });
     std::vector<Run> runs;
     uint64_t s0 = 0;
@@ -188,21 +258,35 @@ static std::vector<Run> chromosome_runs(const ldsc_snp* snps, uint64_t m) {
             s0 = k;
         }
     runs.push_back({s0, m});
+    if (unsorted) {
+        unsorted->assign(runs.size(), 0);
+        for (const auto& v : drops)
+            for (uint64_t k : v) {
+                // the run holding k: the last with start <= k
+                const auto it = std::upper_bound(runs.begin(), runs.end(), k,
+                                                 [](uint64_t x, const Run& r) { return x < r.start; });
+                (*unsorted)[(size_t)(it - runs.begin()) - 1] = 1;
+            }
+    }
     return runs;
 }
 
-bool block_lefts(const ldsc_snp* snps, uint64_t m, int mode, double value,
-                 std::vector<uint64_t>& bl) {
+// Window coordinates of the calling thread's last call, kept with their
+// capacity so that a repeated call maps no fresh pages.
+static thread_local std::vector<double> tl_coord;
+
+static bool block_lefts_runs(const ldsc_snp* snps, uint64_t m, int mode, double value,
+                             std::vector<uint64_t>& bl, std::vector<Run>& runs) {
     bl.resize(m);
+    runs.clear();
     if (m == 0) return false;
-    const bool cm = mode == LDSC_WINDOW_CM;
-    const std::vector<Run> runs = chromosome_runs(snps, m);
-    const int T = task_count(m);
     bool any_full = false;
     if (mode == LDSC_WINDOW_SNP) {
+        runs = chromosome_runs(snps, m);
         const uint64_t half = (uint64_t)value;
         for (const Run& r : runs)
             if (half >= r.end - r.start - 1) any_full = true;
+        const int T = task_count(m);
         parallel_tasks(T, [&](int t) {
             const uint64_t a = m * t / T, b = m * (t + 1) / T;
             size_t r = 0;
@@ -214,34 +298,38 @@ bool block_lefts(const ldsc_snp* snps, uint64_t m, int mode, double value,
         });
         return any_full;
     }
-    // Each task scans its share of every run as if the run's coordinates
-    // were non-decreasing, and checks that they are (NaN counts as not);
-    // runs found otherwise are scanned again from their start afterwards.
-    std::vector<std::vector<size_t>> unsorted_by_task(T);
+    // one pass for the coordinates, the runs and which runs are unsorted;
+    // sorted runs are scanned in parallel pieces, unsorted ones each whole
+    std::vector<double>& c = tl_coord;
+    if (c.size() < m) c.resize(m);
+    std::vector<uint8_t> unsorted;
+    runs = chromosome_runs(snps, m, mode == LDSC_WINDOW_CM, c.data(), &unsorted);
+    const int T = task_count(m);
     parallel_tasks(T, [&](int t) {
         const uint64_t a = m * t / T, b = m * (t + 1) / T;
         for (size_t r = 0; r < runs.size(); ++r) {
             const Run& R = runs[r];
-            if (R.end <= a || R.start >= b) continue;
-            const uint64_t i0 = std::max(a, R.start), i1 = std::min(b, R.end);
-            for (uint64_t k = std::max(i0, R.start + 1); k < i1; ++k)
-                if (!(coord_of(snps[k], cm) >= coord_of(snps[k - 1], cm))) {
-                    unsorted_by_task[t].push_back(r);
-                    break;
-                }
-            block_lefts_range(snps, R.start, i0, i1, R.end, cm, value, true, bl.data());
+            if (unsorted[r] || R.end <= a || R.start >= b) continue;
+            block_lefts_sorted(c.data(), R.start, std::max(a, R.start), std::min(b, R.end), R.end,
+                               value, bl.data());
         }
     });
-    std::vector<uint8_t> unsorted(runs.size(), 0);
-    for (const auto& v : unsorted_by_task)
-        for (size_t r : v) unsorted[r] = 1;
-    for (size_t r = 0; r < runs.size(); ++r) {
-        const Run& R = runs[r];
-        if (unsorted[r])
-            block_lefts_range(snps, R.start, R.start, R.end, R.end, cm, value, false, bl.data());
+    std::vector<size_t> whole;
+    for (size_t r = 0; r < runs.size(); ++r)
+        if (unsorted[r]) whole.push_back(r);
+    parallel_tasks((int)whole.size(), [&](int t) {
+        const Run& R = runs[whole[t]];
+        block_lefts_scan(c.data(), R.start, R.start, R.end, R.end, value, bl.data());
+    });
+    for (const Run& R : runs)
         if (bl[R.end - 1] == R.start) any_full = true;
-    }
     return any_full;
+}
+
+bool block_lefts(const ldsc_snp* snps, uint64_t m, int mode, double value,
+                 std::vector<uint64_t>& bl) {
+    std::vector<Run> runs;
+    return block_lefts_runs(snps, m, mode, value, bl, runs);
 }
 
 int check_config(const ldsc_l2_config* cfg, char* err, uint64_t err_len) {
@@ -275,13 +363,13 @@ void pair_bounds(const ldsc_snp* snps, uint64_t m, const ldsc_l2_config* cfg,
     // lo starts as block_left and is rewritten in place chunk by chunk: each
     // chunk reads block_left only at its own start, before writing over it,
     // so chunks are independent tasks (owned by the task holding their start)
-    const bool full = block_lefts(snps, m, cfg->window_mode, cfg->window_value, lo);
+    std::vector<Run> runs;
+    const bool full = block_lefts_runs(snps, m, cfg->window_mode, cfg->window_value, lo, runs);
     if (any_full) *any_full = full;
     if (cfg->snp_level_masking || m == 0) return;
     const uint64_t c = cfg->chunk_size;
     // chunk grids: one per chromosome run, or one over all SNPs
-    const std::vector<Run> runs =
-        cfg->per_chromosome ? chromosome_runs(snps, m) : std::vector<Run>{{0, m}};
+    if (!cfg->per_chromosome) runs.assign(1, Run{0, m});
     const int T = task_count(m);
     parallel_tasks(T, [&](int t) {
         const uint64_t a = m * t / T, b = m * (t + 1) / T;

@@ -79,6 +79,39 @@ def test_block_left_large_split_scan(oracle):
                                   _a_left_rule(exp, m, chr_, 200, per_chr)), (mode, per_chr)
 
 
+@pytest.mark.parametrize("seed", [3, 4, 5])
+def test_block_left_split_scan_edges(oracle, seed):
+    """The split scan's edges against the restatement: hundreds of runs of
+    1-8 SNPs (shorter than the four-wide test), chromosome codes that come
+    back (a new run each time), gaps far wider than the window (the scan
+    jumps many SNPs at once), heavy ties at W = 0, an infinite coordinate
+    closing a run, and out-of-order positions scattered over the list."""
+    rng = np.random.default_rng(seed)
+    lens = list(rng.integers(1, 9, size=400)) + [60_000, 7, 90_000, 3, 80_000]
+    chr_ = np.concatenate([np.full(n, 1 + (k % 3), np.uint8) for k, n in enumerate(lens)])
+    m = chr_.size
+    gaps = np.where(rng.random(m) < 0.02, rng.integers(50_000, 400_000, size=m),
+                    rng.integers(0, 4, size=m) * rng.integers(0, 900, size=m))
+    bp = np.cumsum(gaps).astype(np.uint32)
+    cm = bp * 1e-6
+    cm_inf = cm.copy()
+    ends = np.cumsum(lens) - 1
+    cm_inf[ends[400]] = np.inf
+    bp_bad = bp.copy()
+    for k in rng.choice(np.arange(1, m), size=25, replace=False):
+        bp_bad[k] = bp_bad[k - 1] // 2
+    kinds = [L.WindowMode.Cm, L.WindowMode.Kb, L.WindowMode.Snp]
+    cases = [(1, 5.0, cm, bp), (1, 0.0, cm, bp), (0, 0.2, cm_inf, bp), (1, 30.0, cm, bp_bad),
+             (0, 0.0, cm, bp)]
+    for mode, w, c, b in cases:
+        snps = L.make_snp_array(chr_, c, b)
+        got, full = L.block_left(snps, kinds[mode](w))
+        exp, efull = oracle.block_left(chr_, c, b, mode, w)
+        assert np.array_equal(got, exp) and full == efull, (mode, w)
+        cfg = L.L2Config(mode=kinds[mode](w), chunk_size=200, per_chromosome=True)
+        assert np.array_equal(L.pair_bounds(snps, cfg), _a_left_rule(exp, m, chr_, 200, True))
+
+
 def _a_left_rule(bl, m, chr_, chunk, per_chr):
     lo = np.zeros(m, np.uint64)
     run0 = 0
