@@ -248,7 +248,6 @@ struct GzOut {
     std::string window;  // the last <= 32 KiB of text written so far
 };
 
-constexpr size_t kPiece = 256u << 10;
 constexpr size_t kDict = 32768;
 
 bool write_gz_header(GzOut& g) {
@@ -256,28 +255,43 @@ bool write_gz_header(GzOut& g) {
     return fwrite(h, 1, 10, g.f) == 10;
 }
 
-// Deflate `text` (continuing the stream in g); `last` closes it.
-bool gz_append(GzOut& g, const std::string& text, bool last, int threads, std::string& why) {
-    size_t n = text.size();
-    size_t np = std::max<size_t>(1, (n + kPiece - 1) / kPiece);
+// The kDict bytes of text that come before piece i of `pieces`: the end of
+// the pieces before it, then of the text already written (g.window).
+static std::string dict_before(const GzOut& g, const std::vector<std::string>& pieces, size_t i) {
+    std::string d(kDict, '\0');
+    size_t have = 0;  // bytes filled at the end of d
+    for (size_t k = i; k-- > 0 && have < kDict;) {
+        const std::string& p = pieces[k];
+        const size_t take = std::min(p.size(), kDict - have);
+        memcpy(&d[kDict - have - take], p.data() + p.size() - take, take);
+        have += take;
+    }
+    if (have < kDict) {
+        const size_t take = std::min(g.window.size(), kDict - have);
+        memcpy(&d[kDict - have - take], g.window.data() + g.window.size() - take, take);
+        have += take;
+    }
+    return d.substr(kDict - have);
+}
+
+// Deflate `pieces` (consecutive text, continuing the stream in g), each on
+// its own thread, primed with the 32 KiB of text before it and ended with a
+// sync flush, so that the outputs concatenate into one deflate stream;
+// `last` closes the stream after the final piece.
+bool gz_append(GzOut& g, const std::vector<std::string>& pieces, bool last, int threads,
+               std::string& why) {
+    const size_t np = pieces.size();
     std::vector<std::string> outs(np);
     std::vector<uLong> crcs(np);
     std::vector<int> ok(np, 1);
 #pragma omp parallel for schedule(dynamic, 1) num_threads(threads)
     for (size_t i = 0; i < np; ++i) {
-        size_t a = i * kPiece, b = std::min(n, a + kPiece);
+        const std::string& in = pieces[i];
+        if (in.size() > 0xFFFF0000u) { ok[i] = 0; continue; }  // one deflate call's uInt
         z_stream zs;
         memset(&zs, 0, sizeof(zs));
         if (deflateInit2(&zs, 1, Z_DEFLATED, -15, 8, Z_DEFAULT_STRATEGY) != Z_OK) { ok[i] = 0; continue; }
-        // prime with the 32 KiB before this piece (from this text, else the carried window)
-        std::string dict;
-        if (a >= kDict) {
-            dict.assign(text.data() + a - kDict, kDict);
-        } else {
-            size_t from_prev = std::min(g.window.size(), kDict - a);
-            dict.assign(g.window.data() + g.window.size() - from_prev, from_prev);
-            dict.append(text.data(), a);
-        }
+        const std::string dict = dict_before(g, pieces, i);
         if (!dict.empty() &&
             deflateSetDictionary(&zs, (const Bytef*)dict.data(), (uInt)dict.size()) != Z_OK) {
             ok[i] = 0;
@@ -285,9 +299,9 @@ bool gz_append(GzOut& g, const std::string& text, bool last, int threads, std::s
             continue;
         }
         std::string& o = outs[i];
-        o.resize(deflateBound(&zs, b - a) + 64);
-        zs.next_in = (Bytef*)(text.data() + a);
-        zs.avail_in = (uInt)(b - a);
+        o.resize(deflateBound(&zs, in.size()) + 64);
+        zs.next_in = (Bytef*)in.data();
+        zs.avail_in = (uInt)in.size();
         zs.next_out = (Bytef*)&o[0];
         zs.avail_out = (uInt)o.size();
         int flush = (last && i + 1 == np) ? Z_FINISH : Z_SYNC_FLUSH;
@@ -297,24 +311,18 @@ bool gz_append(GzOut& g, const std::string& text, bool last, int threads, std::s
             ok[i] = 0;
         o.resize(o.size() - zs.avail_out);
         deflateEnd(&zs);
-        crcs[i] = crc32(0L, (const Bytef*)(text.data() + a), (uInt)(b - a));
+        crcs[i] = crc32(0L, (const Bytef*)in.data(), (uInt)in.size());
     }
     for (size_t i = 0; i < np; ++i) {
         if (!ok[i]) { why = "deflate failed"; return false; }
-        size_t len = std::min(n, (i + 1) * kPiece) - std::min(n, i * kPiece);
-        g.crc = crc32_combine(g.crc, crcs[i], (z_off_t)len);
+        g.crc = crc32_combine(g.crc, crcs[i], (z_off_t)pieces[i].size());
+        g.total += pieces[i].size();
         if (!outs[i].empty() && fwrite(outs[i].data(), 1, outs[i].size(), g.f) != outs[i].size()) {
             why = "write failed";
             return false;
         }
     }
-    g.total += n;
-    if (n >= kDict) {
-        g.window.assign(text.data() + n - kDict, kDict);
-    } else {
-        g.window.append(text);
-        if (g.window.size() > kDict) g.window.erase(0, g.window.size() - kDict);
-    }
+    if (np) g.window = dict_before(g, pieces, np);
     return true;
 }
 
@@ -613,46 +621,45 @@ extern "C" int ldsc_io_write_ldscore(const char* path, const char* header_tail, 
     }
     std::string why;
     bool ok = write_gz_header(g);
-    std::string head = std::string("CHR\tSNP\tBP") + header_tail + "\n";
-    // rows per formatting block and per deflate batch
-    const uint64_t kBlock = 4096;
-    const uint64_t kBatch = kBlock * 256;
-    std::string text;
+    const std::vector<std::string> head{std::string("CHR\tSNP\tBP") + header_tail + "\n"};
+    // rows per formatting block (one deflate piece each) and per batch
+    const uint64_t kBlock = 8192;
+    const uint64_t kBatch = kBlock * 128;
     ok = ok && gz_append(g, head, n_rows == 0, T, why);
-    std::vector<std::string> blocks(kBatch / kBlock);
+    std::vector<std::string> blocks;
     for (uint64_t b0 = 0; ok && b0 < n_rows; b0 += kBatch) {
-        uint64_t b1 = std::min(n_rows, b0 + kBatch);
-        uint64_t nb = (b1 - b0 + kBlock - 1) / kBlock;
+        const uint64_t b1 = std::min(n_rows, b0 + kBatch);
+        const uint64_t nb = (b1 - b0 + kBlock - 1) / kBlock;
+        blocks.resize(nb);
 #pragma omp parallel for schedule(dynamic, 1) num_threads(T)
         for (uint64_t bi = 0; bi < nb; ++bi) {
             std::string& s = blocks[bi];
-            s.clear();
-            uint64_t r0 = b0 + bi * kBlock, r1 = std::min(b1, r0 + kBlock);
-            char num[400];
+            const uint64_t r0 = b0 + bi * kBlock, r1 = std::min(b1, r0 + kBlock);
+            size_t used = 0;
             for (uint64_t i = r0; i < r1; ++i) {
-                int64_t r = rows[i];
-                char* p = put_u64(num, chr[r]);
+                const int64_t r = rows[i];
+                // a row is at most 3 + 1 + name + 1 + 10 + K * (1 + 330) + 1 bytes
+                const size_t need = 16 + (size_t)name_len[r] + 16 + (size_t)K * 332;
+                if (s.size() - used < need) s.resize(std::max(2 * s.size(), used + need));
+                char* p = &s[used];
+                p = put_u64(p, chr[r]);
                 *p++ = '\t';
-                s.append(num, (size_t)(p - num));
-                s.append(names + name_off[r], name_len[r]);
-                p = num;
+                memcpy(p, names + name_off[r], name_len[r]);
+                p += name_len[r];
                 *p++ = '\t';
                 p = put_u64(p, bp[r]);
-                s.append(num, (size_t)(p - num));
                 for (uint64_t k = 0; k < K; ++k) {
-                    num[0] = '\t';
-                    size_t w = fmt3(l2[(uint64_t)r * K + k], num + 1);
-                    s.append(num, w + 1);
+                    *p++ = '\t';
+                    p += fmt3(l2[(uint64_t)r * K + k], p);
                 }
-                s.push_back('\n');
+                *p++ = '\n';
+                used = (size_t)(p - s.data());
             }
+            s.resize(used);
         }
-        size_t tot = 0;
-        for (uint64_t bi = 0; bi < nb; ++bi) tot += blocks[bi].size();
-        text.clear();
-        text.reserve(tot);
-        for (uint64_t bi = 0; bi < nb; ++bi) text.append(blocks[bi]);
-        ok = gz_append(g, text, b1 == n_rows, T, why);
+        ok = gz_append(g, blocks, b1 == n_rows, T, why);
+        if (ok && b1 < n_rows)
+            for (std::string& s : blocks) s.clear();
     }
     ok = ok && write_gz_trailer(g);
     if (fclose(g.f) != 0) ok = false;

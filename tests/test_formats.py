@@ -137,7 +137,8 @@ def _one_member(path):
     return text.decode()
 
 
-@pytest.mark.parametrize("m,K,threads", [(10, 1, 1), (300_000, 1, 8), (20_000, 3, 4)])
+@pytest.mark.parametrize("m,K,threads", [(10, 1, 1), (300_000, 1, 8), (20_000, 3, 4),
+                                         (1_600_000, 1, 8)])
 def test_write_ldscore_round_trip(tmp_path, m, K, threads):
     rng = np.random.default_rng(m + K)
     text = "".join(f"{1 + i * 3 // m}\trs{i}\t0\t{i * 10}\tA\tG\n" for i in range(m))
