@@ -35,7 +35,10 @@ enum {
     LDSC_ERR_OOM = 7,          /* device allocation failed / over budget     */
     LDSC_ERR_BUSY = 8,         /* serving queue is full                      */
     LDSC_ERR_NOT_FOUND = 9,    /* unknown request id                         */
-    LDSC_ERR_PENDING = 10      /* request not finished yet (poll)            */
+    LDSC_ERR_PENDING = 10,     /* request not finished yet (poll)            */
+    LDSC_ERR_CANCELLED = 11,   /* request cancelled (ldsc_l2_cancel) or the  */
+                               /* server destroyed before it ran             */
+    LDSC_ERR_TIMEOUT = 12      /* request queued longer than max_queue_ms    */
 };
 
 /* WindowMode (src/l2/window.rs:5-12). */
@@ -150,7 +153,18 @@ uint64_t ldsc_l2_panel_n_indiv(const ldsc_panel* panel);
  *   annot   m x K column-major annotation matrix, NULL = all-ones (K must be 1)
  *   l2_out  m x K column-major LD scores (caller-owned)
  *   maf_out m MAFs (caller-owned, nullable)
- *   perf    nullable; cb nullable (called once per reference-sized chunk) */
+ *   perf    nullable
+ *   cb      nullable: progress while the call runs (L2Progress,
+ *           src/l2/mod.rs:22-39; called per chunk by compute.rs:2391-2398).
+ *           The reference's chunks are chunk_size SNPs on the call's grid
+ *           (restarted at every chromosome run when per_chromosome).  With a
+ *           callback the band runs in up to 8 pieces of SNP rows; after each
+ *           piece that completes further chunks, cb receives the number of
+ *           chunks whose SNPs are complete (monotone), the end of the last of
+ *           them in snps_done and the wall time since the previous call.  A
+ *           last call with chunks_done == chunks_total and snps_done == m
+ *           comes once the results are back: at most 9 calls in all, on the
+ *           calling thread. */
 int ldsc_l2_compute(const ldsc_panel* panel, const ldsc_snp* snps, uint64_t m,
                     const ldsc_l2_config* cfg, const double* annot, uint64_t K,
                     double* l2_out, double* maf_out, ldsc_l2_perf* perf,
@@ -211,23 +225,35 @@ int ldsc_l2_sketch_buckets(const ldsc_panel* panel, const uint64_t* bed_idx, uin
 typedef struct ldsc_server ldsc_server;
 
 typedef struct {
-    uint64_t hbm_budget_bytes; /* admission budget for request scratch     */
+    uint64_t hbm_budget_bytes; /* device bytes the workers' workspaces may  */
+                               /* hold, buffers kept between requests       */
+                               /* included (see ldsc_server_stats)          */
     uint32_t max_inflight;     /* requests resident on the device at once   */
     uint32_t queue_capacity;   /* pending requests accepted before BUSY     */
     uint32_t batch_window_us;  /* 0: one band launch per request.  Else the */
                                /* band tiles of requests prepared together  */
                                /* share one launch; a request waits at most */
                                /* this long for others still being prepared */
-    uint32_t reserved;
+    uint32_t max_queue_ms;     /* 0: none.  Else a request still queued     */
+                               /* after this long fails with TIMEOUT        */
 } ldsc_server_config;
 
-/* Counters of a server since it was created. */
+/* Counters of a server since it was created.  The budget is charged with
+ * each worker's workspace: the device bytes it holds while idle, plus the
+ * running request's estimate (an upper bound of what it can add) while busy.
+ * hbm_charged is the current sum, hbm_charged_max its largest value; both
+ * stay within hbm_budget_bytes. */
 typedef struct {
-    uint64_t requests_done;
+    uint64_t requests_done;       /* finished, in error or cancelled too         */
     uint64_t band_launches;       /* band kernel launches, batched or not        */
     uint64_t batched_requests;    /* requests whose band shared a launch         */
     uint32_t max_batch;           /* most requests in one band launch            */
     uint32_t reserved;
+    uint64_t requests_cancelled;
+    uint64_t requests_timed_out;
+    uint64_t hbm_charged;
+    uint64_t hbm_charged_max;
+    uint64_t workspace_trims;     /* times a worker freed its buffers            */
 } ldsc_server_stats;
 
 int ldsc_server_create(const ldsc_server_config* cfg, ldsc_server** out, char* err,
@@ -252,6 +278,16 @@ int ldsc_l2_poll(ldsc_server* server, uint64_t request_id, double* latency_secs,
 /* Blocks until the request finishes; same return convention as poll. */
 int ldsc_l2_wait(ldsc_server* server, uint64_t request_id, double* latency_secs,
                  char* err, uint64_t err_len);
+
+/* Cancels a request.  A queued one is done at once; a running one stops at
+ * the next phase boundary of its computation (after tiling and planning,
+ * after the band contraction) once its device work has drained, or
+ * finishes normally when it is past the last boundary.  poll / wait then
+ * report LDSC_ERR_CANCELLED (or the request's own result); the contents of
+ * l2_out / maf_out of a cancelled request are unspecified.  Returns
+ * LDSC_ERR_NOT_FOUND for an unknown or already collected id.  Cancelling a
+ * finished request changes nothing. */
+int ldsc_l2_cancel(ldsc_server* server, uint64_t request_id, char* err, uint64_t err_len);
 
 #ifdef __cplusplus
 }

@@ -17,9 +17,12 @@ LIB_PATH = os.path.join(HERE, "libldsc_l2.so")
 LDSC_OK = 0
 ERR_NAMES = {
     1: "INVALID_ARG", 2: "BAD_BED", 3: "EMPTY", 4: "SKETCH_RANGE", 5: "ANNOT",
-    6: "DEVICE", 7: "OOM", 8: "BUSY", 9: "NOT_FOUND", 10: "PENDING",
+    6: "DEVICE", 7: "OOM", 8: "BUSY", 9: "NOT_FOUND", 10: "PENDING", 11: "CANCELLED",
+    12: "TIMEOUT",
 }
 ERR_PENDING = 10
+ERR_CANCELLED = 11
+ERR_TIMEOUT = 12
 WINDOW_CM, WINDOW_KB, WINDOW_SNP = 0, 1, 2
 PREC_EXACT, PREC_FP32, PREC_EXACT_I8 = 0, 1, 2
 PREC_INT8_FP64 = PREC_EXACT
@@ -28,7 +31,7 @@ EXPORTED = [
     "ldsc_l2_open_panel", "ldsc_l2_close_panel", "ldsc_l2_panel_n_indiv", "ldsc_l2_compute",
     "ldsc_l2_block_left", "ldsc_l2_pair_bounds", "ldsc_l2_decode", "ldsc_l2_countsketch_hash",
     "ldsc_l2_sketch_buckets", "ldsc_server_create", "ldsc_server_destroy", "ldsc_l2_submit",
-    "ldsc_l2_poll", "ldsc_l2_wait", "ldsc_server_stats_get",
+    "ldsc_l2_poll", "ldsc_l2_wait", "ldsc_server_stats_get", "ldsc_l2_cancel",
 ]
 
 
@@ -71,13 +74,15 @@ class Progress(C.Structure):
 class ServerConfig(C.Structure):
     _fields_ = [("hbm_budget_bytes", C.c_uint64), ("max_inflight", C.c_uint32),
                 ("queue_capacity", C.c_uint32), ("batch_window_us", C.c_uint32),
-                ("reserved", C.c_uint32)]
+                ("max_queue_ms", C.c_uint32)]
 
 
 class ServerStats(C.Structure):
     _fields_ = [("requests_done", C.c_uint64), ("band_launches", C.c_uint64),
                 ("batched_requests", C.c_uint64), ("max_batch", C.c_uint32),
-                ("reserved", C.c_uint32)]
+                ("reserved", C.c_uint32), ("requests_cancelled", C.c_uint64),
+                ("requests_timed_out", C.c_uint64), ("hbm_charged", C.c_uint64),
+                ("hbm_charged_max", C.c_uint64), ("workspace_trims", C.c_uint64)]
 
 
 PROGRESS_CB = C.CFUNCTYPE(None, C.POINTER(Progress), C.c_void_p)
@@ -113,6 +118,7 @@ def lib():
     L.ldsc_l2_poll.argtypes = [vp, u64, C.POINTER(dbl), cp, u64]
     L.ldsc_l2_wait.argtypes = [vp, u64, C.POINTER(dbl), cp, u64]
     L.ldsc_server_stats_get.argtypes = [vp, C.POINTER(ServerStats)]
+    L.ldsc_l2_cancel.argtypes = [vp, u64, cp, u64]
     for name in EXPORTED:
         getattr(L, name)  # every declared symbol must resolve
     _lib = L

@@ -87,7 +87,9 @@ struct HostScratch {
     std::vector<uint64_t> lo;
     BandPlan plan;
     std::vector<double> maf, aux;
+    std::vector<double> coord;  // window coordinates of the pair-bound scan
     std::vector<uint8_t> round32;
+    std::vector<uint64_t> chunk_end;  // ends of the reference's chunks (progress)
 };
 
 // One in-flight computation: its own stream, events and grow-only buffers.
@@ -113,9 +115,14 @@ struct Workspace {
     // each copied back as soon as it is done
     static constexpr uint32_t kRedGroups = 8;
     hipEvent_t red_ev[kRedGroups] = {};
+    // with a progress callback the band runs in up to kProgPieces launches
+    // of consecutive block rows, each followed by one of these events
+    static constexpr uint32_t kProgPieces = 8;
+    hipEvent_t prog_ev[kProgPieces] = {};
     HostScratch host;
-    size_t bytes() const;
+    size_t bytes() const;  // device bytes held by the buffers
     hipError_t init(int dev);
+    void trim();           // frees the device buffers (streams and events stay)
     void destroy();
 };
 
@@ -234,6 +241,12 @@ struct L2Call {
     uint32_t seg_I[kMaxSeg + 1] = {};
     uint32_t seg_dl_a = 0, seg_dl_b = 0;  // reduced rows l2_end brings back
     bool has_pq = false;
+    // Progress points of the band (callback given): once prog_ev[p] has
+    // fired, every pair of the SNPs below prog_snps[p] has been evaluated.
+    static constexpr uint32_t kMaxProg = 8;
+    uint32_t nprog = 0;
+    uint32_t prog_snps[kMaxProg] = {};
+    hipEvent_t prog_ev[kMaxProg] = {};
 };
 // Argument table of one batched band launch (device copy + pinned staging),
 // reused once the launch that read it has finished.
@@ -258,8 +271,13 @@ int band_batch_launch(L2Call* const* calls, uint32_t ncalls, BandBatchSlot* slot
                       char* err, uint64_t err_len);
 int l2_end(L2Call* call, char* err, uint64_t err_len);
 
-// Device bytes a call of this shape needs (admission control).
+// Tiles of the call's band: the pair bounds of its SNP list (into `lo`) and
+// the 64x64 tiles (I, J), lo[64 I] / 64 <= J <= I, they span.
+uint64_t band_tile_count(const ldsc_snp* snps, uint64_t m, const ldsc_l2_config* cfg,
+                         std::vector<uint64_t>& lo);
+// Device bytes a call of this shape allocates in a fresh workspace (upper
+// bound, for admission control); `tiles` from band_tile_count.
 size_t l2_workspace_bytes(const ldsc_panel* panel, uint64_t m, const ldsc_l2_config* cfg,
-                          uint64_t K);
+                          uint64_t K, uint64_t tiles, bool annotated);
 
 }  // namespace ldsc

@@ -119,13 +119,24 @@ hipError_t Workspace::init(int dev) {
         e = hipEventCreateWithFlags(&x, hipEventDisableTiming);
         if (e != hipSuccess) return e;
     }
+    for (auto& x : prog_ev) {
+        e = hipEventCreateWithFlags(&x, hipEventDisableTiming);
+        if (e != hipSuccess) return e;
+    }
     return hipSuccess;
 }
 
-void Workspace::destroy() {
+void Workspace::trim() {
     DevBuf* all[] = {&tiled, &stats, &bed_idx, &lo, &info, &aeff, &annot, &pq, &maf, &mafd,
                      &tiles, &j0, &tstart, &rowp, &colp, &l2, &sk_cols, &sk_norm, &scratch};
+    if (stream) (void)hipStreamSynchronize(stream);
+    if (copy_stream) (void)hipStreamSynchronize(copy_stream);
+    if (up_stream) (void)hipStreamSynchronize(up_stream);
     for (auto* b : all) b->release();
+}
+
+void Workspace::destroy() {
+    trim();
     pinned.release();
     pinned_meta.release();
     pinned_maf.release();
@@ -135,6 +146,8 @@ void Workspace::destroy() {
     for (auto& x : ring_ev)
         if (x) (void)hipEventDestroy(x);
     for (auto& x : red_ev)
+        if (x) (void)hipEventDestroy(x);
+    for (auto& x : prog_ev)
         if (x) (void)hipEventDestroy(x);
     if (stream) (void)hipStreamDestroy(stream);
     if (copy_stream) (void)hipStreamDestroy(copy_stream);
@@ -1542,32 +1555,37 @@ static double now_s() {
         .count();
 }
 
+uint64_t band_tile_count(const ldsc_snp* snps, uint64_t m, const ldsc_l2_config* cfg,
+                         std::vector<uint64_t>& lo) {
+    if (m == 0) return 0;
+    pair_bounds(snps, m, cfg, lo, nullptr);
+    uint64_t tiles = 0;
+    for (uint64_t I = 0; I * kTile < m; ++I) tiles += I - lo[I * kTile] / kTile + 1;
+    return tiles;
+}
+
 size_t l2_workspace_bytes(const ldsc_panel* panel, uint64_t m, const ldsc_l2_config* cfg,
-                          uint64_t K) {
+                          uint64_t K, uint64_t tiles, bool annotated) {
     if (m == 0) return 0;
     const uint64_t m_pad = round_up(m, kTile);
     const uint64_t n_pad = round_up(panel->n, kKBlock);
     const uint64_t nb = m_pad / kTile;
-    // band width of the call, from its own pair bounds would need the SNP
-    // list; bound it by the window in SNPs when the mode gives one, else by a
-    // generous default of 16 tiles per block row
-    uint64_t tiles = nb * 16;
-    if (cfg && cfg->window_mode == LDSC_WINDOW_SNP)
-        tiles = nb * (2 + (uint64_t)(cfg->window_value + cfg->chunk_size) / kTile);
-    size_t bytes = m_pad * n_pad / 4;                           // tiled SNP slice
-    bytes += m_pad * 16 * (1 + tile_parts(panel->n));                // stats + chunk slots
-    bytes += m_pad * (8 + 4 + 48 + 8);                          // idx, lo, info, maf
-    bytes += m_pad * aeff_stride((uint32_t)K) * 8 + m * K * 8;  // A_eff, l2
-    if (K > 1) bytes += m * K * 8;                              // uploaded annotation
-    bytes += tiles * (kTile * K * 8 * 2 + 8);                   // tile partials + list
+    size_t bytes = m_pad * n_pad / 4;                            // tiled SNP slice
+    bytes += m_pad * 16 * (1 + tile_parts(panel->n));            // stats + chunk slots
+    bytes += m * 8 + m_pad * (4 + 48 + 8 + 8);                   // idx, lo, info, kappa, maf
+    bytes += m_pad * aeff_stride((uint32_t)K) * 8 + m * K * 8;   // A_eff, l2
+    if (annotated) bytes += m * K * 8;                           // uploaded annotation
+    if (cfg && !std::isnan(cfg->pq_exp)) bytes += m * 8;         // pq
+    bytes += tiles * (kTile * K * 8 * 2 + 8) + (2 * nb + 1) * 4; // partials, tile list, plan
     const bool gather = cfg && (cfg->sketch_d || panel->d_keep32);
-    if (gather) bytes += m * round_up((panel->n + 3) / 4, 4);   // permuted rows
+    if (gather) bytes += m * round_up((panel->n + 3) / 4, 4);    // permuted rows
     if (cfg && cfg->sketch_d) {
         const uint64_t nblocks = round_up(cfg->sketch_d, 32) / 32;
-        bytes += m_pad * nblocks * 32 * 4 + m_pad * 8;  // sketch tiles, kappa
-        bytes += m_pad * 8 * ((nblocks + 3) / 4);        // norm shares
+        bytes += m_pad * nblocks * 32 * 4;                       // sketch tiles
+        bytes += m_pad * 8 * ((nblocks + 3) / 4) + m_pad * 4;    // norm shares, scales
     }
-    return bytes;
+    // DevBuf rounds each of its ~20 buffers up to at least 256 bytes
+    return bytes + 20 * 256;
 }
 
 // MAF of one SNP from exact integer stats (normalize.rs:71-81).
@@ -1931,7 +1949,7 @@ int l2_begin(const ldsc_panel* panel, const ldsc_snp* snps, uint64_t m, const ld
     //    buffer of their own while the device tiles
     std::vector<uint64_t>& lo = ws->host.lo;
     bool any_full = false;
-    pair_bounds(snps, m, cfg, lo, &any_full);
+    pair_bounds(snps, m, cfg, lo, &any_full, &ws->host.coord);
     c.any_full = any_full;
     if (!cfg->yes_really && any_full)
         printf("WARNING: LD window spans the entire chromosome. "
@@ -2023,8 +2041,10 @@ int l2_begin(const ldsc_panel* panel, const ldsc_snp* snps, uint64_t m, const ld
             const double cube = 8.0 * n2 + H, fourth = 16.0 * n2 + H;
             if (C == 0) continue;
             const double cf = (double)C, mean = (double)S / cf;
+            // mean.powi(4) of the reference: (x*x)*(x*x)
+            const double mean2 = mean * mean;
             const double m4 = fourth - 4.0 * mean * cube + 6.0 * mean * mean * (double)SS -
-                              3.0 * std::pow(mean, 4) * cf;
+                              3.0 * (mean2 * mean2) * cf;
             const double css = (double)SS - cf * mean * mean;
             kap[j] = css <= 0.0 ? 0.0 : m4 / (css * css);
         }
@@ -2109,6 +2129,34 @@ int l2_band(L2Call* call, char* err, uint64_t err_len) {
     L2Call& c = *call;
     Workspace* ws = c.ws;
     hipStream_t st = ws->stream;
+    c.nprog = 0;
+    const std::vector<int>& ts = ws->host.plan.tstart;
+    const std::vector<int>& j0v = ws->host.plan.j0;
+    const uint32_t nb = ws->host.plan.nb;
+    if (c.nseg <= 1 && c.cb && nb > 1) {
+        // With a progress callback the band runs as consecutive pieces of
+        // block rows, an event after each.  Once the pieces up to block row
+        // I are done, the SNPs below 64 j0[I] are complete: their row tiles
+        // ran, and no later block row reaches their column block.  The tiles
+        // and the per-tile arithmetic are those of one launch.
+        const uint32_t P = std::min<uint32_t>(Workspace::kProgPieces, nb);
+        for (uint32_t p = 0; p < P; ++p) {
+            const uint32_t I0 = (uint32_t)((uint64_t)nb * p / P);
+            const uint32_t I1 = (uint32_t)((uint64_t)nb * (p + 1) / P);
+            BandArgs b = c.ba;
+            b.tile0 = (uint32_t)ts[I0];
+            b.ntiles = (uint32_t)ts[I1] - b.tile0;
+            b.per_xcd = (b.ntiles + 7) / 8;
+            if (b.ntiles) HIPC(launch_band_form(c.form, b, st));
+            HIPC(hipEventRecord(ws->prog_ev[p], st));
+            c.prog_ev[p] = ws->prog_ev[p];
+            c.prog_snps[p] = I1 < nb ? std::min<uint32_t>((uint32_t)c.m, (uint32_t)j0v[I1] * kTile)
+                                     : (uint32_t)c.m;
+        }
+        c.nprog = P;
+        HIPC(hipEventRecord(ws->ev[4], st));
+        return LDSC_OK;
+    }
     if (c.nseg <= 1) {
         HIPC(launch_band_form(c.form, c.ba, st));
     } else {
@@ -2116,8 +2164,6 @@ int l2_band(L2Call* call, char* err, uint64_t err_len) {
         // the band of the segments before it runs, then its A_eff rows, then
         // the band of its row blocks, whose columns lie in this segment or
         // the ones before
-        const std::vector<int>& ts = ws->host.plan.tstart;
-        const std::vector<int>& j0v = ws->host.plan.j0;
         uint32_t red_to = 0;            // rows reduced so far
         uint32_t dl_a = 0, dl_b = 0;    // rows reduced after the last band, not yet back
         for (uint32_t sg = 0; sg < c.nseg; ++sg) {
@@ -2161,6 +2207,9 @@ int l2_band(L2Call* call, char* err, uint64_t err_len) {
                 red_to = done_to;
             }
             HIPC(hipEventRecord(ws->red_ev[sg], st));
+            c.prog_ev[sg] = ws->red_ev[sg];
+            c.prog_snps[sg] = done_to;
+            c.nprog = sg + 1;
             // the rows reduced after the previous segment's band go back while
             // this band runs
             if (prev_b > prev_a) {
@@ -2230,6 +2279,24 @@ int band_batch_launch(L2Call* const* calls, uint32_t ncalls, BandBatchSlot* slot
     return LDSC_OK;
 }
 
+// Ends of the reference's chunks over the call's SNPs: one grid over all of
+// them (compute.rs:1281-1283), or one per chromosome run when the call
+// stands for run()'s per-chromosome calls (mod.rs:478-553).  Their number is
+// L2Progress::chunks_total.
+static void chunk_ends(const L2Call& c, std::vector<uint64_t>& ends) {
+    ends.clear();
+    const uint64_t m = c.m, cs = c.cfg->chunk_size;
+    for (uint64_t r0 = 0; r0 < m;) {
+        uint64_t r1 = m;
+        if (c.cfg->per_chromosome) {
+            r1 = r0 + 1;
+            while (r1 < m && c.snps[r1].chr == c.snps[r0].chr) ++r1;
+        }
+        for (uint64_t s = r0; s < r1; s += cs) ends.push_back(std::min(s + cs, r1));
+        r0 = r1;
+    }
+}
+
 int l2_end(L2Call* call, char* err, uint64_t err_len) {
     L2Call& c = *call;
     Workspace* ws = c.ws;
@@ -2264,6 +2331,27 @@ int l2_end(L2Call* call, char* err, uint64_t err_len) {
         if (!stage_l2) HIPC(hipEventRecord(ws->red_ev[g], st));
     }
     if (!seg) HIPC(hipEventRecord(ws->ev[5], st));
+    // progress while the band runs: one call per progress point that
+    // completes a chunk not reported yet (the last call, with every chunk
+    // done, comes once the results are back)
+    std::vector<uint64_t>& ends = ws->host.chunk_end;
+    double t_cb = c.t_start;
+    if (c.cb) {
+        chunk_ends(c, ends);
+        uint64_t reported = 0;
+        for (uint32_t p = 0; p < c.nprog; ++p) {
+            HIPC(hipEventSynchronize(c.prog_ev[p]));
+            const uint64_t done =
+                (uint64_t)(std::upper_bound(ends.begin(), ends.end(), (uint64_t)c.prog_snps[p]) -
+                           ends.begin());
+            if (done <= reported || done >= ends.size()) continue;
+            const double t = now_s();
+            ldsc_l2_progress pr{done, ends.size(), ends[done - 1], m, (t - t_cb) * 1e3};
+            c.cb(&pr, c.cb_ctx);
+            reported = done;
+            t_cb = t;
+        }
+    }
     if (early_maf) {  // copy it out while the band kernel runs
         HIPC(hipEventSynchronize(ws->ev[7]));
         std::memcpy(maf_out, ws->pinned_maf.p, m * 8);
@@ -2323,8 +2411,7 @@ int l2_end(L2Call* call, char* err, uint64_t err_len) {
         perf->any_full_chr_window = c.any_full ? 1 : 0;
     }
     if (c.cb) {
-        const uint64_t chunks = (m + c.cfg->chunk_size - 1) / c.cfg->chunk_size;
-        ldsc_l2_progress pr{chunks, chunks, m, m, (now_s() - c.t_start) * 1e3};
+        ldsc_l2_progress pr{ends.size(), ends.size(), m, m, (now_s() - t_cb) * 1e3};
         c.cb(&pr, c.cb_ctx);
     }
     c.done = true;

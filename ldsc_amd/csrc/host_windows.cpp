@@ -271,12 +271,11 @@ static std::vector<Run> chromosome_runs(const ldsc_snp* snps, uint64_t m, bool c
     return runs;
 }
 
-// Window coordinates of the calling thread's last call, kept with their
-// capacity so that a repeated call maps no fresh pages.
-static thread_local std::vector<double> tl_coord;
-
+// `coord` (nullable): scratch for the window coordinates that the caller
+// keeps with its capacity, so that a repeated call maps no fresh pages.
 static bool block_lefts_runs(const ldsc_snp* snps, uint64_t m, int mode, double value,
-                             std::vector<uint64_t>& bl, std::vector<Run>& runs) {
+                             std::vector<uint64_t>& bl, std::vector<Run>& runs,
+                             std::vector<double>* coord) {
     bl.resize(m);
     runs.clear();
     if (m == 0) return false;
@@ -300,7 +299,8 @@ static bool block_lefts_runs(const ldsc_snp* snps, uint64_t m, int mode, double 
     }
     // one pass for the coordinates, the runs and which runs are unsorted;
     // sorted runs are scanned in parallel pieces, unsorted ones each whole
-    std::vector<double>& c = tl_coord;
+    std::vector<double> local;
+    std::vector<double>& c = coord ? *coord : local;
     if (c.size() < m) c.resize(m);
     std::vector<uint8_t> unsorted;
     runs = chromosome_runs(snps, m, mode == LDSC_WINDOW_CM, c.data(), &unsorted);
@@ -327,9 +327,9 @@ static bool block_lefts_runs(const ldsc_snp* snps, uint64_t m, int mode, double 
 }
 
 bool block_lefts(const ldsc_snp* snps, uint64_t m, int mode, double value,
-                 std::vector<uint64_t>& bl) {
+                 std::vector<uint64_t>& bl, std::vector<double>* coord) {
     std::vector<Run> runs;
-    return block_lefts_runs(snps, m, mode, value, bl, runs);
+    return block_lefts_runs(snps, m, mode, value, bl, runs, coord);
 }
 
 int check_config(const ldsc_l2_config* cfg, char* err, uint64_t err_len) {
@@ -359,12 +359,13 @@ int check_config(const ldsc_l2_config* cfg, char* err, uint64_t err_len) {
 // without masking pair (i<j) enters iff i >= a_left(chunk_start(j)).  Masking
 // zeroes i < bl[j]; a_left(chunk_start(j)) <= bl[j], so masked lo[j] = bl[j].
 void pair_bounds(const ldsc_snp* snps, uint64_t m, const ldsc_l2_config* cfg,
-                 std::vector<uint64_t>& lo, bool* any_full) {
+                 std::vector<uint64_t>& lo, bool* any_full, std::vector<double>* coord) {
     // lo starts as block_left and is rewritten in place chunk by chunk: each
     // chunk reads block_left only at its own start, before writing over it,
     // so chunks are independent tasks (owned by the task holding their start)
     std::vector<Run> runs;
-    const bool full = block_lefts_runs(snps, m, cfg->window_mode, cfg->window_value, lo, runs);
+    const bool full =
+        block_lefts_runs(snps, m, cfg->window_mode, cfg->window_value, lo, runs, coord);
     if (any_full) *any_full = full;
     if (cfg->snp_level_masking || m == 0) return;
     const uint64_t c = cfg->chunk_size;

@@ -41,8 +41,10 @@ void parallel_tasks_side(int n, const std::function<void(int)>& f);
 int copy_task_count(size_t bytes, bool side);
 
 // get_block_lefts_by_chr (src/l2/window.rs:32-90).  Returns any_full_chr_window.
+// `coord` (nullable) is scratch for the window coordinates, kept by the
+// caller with its capacity from call to call.
 bool block_lefts(const ldsc_snp* snps, uint64_t m, int mode, double value,
-                 std::vector<uint64_t>& bl);
+                 std::vector<uint64_t>& bl, std::vector<double>* coord = nullptr);
 
 // Validates the window part of a config; returns LDSC_OK or an error code.
 int check_config(const ldsc_l2_config* cfg, char* err, uint64_t err_len);
@@ -52,7 +54,8 @@ int check_config(const ldsc_l2_config* cfg, char* err, uint64_t err_len);
 // cfg->per_chromosome.  Also reports whether any chromosome run is covered by
 // one window.
 void pair_bounds(const ldsc_snp* snps, uint64_t m, const ldsc_l2_config* cfg,
-                 std::vector<uint64_t>& lo, bool* any_full);
+                 std::vector<uint64_t>& lo, bool* any_full,
+                 std::vector<double>* coord = nullptr);
 
 // Gather map in lane-major layout: map_t[e * n16 + q] is the FAM row of
 // output member 16q + e (-1 past the last member), so that the 16 loads a

@@ -6,12 +6,13 @@
 // first failing line over all segments is the error reported, as a
 // sequential reader would report it.
 //
-// ldscore: rows are formatted in parallel blocks into one text buffer per
-// batch of rows, and the buffer is deflated in parallel 1 MiB pieces.
-// Each piece is a raw deflate stream primed with the 32 KiB of text before
-// it; all but the last end on a byte-aligned sync flush, so their
-// concatenation is one valid deflate stream and the file is a single gzip
-// member (readable by any gzip decoder, including single-member ones).
+// ldscore: the rows of a batch are formatted in blocks of 8192 rows, each
+// into its own text buffer, and each block is deflated as one piece of the
+// stream, on a thread of its own.  A piece is a raw
+// deflate stream primed with the 32 KiB of text before it (the end of the
+// previous block); all but the last end on a byte-aligned sync flush, so
+// their concatenation is one valid deflate stream and the file is a single
+// gzip member (readable by any gzip decoder, including single-member ones).
 #include "../../include/ldsc_io.h"
 
 #include <locale.h>

@@ -6,12 +6,26 @@
 //
 // Requests enter a bounded queue.  `max_inflight` worker threads each own a
 // device workspace and HIP stream; a worker takes the oldest queued request
-// whose device-memory estimate fits the remaining HBM budget (first fit in
-// arrival order; once the oldest request has waited longer than
-// kStarveSecs, nothing may overtake it), runs it on its own stream and
-// returns its budget.  Requests that arrive while others run are admitted as
-// soon as a slot and memory free up, and their kernels overlap on the device
-// with those already running.
+// that fits the HBM budget (first fit in arrival order; once the oldest
+// request has waited longer than kStarveSecs, nothing may overtake it) and
+// runs it on its own stream.  Requests that arrive while others run are
+// admitted as soon as a slot and memory free up, and their kernels overlap on
+// the device with those already running.
+//
+// HBM accounting.  A workspace keeps its buffers from request to request
+// (grow-only), so the budget is charged with what each worker holds: while
+// idle, its workspace's device bytes; while running, those plus the
+// request's estimate (l2_workspace_bytes over the request's own band, an
+// upper bound of what the request can add).  After a request the charge
+// falls back to the bytes the workspace then holds.  A worker frees its
+// buffers when that would exceed the budget, and an idle worker frees them
+// when a queued request fits nowhere, so retained buffers never keep a
+// request out for good.
+//
+// Cancel and deadline.  A queued request is cancelled at once; a running one
+// stops at the next phase boundary (after the tiling and planning, after the
+// band), once its device work has drained.  With max_queue_ms a request that
+// waits longer than that in the queue fails with LDSC_ERR_TIMEOUT.
 //
 // Band batching (batch_window_us > 0): a worker prepares its request
 // (tiling, planning, uploads: l2_begin) on its own stream and then hands the
@@ -28,6 +42,7 @@
 // every reduction runs in a fixed order.
 #include <algorithm>
 #include <chrono>
+#include <atomic>
 #include <condition_variable>
 #include <cstring>
 #include <deque>
@@ -57,11 +72,20 @@ struct Request {
     uint64_t K = 1;
     double* l2_out = nullptr;
     double* maf_out = nullptr;
-    size_t bytes = 0;
+    size_t bytes = 0;  // device bytes the request may add to a workspace
     double t_arrival = 0, t_done = 0;
     int state = 0;  // 0 queued, 1 running, 2 done
     int rc = LDSC_OK;
     std::string err;
+    std::atomic<bool> cancel{false};
+};
+
+// A worker thread's device state, under ldsc_server::mu.
+struct Worker {
+    ldsc::Workspace* ws = nullptr;
+    int device = -1;
+    size_t charge = 0;  // bytes of the budget charged to this worker
+    bool trim = false;  // free the workspace's buffers before the next take
 };
 
 }  // namespace
@@ -211,14 +235,16 @@ struct ldsc_server {
     std::deque<Request*> queue;
     std::unordered_map<uint64_t, std::unique_ptr<Request>> reqs;
     uint64_t next_id = 1;
-    size_t in_use = 0;
+    size_t in_use = 0, in_use_max = 0;  // sum of the workers' charges
     bool stop = false;
     std::vector<std::thread> workers;
     BandBatcher batcher;
-    uint64_t done_count = 0;
+    uint64_t done_count = 0, cancelled = 0, timed_out = 0, trims = 0;
 
     void worker_main();
-    Request* take_locked();
+    Request* take_locked(Worker& w);
+    void expire_locked(double now);
+    void finish_locked(Request* r, int rc, const char* msg);
     int run_request(Request* r, ldsc::Workspace* ws, char* err, uint64_t err_len);
 };
 
@@ -233,13 +259,25 @@ int ldsc_server::run_request(Request* r, ldsc::Workspace* ws, char* err, uint64_
     int rc = ldsc::l2_begin(r->panel, r->snps.data(), r->snps.size(), &r->cfg,
                             r->annot.empty() ? nullptr : r->annot.data(), r->K, r->l2_out,
                             r->maf_out, nullptr, nullptr, nullptr, ws, &call, err, err_len);
-    const bool batched = batch && rc == LDSC_OK && !call.done && call.form != ldsc::kBandSketch;
+    // a cancel seen at a phase boundary: the work already queued on the
+    // workspace's streams drains before the workspace serves another request
+    auto cancelled = [&]() {
+        (void)hipStreamSynchronize(ws->stream);
+        (void)hipStreamSynchronize(ws->copy_stream);
+        (void)hipStreamSynchronize(ws->up_stream);
+        return ldsc::fail(err, err_len, LDSC_ERR_CANCELLED, "request %llu cancelled",
+                          (unsigned long long)r->id);
+    };
+    const bool stop_now = rc == LDSC_OK && !call.done && r->cancel.load();
+    const bool batched =
+        batch && rc == LDSC_OK && !call.done && !stop_now && call.form != ldsc::kBandSketch;
     if (batch && !batched) {
         std::lock_guard<std::mutex> lk(batcher.mu);
         --batcher.preparing;
         batcher.cv.notify_all();
     }
     if (rc != LDSC_OK || call.done) return rc;
+    if (stop_now) return cancelled();
     if (batched) {
         rc = batcher.run(&call, err, err_len);
     } else {
@@ -251,61 +289,139 @@ int ldsc_server::run_request(Request* r, ldsc::Workspace* ws, char* err, uint64_
         }
     }
     if (rc != LDSC_OK) return rc;
+    if (r->cancel.load()) {
+        // the band may run on a batch stream: it recorded ev[4] after it
+        (void)hipEventSynchronize(ws->ev[4]);
+        return cancelled();
+    }
     return ldsc::l2_end(&call, err, err_len);
 }
 
-Request* ldsc_server::take_locked() {
+void ldsc_server::finish_locked(Request* r, int rc, const char* msg) {
+    r->state = 2;
+    r->rc = rc;
+    r->err = msg;
+    r->t_done = now_s();
+}
+
+// Queued requests past the queue deadline fail with LDSC_ERR_TIMEOUT.
+void ldsc_server::expire_locked(double now) {
+    if (cfg.max_queue_ms == 0) return;
+    const double limit = cfg.max_queue_ms * 1e-3;
+    bool any = false;
+    for (auto it = queue.begin(); it != queue.end();) {
+        if (now - (*it)->t_arrival > limit) {
+            finish_locked(*it, LDSC_ERR_TIMEOUT, "request waited longer than max_queue_ms");
+            ++timed_out;
+            it = queue.erase(it);
+            any = true;
+        } else {
+            ++it;
+        }
+    }
+    if (any) cv_done.notify_all();
+}
+
+// The oldest queued request that fits the budget next to what the other
+// workers hold, with this worker's own buffers counted as they stand.
+Request* ldsc_server::take_locked(Worker& w) {
+    expire_locked(now_s());
     if (queue.empty()) return nullptr;
-    const size_t room = cfg.hbm_budget_bytes > in_use ? cfg.hbm_budget_bytes - in_use : 0;
+    const size_t others = in_use - w.charge;
+    const size_t room = cfg.hbm_budget_bytes > others ? cfg.hbm_budget_bytes - others : 0;
     const bool head_starving = now_s() - queue.front()->t_arrival > kStarveSecs;
     for (auto it = queue.begin(); it != queue.end(); ++it) {
-        if ((*it)->bytes <= room) {
+        const size_t held = (*it)->panel->device == w.device ? w.charge : 0;
+        if (held + (*it)->bytes <= room) {
             Request* r = *it;
             queue.erase(it);
             return r;
         }
         if (head_starving) break;
     }
+    // nothing fits: buffers this idle worker keeps are given back
+    if (w.charge > 0) w.trim = true;
     return nullptr;
 }
 
 void ldsc_server::worker_main() {
-    ldsc::Workspace* ws = nullptr;
-    int ws_device = -1;
+    Worker w;
+    std::unique_lock<std::mutex> lk(mu);
     for (;;) {
         Request* r = nullptr;
-        {
-            std::unique_lock<std::mutex> lk(mu);
-            cv_work.wait(lk, [&] { return stop || (r = take_locked()) != nullptr; });
-            if (!r) break;  // stop
-            in_use += r->bytes;
-            r->state = 1;
+        cv_work.wait(lk, [&] {
+            if (stop) return true;
+            r = take_locked(w);
+            return r != nullptr || w.trim;
+        });
+        if (!r && w.trim) {
+            w.trim = false;
+            lk.unlock();
+            if (w.ws) w.ws->trim();
+            lk.lock();
+            in_use -= w.charge;
+            w.charge = w.ws ? w.ws->bytes() : 0;
+            in_use += w.charge;
+            ++trims;
+            cv_work.notify_all();
+            continue;
         }
-        if (!ws || ws_device != r->panel->device) {
-            if (ws) ldsc::release_workspace(ws);
-            ws = ldsc::acquire_workspace(r->panel->device);
-            ws_device = r->panel->device;
+        if (!r) break;  // stop
+        // charge: the buffers held (on this device) plus what the request adds
+        const bool same = w.ws && w.device == r->panel->device;
+        in_use -= w.charge;
+        w.charge = (same ? w.charge : 0) + r->bytes;
+        in_use += w.charge;
+        in_use_max = std::max(in_use_max, in_use);
+        r->state = 1;
+        lk.unlock();
+        if (w.ws && !same) {
+            w.ws->destroy();
+            delete w.ws;
+            w.ws = nullptr;
+        }
+        if (!w.ws) {
+            w.ws = new ldsc::Workspace();
+            if (w.ws->init(r->panel->device) != hipSuccess) {
+                w.ws->destroy();
+                delete w.ws;
+                w.ws = nullptr;
+            }
+            w.device = r->panel->device;
         }
         char err[512] = {0};
         int rc;
-        if (!ws) {
+        if (!w.ws) {
             rc = ldsc::fail(err, sizeof(err), LDSC_ERR_DEVICE, "could not create a device workspace");
         } else {
-            rc = run_request(r, ws, err, sizeof(err));
+            rc = run_request(r, w.ws, err, sizeof(err));
         }
-        {
-            std::lock_guard<std::mutex> lk(mu);
-            ++done_count;
-            in_use -= r->bytes;
-            r->rc = rc;
-            r->err = err;
-            r->t_done = now_s();
-            r->state = 2;
+        // the charge falls to what the workspace now holds; over the budget
+        // (next to the other workers) its buffers are freed
+        size_t held = w.ws ? w.ws->bytes() : 0;
+        lk.lock();
+        if (in_use - w.charge + held > cfg.hbm_budget_bytes) {
+            lk.unlock();
+            w.ws->trim();
+            held = w.ws->bytes();
+            lk.lock();
+            ++trims;
         }
+        in_use -= w.charge;
+        w.charge = held;
+        in_use += w.charge;
+        ++done_count;
+        if (rc == LDSC_ERR_CANCELLED) ++cancelled;
+        finish_locked(r, rc, err);
         cv_done.notify_all();
         cv_work.notify_all();
     }
-    if (ws) ldsc::release_workspace(ws);
+    in_use -= w.charge;
+    lk.unlock();
+    if (w.ws) {
+        w.ws->destroy();
+        delete w.ws;
+    }
 }
 
 extern "C" int ldsc_server_create(const ldsc_server_config* cfg, ldsc_server** out, char* err,
@@ -317,6 +433,8 @@ extern "C" int ldsc_server_create(const ldsc_server_config* cfg, ldsc_server** o
         return ldsc::fail(err, err_len, LDSC_ERR_INVALID_ARG, "queue_capacity must be >= 1");
     if (cfg->batch_window_us > 1000000)
         return ldsc::fail(err, err_len, LDSC_ERR_INVALID_ARG, "batch_window_us must be <= 1000000");
+    if (cfg->hbm_budget_bytes == 0)
+        return ldsc::fail(err, err_len, LDSC_ERR_INVALID_ARG, "hbm_budget_bytes must be > 0");
     auto* s = new ldsc_server();
     s->cfg = *cfg;
     s->batcher.window = cfg->batch_window_us * 1e-6;
@@ -332,11 +450,8 @@ extern "C" int ldsc_server_destroy(ldsc_server* s) {
         std::lock_guard<std::mutex> lk(s->mu);
         s->stop = true;
         // queued (not yet running) requests are abandoned
-        for (Request* r : s->queue) {
-            r->state = 2;
-            r->rc = LDSC_ERR_INVALID_ARG;
-            r->err = "server destroyed before the request ran";
-        }
+        for (Request* r : s->queue)
+            s->finish_locked(r, LDSC_ERR_CANCELLED, "server destroyed before the request ran");
         s->queue.clear();
     }
     s->cv_work.notify_all();
@@ -352,6 +467,11 @@ extern "C" int ldsc_server_stats_get(ldsc_server* s, ldsc_server_stats* out) {
     {
         std::lock_guard<std::mutex> lk(s->mu);
         out->requests_done = s->done_count;
+        out->requests_cancelled = s->cancelled;
+        out->requests_timed_out = s->timed_out;
+        out->hbm_charged = s->in_use;
+        out->hbm_charged_max = s->in_use_max;
+        out->workspace_trims = s->trims;
     }
     std::lock_guard<std::mutex> lk(s->batcher.mu);
     out->band_launches = s->batcher.launches;
@@ -380,7 +500,12 @@ extern "C" int ldsc_l2_submit(ldsc_server* s, const ldsc_panel* panel, const lds
     r->K = K;
     r->l2_out = l2_out;
     r->maf_out = maf_out;
-    r->bytes = ldsc::l2_workspace_bytes(panel, m, cfg, K);
+    // the estimate uses the request's own band: its pair bounds and tiles
+    {
+        std::vector<uint64_t> lo;
+        const uint64_t tiles = ldsc::band_tile_count(snps, m, cfg, lo);
+        r->bytes = ldsc::l2_workspace_bytes(panel, m, cfg, K, tiles, annot != nullptr);
+    }
     r->t_arrival = now_s();
     if (r->bytes > s->cfg.hbm_budget_bytes)
         return ldsc::fail(err, err_len, LDSC_ERR_OOM,
@@ -402,8 +527,8 @@ extern "C" int ldsc_l2_submit(ldsc_server* s, const ldsc_panel* panel, const lds
     return LDSC_OK;
 }
 
-static int finish_locked(ldsc_server* s, uint64_t id, double* latency, char* err,
-                         uint64_t err_len) {
+static int collect_locked(ldsc_server* s, uint64_t id, double* latency, char* err,
+                          uint64_t err_len) {
     auto it = s->reqs.find(id);
     Request* r = it->second.get();
     if (latency) *latency = r->t_done - r->t_arrival;
@@ -421,8 +546,9 @@ extern "C" int ldsc_l2_poll(ldsc_server* s, uint64_t id, double* latency, char* 
     if (it == s->reqs.end())
         return ldsc::fail(err, err_len, LDSC_ERR_NOT_FOUND, "unknown request %llu",
                           (unsigned long long)id);
+    s->expire_locked(now_s());
     if (it->second->state != 2) return LDSC_ERR_PENDING;
-    return finish_locked(s, id, latency, err, err_len);
+    return collect_locked(s, id, latency, err, err_len);
 }
 
 extern "C" int ldsc_l2_wait(ldsc_server* s, uint64_t id, double* latency, char* err,
@@ -434,6 +560,40 @@ extern "C" int ldsc_l2_wait(ldsc_server* s, uint64_t id, double* latency, char* 
         return ldsc::fail(err, err_len, LDSC_ERR_NOT_FOUND, "unknown request %llu",
                           (unsigned long long)id);
     Request* r = it->second.get();
-    s->cv_done.wait(lk, [&] { return r->state == 2; });
-    return finish_locked(s, id, latency, err, err_len);
+    // a queued request's deadline is enforced here too, whether or not a
+    // worker looks at the queue meanwhile
+    while (r->state != 2) {
+        if (s->cfg.max_queue_ms && r->state == 0) {
+            const double due = r->t_arrival + s->cfg.max_queue_ms * 1e-3;
+            s->cv_done.wait_for(lk, std::chrono::duration<double>(std::max(0.0, due - now_s()) + 1e-4));
+            s->expire_locked(now_s());
+        } else {
+            s->cv_done.wait(lk);
+        }
+    }
+    return collect_locked(s, id, latency, err, err_len);
+}
+
+extern "C" int ldsc_l2_cancel(ldsc_server* s, uint64_t id, char* err, uint64_t err_len) {
+    if (!s) return ldsc::fail(err, err_len, LDSC_ERR_INVALID_ARG, "NULL server");
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->reqs.find(id);
+    if (it == s->reqs.end())
+        return ldsc::fail(err, err_len, LDSC_ERR_NOT_FOUND, "unknown request %llu",
+                          (unsigned long long)id);
+    Request* r = it->second.get();
+    if (r->state == 0) {
+        for (auto q = s->queue.begin(); q != s->queue.end(); ++q)
+            if (*q == r) {
+                s->queue.erase(q);
+                break;
+            }
+        s->finish_locked(r, LDSC_ERR_CANCELLED, "request cancelled while queued");
+        ++s->cancelled;
+        s->cv_done.notify_all();
+        s->cv_work.notify_all();
+    } else if (r->state == 1) {
+        r->cancel.store(true);
+    }
+    return LDSC_OK;
 }

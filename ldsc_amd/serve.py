@@ -36,10 +36,10 @@ class Ticket:
 
 class Server:
     def __init__(self, hbm_budget_bytes: int = 64 << 30, max_inflight: int = 4,
-                 queue_capacity: int = 1024, batch_window_us: int = 0):
+                 queue_capacity: int = 1024, batch_window_us: int = 0, max_queue_ms: int = 0):
         L = N.lib()
         cfg = N.ServerConfig(int(hbm_budget_bytes), int(max_inflight), int(queue_capacity),
-                             int(batch_window_us), 0)
+                             int(batch_window_us), int(max_queue_ms))
         h = C.c_void_p()
         err = N.errbuf()
         N.check(L.ldsc_server_create(C.byref(cfg), C.byref(h), err, 512), err)
@@ -82,13 +82,24 @@ class Server:
         N.check(N.lib().ldsc_l2_wait(self._h, t.id, C.byref(lat), err, 512), err)
         return lat.value
 
+    def cancel(self, t: Ticket) -> None:
+        """Cancel a queued or running request (ldsc_l2_cancel); its poll / wait
+        then raises LdscError with code ERR_CANCELLED unless it had finished."""
+        err = N.errbuf()
+        N.check(N.lib().ldsc_l2_cancel(self._h, t.id, err, 512), err)
+
     def stats(self) -> dict:
         """Counters since creation: requests done, band launches, requests
-        whose band shared a launch, and the largest launch (in requests)."""
+        whose band shared a launch, the largest launch (in requests),
+        cancelled and timed-out requests, the HBM charged to the workers
+        (now and at most) and how often a worker freed its buffers."""
         st = N.ServerStats()
         N.check(N.lib().ldsc_server_stats_get(self._h, C.byref(st)), N.errbuf())
         return {"requests_done": st.requests_done, "band_launches": st.band_launches,
-                "batched_requests": st.batched_requests, "max_batch": st.max_batch}
+                "batched_requests": st.batched_requests, "max_batch": st.max_batch,
+                "requests_cancelled": st.requests_cancelled,
+                "requests_timed_out": st.requests_timed_out, "hbm_charged": st.hbm_charged,
+                "hbm_charged_max": st.hbm_charged_max, "workspace_trims": st.workspace_trims}
 
     def close(self) -> None:
         if getattr(self, "_h", None) is not None:

@@ -241,8 +241,9 @@ static double per_snp_kappa(uint32_t sum, uint32_t count, uint32_t sum_sq, uint3
     if (count == 0) return 0.0;
     double cf = (double)count;
     double mean = (double)sum / cf;
+    double mean2 = mean * mean; /* mean.powi(4): (x*x)*(x*x) */
     double m4 = (double)sum_4th - 4.0 * mean * (double)sum_cube +
-                6.0 * mean * mean * (double)sum_sq - 3.0 * pow(mean, 4) * cf;
+                6.0 * mean * mean * (double)sum_sq - 3.0 * (mean2 * mean2) * cf;
     double css = (double)sum_sq - cf * mean * mean;
     if (css <= 0.0) return 0.0;
     return m4 / (css * css);
@@ -377,6 +378,46 @@ static void gram_f32(const float* const* A, uint64_t na, const float* const* B, 
 /* sketching, mod.rs:393).  Returns 0 ok, 1 chunk_c == 0,                */
 /* 2 sketch d out of range.  l2 is m x K column-major, maf is m.         */
 /* ------------------------------------------------------------------ */
+/* Test hook: a caller-given projection (bucket, sign over the n individuals
+ * in panel order, sketch dimension d) replaces CountSketchProj::new(n, d, 42)
+ * in orc_l2 while it is set, and d = n is then allowed (the reference's
+ * range check, compute.rs:1083-1088, stays for the hashed projection).  With
+ * an identity projection (d = n, bucket i = i, all signs +1) the sketched
+ * column is the standardized column itself, so the sketch path's chain
+ * (scatter, renorm, product, quadratic inversion) can be checked apart from
+ * the hash.  NULL arrays clear the hook. */
+static const uint32_t* g_proj_bucket = NULL;
+static const int8_t* g_proj_sign = NULL;
+static uint64_t g_proj_n = 0, g_proj_d = 0;
+void orc_set_projection(const uint32_t* bucket, const int8_t* sign, uint64_t n, uint64_t d) {
+    g_proj_bucket = bucket && sign ? bucket : NULL;
+    g_proj_sign = bucket && sign ? sign : NULL;
+    g_proj_n = n;
+    g_proj_d = d;
+}
+
+/* per_snp_kappa of each listed SNP from the integer moments the sketch
+ * path counts (compute.rs:525-538, moments as in compute.rs:429-523). */
+void orc_kappa(const uint8_t* slabs, uint64_t bps, uint64_t n_fam, const int64_t* keep,
+               uint64_t n_keep, const uint64_t* bed_idx, uint64_t m, double* out) {
+    const uint64_t n = keep ? n_keep : n_fam;
+    for (uint64_t j = 0; j < m; ++j) {
+        uint32_t st[4];
+        orc_stats(slabs, bps, n_fam, keep, n_keep, bed_idx + j, 1, st);
+        const uint8_t* slab = slabs + bed_idx[j] * bps;
+        uint32_t cube = 0, fourth = 0;
+        for (uint64_t i = 0; i < n; ++i) {
+            const uint64_t row = keep ? (uint64_t)keep[i] : i;
+            const int gg = DOSAGE_LUT[code_at(slab, row)];
+            if (gg > 0) {
+                cube += (uint32_t)(gg * gg * gg);
+                fourth += (uint32_t)(gg * gg * gg * gg);
+            }
+        }
+        out[j] = per_snp_kappa(st[0], st[1], st[2], cube, fourth);
+    }
+}
+
 int orc_l2(const uint8_t* slabs, uint64_t bps, uint64_t n_fam, const int64_t* keep,
            uint64_t n_keep, const uint8_t* chr, const double* cm, const uint32_t* bp,
            const uint64_t* bed_idx, uint64_t m, int mode, double wval, uint64_t chunk_c,
@@ -385,7 +426,8 @@ int orc_l2(const uint8_t* slabs, uint64_t bps, uint64_t n_fam, const int64_t* ke
     if (chunk_c < 1) return 1; /* compute.rs:1035-1039 */
     if (m == 0) return 0;
     const uint64_t n = keep ? n_keep : n_fam;
-    if (sketch_d && !(sketch_d >= 3 && sketch_d < n)) return 2; /* :1083-1088 */
+    const int given_proj = sketch_d && g_proj_bucket && g_proj_n == n && g_proj_d == sketch_d;
+    if (sketch_d && !given_proj && !(sketch_d >= 3 && sketch_d < n)) return 2; /* :1083-1088 */
 #ifdef _OPENMP
     if (nthreads > 0) omp_set_num_threads(nthreads);
 #else
@@ -408,7 +450,12 @@ int orc_l2(const uint8_t* slabs, uint64_t bps, uint64_t n_fam, const int64_t* ke
     if (sketch_d) {
         bucket = (uint32_t*)malloc(sizeof(uint32_t) * n);
         sgn = (int8_t*)malloc(n);
-        orc_countsketch_hash(n, sketch_d, 42, bucket, sgn);
+        if (given_proj) {
+            memcpy(bucket, g_proj_bucket, sizeof(uint32_t) * n);
+            memcpy(sgn, g_proj_sign, n);
+        } else {
+            orc_countsketch_hash(n, sketch_d, 42, bucket, sgn);
+        }
     }
 
     /* block sizes and ring, compute.rs:1164-1181 */

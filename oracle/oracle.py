@@ -63,6 +63,9 @@ def lib():
                              _f64p, _f64p, C.c_int]
         L.orc_l2.restype = C.c_int
         L.orc_num_threads.restype = C.c_int
+        L.orc_set_projection.argtypes = [vp, vp, C.c_uint64, C.c_uint64]
+        L.orc_kappa.argtypes = [vp, C.c_uint64, C.c_uint64, vp, C.c_uint64, _u64p,
+                                C.c_uint64, _f64p]
         _lib = L
     return _lib
 
@@ -212,6 +215,38 @@ def l2(bed, n_fam, chr_, cm, bp, bed_idx=None, *, mode=WINDOW_CM, window=1.0, ch
     if rc == 2:
         raise ValueError(f"--sketch {sketch} is out of bounds")
     return out.reshape(K, m).T, maf
+
+
+def kappa(bed, n_fam, bed_idx, keep=None) -> np.ndarray:
+    """per_snp_kappa (src/l2/compute.rs:525-538) of the listed SNPs, from the
+    integer moments of the decoded column."""
+    body = _body(bed)
+    bps = (n_fam + 3) // 4
+    k, nk = _keep(keep)
+    idx = np.ascontiguousarray(bed_idx, dtype=np.uint64)
+    out = np.zeros(idx.size, np.float64)
+    lib().orc_kappa(_ptr(body), bps, n_fam, _ptr(k), nk, idx, idx.size, out)
+    return out
+
+
+class projection:
+    """Context manager: l2(..., sketch=d) inside it uses the given bucket /
+    sign arrays (length n, panel order) instead of the seed-42 hash, and
+    allows d = n.  Test hook for checking the sketch chain apart from the
+    hash."""
+
+    def __init__(self, bucket, sign, d):
+        self.b = np.ascontiguousarray(bucket, dtype=np.uint32)
+        self.s = np.ascontiguousarray(sign, dtype=np.int8)
+        self.d = int(d)
+
+    def __enter__(self):
+        lib().orc_set_projection(_ptr(self.b), _ptr(self.s), self.b.size, self.d)
+        return self
+
+    def __exit__(self, *exc):
+        lib().orc_set_projection(None, None, 0, 0)
+        return False
 
 
 def num_threads() -> int:

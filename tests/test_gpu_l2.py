@@ -73,3 +73,48 @@ def test_cpu_runnable_config(oracle, form):
     assert _rel(got, exp) < (TOL_F32 if use_f32 else TOL_EXACT)
     assert np.array_equal(maf, emaf)
     assert perf.pairs_included > 9_000_000
+
+
+def _chunk_total(chr_, chunk, per_chr):
+    if not per_chr:
+        return -(-len(chr_) // chunk)
+    tot, s = 0, 0
+    while s < len(chr_):
+        e = s + 1
+        while e < len(chr_) and chr_[e] == chr_[s]:
+            e += 1
+        tot += -(-(e - s) // chunk)
+        s = e
+    return tot
+
+
+@pytest.mark.parametrize("per_chr,K", [(False, 1), (True, 1), (False, 220)])
+def test_progress_callbacks(per_chr, K):
+    """L2Progress (mod.rs:22-39, called per chunk at compute.rs:2391-2398):
+    progress arrives while the band runs, chunks_done is monotone over the
+    reference's chunk grid (restarted per chromosome when per_chromosome),
+    the last call reports every chunk and every SNP, and the results are
+    bit-identical to a call without a callback.  K = 220 takes the segmented
+    annotation upload (m x K x 8 bytes >= 32 MiB)."""
+    m, chunk = 20000, 50
+    p = synth.make_panel(n=700, m=m, seed=3, miss_rate=0.01)
+    if per_chr:
+        p.chr[:] = 1 + (np.arange(m) * 3) // m
+    snps = L.make_snp_array(p.chr, p.cm, p.bp)
+    rng = np.random.default_rng(1)
+    annot = rng.random((m, K)) if K > 1 else None
+    cfg = L.L2Config(mode=L.WindowMode.Cm(1.0), chunk_size=chunk, per_chromosome=per_chr)
+    print(f"progress per_chr={per_chr} K={K}", flush=True)
+    calls = []
+    with L.Panel(p.bed_bytes(), p.n, p.m) as panel:
+        ref, _, _ = L.compute_ldscore(panel, snps, cfg, annot)
+        got, _, _ = L.compute_ldscore(panel, snps, cfg, annot, on_progress=calls.append)
+    assert np.array_equal(got, ref)
+    total = _chunk_total(p.chr, chunk, per_chr)
+    assert 2 <= len(calls) <= 9
+    assert all(c.chunks_total == total and c.snps_total == m for c in calls)
+    done = [c.chunks_done for c in calls]
+    assert all(b > a for a, b in zip(done, done[1:])) and done[0] >= 1
+    assert calls[-1].chunks_done == total and calls[-1].snps_done == m
+    assert all(0 < c.snps_done < m for c in calls[:-1])
+    assert all(c.chunk_wall_ms >= 0 for c in calls)

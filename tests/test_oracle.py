@@ -372,3 +372,108 @@ def test_oracle_sketch_tracks_exact(oracle):
     sk, maf = oracle.l2(p.bed_bytes(), p.n, p.chr, p.cm, p.bp, mode=0, window=0.1, chunk=50,
                         masking=True, sketch=1000, use_f32=True)
     assert np.corrcoef(ex[:, 0], sk[:, 0])[0, 1] > 0.98
+
+
+# ---------------------------------------------------------------------------
+# sketch chain apart from the hash, and the MAF-aware kurtosis
+# ---------------------------------------------------------------------------
+def brute_kappa(dos):
+    """kappa_j = sum (g - mu)^4 / css^2 over the present individuals, the
+    definition per_snp_kappa (compute.rs:525-538) computes from raw moments;
+    0 for a column with no variance or no present individual."""
+    out = np.zeros(dos.shape[0])
+    for j in range(dos.shape[0]):
+        g = dos[j][dos[j] >= 0].astype(float)
+        if g.size == 0:
+            continue
+        c = g - g.mean()
+        css = (c * c).sum()
+        if css > 0:
+            out[j] = (c ** 4).sum() / css ** 2
+    return out
+
+
+def test_kappa_matches_definition(oracle):
+    """The oracle's raw-moment kurtosis equals the central-moment definition
+    on random, missing-heavy, monomorphic and all-missing columns."""
+    rng = np.random.default_rng(17)
+    n, m = 1503, 60
+    bed, codes, _, _, _ = random_panel(rng, n, m, miss=0.1)
+    body = bed[3:].reshape(m, -1).copy()
+    codes[3] = 3                      # monomorphic dosage 0
+    codes[4] = 0                      # monomorphic dosage 2
+    codes[5] = 1                      # all missing
+    codes[6] = np.where(rng.random(n) < 0.5, 1, 2)  # dosage 1 or missing: no variance
+    codes[7] = np.where(rng.random(n) < 0.002, 0, 3)  # rare allele
+    body[:] = 0
+    for i in range(n):
+        body[:, i // 4] |= (codes[:, i] << (2 * (i % 4))).astype(np.uint8)
+    bed = np.concatenate([bed[:3], body.reshape(-1)])
+    got = oracle.kappa(bed, n, np.arange(m, dtype=np.uint64))
+    exp = brute_kappa(dosages(codes))
+    assert np.all(got[[3, 4, 5, 6]] == 0.0)
+    assert np.max(np.abs(got - exp) / np.maximum(1.0, np.abs(exp))) < 1e-12
+    # with --keep: the moments are over the kept individuals only
+    keep = rng.permutation(n)[:700]
+    got_k = oracle.kappa(bed, n, np.arange(m, dtype=np.uint64), keep=keep)
+    exp_k = brute_kappa(dosages(codes[:, keep]))
+    assert np.max(np.abs(got_k - exp_k) / np.maximum(1.0, np.abs(exp_k))) < 1e-12
+
+
+def _quad_inv(t, d, D, kj=0.0, kk=0.0):
+    """Inverse of the sketch bias model t = r^2 + (1 - r^2)(1 - 2 r^2)/d (with
+    the MAF-aware d - 3 - 2(kj + kk) term), then the unbiased r^2; pinned
+    below by feeding it the model's own output."""
+    dm3 = d - 3.0 - 2.0 * (kj + kk)
+    disc = dm3 * dm3 + 8.0 * (d * t - 1.0)
+    rs = np.where(disc >= 0, (-dm3 + np.sqrt(np.maximum(disc, 0))) / 4.0, 0.0)
+    return rs * (1 + 1 / D) - 1 / D
+
+
+@pytest.mark.parametrize("maf_aware", [False, True])
+def test_sketch_chain_with_injective_projection(oracle, maf_aware):
+    """With an injective projection (each individual alone in its bucket,
+    random signs, d > N so that d and N differ) the sketched column is a
+    signed copy of the standardized column: no sketch noise.  The sketch
+    path must then give exactly the window sum of the quadratic inversion
+    applied to the exact r^2 -- which pins the scatter, the renormalization
+    to ||x||^2 = N, the product scaling and the per-pair kurtosis wiring apart
+    from the hash (whose stream stays unpinned)."""
+    rng = np.random.default_rng(5 + maf_aware)
+    n, m, d = 96, 48, 113
+    bed, codes, chr_, cm, bp = random_panel(rng, n, m, miss=0.04)
+    bucket = rng.permutation(d)[:n].astype(np.uint32)
+    sign = np.where(rng.random(n) < 0.5, 1, -1).astype(np.int8)
+    with oracle.projection(bucket, sign, d):
+        got, _ = oracle.l2(bed, n, chr_, cm, bp, mode=1, window=30.0, chunk=7, masking=True,
+                           use_f32=True, sketch=d, maf_aware=maf_aware)
+    # the same call outside the hook is the hashed projection's range error
+    with pytest.raises(ValueError):
+        oracle.l2(bed, n, chr_, cm, bp, mode=1, window=30.0, chunk=7, masking=True,
+                  use_f32=True, sketch=d)
+    D = n - 2.0
+    # the inversion used for the expectation is itself pinned to the model
+    for r2 in np.linspace(0, 1, 11):
+        t = r2 + (1 - r2) * (1 - 2 * r2) / d
+        assert abs(_quad_inv(t, d, D) - (r2 * (1 + 1 / D) - 1 / D)) < 1e-12
+    dos = dosages(codes)
+    X = std_cols(dos)
+    r2 = (X.T @ X / n) ** 2
+    kap = brute_kappa(dos) if maf_aware else np.zeros(m)
+    L = np.ones(m)
+    for j in range(m):
+        for i in range(j):
+            if window_ok("kb", 30.0, chr_, cm, bp, i, j):
+                v = _quad_inv(r2[i, j], float(d), D, kap[i], kap[j])
+                L[j] += v
+                L[i] += v
+    assert np.max(np.abs(got[:, 0] - L)) < 2e-5
+    # a planted mistake (the inversion run with N in place of d) is visible
+    Lbad = np.ones(m)
+    for j in range(m):
+        for i in range(j):
+            if window_ok("kb", 30.0, chr_, cm, bp, i, j):
+                v = _quad_inv(r2[i, j], float(n), D, kap[i], kap[j])
+                Lbad[j] += v
+                Lbad[i] += v
+    assert np.max(np.abs(got[:, 0] - Lbad)) > 1e-3
