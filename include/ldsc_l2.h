@@ -170,6 +170,23 @@ int ldsc_l2_compute(const ldsc_panel* panel, const ldsc_snp* snps, uint64_t m,
                     double* l2_out, double* maf_out, ldsc_l2_perf* perf,
                     ldsc_progress_cb cb, void* cb_ctx, char* err, uint64_t err_len);
 
+/* ldsc_l2_compute without allocation.  No counterpart in the reference: it
+ * is for callers that cannot allocate (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  Same arguments, kernels
+ * and results as ldsc_l2_compute (bit-identical), but every device buffer
+ * of the call is carved from one fixed 256 MiB device array per GPU (held
+ * from the first call on that GPU until the process ends), and host staging
+ * is pageable memory: no hipMalloc, hipFree or hipHostMalloc after the first
+ * call on a device, which sets up the array's address and the call's
+ * streams.  A call whose buffers would exceed 256 MiB fails with
+ * LDSC_ERR_OOM (message names the scratch size); the CountSketch path
+ * (sketch_d > 0) fails with LDSC_ERR_INVALID_ARG.  Calls on one GPU share its
+ * array and are serialized by a lock inside the library. */
+int ldsc_l2_compute_noalloc(const ldsc_panel* panel, const ldsc_snp* snps, uint64_t m,
+                            const ldsc_l2_config* cfg, const double* annot, uint64_t K,
+                            double* l2_out, double* maf_out, ldsc_l2_perf* perf,
+                            ldsc_progress_cb cb, void* cb_ctx, char* err, uint64_t err_len);
+
 /* ---------------------------------------------------------------------- */
 /* Sub-steps exposed for bit-exact checks and for callers that need them.  */
 /* ---------------------------------------------------------------------- */

@@ -32,6 +32,7 @@ EXPORTED = [
     "ldsc_l2_block_left", "ldsc_l2_pair_bounds", "ldsc_l2_decode", "ldsc_l2_countsketch_hash",
     "ldsc_l2_sketch_buckets", "ldsc_server_create", "ldsc_server_destroy", "ldsc_l2_submit",
     "ldsc_l2_poll", "ldsc_l2_wait", "ldsc_server_stats_get", "ldsc_l2_cancel",
+    "ldsc_l2_compute_noalloc",
 ]
 
 
@@ -119,6 +120,8 @@ def lib():
     L.ldsc_l2_wait.argtypes = [vp, u64, C.POINTER(dbl), cp, u64]
     L.ldsc_server_stats_get.argtypes = [vp, C.POINTER(ServerStats)]
     L.ldsc_l2_cancel.argtypes = [vp, u64, cp, u64]
+    L.ldsc_l2_compute_noalloc.argtypes = [vp, vp, u64, C.POINTER(Config), vp, u64, vp, vp,
+                                          C.POINTER(Perf), PROGRESS_CB, vp, cp, u64]
     for name in EXPORTED:
         getattr(L, name)  # every declared symbol must resolve
     _lib = L

@@ -19,7 +19,7 @@ ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 HIPCC = os.path.join(ROCM, "bin", "hipcc")
 ARCH = os.environ.get("LDSC_OFFLOAD_ARCH", "gfx950")
 
-ENGINE_SOURCES = ["host_windows.cpp", "engine.hip", "sketch.hip", "server.cpp"]
+ENGINE_SOURCES = ["host_windows.cpp", "engine.hip", "sketch.hip", "server.cpp", "noalloc.hip"]
 HIP_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-fPIC", "-shared", "-std=c++17",
     "-Wall", "-Wno-unused-result",

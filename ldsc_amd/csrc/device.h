@@ -53,20 +53,33 @@ struct BandArgs {
     double n_pad;          // individuals of the tiles, padding included
 };
 
+// A fixed block of device memory that buffers are carved from, front to
+// back, instead of being allocated (the allocation-free entry point).
+struct Arena {
+    char* base = nullptr;
+    size_t cap = 0, off = 0;
+};
+
+// Grow-only device buffer: hipMalloc, or a piece of `arena` when set (then
+// release() only forgets the piece; the arena is reset as a whole).
 struct DevBuf {
     void* p = nullptr;
     size_t cap = 0;
     int device = -1;
+    Arena* arena = nullptr;
     hipError_t ensure(size_t bytes);
     void release();
     template <class T>
     T* as() const { return static_cast<T*>(p); }
 };
 
-// Grow-only pinned host staging buffer (hipHostMalloc).
+// Grow-only host staging buffer: pinned (hipHostMalloc), or pageable host
+// memory when `pageable` (no HIP allocation; copies through it are staged
+// by the runtime and return once the host side is consumed or filled).
 struct PinnedBuf {
     void* p = nullptr;
     size_t cap = 0;
+    bool pageable = false;
     hipError_t ensure(size_t bytes);
     void release();
     template <class T>
@@ -124,6 +137,10 @@ struct Workspace {
     hipError_t init(int dev);
     void trim();           // frees the device buffers (streams and events stay)
     void destroy();
+    // Allocation-free mode: device buffers come from `a`, host staging is
+    // pageable.  reset_arena() starts the next call's carving at the front.
+    void use_arena(Arena* a);
+    void reset_arena();
 };
 
 Workspace* acquire_workspace(int device);

@@ -21,6 +21,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <functional>
 #include <thread>
@@ -44,6 +45,14 @@ namespace ldsc {
 // ------------------------------------------------------------------------
 hipError_t DevBuf::ensure(size_t bytes) {
     if (bytes <= cap && p) return hipSuccess;
+    if (arena) {
+        const size_t want = round_up(std::max<size_t>(bytes, 256), 256);
+        if (arena->off + want > arena->cap) return hipErrorOutOfMemory;
+        p = arena->base + arena->off;
+        arena->off += want;
+        cap = want;
+        return hipSuccess;
+    }
     release();
     size_t want = std::max<size_t>(bytes, 256);
     hipError_t e = hipMalloc(&p, want);
@@ -56,7 +65,7 @@ hipError_t DevBuf::ensure(size_t bytes) {
     return hipSuccess;
 }
 void DevBuf::release() {
-    if (p) (void)hipFree(p);
+    if (p && !arena) (void)hipFree(p);
     p = nullptr;
     cap = 0;
 }
@@ -74,6 +83,12 @@ hipError_t PinnedBuf::ensure(size_t bytes) {
     if (bytes <= cap && p) return hipSuccess;
     release();
     const size_t want = std::max<size_t>(bytes, 4096);
+    if (pageable) {
+        p = std::aligned_alloc(4096, round_up(want, 4096));
+        if (!p) return hipErrorOutOfMemory;
+        cap = want;
+        return hipSuccess;
+    }
     hipError_t e = hipHostMalloc(&p, want, hipHostMallocDefault);
     if (e != hipSuccess) {
         p = nullptr;
@@ -84,7 +99,10 @@ hipError_t PinnedBuf::ensure(size_t bytes) {
     return hipSuccess;
 }
 void PinnedBuf::release() {
-    if (p) (void)hipHostFree(p);
+    if (p && pageable)
+        std::free(p);
+    else if (p)
+        (void)hipHostFree(p);
     p = nullptr;
     cap = 0;
 }
@@ -133,6 +151,24 @@ void Workspace::trim() {
     if (copy_stream) (void)hipStreamSynchronize(copy_stream);
     if (up_stream) (void)hipStreamSynchronize(up_stream);
     for (auto* b : all) b->release();
+}
+
+void Workspace::use_arena(Arena* a) {
+    DevBuf* all[] = {&tiled, &stats, &bed_idx, &lo, &info, &aeff, &annot, &pq, &maf, &mafd,
+                     &tiles, &j0, &tstart, &rowp, &colp, &l2, &sk_cols, &sk_norm, &scratch};
+    for (auto* b : all) b->arena = a;
+    PinnedBuf* host[] = {&pinned, &pinned_meta, &pinned_maf, &ring};
+    for (auto* b : host) b->pageable = true;
+}
+
+void Workspace::reset_arena() {
+    DevBuf* all[] = {&tiled, &stats, &bed_idx, &lo, &info, &aeff, &annot, &pq, &maf, &mafd,
+                     &tiles, &j0, &tstart, &rowp, &colp, &l2, &sk_cols, &sk_norm, &scratch};
+    for (auto* b : all) {
+        b->p = nullptr;
+        b->cap = 0;
+        if (b->arena) b->arena->off = 0;
+    }
 }
 
 void Workspace::destroy() {
