@@ -17,8 +17,15 @@ Default run (one JSON line):
   * cpu_baseline -- the C restatement of the reference's f64 path on the
     host cores, on a bounded slice of the same panel.
 
-`--workload sketch` (CountSketch, N=100,000 x M=200,000, d in {200, 1000, 5000}),
-`--workload partitioned` (partitioned, N=20,000 x M=500,000, K=50, kb 1000) and
+  * sketch -- CountSketch LD scores, N=100,000 x M=200,000, cM 1, masked,
+    d in {200, 1000, 5000}: SNP-pairs/s, the preparation (decode, bucket
+    sums) against HBM bandwidth and the d-length contraction against its
+    MFMA peak, and the restatement of the reference's sketch path on a slice.
+  * partitioned -- N=20,000 x M=500,000, K=50 annotations, kb 1000: the
+    exact form on both exact kernels and the FP32 form, with the band's
+    roofline and the restatement of the f64 path on a slice.
+
+`--workload sketch` / `--workload partitioned` print those legs alone;
 `--workload reference_shapes` (the panel shapes of the reference's published
 CPU wall times) and `--workload cli` (the `ldsc l2` command end to end, from
 the PLINK files to the written LD-score files, at the 1000G shape) print
@@ -49,18 +56,48 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def profiled_traffic(kernel: str):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 summary
-    (FETCH_SIZE/WRITE_SIZE scaled by a copy-kernel calibration), or None."""
-    path = os.path.join(ROOT, "profiles", "k_band_fp4_50k_x_100k.md")
+PANEL_PROFILE = "profiles/k_band_fp4_50k_x_100k.md"
+
+
+def profiled(key: str, kernel: str, path: str = PANEL_PROFILE):
+    """A figure of `kernel` from a committed rocprofv3 summary: its lines
+    `<key> <kernel> <value>`, e.g. traffic_bytes_per_launch (FETCH_SIZE and
+    WRITE_SIZE scaled by a copy-kernel calibration) or trace_avg_ms (the
+    kernel-trace average duration of the same launches), or None."""
     try:
-        for line in open(path):
+        for line in open(os.path.join(ROOT, path)):
             parts = line.split()
-            if len(parts) == 3 and parts[0] == "traffic_bytes_per_launch" and parts[1] == kernel:
-                return int(parts[2])
+            if len(parts) == 3 and parts[0] == key and parts[1] == kernel:
+                return float(parts[2])
     except OSError:
         pass
     return None
+
+
+def profiled_traffic(kernel: str, path: str = PANEL_PROFILE):
+    v = profiled("traffic_bytes_per_launch", kernel, path)
+    return None if v is None else int(v)
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def time_cpu(fn, runs=3):
+    """Median wall of `runs` calls after one warm-up, and the last result."""
+    fn()
+    walls, out = [], None
+    for _ in range(runs):
+        t = time.perf_counter()
+        out = fn()
+        walls.append(time.perf_counter() - t)
+    return float(np.median(walls)), out
 
 
 def measured_peaks():
@@ -169,6 +206,8 @@ def leg_panel(args, L, synth):
                      "peak": peak, "unit": "TFLOP/s",
                      "frac": algo / k8 / 1e12 / peak,
                      "traffic": profiled_traffic(kname), "band_kernel_ms": k8 * 1e3,
+                     "profile_trace_avg_ms": profiled("trace_avg_ms", kname),
+                     "profile": PANEL_PROFILE,
                      "ops_per_launch": algo,
                      "note": "algorithmic 2*N*P per launch (one N-long dot product per "
                              "included pair); peak = dense " +
@@ -224,14 +263,7 @@ def leg_panel(args, L, synth):
             "fp32_vs_f32_restatement_max_rel": float(np.max(np.abs(got32 - exp32) /
                                                             np.abs(exp32))),
         }
-        model = "unknown"
-        try:
-            for line in open("/proc/cpuinfo"):
-                if line.startswith("model name"):
-                    model = line.split(":", 1)[1].strip()
-                    break
-        except OSError:
-            pass
+        model = cpu_model()
         P_s = perf_s.pairs_included
         res["cpu_baseline"] = {
             "value": P_s / cpu_wall, "unit": "SNP-pairs/s",
@@ -432,36 +464,88 @@ def leg_serving(args, L, synth):
 # ---------------------------------------------------------------------------
 # CountSketch
 # ---------------------------------------------------------------------------
+SKETCH_PROFILE = "profiles/sketch_100k_x_200k.md"
+PART_PROFILE = "profiles/partitioned_20k_x_500k.md"
+BF16_PEAK_TFLOPS = 2500.0  # MI355X dense bf16 MFMA, TFLOP/s
+# the d-length contraction of the sketch path
+SKETCH_CONTRACTION = {"kernel": "k_band_sk", "peak": BF16_PEAK_TFLOPS,
+                      "note": "algorithmic 2*d*P f32-accuracy flops per launch; the kernel runs "
+                              "them on the bf16 MFMA as six part products of a three-part "
+                              "split of each f32 value; peak = dense bf16"}
+
+
 def leg_sketch(args, L, synth):
+    """C3: CountSketch LD scores at N=100,000 x M=200,000 for d = 200, 1000,
+    5000.  The preparation (tiling / bucket-order gather and the exact
+    integer bucket sums, HBM-bound) and the d-length contraction (MFMA) are
+    timed with HIP events inside the call (L2Perf); algorithmic bytes of the
+    preparation: M*ceil(N/4) .bed bytes + 8N bucket/sign bytes + 4*M*d
+    sketched values; of the contraction, 2*d*P flops."""
     n, m = 100_000, args.sketch_m
     p = synth.make_panel(n, m, 20261206, miss_rate=0.01)
     snps = L.make_snp_array(p.chr, p.cm, p.bp)
-    out = {"workload": f"CountSketch N={n} x M={m}, cM 1, chunk 200, masked", "d": {}}
+    out = {"workload": f"CountSketch N={n} x M={m}, 1% missing, cM 1, chunk 200, masked",
+           "d": {}}
     with L.Panel(p.bed_bytes(), n, m) as panel:
         ex_cfg = L.L2Config(mode=L.WindowMode.Cm(1.0), snp_level_masking=True)
         w, pf, (exact, _) = timed_steps(L, panel, snps, ex_cfg, 1, 1, "sketch exact")
-        out["exact_int8_ms"] = w[0] * 1e3
+        out["exact_ms_per_step"] = w[0] * 1e3
         P = pf[0].pairs_included
         for d in (200, 1000, 5000):
             cfg = L.L2Config(mode=L.WindowMode.Cm(1.0), snp_level_masking=True, sketch=d)
             w, pf, (sk, _) = timed_steps(L, panel, snps, cfg, args.steps, 1, f"sketch d={d}")
             t = float(np.mean(w))
-            sk_s = float(np.mean([q.sketch_secs for q in pf]))
-            tile_s = float(np.mean([q.norm_secs for q in pf]))
+            prep_s = float(np.mean([q.norm_secs + (q.sketch_secs or 0.0) for q in pf]))
             band_s = float(np.mean([q.bb_dot_secs for q in pf]))
             bytes_sk = m * ((n + 3) // 4) + 8 * n + 4 * m * d
             out["d"][str(d)] = {
                 "snp_pairs_per_s": P / t, "ms_per_step": t * 1e3,
-                "gather_tile_ms": tile_s * 1e3, "sketch_kernel_ms": sk_s * 1e3,
-                "sketch_gbs_algorithmic": bytes_sk / sk_s / 1e9,
-                "sketch_frac_hbm": bytes_sk / sk_s / 1e9 / HBM_PEAK_GBS,
-                "contraction_ms": band_s * 1e3,
-                "contraction_tflops": 2.0 * d * P / band_s / 1e12,
+                "prep_ms": prep_s * 1e3, "contraction_ms": band_s * 1e3,
+                "prep_roofline": {"bound": "hbm", "achieved": bytes_sk / prep_s / 1e9,
+                                  "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                  "frac": bytes_sk / prep_s / 1e9 / HBM_PEAK_GBS,
+                                  "bytes": bytes_sk,
+                                  "traffic": profiled_traffic(f"prep_d{d}", SKETCH_PROFILE)},
+                "contraction_roofline": {
+                    "bound": "mfma", "kernel": SKETCH_CONTRACTION["kernel"],
+                    "achieved": 2.0 * d * P / band_s / 1e12,
+                    "peak": SKETCH_CONTRACTION["peak"], "unit": "TFLOP/s",
+                    "frac": 2.0 * d * P / band_s / 1e12 / SKETCH_CONTRACTION["peak"],
+                    "profile_trace_avg_ms": profiled("trace_avg_ms",
+                                                     f"{SKETCH_CONTRACTION['kernel']}_d{d}",
+                                                     SKETCH_PROFILE),
+                    "note": SKETCH_CONTRACTION["note"]},
+                "speedup_vs_exact": out["exact_ms_per_step"] / (t * 1e3),
                 "pearson_r_vs_exact": float(np.corrcoef(sk[:, 0], exact[:, 0])[0, 1]),
                 "max_abs_diff_vs_exact": float(np.max(np.abs(sk - exact))),
             }
             log(f"sketch d={d}: {out['d'][str(d)]}")
-    out["pairs"] = P
+        out["pairs"] = P
+        if not args.skip_cpu:
+            from oracle import oracle as O
+
+            ms, d = min(args.cpu_sample_snps, m), 1000
+            sl = slice(m // 2, m // 2 + ms)
+            idx = np.arange(sl.start, sl.stop, dtype=np.uint64)
+            cfg = L.L2Config(mode=L.WindowMode.Cm(1.0), snp_level_masking=True, sketch=d)
+            got, _, perf_s = L.compute_ldscore(panel, snps[sl], cfg)
+            wall, exp = time_cpu(lambda: O.l2(p.bed_bytes(), n, p.chr[sl], p.cm[sl], p.bp[sl],
+                                              idx, mode=O.WINDOW_CM, window=1.0, chunk=200,
+                                              masking=True, use_f32=True, sketch=d,
+                                              threads=args.cpu_threads)[0])
+            Ps = perf_s.pairs_included
+            out["cpu_baseline"] = {
+                "value": Ps / wall, "unit": "SNP-pairs/s", "cores": args.cpu_threads,
+                "kind": "port",
+                "sample": f"{ms} SNPs from the middle of the same panel (N={n}, cM 1, masked), "
+                          f"{Ps} pairs, d={d}: C restatement of the reference's f32 sketch path "
+                          f"(sequential f32 scatter, f32 GEMM), median of 3 runs after 1 "
+                          f"warmup: {wall:.2f}s",
+                "host": {"cpu_model": cpu_model(), "nproc": os.cpu_count()},
+                "device_vs_restatement_max_rel": float(np.max(np.abs(got - exp) /
+                                                              np.maximum(1.0, np.abs(exp)))),
+            }
+            log(f"sketch cpu baseline: {out['cpu_baseline']}")
     return out
 
 
@@ -469,6 +553,10 @@ def leg_sketch(args, L, synth):
 # Partitioned LD scores
 # ---------------------------------------------------------------------------
 def leg_partitioned(args, L, synth):
+    """C4: N=20,000 x M=500,000, K=50 annotations (40 binary in runs, 10
+    LogNormal), --ld-wind-kb 1000, chunk 200.  The 200 MB annotation goes
+    up and the 200 MB result comes back within each step (PCIe-inclusive);
+    the band's roofline counts 2*N*P per launch over its HIP-event time."""
     n, m = 20_000, args.partitioned_m
     p = synth.make_panel(n, m, 20261207 + 100, miss_rate=0.01)
     snps = L.make_snp_array(p.chr, p.cm, p.bp)
@@ -476,26 +564,61 @@ def leg_partitioned(args, L, synth):
     out = {"workload": f"partitioned N={n} x M={m}, K=50, kb 1000, chunk 200"}
     res = {}
     with L.Panel(p.bed_bytes(), n, m) as panel:
-        for tag, f32, kern in (("exact", False, "auto"), ("exact_int8", False, "i8"),
-                               ("fp32", True, "auto")):
+        for tag, f32, kern, steps in (("exact", False, "auto", args.steps),
+                                      ("exact_int8", False, "i8", args.i8_steps),
+                                      ("fp32", True, "auto", args.fp32_steps)):
+            if steps <= 0:
+                continue
             cfg = L.L2Config(mode=L.WindowMode.Kb(1000.0), chunk_size=200, use_f32=f32,
                              exact_kernel=kern)
             # the 200 MB result lands in one reused buffer, as in a loop over
             # annotation sets; a fresh array per call adds its page faults
             dest = np.empty((annot.shape[1], m)).T
-            w, pf, (l2, _) = timed_steps(L, panel, snps, cfg, args.steps, 1, f"partitioned {tag}",
+            w, pf, (l2, _) = timed_steps(L, panel, snps, cfg, steps, 1, f"partitioned {tag}",
                                          annot=annot, dest=dest)
             P = pf[0].pairs_included
             t = float(np.mean(w))
             band = float(np.mean([q.bb_dot_secs for q in pf]))
+            peak = FP32_PEAK_TFLOPS if f32 else INT8_PEAK_TOPS if kern == "i8" else FP4_PEAK_TFLOPS
             out[tag] = {"snp_pairs_per_s": P / t, "ms_per_step": t * 1e3,
-                        "band_kernel_ms": band * 1e3, "pairs": P,
-                        "algorithmic_tops": 2.0 * n * P / band / 1e12}
-            res[tag] = l2
-        out["fp32_vs_exact_max_rel"] = float(np.max(np.abs(res["fp32"] - res["exact"]) /
-                                                    np.maximum(1.0, np.abs(res["exact"]))))
-        out["exact_int8_max_rel"] = float(np.max(np.abs(res["exact_int8"] - res["exact"]) /
-                                                 np.maximum(1.0, np.abs(res["exact"]))))
+                        "band_ms": band * 1e3, "pairs": P,
+                        "roofline": {"bound": "mfma", "achieved": 2.0 * n * P / band / 1e12,
+                                     "peak": peak, "unit": "TFLOP/s",
+                                     "frac": 2.0 * n * P / band / 1e12 / peak,
+                                     "note": "2*N*P per call over the band's HIP-event time "
+                                             "(its row-segment launches together); the FP64 "
+                                             "annotation contraction, 4*P*K flops, runs in the "
+                                             "same kernels"}}
+            res[tag] = l2.copy()
+        if "fp32" in res:
+            out["fp32_vs_exact_max_rel"] = float(np.max(np.abs(res["fp32"] - res["exact"]) /
+                                                        np.maximum(1.0, np.abs(res["exact"]))))
+        if "exact_int8" in res:
+            out["exact_int8_max_rel"] = float(np.max(np.abs(res["exact_int8"] - res["exact"]) /
+                                                     np.maximum(1.0, np.abs(res["exact"]))))
+        if not args.skip_cpu:
+            from oracle import oracle as O
+
+            ms = min(args.cpu_sample_snps, m)
+            sl = slice(m // 2, m // 2 + ms)
+            idx = np.arange(sl.start, sl.stop, dtype=np.uint64)
+            cfg = L.L2Config(mode=L.WindowMode.Kb(1000.0), chunk_size=200)
+            got, _, perf_s = L.compute_ldscore(panel, snps[sl], cfg, annot=annot[sl])
+            wall, exp = time_cpu(lambda: O.l2(p.bed_bytes(), n, p.chr[sl], p.cm[sl], p.bp[sl],
+                                              idx, mode=O.WINDOW_KB, window=1000.0, chunk=200,
+                                              annot=annot[sl], threads=args.cpu_threads)[0])
+            Ps = perf_s.pairs_included
+            out["cpu_baseline"] = {
+                "value": Ps / wall, "unit": "SNP-pairs/s", "cores": args.cpu_threads,
+                "kind": "port",
+                "sample": f"{ms} SNPs from the middle of the same panel (N={n}, kb 1000, K=50), "
+                          f"{Ps} pairs: C restatement of the reference's f64 path, median of 3 "
+                          f"runs after 1 warmup: {wall:.2f}s",
+                "host": {"cpu_model": cpu_model(), "nproc": os.cpu_count()},
+                "device_vs_restatement_max_rel": float(np.max(np.abs(got - exp) /
+                                                              np.maximum(1.0, np.abs(exp)))),
+            }
+            log(f"partitioned cpu baseline: {out['cpu_baseline']}")
     return out
 
 
@@ -603,6 +726,8 @@ def main() -> int:
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-serving", action="store_true")
+    ap.add_argument("--skip-sketch", action="store_true")
+    ap.add_argument("--skip-partitioned", action="store_true")
     ap.add_argument("--skip-peaks", action="store_true",
                     help="do not measure the sustained MFMA and copy peaks")
     ap.add_argument("--qps", type=float, default=200.0)
@@ -663,6 +788,8 @@ def main() -> int:
 
     pan = leg_panel(args, L, synth)
     serving = None if args.skip_serving else leg_serving(args, L, synth)
+    sketch = None if args.skip_sketch else leg_sketch(args, L, synth)
+    part = None if args.skip_partitioned else leg_partitioned(args, L, synth)
     out = {
         "metric": METRIC,
         "value": pan["value"],
@@ -694,6 +821,8 @@ def main() -> int:
         "sample_accuracy": pan.get("sample_accuracy"),
         "pair_accuracy": pan.get("pair_accuracy"),
         "serving": serving,
+        "sketch": sketch,
+        "partitioned": part,
     }
     print(json.dumps(out), flush=True)
     return 0
