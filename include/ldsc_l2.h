@@ -105,7 +105,8 @@ typedef struct {
     uint64_t pairs_evaluated;/* pairs the band kernel computed                  */
     uint64_t tiles;          /* 64x64 SNP tiles launched                        */
     int32_t any_full_chr_window;
-    int32_t pad_;
+    int32_t sketch_int8;     /* 1: the CountSketch contraction ran on exact int8
+                                operands; 0: on f32 values (or no sketch)       */
 } ldsc_l2_perf;
 
 /* L2Progress (src/l2/mod.rs:22-39). */

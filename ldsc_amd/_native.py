@@ -63,7 +63,7 @@ class Perf(C.Structure):
                 ("bb_dot_secs", C.c_double), ("ab_dot_secs", C.c_double),
                 ("ring_store_secs", C.c_double), ("total_secs", C.c_double),
                 ("pairs_included", C.c_uint64), ("pairs_evaluated", C.c_uint64),
-                ("tiles", C.c_uint64), ("any_full_chr_window", C.c_int32), ("pad_", C.c_int32)]
+                ("tiles", C.c_uint64), ("any_full_chr_window", C.c_int32), ("sketch_int8", C.c_int32)]
 
 
 class Progress(C.Structure):

@@ -272,15 +272,29 @@ inline void fill_band_tiles(const BandPlan& p, int2* tiles) {
     });
 }
 
-// CountSketch (sketch.hip)
+// CountSketch (sketch.hip).  Forms of the bucket sums:
+//  * kSketchRaw: the integer sums SG, SM (d x m column-major int32 each);
+//  * kSketchF32: f32 bucket values in the lane order of k_band_sk, their
+//    norm shares and per-SNP renorm scales;
+//  * kSketchI8: the int8 W and E tiles of k_band_sk8 (m_pad x d_pad bytes
+//    each, back to back in `cols`), per-SNP constants and the range flag.
+enum SketchMode { kSketchRaw = 0, kSketchF32 = 1, kSketchI8 = 2 };
 int sketch_plan(const ldsc_panel* panel, uint64_t d, SketchPlan** out, char* err,
                 uint64_t err_len);
-hipError_t launch_sketch(const SketchPlan* sp, const uint32_t* tiled, uint32_t nkb,
+hipError_t launch_sketch(const SketchPlan* sp, int mode, const uint32_t* tiled, uint32_t nkb,
                          const float2* musc, uint32_t m, uint32_t m_pad, double n, double* norm,
-                         float* sk, int32_t* sg, int32_t* sm, hipStream_t st);
+                         void* cols, int32_t* sg, int32_t* sm, hipStream_t st);
 size_t sketch_norm_bytes(const SketchPlan* sp, uint64_t m_pad);
-// The per-SNP renorm scales launch_sketch leaves after the norm shares.
+// The per-SNP renorm scales kSketchF32 leaves after the norm shares.
 const float* sketch_scales(const SketchPlan* sp, const double* norm, uint64_t m_pad);
+// kSketchI8: per SNP scale, delta and A (3 x m_pad doubles), and the flag
+// (non-zero when a W or E value of a real SNP left [-127, 127]).
+const double* sketch_i8_consts(const SketchPlan* sp, const double* norm, uint64_t m_pad);
+int32_t* sketch_i8_flag(const SketchPlan* sp, double* norm, uint64_t m_pad);
+// The largest d_pad of the integer-operand form: the int32 sums of its
+// products stay exact (d_pad * 127^2 < 2^31).
+constexpr uint64_t kSketchI8MaxDpad = 131072;
 hipError_t launch_band_sk(const BandArgs& a, hipStream_t st);
+hipError_t launch_band_sk8(const BandArgs& a, hipStream_t st);
 
 }  // namespace ldsc

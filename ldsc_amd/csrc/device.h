@@ -47,6 +47,12 @@ struct BandArgs {
     const float* sk;       // sketched columns (CountSketch path) or nullptr
     const float* sk_scale; // their per-SNP renorm scales
     const double* kappa;   // per-SNP kurtosis of the MAF-aware sketch or nullptr
+    // integer-operand sketch form: W and E int8 tiles, and per SNP (m_pad
+    // each, in this order) the scale inv*sc, delta and A (sketch.hip)
+    const u32x4* skw;
+    const u32x4* ske;
+    const double* sk8;
+    double sk_tt;
     uint32_t nkb, ntiles, per_xcd, m, m_pad, K;
     uint32_t tile0;        // first tile of this launch in the call's tile list
     double r2u_a, r2u_b, n_inv_sq_r2u_a, n_inv_sq, d_f;
@@ -111,12 +117,13 @@ struct Workspace {
     hipStream_t stream = nullptr;
     hipStream_t copy_stream = nullptr;  // read-backs that overlap the band kernel
     hipStream_t up_stream = nullptr;    // annotation upload beside the tiling
-    hipEvent_t ev[9] = {};
+    hipEvent_t ev[10] = {};
     DevBuf tiled, stats, bed_idx, lo, info, aeff, annot, pq, maf, mafd, tiles, j0, tstart,
         rowp, colp, l2, sk_cols, sk_norm, scratch;
     PinnedBuf pinned;       // SNP row indices (upload in flight while the host plans)
     PinnedBuf pinned_meta;  // pair bounds, band plan, stats read-back
     PinnedBuf pinned_maf;   // MAF read-back (during the band kernel)
+    PinnedBuf pinned_flag;  // the sketch's integer-range flag (read before the band)
     // Large copies of caller memory (annotation upload, multi-annotation
     // results) pass through kRingSlots pinned slots of kRingSlot bytes: the
     // host's task pool fills or drains one slot while DMA moves another.
@@ -170,6 +177,8 @@ struct SketchPlan {
     void* d_ptiles = nullptr;     // ntiles x 1 KiB signed int8 MFMA A-operand tiles
     int4* d_blocks = nullptr;     // per bucket block: first tile, number of tiles
     int32_t* d_tile_ks = nullptr; // K-step (32 positions) of each tile
+    int32_t* d_tsum = nullptr;    // per bucket (d_pad, zero padded): T_b = sum of its signs
+    double tt = 0.0;              // sum over buckets of T_b^2
     void release();
 };
 }  // namespace ldsc
@@ -228,7 +237,8 @@ int run_l2(const ldsc_panel* panel, const ldsc_snp* snps, uint64_t m, const ldsc
 //    call's ws->ev[4] after the launch.
 //  * l2_end: waits for ev[4] on ws->stream, reduces, copies the results
 //    back; synchronous.
-enum BandForm { kBandFp4 = 0, kBandI8 = 1, kBandF32 = 2, kBandSketch = 3 };
+enum BandForm { kBandFp4 = 0, kBandI8 = 1, kBandF32 = 2, kBandSketch = 3, kBandSketch8 = 4 };
+inline bool is_sketch_form(int form) { return form == kBandSketch || form == kBandSketch8; }
 struct SketchPlan;
 struct L2Call {
     const ldsc_panel* panel = nullptr;

@@ -157,7 +157,7 @@ void Workspace::use_arena(Arena* a) {
     DevBuf* all[] = {&tiled, &stats, &bed_idx, &lo, &info, &aeff, &annot, &pq, &maf, &mafd,
                      &tiles, &j0, &tstart, &rowp, &colp, &l2, &sk_cols, &sk_norm, &scratch};
     for (auto* b : all) b->arena = a;
-    PinnedBuf* host[] = {&pinned, &pinned_meta, &pinned_maf, &ring};
+    PinnedBuf* host[] = {&pinned, &pinned_meta, &pinned_maf, &pinned_flag, &ring};
     for (auto* b : host) b->pageable = true;
 }
 
@@ -176,6 +176,7 @@ void Workspace::destroy() {
     pinned.release();
     pinned_meta.release();
     pinned_maf.release();
+    pinned_flag.release();
     ring.release();
     for (auto& x : ev)
         if (x) (void)hipEventDestroy(x);
@@ -1617,8 +1618,9 @@ size_t l2_workspace_bytes(const ldsc_panel* panel, uint64_t m, const ldsc_l2_con
     if (gather) bytes += m * round_up((panel->n + 3) / 4, 4);    // permuted rows
     if (cfg && cfg->sketch_d) {
         const uint64_t nblocks = round_up(cfg->sketch_d, 32) / 32;
+        const uint64_t shares = m_pad * 8 * ((nblocks + 3) / 4);
         bytes += m_pad * nblocks * 32 * 4;                       // sketch tiles
-        bytes += m_pad * 8 * ((nblocks + 3) / 4) + m_pad * 4;    // norm shares, scales
+        bytes += shares + std::max<uint64_t>(m_pad * 4, shares + 3 * m_pad * 8 + 16);
     }
     // DevBuf rounds each of its ~20 buffers up to at least 256 bytes
     return bytes + 20 * 256;
@@ -1902,7 +1904,12 @@ int l2_begin(const ldsc_panel* panel, const ldsc_snp* snps, uint64_t m, const ld
     // exact form: e2m1 MFMA while f32 holds the products exactly, int8 beyond
     const bool use_fp4 = !d && cfg->precision == LDSC_PREC_EXACT && n <= kFp4MaxIndiv;
     c.use_f32 = use_f32;
-    c.form = d ? kBandSketch : use_f32 ? kBandF32 : use_fp4 ? kBandFp4 : kBandI8;
+    // the sketch's d-length contraction runs on exact int8 operands when its
+    // int32 sums stay exact (falling back to f32 values if a value leaves
+    // the int8 range, l2_band)
+    const bool sk8 = d && sp->d_pad <= kSketchI8MaxDpad;
+    c.form = d ? (sk8 ? kBandSketch8 : kBandSketch)
+               : use_f32 ? kBandF32 : use_fp4 ? kBandFp4 : kBandI8;
     uint32_t parts = 1;
     if (d && sp->d_local)
         rc = gather_tile_chunks(panel, contiguous ? nullptr : ws->bed_idx.as<uint64_t>(),
@@ -1975,9 +1982,16 @@ int l2_begin(const ldsc_panel* panel, const ldsc_snp* snps, uint64_t m, const ld
     const uint32_t nkb = (uint32_t)(round_up(n, kKBlock) / kKBlock);
     if (d) {
         HIPC(ws->sk_norm.ensure(sketch_norm_bytes(sp, m_pad)));
-        HIPC(launch_sketch(sp, ws->tiled.as<uint32_t>(), nkb, d_musc, (uint32_t)m, m_pad,
-                           (double)n, ws->sk_norm.as<double>(), ws->sk_cols.as<float>(), nullptr,
-                           nullptr, st));
+        HIPC(launch_sketch(sp, sk8 ? kSketchI8 : kSketchF32, ws->tiled.as<uint32_t>(), nkb,
+                           d_musc, (uint32_t)m, m_pad, (double)n, ws->sk_norm.as<double>(),
+                           ws->sk_cols.p, nullptr, nullptr, st));
+        if (sk8) {  // the range flag comes back for l2_band's choice
+            HIPC(ws->pinned_flag.ensure(16));
+            HIPC(hipMemcpyAsync(ws->pinned_flag.p,
+                                sketch_i8_flag(sp, ws->sk_norm.as<double>(), m_pad), 4,
+                                hipMemcpyDeviceToHost, st));
+            HIPC(hipEventRecord(ws->ev[9], st));
+        }
     }
     HIPC(hipEventRecord(ws->ev[2], st));
 
@@ -2125,7 +2139,13 @@ int l2_begin(const ldsc_panel* panel, const ldsc_snp* snps, uint64_t m, const ld
     ba.rowp = ws->rowp.as<double>();
     ba.colp = ws->colp.as<double>();
     ba.sk = d ? ws->sk_cols.as<float>() : nullptr;
-    ba.sk_scale = d ? sketch_scales(sp, ws->sk_norm.as<double>(), m_pad) : nullptr;
+    ba.sk_scale = d && !sk8 ? sketch_scales(sp, ws->sk_norm.as<double>(), m_pad) : nullptr;
+    if (sk8) {
+        ba.skw = ws->sk_cols.as<u32x4>();
+        ba.ske = ba.skw + (size_t)m_pad * sp->d_pad / 16;
+        ba.sk8 = sketch_i8_consts(sp, ws->sk_norm.as<double>(), m_pad);
+        ba.sk_tt = sp->tt;
+    }
     ba.kappa = maf_aware ? ws->maf.as<double>() : nullptr;
     ba.nkb = d ? (uint32_t)sp->nblocks : nkb;
     ba.ntiles = plan.ntiles;
@@ -2152,6 +2172,7 @@ int l2_begin(const ldsc_panel* panel, const ldsc_snp* snps, uint64_t m, const ld
 
 static hipError_t launch_band_form(int form, const BandArgs& ba, hipStream_t st) {
     if (form == kBandSketch) return launch_band_sk(ba, st);
+    if (form == kBandSketch8) return launch_band_sk8(ba, st);
     if (form == kBandF32)
         hipLaunchKernelGGL(k_band_f32, dim3(8 * ba.per_xcd), dim3(64), 0, st, ba);
     else if (form == kBandFp4)
@@ -2166,6 +2187,22 @@ int l2_band(L2Call* call, char* err, uint64_t err_len) {
     Workspace* ws = c.ws;
     hipStream_t st = ws->stream;
     c.nprog = 0;
+    if (c.form == kBandSketch8) {
+        // a W or E value outside [-127, 127]: the bucket sums again as f32
+        // values (from the same tiles) and the f32 contraction
+        HIPC(hipEventSynchronize(ws->ev[9]));
+        if (*ws->pinned_flag.as<int32_t>()) {
+            const SketchPlan* sp = c.sketch_plan;
+            const uint32_t nkb = (uint32_t)(round_up(c.panel->n, kKBlock) / kKBlock);
+            const float2* d_musc =
+                reinterpret_cast<const float2*>(ws->info.as<char>() + (size_t)c.m_pad * 32);
+            HIPC(launch_sketch(sp, kSketchF32, ws->tiled.as<uint32_t>(), nkb, d_musc,
+                               (uint32_t)c.m, c.m_pad, (double)c.panel->n,
+                               ws->sk_norm.as<double>(), ws->sk_cols.p, nullptr, nullptr, st));
+            c.ba.sk_scale = sketch_scales(sp, ws->sk_norm.as<double>(), c.m_pad);
+            c.form = kBandSketch;
+        }
+    }
     const std::vector<int>& ts = ws->host.plan.tstart;
     const std::vector<int>& j0v = ws->host.plan.j0;
     const uint32_t nb = ws->host.plan.nb;
@@ -2445,6 +2482,7 @@ int l2_end(L2Call* call, char* err, uint64_t err_len) {
         perf->pairs_evaluated = (uint64_t)plan.ntiles * kTile * kTile;
         perf->tiles = plan.ntiles;
         perf->any_full_chr_window = c.any_full ? 1 : 0;
+        perf->sketch_int8 = c.form == kBandSketch8 ? 1 : 0;
     }
     if (c.cb) {
         ldsc_l2_progress pr{ends.size(), ends.size(), m, m, (now_s() - t_cb) * 1e3};

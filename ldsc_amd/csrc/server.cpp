@@ -270,7 +270,7 @@ int ldsc_server::run_request(Request* r, ldsc::Workspace* ws, char* err, uint64_
     };
     const bool stop_now = rc == LDSC_OK && !call.done && r->cancel.load();
     const bool batched =
-        batch && rc == LDSC_OK && !call.done && !stop_now && call.form != ldsc::kBandSketch;
+        batch && rc == LDSC_OK && !call.done && !stop_now && !ldsc::is_sketch_form(call.form);
     if (batch && !batched) {
         std::lock_guard<std::mutex> lk(batcher.mu);
         --batcher.preparing;

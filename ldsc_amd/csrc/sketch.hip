@@ -37,6 +37,8 @@ void SketchPlan::release() {
     if (d_ptiles) (void)hipFree(d_ptiles);
     if (d_blocks) (void)hipFree(d_blocks);
     if (d_tile_ks) (void)hipFree(d_tile_ks);
+    if (d_tsum) (void)hipFree(d_tsum);
+    d_tsum = nullptr;
     d_fam = nullptr;
     d_local = nullptr;
     d_ptiles = nullptr;
@@ -82,6 +84,10 @@ int sketch_plan(const ldsc_panel* panel_c, uint64_t d, SketchPlan** out, char* e
             for (uint64_t i = c0; i < c1; ++i) perm[off[bucket[i]]++] = (uint32_t)i;
         }
     }
+    // T_b = sum of the signs of bucket b's members, and sum_b T_b^2
+    std::vector<int32_t> tsum(sp->d_pad, 0);
+    for (uint64_t i = 0; i < n; ++i) tsum[bucket[i]] += sign[i];
+    for (uint64_t b = 0; b < d; ++b) sp->tt += (double)tsum[b] * (double)tsum[b];
     // P tiles, in (bucket block, chunk, K-step) order
     std::vector<int4> blocks(sp->nblocks);
     std::vector<int32_t> tile_ks;
@@ -138,6 +144,9 @@ int sketch_plan(const ldsc_panel* panel_c, uint64_t d, SketchPlan** out, char* e
         e = hipMemcpy(sp->d_tile_ks, tile_ks.data(), tile_ks.size() * 4, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMalloc(&sp->d_ptiles, tiles.size());
     if (e == hipSuccess) e = hipMemcpy(sp->d_ptiles, tiles.data(), tiles.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMalloc(&sp->d_tsum, tsum.size() * 4);
+    if (e == hipSuccess)
+        e = hipMemcpy(sp->d_tsum, tsum.data(), tsum.size() * 4, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMalloc(&sp->d_blocks, blocks.size() * sizeof(int4));
     if (e == hipSuccess)
         e = hipMemcpy(sp->d_blocks, blocks.data(), blocks.size() * sizeof(int4), hipMemcpyHostToDevice);
@@ -162,20 +171,34 @@ static inline uint32_t sketch_chunks(const SketchPlan* sp) {
     return (uint32_t)((sp->nblocks + kSkChunk - 1) / kSkChunk);
 }
 
+// Layout of the norm buffer: the norm shares (chunks x m_pad doubles), then
+// for kSketchF32 the scales (m_pad floats), for kSketchI8 the shares of A
+// (chunks x m_pad doubles), the per-SNP constants (3 x m_pad doubles) and
+// the range flag.
 size_t sketch_norm_bytes(const SketchPlan* sp, uint64_t m_pad) {
-    return (size_t)sketch_chunks(sp) * m_pad * 8 + m_pad * 4;  // norm shares, scales
+    const size_t shares = (size_t)sketch_chunks(sp) * m_pad * 8;
+    return shares + std::max<size_t>(m_pad * 4, shares + 3 * m_pad * 8 + 16);
 }
 
 // Wave (gp, c) handles the SNP groups 2gp, 2gp+1 (64 SNPs) and bucket
 // blocks [c * kSkChunk, (c+1) * kSkChunk).  Within a block it walks the
 // block's tiles (each with its K-step) with the next kSkAhead tiles' loads
-// in flight.  RAW writes the
-// integer bucket sums (d x m column-major); otherwise unscaled f32 bucket
-// values into the sketch tiles sk[((g * nblocks + B) * 64 + lane) * 16 + r]
-// (bucket 32B + 8(r/4) + 4(lane/32) + r%4 of SNP 32g + lane%32) and the
-// wave's share of each SNP's squared norm into norm[c * m_pad + snp];
-// k_sketch_scale then forms each SNP's renorm scale.
-template <bool RAW>
+// in flight.  The exact integer sums SG, SM of a block are then written as:
+//  * kSketchRaw: the integer bucket sums (d x m column-major);
+//  * kSketchF32: unscaled f32 bucket values y = inv (SG - mean SM) into the
+//    sketch tiles sk[((g * nblocks + B) * 64 + lane) * 16 + r] (bucket
+//    32B + 8(r/4) + 4(lane/32) + r%4 of SNP 32g + lane%32), and the wave's
+//    share of each SNP's squared norm into norm[c * m_pad + snp];
+//  * kSketchI8: the same value as two small integers.  With k = rint(mean)
+//    and delta = mean - k, and T_b the sign sum of bucket b,
+//      w = SG - k SM,  e = T_b - SM  (the signs of b's missing members),
+//      y / inv = w + delta e - delta T_b,
+//    w and e go to int8 tiles of the same lane order (W at `cols`, E m_pad x
+//    d_pad bytes after it), the wave's shares of the squared norm and of
+//    A = sum_b (w + delta e) T_b to norm[] and asum[], and a value of a real
+//    SNP outside [-127, 127] sets *flag.
+//  k_sketch_scale then forms each SNP's renorm scale (and constants).
+template <int MODE>
 __global__ __launch_bounds__(64) void k_sketch(const uint32_t* __restrict__ tiled, uint32_t nkb,
                                                 const v4i* __restrict__ ptiles,
                                                 const int4* __restrict__ blocks,
@@ -183,9 +206,12 @@ __global__ __launch_bounds__(64) void k_sketch(const uint32_t* __restrict__ tile
                                                 uint32_t nblocks,
                                                 const float2* __restrict__ musc, uint32_t m,
                                                 uint32_t m_pad, uint32_t d,
-                                                double* __restrict__ norm, float* __restrict__ sk,
+                                                double* __restrict__ norm, void* __restrict__ cols,
                                                 int32_t* __restrict__ sg_out,
-                                                int32_t* __restrict__ sm_out) {
+                                                int32_t* __restrict__ sm_out,
+                                                const int32_t* __restrict__ tsum,
+                                                double* __restrict__ asum,
+                                                int32_t* __restrict__ flag) {
     const uint32_t gp = blockIdx.x, c = blockIdx.y;
     const int lane = threadIdx.x;
     const int h = lane >> 5;
@@ -193,7 +219,12 @@ __global__ __launch_bounds__(64) void k_sketch(const uint32_t* __restrict__ tile
     const uint32_t* w1p = w0p + (size_t)nkb * 256;
     const uint32_t snp0 = 64 * gp + (lane & 31), snp1 = snp0 + 32;
     const float2 ms0 = musc[snp0], ms1 = musc[snp1];
-    double nrm0 = 0.0, nrm1 = 0.0;
+    double nrm0 = 0.0, nrm1 = 0.0, as0 = 0.0, as1 = 0.0;
+    // integer form: k = rint(mean) in {0, 1, 2} and delta = mean - k (exact)
+    const float kf0 = rintf(ms0.x), kf1 = rintf(ms1.x);
+    const int k0 = (int)kf0, k1 = (int)kf1;
+    const double dl0 = (double)ms0.x - (double)kf0, dl1 = (double)ms1.x - (double)kf1;
+    bool bad = false;
     const uint32_t B1 = min(nblocks, (c + 1) * kSkChunk);
     for (uint32_t B = c * kSkChunk; B < B1; ++B) {
         const int4 blk = blocks[B];
@@ -241,7 +272,7 @@ __global__ __launch_bounds__(64) void k_sketch(const uint32_t* __restrict__ tile
                 sm1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(cP[u], M1, sm1, 0, 0, 0);
             }
         }
-        if (RAW) {
+        if (MODE == kSketchRaw) {
 #pragma unroll
             for (int r = 0; r < 16; ++r) {
                 const uint32_t b = 32 * B + 8 * (r >> 2) + 4 * h + (r & 3);
@@ -256,7 +287,8 @@ __global__ __launch_bounds__(64) void k_sketch(const uint32_t* __restrict__ tile
                     }
                 }
             }
-        } else {
+        } else if (MODE == kSketchF32) {
+            float* sk = static_cast<float*>(cols);
             float* t0 = sk + (((size_t)(2 * gp) * nblocks + B) * 64 + lane) * 16;
             float* t1 = sk + (((size_t)(2 * gp + 1) * nblocks + B) * 64 + lane) * 16;
 #pragma unroll
@@ -268,9 +300,49 @@ __global__ __launch_bounds__(64) void k_sketch(const uint32_t* __restrict__ tile
                 t0[r] = y0;
                 t1[r] = y1;
             }
+        } else {
+            // T_b of the lane's buckets 32B + 8q + 4h + t (t = 0..3)
+            int T[16];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int4 t4 = *reinterpret_cast<const int4*>(tsum + 32 * B + 8 * q + 4 * h);
+                T[4 * q] = t4.x;
+                T[4 * q + 1] = t4.y;
+                T[4 * q + 2] = t4.z;
+                T[4 * q + 3] = t4.w;
+            }
+            uint32_t wd0[4] = {0u, 0u, 0u, 0u}, ed0[4] = {0u, 0u, 0u, 0u};
+            uint32_t wd1[4] = {0u, 0u, 0u, 0u}, ed1[4] = {0u, 0u, 0u, 0u};
+            uint32_t out0 = 0u, out1 = 0u;
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int w0 = sg0[r] - k0 * sm0[r], e0 = T[r] - sm0[r];
+                const int w1 = sg1[r] - k1 * sm1[r], e1 = T[r] - sm1[r];
+                out0 |= ((uint32_t)(w0 + 127) > 254u) | ((uint32_t)(e0 + 127) > 254u);
+                out1 |= ((uint32_t)(w1 + 127) > 254u) | ((uint32_t)(e1 + 127) > 254u);
+                wd0[r >> 2] |= ((uint32_t)w0 & 0xFFu) << (8 * (r & 3));
+                ed0[r >> 2] |= ((uint32_t)e0 & 0xFFu) << (8 * (r & 3));
+                wd1[r >> 2] |= ((uint32_t)w1 & 0xFFu) << (8 * (r & 3));
+                ed1[r >> 2] |= ((uint32_t)e1 & 0xFFu) << (8 * (r & 3));
+                const float y0 = (float)((double)ms0.y * ((double)sg0[r] - (double)ms0.x * (double)sm0[r]));
+                const float y1 = (float)((double)ms1.y * ((double)sg1[r] - (double)ms1.x * (double)sm1[r]));
+                nrm0 += (double)y0 * (double)y0;
+                nrm1 += (double)y1 * (double)y1;
+                as0 += ((double)w0 + dl0 * (double)e0) * (double)T[r];
+                as1 += ((double)w1 + dl1 * (double)e1) * (double)T[r];
+            }
+            bad = bad || (out0 && snp0 < m) || (out1 && snp1 < m);
+            uint4* W = static_cast<uint4*>(cols);
+            uint4* E = W + (size_t)m_pad * nblocks * 2;  // m_pad x d_pad bytes later
+            const size_t o0 = ((size_t)(2 * gp) * nblocks + B) * 64 + lane;
+            const size_t o1 = ((size_t)(2 * gp + 1) * nblocks + B) * 64 + lane;
+            W[o0] = make_uint4(wd0[0], wd0[1], wd0[2], wd0[3]);
+            E[o0] = make_uint4(ed0[0], ed0[1], ed0[2], ed0[3]);
+            W[o1] = make_uint4(wd1[0], wd1[1], wd1[2], wd1[3]);
+            E[o1] = make_uint4(ed1[0], ed1[1], ed1[2], ed1[3]);
         }
     }
-    if (!RAW) {
+    if (MODE != kSketchRaw) {
         nrm0 += __shfl_xor(nrm0, 32);
         nrm1 += __shfl_xor(nrm1, 32);
         if (h == 0) {
@@ -278,44 +350,97 @@ __global__ __launch_bounds__(64) void k_sketch(const uint32_t* __restrict__ tile
             norm[(size_t)c * m_pad + snp1] = nrm1;
         }
     }
+    if (MODE == kSketchI8) {
+        as0 += __shfl_xor(as0, 32);
+        as1 += __shfl_xor(as1, 32);
+        if (h == 0) {
+            asum[(size_t)c * m_pad + snp0] = as0;
+            asum[(size_t)c * m_pad + snp1] = as1;
+        }
+        if (bad) flag[0] = 1;  // a plain store: every writer stores 1
+    }
 }
 
 // Ratio-estimator renorm ||col||^2 = N (wasm_simd.rs:1774-1783): the squared
 // norm is the chunk shares added in a fixed order, and the column's scale
 // sqrt(N / ||y||^2) (f32) is applied by the contraction to each dot
 // product (sc_i sc_j <y_i, y_j>) rather than to the stored values.
+// kSketchI8 also forms, per SNP, the constants of k_band_sk8: scale = sc
+// inv (the renorm scale times the f32 inverse std), delta and A (its shares
+// added in the same fixed order).
+template <int MODE>
 __global__ __launch_bounds__(256) void k_sketch_scale(const double* __restrict__ norm,
                                                        uint32_t nchunks, uint32_t m_pad,
-                                                       double n_norm, float* __restrict__ sc) {
+                                                       double n_norm, float* __restrict__ sc,
+                                                       const float2* __restrict__ musc,
+                                                       const double* __restrict__ asum,
+                                                       double* __restrict__ consts) {
     const uint32_t snp = blockIdx.x * blockDim.x + threadIdx.x;
     if (snp >= m_pad) return;
     double nrm = 0.0;
     for (uint32_t c = 0; c < nchunks; ++c) nrm += norm[(size_t)c * m_pad + snp];
-    sc[snp] = nrm > 0.0 ? (float)sqrt(n_norm / nrm) : 1.0f;
+    const float s = nrm > 0.0 ? (float)sqrt(n_norm / nrm) : 1.0f;
+    if (MODE == kSketchF32) {
+        sc[snp] = s;
+    } else {
+        double A = 0.0;
+        for (uint32_t c = 0; c < nchunks; ++c) A += asum[(size_t)c * m_pad + snp];
+        const float2 ms = musc[snp];
+        consts[snp] = (double)s * (double)ms.y;
+        consts[m_pad + snp] = (double)ms.x - (double)rintf(ms.x);
+        consts[2 * (size_t)m_pad + snp] = A;
+    }
+}
+
+__global__ void k_sketch_clear_flag(int32_t* flag) {
+    if (threadIdx.x == 0) flag[0] = 0;
 }
 
 const float* sketch_scales(const SketchPlan* sp, const double* norm, uint64_t m_pad) {
     return reinterpret_cast<const float*>(norm + (size_t)sketch_chunks(sp) * m_pad);
 }
 
-hipError_t launch_sketch(const SketchPlan* sp, const uint32_t* tiled, uint32_t nkb,
+const double* sketch_i8_consts(const SketchPlan* sp, const double* norm, uint64_t m_pad) {
+    return norm + 2 * (size_t)sketch_chunks(sp) * m_pad;
+}
+
+int32_t* sketch_i8_flag(const SketchPlan* sp, double* norm, uint64_t m_pad) {
+    return reinterpret_cast<int32_t*>(norm + 2 * (size_t)sketch_chunks(sp) * m_pad + 3 * m_pad);
+}
+
+hipError_t launch_sketch(const SketchPlan* sp, int mode, const uint32_t* tiled, uint32_t nkb,
                          const float2* musc, uint32_t m, uint32_t m_pad, double n, double* norm,
-                         float* sk, int32_t* sg, int32_t* sm, hipStream_t st) {
+                         void* cols, int32_t* sg, int32_t* sm, hipStream_t st) {
     const uint32_t nch = sketch_chunks(sp), nb = (uint32_t)sp->nblocks;
     const dim3 grid(m_pad / 64, nch), block(64);
-    if (sg) {
-        hipLaunchKernelGGL(k_sketch<true>, grid, block, 0, st, tiled, nkb,
+    if (mode == kSketchRaw) {
+        hipLaunchKernelGGL(k_sketch<kSketchRaw>, grid, block, 0, st, tiled, nkb,
                            (const v4i*)sp->d_ptiles, sp->d_blocks, sp->d_tile_ks, nb, musc, m, m_pad,
-                           (uint32_t)sp->d, nullptr, nullptr, sg, sm);
+                           (uint32_t)sp->d, nullptr, nullptr, sg, sm, nullptr, nullptr, nullptr);
         return hipGetLastError();
     }
-    hipLaunchKernelGGL(k_sketch<false>, grid, block, 0, st, tiled, nkb, (const v4i*)sp->d_ptiles,
-                       sp->d_blocks, sp->d_tile_ks, nb, musc, m, m_pad, (uint32_t)sp->d, norm, sk, nullptr,
-                       nullptr);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_sketch_scale, dim3((m_pad + 255) / 256), dim3(256), 0, st, norm, nch,
-                       m_pad, n, const_cast<float*>(sketch_scales(sp, norm, m_pad)));
+    hipError_t e;
+    if (mode == kSketchF32) {
+        hipLaunchKernelGGL(k_sketch<kSketchF32>, grid, block, 0, st, tiled, nkb,
+                           (const v4i*)sp->d_ptiles, sp->d_blocks, sp->d_tile_ks, nb, musc, m, m_pad,
+                           (uint32_t)sp->d, norm, cols, nullptr, nullptr, nullptr, nullptr, nullptr);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+        hipLaunchKernelGGL(k_sketch_scale<kSketchF32>, dim3((m_pad + 255) / 256), dim3(256), 0, st,
+                           norm, nch, m_pad, n, const_cast<float*>(sketch_scales(sp, norm, m_pad)),
+                           nullptr, nullptr, nullptr);
+        return hipGetLastError();
+    }
+    double* asum = norm + (size_t)nch * m_pad;
+    double* consts = const_cast<double*>(sketch_i8_consts(sp, norm, m_pad));
+    int32_t* flag = sketch_i8_flag(sp, norm, m_pad);
+    hipLaunchKernelGGL(k_sketch_clear_flag, dim3(1), dim3(64), 0, st, flag);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    hipLaunchKernelGGL(k_sketch<kSketchI8>, grid, block, 0, st, tiled, nkb,
+                       (const v4i*)sp->d_ptiles, sp->d_blocks, sp->d_tile_ks, nb, musc, m, m_pad,
+                       (uint32_t)sp->d, norm, cols, nullptr, nullptr, sp->d_tsum, asum, flag);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    hipLaunchKernelGGL(k_sketch_scale<kSketchI8>, dim3((m_pad + 255) / 256), dim3(256), 0, st,
+                       norm, nch, m_pad, n, nullptr, musc, asum, consts);
     return hipGetLastError();
 }
 
@@ -473,6 +598,121 @@ hipError_t launch_band_sk(const BandArgs& a, hipStream_t st) {
     return hipGetLastError();
 }
 
+// d-length contraction of the integer-operand form over one 64x64 band tile
+// on the int8 MFMA: with a = w + delta e per bucket, the sketched dot
+// product of SNPs r and c is (quadratic_sketch_correction's val before the
+// 1/N^2 of compute.rs:947-998)
+//   v = scale_r scale_c [ W_r.W_c + d_c W_r.E_c + d_r E_r.W_c + d_r d_c E_r.E_c
+//                         - d_c A_r - d_r A_c + d_r d_c TT ],
+// four exact int32 products over the d_pad buckets (32 per K-step), finished
+// in FP64.  One wave per tile: 4 products x 2 x 2 blocks of 32x32 int32
+// accumulators (256 AGPRs); the next K-steps' operands load while the
+// current ones' MFMAs run.
+constexpr int kSk8Ahead = 3;
+
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1))) void k_band_sk8(BandArgs a) {
+    __shared__ EpiShared sh;
+    __shared__ double kr[64], kc[64], scr[64], scc[64], dlr[64], dlc[64], ar[64], ac[64];
+    const int tile = tile_of_block(a);
+    if (tile < 0) return;
+    const int2 tl = a.tiles[tile];
+    const uint32_t I = (uint32_t)tl.x, J = (uint32_t)tl.y;
+    const int lane = threadIdx.x;
+    const size_t gs = (size_t)a.nkb * 64;  // 16-byte operand words per SNP group
+    // operand streams: W and E of the row groups 2I, 2I+1 and column groups 2J, 2J+1
+    const u32x4* src[8] = {a.skw + (2 * I) * gs + lane, a.skw + (2 * I + 1) * gs + lane,
+                           a.skw + (2 * J) * gs + lane, a.skw + (2 * J + 1) * gs + lane,
+                           a.ske + (2 * I) * gs + lane, a.ske + (2 * I + 1) * gs + lane,
+                           a.ske + (2 * J) * gs + lane, a.ske + (2 * J + 1) * gs + lane};
+    v16i acc[4][2][2];
+#pragma unroll
+    for (int p = 0; p < 4; ++p)
+#pragma unroll
+        for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+            for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+                for (int r = 0; r < 16; ++r) acc[p][bi][bj][r] = 0;
+    // ring of kSk8Ahead K-steps of operands; steps past the last reload the
+    // last one (in bounds) and are not used
+    u32x4 ring[kSk8Ahead][8];
+    const uint32_t nk = a.nkb;
+#pragma unroll
+    for (int u = 0; u < kSk8Ahead; ++u) {
+        const size_t o = (size_t)(u < (int)nk ? u : nk - 1) * 64;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) ring[u][q] = src[q][o];
+    }
+    for (uint32_t kb = 0; kb < nk; kb += kSk8Ahead) {
+#pragma unroll
+        for (int u = 0; u < kSk8Ahead; ++u) {
+            if (kb + u < nk) {
+                v4i op[8];
+#pragma unroll
+                for (int q = 0; q < 8; ++q) op[q] = __builtin_bit_cast(v4i, ring[u][q]);
+                // refill this slot with step kb + u + kSk8Ahead
+                const uint32_t nxt = kb + u + kSk8Ahead;
+                const size_t o = (size_t)(nxt < nk ? nxt : nk - 1) * 64;
+#pragma unroll
+                for (int q = 0; q < 8; ++q) ring[u][q] = src[q][o];
+#pragma unroll
+                for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+                    for (int bj = 0; bj < 2; ++bj) {
+                        const v4i& Wr = op[bi];
+                        const v4i& Wc = op[2 + bj];
+                        const v4i& Er = op[4 + bi];
+                        const v4i& Ec = op[6 + bj];
+                        acc[0][bi][bj] = __builtin_amdgcn_mfma_i32_32x32x32_i8(Wr, Wc, acc[0][bi][bj], 0, 0, 0);
+                        acc[1][bi][bj] = __builtin_amdgcn_mfma_i32_32x32x32_i8(Wr, Ec, acc[1][bi][bj], 0, 0, 0);
+                        acc[2][bi][bj] = __builtin_amdgcn_mfma_i32_32x32x32_i8(Er, Wc, acc[2][bi][bj], 0, 0, 0);
+                        acc[3][bi][bj] = __builtin_amdgcn_mfma_i32_32x32x32_i8(Er, Ec, acc[3][bi][bj], 0, 0, 0);
+                    }
+            }
+        }
+    }
+
+    stage_info(a, sh, I, J);
+    const size_t ri = (size_t)I * kTile + lane, ci = (size_t)J * kTile + lane;
+    kr[lane] = a.kappa ? a.kappa[ri] : 0.0;
+    kc[lane] = a.kappa ? a.kappa[ci] : 0.0;
+    scr[lane] = a.sk8[ri];
+    scc[lane] = a.sk8[ci];
+    dlr[lane] = a.sk8[a.m_pad + ri];
+    dlc[lane] = a.sk8[a.m_pad + ci];
+    ar[lane] = a.sk8[2 * (size_t)a.m_pad + ri];
+    ac[lane] = a.sk8[2 * (size_t)a.m_pad + ci];
+    __syncthreads();
+    // quadratic_sketch_correction[_maf_aware] (compute.rs:947-998)
+    double r2u[2][2][16];
+#pragma unroll
+    for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+        for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int row = row_of(bi, r, lane), col = 32 * bj + (lane & 31);
+                const double dr = dlr[row], dc = dlc[col];
+                const double D = (double)acc[0][bi][bj][r] + dc * (double)acc[1][bi][bj][r] +
+                                 dr * (double)acc[2][bi][bj][r] +
+                                 dr * dc * ((double)acc[3][bi][bj][r] + a.sk_tt) - dc * ar[row] -
+                                 dr * ac[col];
+                const double v = D * scr[row] * scc[col];
+                const double t = v * v * a.n_inv_sq;
+                const double dm3 = a.d_f - 3.0 - 2.0 * (kr[row] + kc[col]);
+                const double disc = dm3 * dm3 + 8.0 * (a.d_f * t - 1.0);
+                const double rs = disc >= 0.0 ? (-dm3 + sqrt(disc)) * 0.25 : 0.0;
+                const double val = rs * a.r2u_a + a.r2u_b;
+                r2u[bi][bj][r] = pair_included(a, sh, I, J, row, col) ? val : 0.0;
+            }
+    band_contract(a, sh, tile, I, J, r2u);
+}
+
+hipError_t launch_band_sk8(const BandArgs& a, hipStream_t st) {
+    hipLaunchKernelGGL(k_band_sk8, dim3(8 * a.per_xcd), dim3(64), 0, st, a);
+    return hipGetLastError();
+}
+
 }  // namespace ldsc
 
 using namespace ldsc;
@@ -519,7 +759,7 @@ extern "C" int ldsc_l2_sketch_buckets(const ldsc_panel* panel, const uint64_t* b
         // musc is unused by the integer form; pass any valid per-SNP array
         SCHK(hipMemsetAsync(ws->info.p, 0, m_pad * 48, st));
         const uint32_t nkb = (uint32_t)(round_up(panel->n, kKBlock) / kKBlock);
-        SCHK(launch_sketch(sp, ws->tiled.as<uint32_t>(), nkb,
+        SCHK(launch_sketch(sp, kSketchRaw, ws->tiled.as<uint32_t>(), nkb,
                            reinterpret_cast<const float2*>(ws->info.as<char>()), (uint32_t)m,
                            (uint32_t)m_pad, (double)panel->n, nullptr, nullptr, d_sg, d_sm, st));
         SCHK(hipMemcpyAsync(sg, d_sg, m * d * 4, hipMemcpyDeviceToHost, st));

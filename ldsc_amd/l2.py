@@ -110,13 +110,14 @@ class L2Perf:
     pairs_included: int = 0
     pairs_evaluated: int = 0
     tiles: int = 0
+    sketch_int8: bool = False  # the sketch contraction ran on exact int8 operands
 
     @classmethod
     def from_c(cls, p: N.Perf, sketch: bool) -> "L2Perf":
         return cls(p.m, p.n_indiv, p.bed_read_secs, p.norm_secs,
                    p.sketch_secs if sketch else None, p.bb_dot_secs, p.ab_dot_secs,
                    p.ring_store_secs, p.total_secs, p.pairs_included, p.pairs_evaluated,
-                   p.tiles)
+                   p.tiles, bool(p.sketch_int8))
 
 
 @dataclass

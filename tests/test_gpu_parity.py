@@ -241,8 +241,13 @@ def test_l2_special_columns(oracle):
 
 @pytest.mark.parametrize("n,d,maf_aware", [(3000, 64, False), (3000, 300, False),
                                            (3000, 300, True), (3000, 1000, False),
-                                           (20000, 500, True)])
+                                           (20000, 500, True), (20000, 3, False),
+                                           (20000, 3, True)])
 def test_l2_sketch_matches_oracle(oracle, n, d, maf_aware):
+    """Sketch LD scores against the f32 sequential-scatter restatement.  The
+    contraction runs on exact int8 operands (w = SG - k SM, e = T - SM)
+    while they fit; at d = 3 over 20,000 individuals (buckets of ~6,700
+    members) they do not, and the f32 form runs instead."""
     m = 700
     p = _panel(n, m, seed=d)
     snps = L.make_snp_array(p.chr, p.cm, p.bp)
@@ -253,9 +258,10 @@ def test_l2_sketch_matches_oracle(oracle, n, d, maf_aware):
         got, maf, perf = L.compute_ldscore(panel, snps, cfg)
     exp, emaf = oracle.l2(p.bed_bytes(), n, p.chr, p.cm, p.bp, mode=0, window=0.15, chunk=60,
                           sketch=d, maf_aware=maf_aware, use_f32=True, masking=True)
-    print(f"  max rel {_rel(got, exp):.3e}", flush=True)
+    print(f"  max rel {_rel(got, exp):.3e}, int8 operands: {perf.sketch_int8}", flush=True)
     assert _rel(got, exp) < TOL_SKETCH
     assert np.array_equal(maf, emaf)
+    assert perf.sketch_int8 == (d > 3)
 
 
 def test_sketch_errors_and_unmasked(oracle):
