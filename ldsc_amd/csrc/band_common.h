@@ -3,6 +3,8 @@
 // epilogue and the deterministic per-tile annotation contraction.
 #pragma once
 
+#include <algorithm>
+
 #include "device.h"
 
 namespace ldsc {
@@ -66,6 +68,42 @@ __device__ __forceinline__ void expand_codes(uint32_t w, v4i& G, v4i& M) {
         G[f] = (int)__builtin_amdgcn_perm(0x00010002u, 0x00010002u, D);
         M[f] = (int)__builtin_amdgcn_perm(0x01010001u, 0x01010001u, D);
     }
+}
+
+// Cells of individuals >= nv (the row's real count; 0 for a padding row)
+// of dword q read as code 01 (missing).
+__device__ __forceinline__ uint32_t ts_mask(uint32_t raw, uint64_t q, uint32_t nv) {
+    const int64_t left = (int64_t)nv - (int64_t)(q * 16);
+    const uint32_t keep = left >= 16 ? 0xFFFFFFFFu
+                        : left <= 0  ? 0u
+                                     : (1u << (2 * (uint32_t)left)) - 1u;
+    return (raw & keep) | (0x55555555u & ~keep);
+}
+
+// o[k] byte r = byte k of d_r (4x4 byte transpose).
+__device__ __forceinline__ void transpose_bytes4(uint32_t d0, uint32_t d1, uint32_t d2,
+                                                 uint32_t d3, uint32_t (&o)[4]) {
+    const uint32_t lo01 = __builtin_amdgcn_perm(d1, d0, 0x05010400u);  // d0.b0 d1.b0 d0.b1 d1.b1
+    const uint32_t hi01 = __builtin_amdgcn_perm(d1, d0, 0x07030602u);  // d0.b2 d1.b2 d0.b3 d1.b3
+    const uint32_t lo23 = __builtin_amdgcn_perm(d3, d2, 0x05010400u);
+    const uint32_t hi23 = __builtin_amdgcn_perm(d3, d2, 0x07030602u);
+    o[0] = __builtin_amdgcn_perm(lo23, lo01, 0x05040100u);
+    o[1] = __builtin_amdgcn_perm(lo23, lo01, 0x07060302u);
+    o[2] = __builtin_amdgcn_perm(hi23, hi01, 0x05040100u);
+    o[3] = __builtin_amdgcn_perm(hi23, hi01, 0x07060302u);
+}
+
+// The reference's f32 mean and inverse std of a SNP from its exact integer
+// stats (normalize.rs:115-154): the same operation order, no contraction
+// into FMA.  Returns the f64 mean.
+__device__ __forceinline__ double ref_mean_inv_f32(uint32_t S, uint32_t C, uint32_t SS, uint32_t n,
+                                                   float& a32, float& i32) {
+    const double avg = C > 0 ? (double)S / (double)C : 0.0;
+    const double css = __dsub_rn((double)SS, __dmul_rn(__dmul_rn((double)C, avg), avg));
+    const double sd = sqrt(css / (double)n);
+    a32 = (float)avg;
+    i32 = sd > 0.0 ? (float)(1.0 / sd) : 0.0f;
+    return avg;
 }
 
 struct EpiShared {
@@ -294,6 +332,36 @@ int32_t* sketch_i8_flag(const SketchPlan* sp, double* norm, uint64_t m_pad);
 // The largest d_pad of the integer-operand form: the int32 sums of its
 // products stay exact (d_pad * 127^2 < 2^31).
 constexpr uint64_t kSketchI8MaxDpad = 131072;
+// The one-pass form of the bucket sums (k_sketch_fused): kSfWaves waves per
+// workgroup, each owning sketch_fused_bpw(nblocks) bucket blocks; a
+// workgroup covers kSfWaves * bpw blocks (a range) of one 32-SNP group.
+constexpr uint32_t kSfWaves = 8;
+// The one-pass form is used up to this many bucket blocks (one per wave).
+constexpr uint64_t kSfFusedMaxBlocks = 8;
+inline uint32_t sketch_fused_bpw(uint64_t nblocks) { return nblocks <= 8 ? 1 : nblocks <= 16 ? 2 : 4; }
+inline uint32_t sketch_fused_ranges(uint64_t nblocks) {
+    const uint64_t per = kSfWaves * sketch_fused_bpw(nblocks);
+    return (uint32_t)((nblocks + per - 1) / per);
+}
+// Per-SNP share slots of the norm buffer: enough for k_sketch (one per
+// kSkChunk = 4 blocks) and for k_sketch_fused (one per wave and range).
+inline uint32_t sketch_share_slots(uint64_t nblocks) {
+    return (uint32_t)std::max<uint64_t>((nblocks + 3) / 4, kSfWaves * sketch_fused_ranges(nblocks));
+}
+// One-pass bucket sums of the chunk-local order (SketchPlan::d_local) in the
+// integer-operand form, straight from the resident .bed rows: the W / E
+// tiles, norm and A shares, range flag (as launch_sketch kSketchI8 before
+// k_sketch_scale) and each SNP's integer stats in stats_out (uint4 per SNP).
+// With sg_out / sm_out it writes the exact sums SG, SM instead (d x m
+// column-major int32 each; norm and cols unused).
+hipError_t launch_sketch_fused(const SketchPlan* sp, const ldsc_panel* panel,
+                               const uint64_t* d_bed_idx, uint64_t base, uint32_t m, uint32_t m_pad,
+                               uint32_t* stats_out, double* norm, void* cols, hipStream_t st,
+                               int32_t* sg_out = nullptr, int32_t* sm_out = nullptr);
+// After k_snp_info has the f32 mean / inverse std: per-SNP constants of the
+// integer form from the shares launch_sketch_fused wrote.
+hipError_t launch_sketch_fused_scale(const SketchPlan* sp, const float2* musc, uint32_t m_pad,
+                                     double n, double* norm, hipStream_t st);
 hipError_t launch_band_sk(const BandArgs& a, hipStream_t st);
 hipError_t launch_band_sk8(const BandArgs& a, hipStream_t st);
 

@@ -169,16 +169,30 @@ constexpr uint32_t kSkMembers = 16384;
 // 3-4 puts eight consecutive lanes in eight distinct groups of four banks.
 // A bijection within aligned groups of eight units.
 __host__ __device__ inline uint32_t gc_slot(uint32_t u) { return u ^ ((u >> 3) & 3u); }
+// Where member `mem` of a chunk sits in such a staged quad: its LDS word
+// (16-byte slot gc_slot(mem / 16), word (mem / 4) % 4) times 4, plus its
+// 2-bit field within each byte (mem % 4).  Past the last member: the word
+// after the window (kSkMembers / 4), which reads as code 01 in every byte.
+__host__ __device__ inline uint16_t gc_code(uint32_t mem) {
+    return (uint16_t)((((gc_slot(mem >> 4) << 2) | ((mem >> 2) & 3u)) << 2) | (mem & 3u));
+}
+constexpr uint16_t kGcPastCode = (uint16_t)((kSkMembers / 4) << 2);
 struct SketchPlan {
     uint64_t d = 0, d_pad = 0, nblocks = 0, ntiles = 0, chunk = 0, nchunks = 0;
     int32_t* d_fam = nullptr;     // one chunk: FAM row of each position, lane-major map
-    uint16_t* d_local = nullptr;  // chunks: member of each position within its chunk
-                                  // (0xFFFF past the last member), nkb * 128 entries
+    uint16_t* d_local = nullptr;  // chunks: gc_code of the member at each position (its
+                                  // LDS word and field in the staged chunk window;
+                                  // kGcPastCode past the last member), nkb * 128 entries
     void* d_ptiles = nullptr;     // ntiles x 1 KiB signed int8 MFMA A-operand tiles
     int4* d_blocks = nullptr;     // per bucket block: first tile, number of tiles
     int32_t* d_tile_ks = nullptr; // K-step (32 positions) of each tile
     int32_t* d_tsum = nullptr;    // per bucket (d_pad, zero padded): T_b = sum of its signs
     double tt = 0.0;              // sum over buckets of T_b^2
+    // chunk-local order only: per chunk c, the chunk-local position where
+    // bucket block B starts (nchunks x (nblocks + 1)), and the first P tile of
+    // block B in chunk c (nblocks x nchunks)
+    uint32_t* d_bo = nullptr;
+    int32_t* d_bstart = nullptr;
     void release();
 };
 }  // namespace ldsc
@@ -274,6 +288,8 @@ struct L2Call {
     uint32_t nprog = 0;
     uint32_t prog_snps[kMaxProg] = {};
     hipEvent_t prog_ev[kMaxProg] = {};
+    bool sk_fused = false;        // bucket sums by the one-pass kernel (no tiled rows)
+    bool bed_contiguous = false;  // SNP rows base .. base + m - 1 (no index upload)
 };
 // Argument table of one batched band launch (device copy + pinned staging),
 // reused once the launch that read it has finished.

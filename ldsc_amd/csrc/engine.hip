@@ -418,15 +418,6 @@ constexpr int kTsPass = 8;                 // K-blocks per LDS pass (64 dwords o
 constexpr int kTsChunk = 8 * kTsPass;      // K-blocks per workgroup
 constexpr int kTsStride = 72;              // LDS row stride in dwords, 8 mod 32:
                                            // the transposing reads hit 32 banks
-// Cells of individuals >= nv (the row's real count; 0 for a padding row)
-// of dword q read as code 01 (missing).
-__device__ __forceinline__ uint32_t ts_mask(uint32_t raw, uint64_t q, uint32_t nv) {
-    const int64_t left = (int64_t)nv - (int64_t)(q * 16);
-    const uint32_t keep = left >= 16 ? 0xFFFFFFFFu
-                        : left <= 0  ? 0u
-                                     : (1u << (2 * (uint32_t)left)) - 1u;
-    return (raw & keep) | (0x55555555u & ~keep);
-}
 
 // 4x4 transpose of 2-bit cells: individual 4f+k moves from bit 8f+2k to bit
 // 8k+2f (two delta swaps).
@@ -555,18 +546,6 @@ __global__ __launch_bounds__(256) void k_tile_stats(const uint8_t* __restrict__ 
 // dword's 16 map entries already loading, and stores the four dwords into
 // the gathered rows (`n16` dwords each, coalesced across the wave).  The
 // rows' integer stats (sum, count, sum_sq, het) go to part[j] as uint4.
-__device__ __forceinline__ void transpose_bytes4(uint32_t d0, uint32_t d1, uint32_t d2,
-                                                 uint32_t d3, uint32_t (&o)[4]) {
-    // o[k] byte r = byte k of d_r
-    const uint32_t lo01 = __builtin_amdgcn_perm(d1, d0, 0x05010400u);  // d0.b0 d1.b0 d0.b1 d1.b1
-    const uint32_t hi01 = __builtin_amdgcn_perm(d1, d0, 0x07030602u);  // d0.b2 d1.b2 d0.b3 d1.b3
-    const uint32_t lo23 = __builtin_amdgcn_perm(d3, d2, 0x05010400u);
-    const uint32_t hi23 = __builtin_amdgcn_perm(d3, d2, 0x07030602u);
-    o[0] = __builtin_amdgcn_perm(lo23, lo01, 0x05040100u);
-    o[1] = __builtin_amdgcn_perm(lo23, lo01, 0x07060302u);
-    o[2] = __builtin_amdgcn_perm(hi23, hi01, 0x05040100u);
-    o[3] = __builtin_amdgcn_perm(hi23, hi01, 0x07060302u);
-}
 
 // Output forms of the gather: the gathered rows (n16 dwords each), or
 // straight into the tile layouts of k_tile_stats (PLINK-code tiles, or
@@ -837,10 +816,10 @@ __global__ __launch_bounds__(256) void k_gather_chunk(const uint8_t* __restrict_
         const uint32_t ev[8] = {e0.x, e0.y, e0.z, e0.w, e1.x, e1.y, e1.z, e1.w};
         uint32_t widx[16], shv[16];
 #pragma unroll
-        for (int e = 0; e < 16; ++e) {
-            const uint32_t src = (ev[e >> 1] >> (16 * (e & 1))) & 0xFFFFu;
-            widx[e] = src == 0xFFFFu ? kGcWords : (gc_slot(src >> 4) << 2) | ((src >> 2) & 3u);
-            shv[e] = 2 * (src & 3);
+        for (int e = 0; e < 16; ++e) {  // gc_code: LDS word * 4 + field
+            const uint32_t code = (ev[e >> 1] >> (16 * (e & 1))) & 0xFFFFu;
+            widx[e] = code >> 2;
+            shv[e] = 2 * (code & 3);
         }
         // row dword 2t + h of K-block kb -> lane s + 32h, dword t
         const uint32_t kb = q / 8, dq = q % 8;
@@ -893,12 +872,8 @@ __global__ void k_snp_info(const uint32_t* __restrict__ part, uint32_t nparts,
     const int64_t Q = (int64_t)C * (int64_t)SS - (int64_t)S * (int64_t)S;
     const int64_t V = (int64_t)C * Q;
     rs[j] = V > 0 ? 1.0 / sqrt((double)V) : 0.0;
-    // same operation order as the reference, no contraction into FMA
-    const double avg = C > 0 ? (double)S / (double)C : 0.0;
-    const double css = __dsub_rn((double)SS, __dmul_rn(__dmul_rn((double)C, avg), avg));
-    const double sd = sqrt(css / (double)n);
-    const float a32 = (float)avg;
-    const float i32 = sd > 0.0 ? (float)(1.0 / sd) : 0.0f;
+    float a32, i32;
+    const double avg = ref_mean_inv_f32(S, C, SS, n, a32, i32);
     // (0 - mean) * inv: written so that no backend fold of 0 - x into -x can
     // turn the reference's +0 (mean 0) into -0; otherwise negation is exact.
     const float w0 = a32 == 0.0f ? 0.0f : -(a32 * i32);
@@ -1618,7 +1593,7 @@ size_t l2_workspace_bytes(const ldsc_panel* panel, uint64_t m, const ldsc_l2_con
     if (gather) bytes += m * round_up((panel->n + 3) / 4, 4);    // permuted rows
     if (cfg && cfg->sketch_d) {
         const uint64_t nblocks = round_up(cfg->sketch_d, 32) / 32;
-        const uint64_t shares = m_pad * 8 * ((nblocks + 3) / 4);
+        const uint64_t shares = m_pad * 8 * sketch_share_slots(nblocks);
         bytes += m_pad * nblocks * 32 * 4;                       // sketch tiles
         bytes += shares + std::max<uint64_t>(m_pad * 4, shares + 3 * m_pad * 8 + 16);
     }
@@ -1868,6 +1843,7 @@ int l2_begin(const ldsc_panel* panel, const ldsc_snp* snps, uint64_t m, const ld
     c.t_start = t_start;
     c.m_pad = m_pad;
     c.d = d;
+    c.bed_contiguous = contiguous;
 
     SketchPlan* sp = nullptr;
     if (d) {
@@ -1911,7 +1887,23 @@ int l2_begin(const ldsc_panel* panel, const ldsc_snp* snps, uint64_t m, const ld
     c.form = d ? (sk8 ? kBandSketch8 : kBandSketch)
                : use_f32 ? kBandF32 : use_fp4 ? kBandFp4 : kBandI8;
     uint32_t parts = 1;
-    if (d && sp->d_local)
+    // chunk-local bucket order in the integer form with at most 8 bucket
+    // blocks (d <= 256): one pass from the .bed rows to the bucket sums, no
+    // tiled copy (the range fallback of l2_band tiles the rows then).  With
+    // more blocks a wave owns several and the pass measured slower than the
+    // tiled gather plus k_sketch.
+    const bool fused = sk8 && sp->d_local && sp->nblocks <= kSfFusedMaxBlocks;
+    c.sk_fused = fused;
+    if (fused) {
+        // sized for the fallback's chunk slots too, so that it never moves
+        HIPC(ws->stats.ensure((size_t)m_pad * 16 * (1 + sp->nchunks)));
+        HIPC(ws->sk_norm.ensure(sketch_norm_bytes(sp, m_pad)));
+        HIPC(launch_sketch_fused(sp, panel, contiguous ? nullptr : ws->bed_idx.as<uint64_t>(),
+                                 contiguous ? base : 0, (uint32_t)m, m_pad,
+                                 ws->stats.as<uint32_t>() + (size_t)m_pad * 4,
+                                 ws->sk_norm.as<double>(), ws->sk_cols.p, st));
+        parts = 1;
+    } else if (d && sp->d_local)
         rc = gather_tile_chunks(panel, contiguous ? nullptr : ws->bed_idx.as<uint64_t>(),
                                 contiguous ? base : 0, m, sp->d_local, (uint32_t)sp->chunk, m_pad,
                                 ws, &parts, err, err_len);
@@ -1980,11 +1972,15 @@ int l2_begin(const ldsc_panel* panel, const ldsc_snp* snps, uint64_t m, const ld
     }
     HIPC(hipEventRecord(ws->ev[1], st));
     const uint32_t nkb = (uint32_t)(round_up(n, kKBlock) / kKBlock);
-    if (d) {
+    if (fused) {
+        HIPC(launch_sketch_fused_scale(sp, d_musc, m_pad, (double)n, ws->sk_norm.as<double>(), st));
+    } else if (d) {
         HIPC(ws->sk_norm.ensure(sketch_norm_bytes(sp, m_pad)));
         HIPC(launch_sketch(sp, sk8 ? kSketchI8 : kSketchF32, ws->tiled.as<uint32_t>(), nkb,
                            d_musc, (uint32_t)m, m_pad, (double)n, ws->sk_norm.as<double>(),
                            ws->sk_cols.p, nullptr, nullptr, st));
+    }
+    if (d) {
         if (sk8) {  // the range flag comes back for l2_band's choice
             HIPC(ws->pinned_flag.ensure(16));
             HIPC(hipMemcpyAsync(ws->pinned_flag.p,
@@ -2194,6 +2190,15 @@ int l2_band(L2Call* call, char* err, uint64_t err_len) {
         if (*ws->pinned_flag.as<int32_t>()) {
             const SketchPlan* sp = c.sketch_plan;
             const uint32_t nkb = (uint32_t)(round_up(c.panel->n, kKBlock) / kKBlock);
+            if (c.sk_fused) {  // the one-pass sums left no tiled rows
+                const bool contiguous = c.bed_contiguous;
+                uint32_t parts = 0;
+                const int rc = gather_tile_chunks(
+                    c.panel, contiguous ? nullptr : ws->bed_idx.as<uint64_t>(),
+                    contiguous ? c.snps[0].bed_idx : 0, c.m, sp->d_local, (uint32_t)sp->chunk,
+                    c.m_pad, ws, &parts, err, err_len);
+                if (rc) return rc;
+            }
             const float2* d_musc =
                 reinterpret_cast<const float2*>(ws->info.as<char>() + (size_t)c.m_pad * 32);
             HIPC(launch_sketch(sp, kSketchF32, ws->tiled.as<uint32_t>(), nkb, d_musc,

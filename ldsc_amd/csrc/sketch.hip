@@ -38,7 +38,11 @@ void SketchPlan::release() {
     if (d_blocks) (void)hipFree(d_blocks);
     if (d_tile_ks) (void)hipFree(d_tile_ks);
     if (d_tsum) (void)hipFree(d_tsum);
+    if (d_bo) (void)hipFree(d_bo);
+    if (d_bstart) (void)hipFree(d_bstart);
     d_tsum = nullptr;
+    d_bo = nullptr;
+    d_bstart = nullptr;
     d_fam = nullptr;
     d_local = nullptr;
     d_ptiles = nullptr;
@@ -91,15 +95,21 @@ int sketch_plan(const ldsc_panel* panel_c, uint64_t d, SketchPlan** out, char* e
     // P tiles, in (bucket block, chunk, K-step) order
     std::vector<int4> blocks(sp->nblocks);
     std::vector<int32_t> tile_ks;
+    std::vector<int32_t> bstart(sp->nblocks * sp->nchunks);
     for (uint64_t B = 0; B < sp->nblocks; ++B) {
         blocks[B] = make_int4((int)tile_ks.size(), 0, 0, 0);
         for (uint64_t c = 0; c < sp->nchunks; ++c) {
             const uint64_t lo = bo[c][B], hi = bo[c][B + 1];
+            bstart[B * sp->nchunks + c] = (int32_t)tile_ks.size();
             if (hi > lo)
                 for (uint64_t ks = lo / 32; ks < (hi + 31) / 32; ++ks) tile_ks.push_back((int32_t)ks);
         }
         blocks[B].y = (int)tile_ks.size() - blocks[B].x;
     }
+    std::vector<uint32_t> bo_local(sp->nchunks * (sp->nblocks + 1));
+    for (uint64_t c = 0; c < sp->nchunks; ++c)
+        for (uint64_t B = 0; B <= sp->nblocks; ++B)
+            bo_local[c * (sp->nblocks + 1) + B] = (uint32_t)(bo[c][B] - c * sp->chunk);
     sp->ntiles = tile_ks.size();
     // A-operand tile of (bucket block B, K-step ks): lane l is bucket row
     // 32B + l%32; its byte p is position ks*32 + (l/32)*16 + p.
@@ -121,8 +131,9 @@ int sketch_plan(const ldsc_panel* panel_c, uint64_t d, SketchPlan** out, char* e
     std::vector<int32_t> fam;
     std::vector<uint16_t> loc;
     if (local) {
-        loc.assign(round_up(n, kKBlock), 0xFFFF);
-        for (uint64_t k = 0; k < n; ++k) loc[k] = (uint16_t)(perm[k] - (k / sp->chunk) * sp->chunk);
+        loc.assign(round_up(n, kKBlock), kGcPastCode);
+        for (uint64_t k = 0; k < n; ++k)
+            loc[k] = gc_code((uint32_t)(perm[k] - (k / sp->chunk) * sp->chunk));
     } else {
         std::vector<int32_t> fam_rows(n);
         for (uint64_t k = 0; k < n; ++k)
@@ -144,6 +155,14 @@ int sketch_plan(const ldsc_panel* panel_c, uint64_t d, SketchPlan** out, char* e
         e = hipMemcpy(sp->d_tile_ks, tile_ks.data(), tile_ks.size() * 4, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMalloc(&sp->d_ptiles, tiles.size());
     if (e == hipSuccess) e = hipMemcpy(sp->d_ptiles, tiles.data(), tiles.size(), hipMemcpyHostToDevice);
+    if (local) {
+        if (e == hipSuccess) e = hipMalloc(&sp->d_bo, bo_local.size() * 4);
+        if (e == hipSuccess)
+            e = hipMemcpy(sp->d_bo, bo_local.data(), bo_local.size() * 4, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMalloc(&sp->d_bstart, bstart.size() * 4);
+        if (e == hipSuccess)
+            e = hipMemcpy(sp->d_bstart, bstart.data(), bstart.size() * 4, hipMemcpyHostToDevice);
+    }
     if (e == hipSuccess) e = hipMalloc(&sp->d_tsum, tsum.size() * 4);
     if (e == hipSuccess)
         e = hipMemcpy(sp->d_tsum, tsum.data(), tsum.size() * 4, hipMemcpyHostToDevice);
@@ -171,12 +190,18 @@ static inline uint32_t sketch_chunks(const SketchPlan* sp) {
     return (uint32_t)((sp->nblocks + kSkChunk - 1) / kSkChunk);
 }
 
-// Layout of the norm buffer: the norm shares (chunks x m_pad doubles), then
+// Share slots of the norm buffer's layout (the stride of its regions).
+static inline uint32_t sketch_slots(const SketchPlan* sp) {
+    return sketch_share_slots(sp->nblocks);
+}
+
+// Layout of the norm buffer: the norm shares (slots x m_pad doubles), then
 // for kSketchF32 the scales (m_pad floats), for kSketchI8 the shares of A
-// (chunks x m_pad doubles), the per-SNP constants (3 x m_pad doubles) and
-// the range flag.
+// (slots x m_pad doubles), the per-SNP constants (3 x m_pad doubles) and
+// the range flag.  Each writer fills its own number of slots from the
+// front.
 size_t sketch_norm_bytes(const SketchPlan* sp, uint64_t m_pad) {
-    const size_t shares = (size_t)sketch_chunks(sp) * m_pad * 8;
+    const size_t shares = (size_t)sketch_slots(sp) * m_pad * 8;
     return shares + std::max<size_t>(m_pad * 4, shares + 3 * m_pad * 8 + 16);
 }
 
@@ -397,15 +422,15 @@ __global__ void k_sketch_clear_flag(int32_t* flag) {
 }
 
 const float* sketch_scales(const SketchPlan* sp, const double* norm, uint64_t m_pad) {
-    return reinterpret_cast<const float*>(norm + (size_t)sketch_chunks(sp) * m_pad);
+    return reinterpret_cast<const float*>(norm + (size_t)sketch_slots(sp) * m_pad);
 }
 
 const double* sketch_i8_consts(const SketchPlan* sp, const double* norm, uint64_t m_pad) {
-    return norm + 2 * (size_t)sketch_chunks(sp) * m_pad;
+    return norm + 2 * (size_t)sketch_slots(sp) * m_pad;
 }
 
 int32_t* sketch_i8_flag(const SketchPlan* sp, double* norm, uint64_t m_pad) {
-    return reinterpret_cast<int32_t*>(norm + 2 * (size_t)sketch_chunks(sp) * m_pad + 3 * m_pad);
+    return reinterpret_cast<int32_t*>(norm + 2 * (size_t)sketch_slots(sp) * m_pad + 3 * m_pad);
 }
 
 hipError_t launch_sketch(const SketchPlan* sp, int mode, const uint32_t* tiled, uint32_t nkb,
@@ -430,7 +455,7 @@ hipError_t launch_sketch(const SketchPlan* sp, int mode, const uint32_t* tiled, 
                            nullptr, nullptr, nullptr);
         return hipGetLastError();
     }
-    double* asum = norm + (size_t)nch * m_pad;
+    double* asum = norm + (size_t)sketch_slots(sp) * m_pad;
     double* consts = const_cast<double*>(sketch_i8_consts(sp, norm, m_pad));
     int32_t* flag = sketch_i8_flag(sp, norm, m_pad);
     hipLaunchKernelGGL(k_sketch_clear_flag, dim3(1), dim3(64), 0, st, flag);
@@ -441,6 +466,364 @@ hipError_t launch_sketch(const SketchPlan* sp, int mode, const uint32_t* tiled, 
     if ((e = hipGetLastError()) != hipSuccess) return e;
     hipLaunchKernelGGL(k_sketch_scale<kSketchI8>, dim3((m_pad + 255) / 256), dim3(256), 0, st,
                        norm, nch, m_pad, n, nullptr, musc, asum, consts);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------------
+// One-pass bucket sums (chunk-local bucket order, integer-operand form)
+// ------------------------------------------------------------------------
+// Workgroup (g, r) owns the 32 SNPs of group g and bucket blocks
+// [8 BPW r, 8 BPW (r+1)); its wave v owns BPW consecutive blocks of them.
+// For each member chunk c (kSkMembers FAM rows, one 4 KiB window of each
+// .bed row) the workgroup
+//  1. loads the window of its 32 rows (wave v: rows 4v .. 4v+3, one 16-byte
+//     load per lane and row, the next pass's loads in flight), counts their
+//     integer stats and stages them in LDS byte-interleaved by row quad
+//     (word k of quad Q = byte k of rows 4Q .. 4Q+3, in 16-byte slots placed
+//     by gc_slot) -- as k_gather_chunk;
+//  2. per wave, walks the K-steps (32 positions of the chunk's bucket order)
+//     that its blocks cover, four at a time.  A lane gathers 16 positions
+//     (one K-step half) of one row quad: the plan's gc_code of each position
+//     gives the member's LDS word and field, and one 4-byte read its code in
+//     the quad's four rows.  The row dwords, in the tiles' cell order, are
+//     staged; the wave then runs, for each K-step and each of its blocks the
+//     K-step reaches, v_mfma_i32_32x32x32_i8 of that block's +-1 tile with
+//     the step's G and M operands.  The first block's tiles of the four
+//     steps load before the gather.
+// The exact SG, SM of every block are complete after the last chunk; the
+// epilogue forms w, e, the norm and A shares and the range flag exactly as
+// k_sketch<kSketchI8>, with the f32 mean / inverse std from the stats the
+// workgroup counted itself (the same operations as k_snp_info).  No tiled
+// copy of the rows is written.  Workgroups are mapped XCD-aware so that the
+// ranges of one group (which stage the same windows) share an L2.
+constexpr uint32_t kSfWords = kSkMembers / 4;   // LDS words per quad and chunk
+constexpr uint32_t kSfStride = kSfWords + 4;    // word kSfWords reads as code 01
+constexpr uint32_t kSfSteps = 4;                // K-steps per gather pass
+constexpr uint32_t kSfStage = kSfSteps * 64;    // staging dwords per wave
+
+// RAW: the epilogue writes the exact sums SG, SM (d x m column-major int32,
+// sg_out / sm_out) instead of the int8 form, for ldsc_l2_sketch_buckets.
+template <int BPW, bool RAW>
+__global__ __launch_bounds__(512) void k_sketch_fused(
+    const uint8_t* __restrict__ slabs, uint64_t pitch, uint32_t src_dw, uint32_t n,
+    const uint64_t* __restrict__ bed_idx, uint64_t base, uint32_t m, uint32_t m_pad,
+    const uint16_t* __restrict__ local, uint32_t nchunks, const v4i* __restrict__ ptiles,
+    const uint32_t* __restrict__ bo, const int32_t* __restrict__ bstart, uint32_t nblocks,
+    uint32_t nranges, uint32_t ngroups, uint32_t per_xcd, const int32_t* __restrict__ tsum,
+    uint32_t* __restrict__ stats_out, double* __restrict__ norm, double* __restrict__ asum,
+    void* __restrict__ cols, int32_t* __restrict__ flag, uint32_t d, int32_t* __restrict__ sg_out,
+    int32_t* __restrict__ sm_out) {
+    extern __shared__ uint32_t lds[];  // 8 quads x kSfStride, then kSfWaves x kSfStage staging
+    __shared__ uint32_t srow[32][3];
+    const uint32_t xcd = blockIdx.x & 7u, slot = blockIdx.x >> 3;
+    const uint32_t logical = xcd * per_xcd + slot;
+    if (logical >= ngroups * nranges) return;
+    const uint32_t g = logical / nranges, r = logical % nranges;
+    const int tid = threadIdx.x, wv = tid >> 6, ln = tid & 63, h = ln >> 5;
+    constexpr uint32_t cw = kSkMembers / 16;  // window dwords per row
+    constexpr int kPasses = (int)(cw / 256);
+    uint32_t* stage = lds + 8 * kSfStride + wv * kSfStage;
+    if (ln == 0) lds[wv * kSfStride + kSfWords] = 0x55555555u;
+    // load role: rows 4wv + q (quad wv); padding rows read as missing
+    const uint32_t* rp[4];
+    uint32_t pad_or = 0;  // bit q: row 4wv + q is padding
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const uint32_t j = 32 * g + 4 * wv + q;
+        const uint64_t row = j < m ? (bed_idx ? bed_idx[j] : base + j) : 0;
+        rp[q] = reinterpret_cast<const uint32_t*>(slabs + row * pitch);
+        pad_or |= (j < m ? 0u : 1u) << q;
+    }
+    const uint32_t B0 = r * (kSfWaves * BPW) + wv * BPW;
+    v16i sg[BPW], sm[BPW];
+#pragma unroll
+    for (int k = 0; k < BPW; ++k)
+#pragma unroll
+        for (int x = 0; x < 16; ++x) sg[k][x] = sm[k][x] = 0;
+    uint32_t cnt[4][3] = {};
+    // gather role of the lane: K-step gi of a pass, row quad gQ, half gh
+    const uint32_t gi = ln >> 4, gQ = (ln >> 1) & 7, gh = ln & 1;
+    for (uint32_t c = 0; c < nchunks; ++c) {
+        __syncthreads();  // the previous chunk's gathers are done with the quads
+        // 1. stage the chunk's window of the wave's four rows.  Only the last
+        //    chunk holds positions past the last member (read as code 01).
+        const bool tail = (uint64_t)(c + 1) * kSkMembers > n;
+        auto load_pass = [&](int i, uint4 (&v)[4]) {
+            const uint64_t q0 = (uint64_t)c * cw + i * 256 + 4 * ln;
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                v[q] = q0 < src_dw ? *reinterpret_cast<const uint4*>(rp[q] + q0)
+                                   : make_uint4(0u, 0u, 0u, 0u);
+        };
+        uint4 vn[4];
+        load_pass(0, vn);
+#pragma unroll 1
+        for (int i = 0; i < kPasses; ++i) {
+            const uint32_t k0 = i * 256 + 4 * ln;  // window dword of the lane's first word
+            const uint64_t q0 = (uint64_t)c * cw + k0;
+            uint4 v[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) v[q] = vn[q];
+            if (i + 1 < kPasses) load_pass(i + 1, vn);
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                uint32_t w[4];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    uint32_t x = t == 0 ? v[q].x : t == 1 ? v[q].y : t == 2 ? v[q].z : v[q].w;
+                    if (tail) x = ts_mask(x, q0 + t, n);
+                    w[q] = (pad_or >> q) & 1u ? 0x55555555u : x;
+                    const uint32_t L = w[q] & 0x55555555u, H = (w[q] >> 1) & 0x55555555u;
+                    cnt[q][0] += __popc(~(L | H) & 0x55555555u);  // 00: dosage 2
+                    cnt[q][1] += __popc(~L & H);                   // 10: dosage 1
+                    cnt[q][2] += __popc(L & H);                    // 11: dosage 0
+                }
+                uint32_t o[4];
+                transpose_bytes4(w[0], w[1], w[2], w[3], o);
+                reinterpret_cast<uint4*>(lds + wv * kSfStride)[gc_slot(k0 + t)] =
+                    make_uint4(o[0], o[1], o[2], o[3]);
+            }
+        }
+        __syncthreads();
+        // 2. the wave's K-steps in this chunk: the union of its blocks' runs
+        uint32_t blo[BPW], bhi[BPW];
+        int32_t bts[BPW];
+        uint32_t ks_lo = 0xFFFFFFFFu, ks_hi = 0;
+#pragma unroll
+        for (int k = 0; k < BPW; ++k) {
+            const uint32_t B = B0 + k;
+            blo[k] = bhi[k] = 0;
+            bts[k] = 0;
+            if (B < nblocks) {
+                const uint32_t p0 = bo[(size_t)c * (nblocks + 1) + B];
+                const uint32_t p1 = bo[(size_t)c * (nblocks + 1) + B + 1];
+                if (p1 > p0) {
+                    blo[k] = p0 / 32;
+                    bhi[k] = (p1 + 31) / 32;
+                    bts[k] = bstart[(size_t)B * nchunks + c];
+                    ks_lo = min(ks_lo, blo[k]);
+                    ks_hi = max(ks_hi, bhi[k]);
+                }
+            }
+        }
+        const uint16_t* lc = local + (size_t)c * kSkMembers;
+        uint4 le0 = make_uint4(0u, 0u, 0u, 0u), le1 = le0;  // the pass's 16 codes
+        auto load_codes = [&](uint32_t ks0) {
+            const uint32_t ks = ks0 + gi;
+            if (ks < ks_hi) {
+                const uint4* lp = reinterpret_cast<const uint4*>(lc + 32 * ks + 16 * gh);
+                le0 = lp[0];
+                le1 = lp[1];
+            }
+        };
+        if (ks_lo < ks_hi) load_codes(ks_lo);
+        for (uint32_t ks0 = ks_lo; ks0 < ks_hi; ks0 += kSfSteps) {
+            // the +-1 tiles of the pass's K-steps for the first of the wave's
+            // blocks reaching each, loaded before the gather (further blocks
+            // reaching a K-step -- at block boundaries, or several short runs
+            // in the last, partial chunk -- load their tiles where used)
+            v4i P1[kSfSteps];
+            int k1[kSfSteps];
+#pragma unroll
+            for (int i = 0; i < (int)kSfSteps; ++i) {
+                const uint32_t ks = ks0 + i;
+                k1[i] = -1;
+#pragma unroll
+                for (int k = BPW - 1; k >= 0; --k)
+                    if (ks >= blo[k] && ks < bhi[k]) k1[i] = k;
+                if (k1[i] >= 0)
+                    P1[i] = ptiles[(size_t)(bts[k1[i]] + (int32_t)(ks - blo[k1[i]])) * 64 + ln];
+            }
+            const uint4 e0 = le0, e1 = le1;
+            if (ks0 + kSfSteps < ks_hi) load_codes(ks0 + kSfSteps);
+            // gather: lane -> (K-step ks0 + gi, row quad gQ, half gh)
+            if (ks0 + gi < ks_hi) {
+                const uint32_t ev[8] = {e0.x, e0.y, e0.z, e0.w, e1.x, e1.y, e1.z, e1.w};
+                const uint32_t* quad = lds + gQ * kSfStride;
+                uint32_t acc[4] = {0u, 0u, 0u, 0u};
+#pragma unroll
+                for (int e = 0; e < 16; ++e) {  // gc_code: LDS word * 4 + field
+                    const uint32_t code = (ev[e >> 1] >> (16 * (e & 1))) & 0xFFFFu;
+                    acc[e & 3] |= ((quad[code >> 2] >> (2 * (code & 3))) & 0x03030303u)
+                                  << (2 * (e >> 2));
+                }
+                uint32_t o[4];
+                transpose_bytes4(acc[0], acc[1], acc[2], acc[3], o);  // o[x] = row 4gQ + x
+                reinterpret_cast<uint4*>(stage)[(gi * 2 + gh) * 8 + gQ] =
+                    make_uint4(o[0], o[1], o[2], o[3]);
+            }
+            // the staging writes above are read by other lanes of this wave
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+            for (int i = 0; i < (int)kSfSteps; ++i) {
+                if (k1[i] < 0) continue;  // past the last K-step
+                const uint32_t dw = stage[(i * 2 + h) * 32 + (ln & 31)];
+                v4i G, M;
+                expand_codes(dw, G, M);
+                const uint32_t ks = ks0 + i;
+#pragma unroll
+                for (int k = 0; k < BPW; ++k) {
+                    if (k == k1[i]) {
+                        sg[k] = __builtin_amdgcn_mfma_i32_32x32x32_i8(P1[i], G, sg[k], 0, 0, 0);
+                        sm[k] = __builtin_amdgcn_mfma_i32_32x32x32_i8(P1[i], M, sm[k], 0, 0, 0);
+                    } else if (ks >= blo[k] && ks < bhi[k]) {
+                        const v4i P2 = ptiles[(size_t)(bts[k] + (int32_t)(ks - blo[k])) * 64 + ln];
+                        sg[k] = __builtin_amdgcn_mfma_i32_32x32x32_i8(P2, G, sg[k], 0, 0, 0);
+                        sm[k] = __builtin_amdgcn_mfma_i32_32x32x32_i8(P2, M, sm[k], 0, 0, 0);
+                    }
+                }
+            }
+            // the next gather rewrites the staging only after these reads
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        }
+    }
+    const uint32_t row = ln & 31, snp = 32 * g + row;
+    if (RAW) {
+#pragma unroll
+        for (int k = 0; k < BPW; ++k) {
+            const uint32_t B = B0 + k;
+            if (B >= nblocks || snp >= m) continue;
+#pragma unroll
+            for (int x = 0; x < 16; ++x) {
+                const uint32_t b = 32 * B + 8 * (x >> 2) + 4 * h + (x & 3);
+                if (b < d) {
+                    sg_out[(size_t)snp * d + b] = sg[k][x];
+                    sm_out[(size_t)snp * d + b] = sm[k][x];
+                }
+            }
+        }
+        return;
+    }
+    // stats of the 32 rows over every chunk: wave wv holds rows 4wv .. 4wv+3
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            uint32_t x = cnt[q][k];
+#pragma unroll
+            for (int o = 32; o >= 1; o >>= 1) x += __shfl_xor(x, o);
+            cnt[q][k] = x;
+        }
+    if (ln < 4) {
+        uint32_t n2 = 0, n1 = 0, n0 = 0;
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            if (q == ln) {
+                n2 = cnt[q][0];
+                n1 = cnt[q][1];
+                n0 = cnt[q][2];
+            }
+        srow[4 * wv + ln][0] = n2;
+        srow[4 * wv + ln][1] = n1;
+        srow[4 * wv + ln][2] = n0;
+    }
+    __syncthreads();
+    const uint32_t n2 = srow[row][0], n1 = srow[row][1], n0 = srow[row][2];
+    const uint32_t S = 2 * n2 + n1, C = n2 + n1 + n0, SS = 4 * n2 + n1;
+    if (r == 0 && wv == 0 && ln < 32)
+        reinterpret_cast<uint4*>(stats_out)[snp] = make_uint4(S, C, SS, n1);
+    float a32, i32;
+    (void)ref_mean_inv_f32(S, C, SS, n, a32, i32);
+    const float kf = rintf(a32);
+    const int kk = (int)kf;
+    const double dl = (double)a32 - (double)kf;
+    double nrm = 0.0, as = 0.0;
+    bool bad = false;
+    uint4* W = static_cast<uint4*>(cols);
+    uint4* E = W + (size_t)m_pad * nblocks * 2;  // m_pad x d_pad bytes later
+#pragma unroll
+    for (int k = 0; k < BPW; ++k) {
+        const uint32_t B = B0 + k;
+        if (B >= nblocks) continue;
+        int T[16];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int4 t4 = *reinterpret_cast<const int4*>(tsum + 32 * B + 8 * q + 4 * h);
+            T[4 * q] = t4.x;
+            T[4 * q + 1] = t4.y;
+            T[4 * q + 2] = t4.z;
+            T[4 * q + 3] = t4.w;
+        }
+        uint32_t wd[4] = {0u, 0u, 0u, 0u}, ed[4] = {0u, 0u, 0u, 0u};
+        uint32_t out = 0u;
+#pragma unroll
+        for (int x = 0; x < 16; ++x) {
+            const int w = sg[k][x] - kk * sm[k][x], e = T[x] - sm[k][x];
+            out |= ((uint32_t)(w + 127) > 254u) | ((uint32_t)(e + 127) > 254u);
+            wd[x >> 2] |= ((uint32_t)w & 0xFFu) << (8 * (x & 3));
+            ed[x >> 2] |= ((uint32_t)e & 0xFFu) << (8 * (x & 3));
+            const float y = (float)((double)i32 * ((double)sg[k][x] - (double)a32 * (double)sm[k][x]));
+            nrm += (double)y * (double)y;
+            as += ((double)w + dl * (double)e) * (double)T[x];
+        }
+        bad = bad || (out && snp < m);
+        const size_t o = ((size_t)g * nblocks + B) * 64 + ln;
+        W[o] = make_uint4(wd[0], wd[1], wd[2], wd[3]);
+        E[o] = make_uint4(ed[0], ed[1], ed[2], ed[3]);
+    }
+    nrm += __shfl_xor(nrm, 32);
+    as += __shfl_xor(as, 32);
+    if (h == 0) {
+        const size_t sl = (size_t)(r * kSfWaves + wv) * m_pad + snp;
+        norm[sl] = nrm;
+        asum[sl] = as;
+    }
+    if (bad) flag[0] = 1;  // a plain store: every writer stores 1
+}
+
+hipError_t launch_sketch_fused(const SketchPlan* sp, const ldsc_panel* panel,
+                               const uint64_t* d_bed_idx, uint64_t base, uint32_t m, uint32_t m_pad,
+                               uint32_t* stats_out, double* norm, void* cols, hipStream_t st,
+                               int32_t* sg_out, int32_t* sm_out) {
+    const bool raw = sg_out != nullptr;
+    const uint32_t nb = (uint32_t)sp->nblocks, bpw = sketch_fused_bpw(nb);
+    const uint32_t nranges = sketch_fused_ranges(nb), ngroups = m_pad / 32;
+    const uint32_t per_xcd = (ngroups * nranges + 7) / 8;
+    double* asum = raw ? nullptr : norm + (size_t)sketch_slots(sp) * m_pad;
+    int32_t* flag = raw ? nullptr : sketch_i8_flag(sp, norm, m_pad);
+    hipError_t e;
+    if (!raw) {
+        hipLaunchKernelGGL(k_sketch_clear_flag, dim3(1), dim3(64), 0, st, flag);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+    }
+    const size_t lds = (size_t)(8 * kSfStride + kSfWaves * kSfStage) * 4;
+    static const void* const fns[2][3] = {
+        {(const void*)k_sketch_fused<1, false>, (const void*)k_sketch_fused<2, false>,
+         (const void*)k_sketch_fused<4, false>},
+        {(const void*)k_sketch_fused<1, true>, (const void*)k_sketch_fused<2, true>,
+         (const void*)k_sketch_fused<4, true>}};
+    const void* kfn = fns[raw ? 1 : 0][bpw == 1 ? 0 : bpw == 2 ? 1 : 2];
+    e = hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+    const uint8_t* slabs = panel->d_slabs;
+    uint64_t pitch = panel->stride, a_base = base;
+    uint32_t src_dw = (uint32_t)((panel->bps + 3) / 4), n = (uint32_t)panel->n;
+    const uint16_t* local = sp->d_local;
+    uint32_t nch = (uint32_t)sp->nchunks, a_m = m, a_mp = m_pad;
+    const v4i* pt = (const v4i*)sp->d_ptiles;
+    const uint32_t* bo = sp->d_bo;
+    const int32_t* bst = sp->d_bstart;
+    const int32_t* ts = sp->d_tsum;
+    uint32_t a_nb = nb, a_nr = nranges, a_ng = ngroups, a_px = per_xcd, a_d = (uint32_t)sp->d;
+    void* kargs[] = {(void*)&slabs, (void*)&pitch, (void*)&src_dw, (void*)&n, (void*)&d_bed_idx,
+                     (void*)&a_base, (void*)&a_m, (void*)&a_mp, (void*)&local, (void*)&nch,
+                     (void*)&pt, (void*)&bo, (void*)&bst, (void*)&a_nb, (void*)&a_nr, (void*)&a_ng,
+                     (void*)&a_px, (void*)&ts, (void*)&stats_out, (void*)&norm, (void*)&asum,
+                     (void*)&cols, (void*)&flag, (void*)&a_d, (void*)&sg_out, (void*)&sm_out};
+    return hipLaunchKernel(kfn, dim3(8 * per_xcd), dim3(64 * kSfWaves), kargs, lds, st);
+}
+
+hipError_t launch_sketch_fused_scale(const SketchPlan* sp, const float2* musc, uint32_t m_pad,
+                                     double n, double* norm, hipStream_t st) {
+    const uint32_t slots = kSfWaves * sketch_fused_ranges(sp->nblocks);
+    double* asum = norm + (size_t)sketch_slots(sp) * m_pad;
+    double* consts = const_cast<double*>(sketch_i8_consts(sp, norm, m_pad));
+    hipLaunchKernelGGL(k_sketch_scale<kSketchI8>, dim3((m_pad + 255) / 256), dim3(256), 0, st,
+                       norm, slots, m_pad, n, nullptr, musc, asum, consts);
     return hipGetLastError();
 }
 
@@ -747,6 +1130,21 @@ extern "C" int ldsc_l2_sketch_buckets(const ldsc_panel* panel, const uint64_t* b
         SCHK(ws->info.ensure(m_pad * 48));
         SCHK(ws->sk_cols.ensure(m * d * 8));
         SCHK(hipMemcpyAsync(ws->bed_idx.p, bed_idx, m * 8, hipMemcpyHostToDevice, st));
+        int32_t* d_sg = ws->sk_cols.as<int32_t>();
+        int32_t* d_sm = d_sg + m * d;
+        if (sp->d_local && sp->d_pad <= kSketchI8MaxDpad) {
+            // (every block count here, so that the kernel's forms with several
+            // blocks per wave stay checked bit for bit)
+            // chunk-local order: the one-pass kernel's exact sums
+            SCHK(ws->stats.ensure(m_pad * 16));
+            SCHK(launch_sketch_fused(sp, panel, ws->bed_idx.as<uint64_t>(), 0, (uint32_t)m,
+                                     (uint32_t)m_pad, ws->stats.as<uint32_t>(), nullptr, nullptr,
+                                     st, d_sg, d_sm));
+            SCHK(hipMemcpyAsync(sg, d_sg, m * d * 4, hipMemcpyDeviceToHost, st));
+            SCHK(hipMemcpyAsync(sm, d_sm, m * d * 4, hipMemcpyDeviceToHost, st));
+            SCHK(hipStreamSynchronize(st));
+            break;
+        }
         uint32_t parts = 0;
         rc = sp->d_local ? gather_tile_chunks(panel, ws->bed_idx.as<uint64_t>(), 0, m,
                                               sp->d_local, (uint32_t)sp->chunk, m_pad, ws,
@@ -754,8 +1152,6 @@ extern "C" int ldsc_l2_sketch_buckets(const ldsc_panel* panel, const uint64_t* b
                          : gather_and_tile(panel, ws->bed_idx.as<uint64_t>(), 0, m, sp->d_fam,
                                            m_pad, false, /*tile=*/true, ws, &parts, err, err_len);
         if (rc) break;
-        int32_t* d_sg = ws->sk_cols.as<int32_t>();
-        int32_t* d_sm = d_sg + m * d;
         // musc is unused by the integer form; pass any valid per-SNP array
         SCHK(hipMemsetAsync(ws->info.p, 0, m_pad * 48, st));
         const uint32_t nkb = (uint32_t)(round_up(panel->n, kKBlock) / kKBlock);
