@@ -933,91 +933,6 @@ __global__ __launch_bounds__(256) void k_aeff(double* __restrict__ out, const do
     }
 }
 
-// Epilogue storage of the two-wave band kernel.  One annotation: each wave
-// stages one 16-row quarter of its 32 x 64 r2u block at a time.  Several:
-// the whole tile's r2u for the FP64 MFMA contraction.  ~35 KiB per
-// workgroup, so four workgroups (two waves per SIMD) fit in a CU's LDS.
-struct EpiShared2 {
-    union {
-        struct {
-            double R[2][16][65];
-            double ar[64];    // A_eff of the tile's row SNPs
-            double ac[64];    // A_eff of the tile's column SNPs
-            double cx[64];    // wave 1's column partials
-        } one;
-        double Rf[64][65];
-    } u;
-    int2 csr[64], csc[64];
-    double rsr[64], rsc[64];
-    uint32_t lor[64];
-};
-
-// One annotation: r2u of wave w (rows 32w .. 32w+31 of the tile, all 64
-// columns, in MFMA output order r2u[bj][r]) -> the tile's partial sums,
-// fixed summation order.
-__device__ __forceinline__ void band_contract2_one(const BandArgs& a, EpiShared2& sh, int tile,
-                                                   uint32_t I, uint32_t J, int w,
-                                                   const double (&r2u)[2][16]) {
-    const int lane = threadIdx.x & 63;
-    __syncthreads();
-    if (w == 0)
-        sh.u.one.ar[lane] = a.aeff[(size_t)I * kTile + lane];
-    else
-        sh.u.one.ac[lane] = a.aeff[(size_t)J * kTile + lane];
-    double cacc = 0.0;
-#pragma unroll
-    for (int Q = 0; Q < 2; ++Q) {
-        __syncthreads();
-#pragma unroll
-        for (int bj = 0; bj < 2; ++bj)
-#pragma unroll
-            for (int i = 0; i < 8; ++i) {
-                const int r = 8 * Q + i;
-                const int local = 8 * ((r >> 2) - 2 * Q) + 4 * (lane >> 5) + (r & 3);
-                sh.u.one.R[w][local][32 * bj + (lane & 31)] = r2u[bj][r];
-            }
-        __syncthreads();
-        // row role: row lane%16 of the quarter, columns 16*(lane/16) .. +16
-        double racc = 0.0;
-        const int rr = lane & 15, b0 = 16 * (lane >> 4);
-        for (int b = 0; b < 16; ++b) racc += sh.u.one.R[w][rr][b0 + b] * sh.u.one.ac[b0 + b];
-        racc += __shfl_xor(racc, 16);
-        racc += __shfl_xor(racc, 32);
-        if (lane < 16) a.rowp[(size_t)tile * kTile + 32 * w + 16 * Q + lane] = racc;
-        // column role: lane = column, the 16 rows of this quarter
-        for (int r = 0; r < 16; ++r)
-            cacc += sh.u.one.R[w][r][lane] * sh.u.one.ar[32 * w + 16 * Q + r];
-    }
-    __syncthreads();
-    if (w == 1) sh.u.one.cx[lane] = cacc;
-    __syncthreads();
-    if (w == 0) a.colp[(size_t)tile * kTile + lane] = cacc + sh.u.one.cx[lane];
-}
-
-__device__ __forceinline__ void band_contract2(const BandArgs& a, EpiShared2& sh, int tile,
-                                               uint32_t I, uint32_t J, int w,
-                                               const double (&r2u)[2][16]) {
-    if (a.K == 1) {
-        band_contract2_one(a, sh, tile, I, J, w, r2u);
-        return;
-    }
-    // several annotations: both waves stage their rows, then wave 0 forms
-    // the row partials and wave 1 the column partials
-    const int lane = threadIdx.x & 63;
-    __syncthreads();
-#pragma unroll
-    for (int bj = 0; bj < 2; ++bj)
-#pragma unroll
-        for (int r = 0; r < 16; ++r)
-            sh.u.Rf[32 * w + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3)][32 * bj + (lane & 31)] =
-                r2u[bj][r];
-    __syncthreads();
-    if (w == 0)
-        band_contract_mfma<true, false>(a, sh.u.Rf, tile, I, J);
-    else
-        band_contract_mfma<false, true>(a, sh.u.Rf, tile, I, J);
-}
-
 __device__ __forceinline__ void finish_2w(const BandArgs& a, EpiShared2& sh, int tile, uint32_t I,
                                           uint32_t J, int w, int lane, const v16i (&acc)[4][2]);
 

@@ -988,111 +988,106 @@ hipError_t launch_band_sk(const BandArgs& a, hipStream_t st) {
 //   v = scale_r scale_c [ W_r.W_c + d_c W_r.E_c + d_r E_r.W_c + d_r d_c E_r.E_c
 //                         - d_c A_r - d_r A_c + d_r d_c TT ],
 // four exact int32 products over the d_pad buckets (32 per K-step), finished
-// in FP64.  One wave per tile: 4 products x 2 x 2 blocks of 32x32 int32
-// accumulators (256 AGPRs); the next K-steps' operands load while the
-// current ones' MFMAs run.
+// in FP64.  The next K-steps' operands load while the current ones' MFMAs
+// run.
 constexpr int kSk8Ahead = 3;
 
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1))) void k_band_sk8(BandArgs a) {
-    __shared__ EpiShared sh;
+// Two waves per tile: wave w owns rows 32w .. 32w+31 (SNP group 2I + w)
+// against both column groups, 4 products x 2 blocks of int32 accumulators
+// (128 registers), so two waves fit per SIMD and one tile's FP64 epilogue
+// overlaps another's MFMAs (measured faster than one wave per tile with all
+// four blocks at d = 200, 1000 and 5000).
+__global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_band_sk8_2w(BandArgs a) {
+    __shared__ EpiShared2 sh;
     __shared__ double kr[64], kc[64], scr[64], scc[64], dlr[64], dlc[64], ar[64], ac[64];
     const int tile = tile_of_block(a);
     if (tile < 0) return;
     const int2 tl = a.tiles[tile];
     const uint32_t I = (uint32_t)tl.x, J = (uint32_t)tl.y;
-    const int lane = threadIdx.x;
-    const size_t gs = (size_t)a.nkb * 64;  // 16-byte operand words per SNP group
-    // operand streams: W and E of the row groups 2I, 2I+1 and column groups 2J, 2J+1
-    const u32x4* src[8] = {a.skw + (2 * I) * gs + lane, a.skw + (2 * I + 1) * gs + lane,
-                           a.skw + (2 * J) * gs + lane, a.skw + (2 * J + 1) * gs + lane,
-                           a.ske + (2 * I) * gs + lane, a.ske + (2 * I + 1) * gs + lane,
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const size_t gs = (size_t)a.nkb * 64;
+    // operand streams: W and E of the wave's row group and both column groups
+    const u32x4* src[6] = {a.skw + (2 * I + w) * gs + lane, a.skw + (2 * J) * gs + lane,
+                           a.skw + (2 * J + 1) * gs + lane, a.ske + (2 * I + w) * gs + lane,
                            a.ske + (2 * J) * gs + lane, a.ske + (2 * J + 1) * gs + lane};
-    v16i acc[4][2][2];
+    v16i acc[4][2];
 #pragma unroll
     for (int p = 0; p < 4; ++p)
 #pragma unroll
-        for (int bi = 0; bi < 2; ++bi)
+        for (int bj = 0; bj < 2; ++bj)
 #pragma unroll
-            for (int bj = 0; bj < 2; ++bj)
-#pragma unroll
-                for (int r = 0; r < 16; ++r) acc[p][bi][bj][r] = 0;
-    // ring of kSk8Ahead K-steps of operands; steps past the last reload the
-    // last one (in bounds) and are not used
-    u32x4 ring[kSk8Ahead][8];
+            for (int r = 0; r < 16; ++r) acc[p][bj][r] = 0;
+    u32x4 ring[kSk8Ahead][6];
     const uint32_t nk = a.nkb;
 #pragma unroll
     for (int u = 0; u < kSk8Ahead; ++u) {
         const size_t o = (size_t)(u < (int)nk ? u : nk - 1) * 64;
 #pragma unroll
-        for (int q = 0; q < 8; ++q) ring[u][q] = src[q][o];
+        for (int q = 0; q < 6; ++q) ring[u][q] = src[q][o];
     }
     for (uint32_t kb = 0; kb < nk; kb += kSk8Ahead) {
 #pragma unroll
         for (int u = 0; u < kSk8Ahead; ++u) {
             if (kb + u < nk) {
-                v4i op[8];
+                v4i op[6];
 #pragma unroll
-                for (int q = 0; q < 8; ++q) op[q] = __builtin_bit_cast(v4i, ring[u][q]);
-                // refill this slot with step kb + u + kSk8Ahead
+                for (int q = 0; q < 6; ++q) op[q] = __builtin_bit_cast(v4i, ring[u][q]);
                 const uint32_t nxt = kb + u + kSk8Ahead;
                 const size_t o = (size_t)(nxt < nk ? nxt : nk - 1) * 64;
 #pragma unroll
-                for (int q = 0; q < 8; ++q) ring[u][q] = src[q][o];
+                for (int q = 0; q < 6; ++q) ring[u][q] = src[q][o];
 #pragma unroll
-                for (int bi = 0; bi < 2; ++bi)
-#pragma unroll
-                    for (int bj = 0; bj < 2; ++bj) {
-                        const v4i& Wr = op[bi];
-                        const v4i& Wc = op[2 + bj];
-                        const v4i& Er = op[4 + bi];
-                        const v4i& Ec = op[6 + bj];
-                        acc[0][bi][bj] = __builtin_amdgcn_mfma_i32_32x32x32_i8(Wr, Wc, acc[0][bi][bj], 0, 0, 0);
-                        acc[1][bi][bj] = __builtin_amdgcn_mfma_i32_32x32x32_i8(Wr, Ec, acc[1][bi][bj], 0, 0, 0);
-                        acc[2][bi][bj] = __builtin_amdgcn_mfma_i32_32x32x32_i8(Er, Wc, acc[2][bi][bj], 0, 0, 0);
-                        acc[3][bi][bj] = __builtin_amdgcn_mfma_i32_32x32x32_i8(Er, Ec, acc[3][bi][bj], 0, 0, 0);
-                    }
+                for (int bj = 0; bj < 2; ++bj) {
+                    acc[0][bj] = __builtin_amdgcn_mfma_i32_32x32x32_i8(op[0], op[1 + bj], acc[0][bj], 0, 0, 0);
+                    acc[1][bj] = __builtin_amdgcn_mfma_i32_32x32x32_i8(op[0], op[4 + bj], acc[1][bj], 0, 0, 0);
+                    acc[2][bj] = __builtin_amdgcn_mfma_i32_32x32x32_i8(op[3], op[1 + bj], acc[2][bj], 0, 0, 0);
+                    acc[3][bj] = __builtin_amdgcn_mfma_i32_32x32x32_i8(op[3], op[4 + bj], acc[3][bj], 0, 0, 0);
+                }
             }
         }
     }
-
-    stage_info(a, sh, I, J);
-    const size_t ri = (size_t)I * kTile + lane, ci = (size_t)J * kTile + lane;
-    kr[lane] = a.kappa ? a.kappa[ri] : 0.0;
-    kc[lane] = a.kappa ? a.kappa[ci] : 0.0;
-    scr[lane] = a.sk8[ri];
-    scc[lane] = a.sk8[ci];
-    dlr[lane] = a.sk8[a.m_pad + ri];
-    dlc[lane] = a.sk8[a.m_pad + ci];
-    ar[lane] = a.sk8[2 * (size_t)a.m_pad + ri];
-    ac[lane] = a.sk8[2 * (size_t)a.m_pad + ci];
+    // per-SNP constants: wave 0 the tile's rows, wave 1 its columns
+    if (w == 0) {
+        const size_t ri = (size_t)I * kTile + lane;
+        sh.lor[lane] = a.lo[ri];
+        kr[lane] = a.kappa ? a.kappa[ri] : 0.0;
+        scr[lane] = a.sk8[ri];
+        dlr[lane] = a.sk8[a.m_pad + ri];
+        ar[lane] = a.sk8[2 * (size_t)a.m_pad + ri];
+    } else {
+        const size_t ci = (size_t)J * kTile + lane;
+        kc[lane] = a.kappa ? a.kappa[ci] : 0.0;
+        scc[lane] = a.sk8[ci];
+        dlc[lane] = a.sk8[a.m_pad + ci];
+        ac[lane] = a.sk8[2 * (size_t)a.m_pad + ci];
+    }
     __syncthreads();
     // quadratic_sketch_correction[_maf_aware] (compute.rs:947-998)
-    double r2u[2][2][16];
+    double r2u[2][16];
 #pragma unroll
-    for (int bi = 0; bi < 2; ++bi)
+    for (int bj = 0; bj < 2; ++bj)
 #pragma unroll
-        for (int bj = 0; bj < 2; ++bj)
-#pragma unroll
-            for (int r = 0; r < 16; ++r) {
-                const int row = row_of(bi, r, lane), col = 32 * bj + (lane & 31);
-                const double dr = dlr[row], dc = dlc[col];
-                const double D = (double)acc[0][bi][bj][r] + dc * (double)acc[1][bi][bj][r] +
-                                 dr * (double)acc[2][bi][bj][r] +
-                                 dr * dc * ((double)acc[3][bi][bj][r] + a.sk_tt) - dc * ar[row] -
-                                 dr * ac[col];
-                const double v = D * scr[row] * scc[col];
-                const double t = v * v * a.n_inv_sq;
-                const double dm3 = a.d_f - 3.0 - 2.0 * (kr[row] + kc[col]);
-                const double disc = dm3 * dm3 + 8.0 * (a.d_f * t - 1.0);
-                const double rs = disc >= 0.0 ? (-dm3 + sqrt(disc)) * 0.25 : 0.0;
-                const double val = rs * a.r2u_a + a.r2u_b;
-                r2u[bi][bj][r] = pair_included(a, sh, I, J, row, col) ? val : 0.0;
-            }
-    band_contract(a, sh, tile, I, J, r2u);
+        for (int r = 0; r < 16; ++r) {
+            const int row = 32 * w + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+            const int col = 32 * bj + (lane & 31);
+            const double dr = dlr[row], dc = dlc[col];
+            const double D = (double)acc[0][bj][r] + dc * (double)acc[1][bj][r] +
+                             dr * (double)acc[2][bj][r] +
+                             dr * dc * ((double)acc[3][bj][r] + a.sk_tt) - dc * ar[row] - dr * ac[col];
+            const double v = D * scr[row] * scc[col];
+            const double t = v * v * a.n_inv_sq;
+            const double dm3 = a.d_f - 3.0 - 2.0 * (kr[row] + kc[col]);
+            const double disc = dm3 * dm3 + 8.0 * (a.d_f * t - 1.0);
+            const double rs = disc >= 0.0 ? (-dm3 + sqrt(disc)) * 0.25 : 0.0;
+            const double val = rs * a.r2u_a + a.r2u_b;
+            const uint32_t jj = I * kTile + row, ii = J * kTile + col;
+            r2u[bj][r] = (ii < jj && jj < a.m && ii >= sh.lor[row]) ? val : 0.0;
+        }
+    band_contract2(a, sh, tile, I, J, w, r2u);
 }
 
 hipError_t launch_band_sk8(const BandArgs& a, hipStream_t st) {
-    hipLaunchKernelGGL(k_band_sk8, dim3(8 * a.per_xcd), dim3(64), 0, st, a);
+    hipLaunchKernelGGL(k_band_sk8_2w, dim3(8 * a.per_xcd), dim3(128), 0, st, a);
     return hipGetLastError();
 }
 
