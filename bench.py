@@ -468,19 +468,20 @@ SKETCH_PROFILE = "profiles/sketch_100k_x_200k.md"
 PART_PROFILE = "profiles/partitioned_20k_x_500k.md"
 BF16_PEAK_TFLOPS = 2500.0  # MI355X dense bf16 MFMA, TFLOP/s
 # the d-length contraction of the sketch path
-SKETCH_CONTRACTION = {"kernel": "k_band_sk", "peak": BF16_PEAK_TFLOPS,
-                      "note": "algorithmic 2*d*P f32-accuracy flops per launch; the kernel runs "
-                              "them on the bf16 MFMA as six part products of a three-part "
-                              "split of each f32 value; peak = dense bf16"}
+SKETCH_CONTRACTION = {"kernel": "k_band_sk8_2w", "peak": INT8_PEAK_TOPS,
+                      "note": "algorithmic 2*d*P ops per launch (one d-long dot product per "
+                              "included pair); the kernel runs four exact int8 products (W.W, "
+                              "W.E, E.W, E.E) per pair, so 25 % of the dense int8 peak is its "
+                              "ceiling in these units"}
 
 
 def leg_sketch(args, L, synth):
     """C3: CountSketch LD scores at N=100,000 x M=200,000 for d = 200, 1000,
-    5000.  The preparation (tiling / bucket-order gather and the exact
-    integer bucket sums, HBM-bound) and the d-length contraction (MFMA) are
-    timed with HIP events inside the call (L2Perf); algorithmic bytes of the
-    preparation: M*ceil(N/4) .bed bytes + 8N bucket/sign bytes + 4*M*d
-    sketched values; of the contraction, 2*d*P flops."""
+    5000.  The preparation (decode, bucket-order gather and the exact integer
+    bucket sums) and the d-length contraction (MFMA) are timed with HIP
+    events inside the call (L2Perf); algorithmic bytes of the preparation:
+    M*ceil(N/4) .bed bytes read + 8N bucket/sign bytes + 2*M*d bytes of the
+    int8 bucket values (W and E) written; of the contraction, 2*d*P ops."""
     n, m = 100_000, args.sketch_m
     p = synth.make_panel(n, m, 20261206, miss_rate=0.01)
     snps = L.make_snp_array(p.chr, p.cm, p.bp)
@@ -497,7 +498,7 @@ def leg_sketch(args, L, synth):
             t = float(np.mean(w))
             prep_s = float(np.mean([q.norm_secs + (q.sketch_secs or 0.0) for q in pf]))
             band_s = float(np.mean([q.bb_dot_secs for q in pf]))
-            bytes_sk = m * ((n + 3) // 4) + 8 * n + 4 * m * d
+            bytes_sk = m * ((n + 3) // 4) + 8 * n + 2 * m * d
             out["d"][str(d)] = {
                 "snp_pairs_per_s": P / t, "ms_per_step": t * 1e3,
                 "prep_ms": prep_s * 1e3, "contraction_ms": band_s * 1e3,
