@@ -6,23 +6,30 @@
 // and the quadratic bias inversion (compute.rs:947-998).
 //
 // MI355X form:
-//  * Individuals are put in bucket order (stable within a bucket) by the
-//    LDS-staged row gather, then tiled like the exact path.  A bucket's
-//    members are then one contiguous run of the K axis.
+//  * Individuals are put in bucket order (stable within a bucket) within
+//    chunks of kSkMembers members.  A bucket's members are then one
+//    contiguous run of each chunk's positions.
 //  * The signed bucket sums are a product with a sparse +-1 matrix P.  For
 //    each block of 32 buckets only the K-steps covering its members are
 //    visited; P's non-zero 32x32 tiles are precomputed as int8 MFMA
 //    A-operands.  v_mfma_i32_32x32x32_i8 then gives exact integers
 //      SG[b, j] = sum_{i in b} s_i g_ij,  SM[b, j] = sum_{i in b} s_i [present].
-//  * Bucket value y_b = inv_j (SG - mean_j SM) with the reference's f32 mean
-//    and inverse std (the same real number its f32 LUT scatter accumulates),
-//    stored f32 in the lane order the contraction's MFMA reads.  The
-//    renormalization to ||y||^2 = N is a per-SNP scale sc_j.
-//  * The d-length contraction runs over 64x64 SNP tiles of the band on the
-//    bf16 MFMA, each f32 value split into three bf16 parts (f32-level
-//    products); each dot product is scaled by sc_i sc_j, then the quadratic
-//    inversion, the unbiased r^2, masking and the annotation contraction
-//    shared with the exact path follow.
+//    For d <= 256 (kSfFusedMaxBlocks blocks) with chunk-local order this is
+//    one pass over the .bed windows (k_sketch_fused: LDS-staged rows, gather
+//    by precomputed word codes, MFMA); otherwise the rows are first tiled in
+//    bucket order (k_gather_chunk) and k_sketch reads the tiles.
+//  * The bucket value y_b = inv_j (SG - mean_j SM) (the reference's f32 mean
+//    and inverse std: the same real number its f32 LUT scatter accumulates)
+//    is kept as two int8 integers, w = SG - k SM and e = T_b - SM with
+//    k = rint(mean) (kSketchI8); a value outside int8 switches the call to
+//    the f32 form (kSketchF32: f32 bucket values, bf16 three-part split
+//    contraction k_band_sk).  The renormalization to ||y||^2 = N is a
+//    per-SNP scale sc_j.
+//  * The d-length contraction runs over 64x64 SNP tiles of the band as four
+//    exact int8 products per pair (k_band_sk8_2w), finished in FP64; each
+//    dot product is scaled by sc_i sc_j, then the quadratic inversion, the
+//    unbiased r^2, masking and the annotation contraction shared with the
+//    exact path follow.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
