@@ -7,15 +7,17 @@
 //                     stats (sum, count, sum_sq, het) -- HBM-bound.
 //   2. k_snp_info     per-SNP constants of the finish: exact-integer (C, S,
 //                     1/sqrt(C*Q)) or the reference's f32 standardization LUT.
-//   3. k_band_i8 / k_band_f32
-//                     one wave per 64x64 tile of the SNP band.  int8 form:
-//                     four exact integer products G.G, G.M, M.G, M.M on
-//                     v_mfma_i32_32x32x32_i8 with int32 accumulation, FP64
-//                     finish of the mean-imputed covariance and r^2.  f32
-//                     form: standardized columns on v_mfma_f32_32x32x2f32.
-//                     Then the unbiased r^2, the pair bound mask and the
-//                     r2u . A_eff contractions for both members of each pair,
-//                     written as per-tile partial sums (no atomics).
+//   3. k_band_fp4 / k_band_i8 / k_band_f32_2w
+//                     the 64x64 tiles of the SNP band.  Exact form: four
+//                     exact integer products G.G, G.M, M.G, M.M on the e2m1
+//                     MFMA (one wave per tile) or v_mfma_i32_32x32x32_i8
+//                     (two waves per tile), FP64 finish of the mean-imputed
+//                     covariance and r^2.  f32 form: the reference's
+//                     standardized values on v_mfma_f32_32x32x2f32 (two
+//                     waves per tile).  Then the unbiased r^2, the pair
+//                     bound mask and the r2u . A_eff contractions for both
+//                     members of each pair, written as per-tile partial sums
+//                     (no atomics).
 //   4. k_reduce       per-SNP sum of its tile partials in a fixed order plus
 //                     the self term -> deterministic l2.
 #include <algorithm>
@@ -1289,84 +1291,134 @@ void k_band_fp4_batch(BandBatch b) {
     if (batch_tile(b, r, tile)) band_fp4_tile(b.args[r], sh, tile);
 }
 
-__device__ __forceinline__ void band_f32_tile(const BandArgs& a, EpiShared& sh, int tile) {
-    const int2 tl = a.tiles[tile];
-    const uint32_t I = (uint32_t)tl.x, J = (uint32_t)tl.y;
-    const int lane = threadIdx.x;
-    const size_t gstride = (size_t)a.nkb * 64;
-    const u32x4* pa0 = a.tiled + (size_t)(2 * I) * gstride + lane;
-    const u32x4* pa1 = pa0 + gstride;
-    const u32x4* pb0 = a.tiled + (size_t)(2 * J) * gstride + lane;
-    const u32x4* pb1 = pb0 + gstride;
-    const float4 la0 = a.lut[(size_t)I * kTile + (lane & 31)];
-    const float4 la1 = a.lut[(size_t)I * kTile + 32 + (lane & 31)];
-    const float4 lb0 = a.lut[(size_t)J * kTile + (lane & 31)];
-    const float4 lb1 = a.lut[(size_t)J * kTile + 32 + (lane & 31)];
+// Standardized f32 value of 2-bit code field `sh` of w from the SNP's LUT
+// [(2-mu)inv, 0, (1-mu)inv, (0-mu)inv] as bit selects: m0, m1 are the
+// code's bits sign-extended to whole words (v_bfe_i32) and each select is
+// (m & x) | (~m & y) (v_bfi_b32): five VALU instructions, no compare.
+// `z` is zero, opaque to the compiler, so that the field offsets stay
+// operands of v_bfe_i32 rather than being folded into shifts whose
+// sign-splat the compiler would turn back into compares and selects.
+// Bit-identical to pick_code.
+__device__ __forceinline__ float pick_code_bits(const float4& L, uint32_t w, uint32_t sh,
+                                                uint32_t z) {
+    const uint32_t m0 = (uint32_t)__builtin_amdgcn_sbfe((int)w, sh + z, 1u);
+    const uint32_t m1 = (uint32_t)__builtin_amdgcn_sbfe((int)w, sh + 1u + z, 1u);
+    const uint32_t lo = (m0 & __float_as_uint(L.y)) | (~m0 & __float_as_uint(L.x));
+    const uint32_t hi = (m0 & __float_as_uint(L.w)) | (~m0 & __float_as_uint(L.z));
+    return __uint_as_float((m1 & hi) | (~m1 & lo));
+}
 
-    v16f acc[2][2];
-#pragma unroll
-    for (int bi = 0; bi < 2; ++bi)
-#pragma unroll
-        for (int bj = 0; bj < 2; ++bj)
-#pragma unroll
-            for (int r = 0; r < 16; ++r) acc[bi][bj][r] = 0.0f;
-
-    u32x4 ca0 = pa0[0], ca1 = pa1[0], cb0 = pb0[0], cb1 = pb1[0];
-    for (uint32_t kb = 0; kb < a.nkb; ++kb) {
-        const size_t nk = (size_t)(kb + 1 < a.nkb ? kb + 1 : kb) * 64;
-        const u32x4 na0 = pa0[nk], na1 = pa1[nk], nb0 = pb0[nk], nb1 = pb1[nk];
+// K loop of one wave of the two-wave FP32 kernel: its row group against the
+// column groups MASK names (bit bj: columns 32bj ..), one
+// v_mfma_f32_32x32x2f32 per column group and pair of individuals.
+template <int MASK>
+__device__ __forceinline__ void f32_2w_kloop(const u32x4* pa, const u32x4* pb0, const u32x4* pb1,
+                                             const float4& la, const float4& lb0,
+                                             const float4& lb1, uint32_t nkb, v16f (&acc)[2]) {
+    constexpr bool kB0 = (MASK & 1) != 0, kB1 = (MASK & 2) != 0;
+    const u32x4 z = {0u, 0u, 0u, 0u};
+    uint32_t zo = 0;
+    asm("" : "+s"(zo));  // zero that the compiler cannot see through
+    u32x4 ca = pa[0], cb0 = kB0 ? pb0[0] : z, cb1 = kB1 ? pb1[0] : z;
+    for (uint32_t kb = 0; kb < nkb; ++kb) {
+        const size_t nk = (size_t)(kb + 1 < nkb ? kb + 1 : kb) * 64;
+        const u32x4 na = pa[nk], nb0 = kB0 ? pb0[nk] : z, nb1 = kB1 ? pb1[nk] : z;
         __builtin_amdgcn_sched_barrier(0);  // keep the prefetch a block ahead
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
-            const uint32_t wa0 = ca0[t], wa1 = ca1[t], wb0 = cb0[t], wb1 = cb1[t];
 #pragma unroll
             for (int s = 0; s < 16; ++s) {
-                const int sh_bits = 8 * (s & 3) + 2 * (s >> 2);
-                const float fa0 = pick_code(la0, (wa0 >> sh_bits) & 3u);
-                const float fa1 = pick_code(la1, (wa1 >> sh_bits) & 3u);
-                const float fb0 = pick_code(lb0, (wb0 >> sh_bits) & 3u);
-                const float fb1 = pick_code(lb1, (wb1 >> sh_bits) & 3u);
-                acc[0][0] = __builtin_amdgcn_mfma_f32_32x32x2f32(fa0, fb0, acc[0][0], 0, 0, 0);
-                acc[0][1] = __builtin_amdgcn_mfma_f32_32x32x2f32(fa0, fb1, acc[0][1], 0, 0, 0);
-                acc[1][0] = __builtin_amdgcn_mfma_f32_32x32x2f32(fa1, fb0, acc[1][0], 0, 0, 0);
-                acc[1][1] = __builtin_amdgcn_mfma_f32_32x32x2f32(fa1, fb1, acc[1][1], 0, 0, 0);
+                const uint32_t sh_bits = 8 * (s & 3) + 2 * (s >> 2);
+                const float fa = pick_code_bits(la, ca[t], sh_bits, zo);
+                if (kB0) {
+                    const float fb = pick_code_bits(lb0, cb0[t], sh_bits, zo);
+                    acc[0] = __builtin_amdgcn_mfma_f32_32x32x2f32(fa, fb, acc[0], 0, 0, 0);
+                }
+                if (kB1) {
+                    const float fb = pick_code_bits(lb1, cb1[t], sh_bits, zo);
+                    acc[1] = __builtin_amdgcn_mfma_f32_32x32x2f32(fa, fb, acc[1], 0, 0, 0);
+                }
             }
         }
-        ca0 = na0;
-        ca1 = na1;
+        ca = na;
         cb0 = nb0;
         cb1 = nb1;
     }
+}
 
-    stage_info(a, sh, I, J);
+// FP32 form, two waves per 64x64 tile: wave w owns rows 32w .. 32w+31 (SNP
+// group 2I + w) against both column groups (2 blocks of 32x32 f32
+// accumulators), so two waves share a SIMD and one wave's code selects and
+// FP64 epilogue run beside the other's MFMAs.  Each wave skips a 32x32
+// block of its rows that cannot hold an included pair.  Each dot product
+// is the same sequence of MFMAs over the individuals as one wave per tile
+// would run.
+__device__ __forceinline__ void band_f32_2w_tile(const BandArgs& a, EpiShared2& sh, int tile) {
+    const int2 tl = a.tiles[tile];
+    const uint32_t I = (uint32_t)tl.x, J = (uint32_t)tl.y;
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const size_t gstride = (size_t)a.nkb * 64;
+    const u32x4* pa = a.tiled + (size_t)(2 * I + w) * gstride + lane;
+    const u32x4* pb0 = a.tiled + (size_t)(2 * J) * gstride + lane;
+    const u32x4* pb1 = pb0 + gstride;
+    const uint32_t r0 = I * kTile + 32 * w, c0 = J * kTile;
+    const float4 la = a.lut[r0 + (lane & 31)];
+    const float4 lb0 = a.lut[c0 + (lane & 31)];
+    const float4 lb1 = a.lut[c0 + 32 + (lane & 31)];
+
+    uint32_t lmin = a.lo[r0 + (lane & 31)];
+#pragma unroll
+    for (int o = 16; o >= 1; o >>= 1) lmin = min(lmin, (uint32_t)__shfl_xor((int)lmin, o));
+    lmin = __builtin_amdgcn_readlane(lmin, 0);
+    int mask = 0;
+#pragma unroll
+    for (int bj = 0; bj < 2; ++bj) {
+        const uint32_t cmin = c0 + 32 * bj;
+        if (cmin + 31 >= lmin && cmin < r0 + 31) mask |= 1 << bj;
+    }
+
+    v16f acc[2];
+#pragma unroll
+    for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) acc[bj][r] = 0.0f;
+    if (mask == 3)
+        f32_2w_kloop<3>(pa, pb0, pb1, la, lb0, lb1, a.nkb, acc);
+    else if (mask == 1)
+        f32_2w_kloop<1>(pa, pb0, pb1, la, lb0, lb1, a.nkb, acc);
+    else if (mask == 2)
+        f32_2w_kloop<2>(pa, pb0, pb1, la, lb0, lb1, a.nkb, acc);
+
+    if (w == 0) sh.lor[lane] = a.lo[(size_t)I * kTile + lane];
     __syncthreads();
     // compute.rs:2064-2077: val (f32 GEMM output) -> f64, val^2 * n_inv^2 * r2u_a + r2u_b
-    double r2u[2][2][16];
+    double r2u[2][16];
 #pragma unroll
-    for (int bi = 0; bi < 2; ++bi)
+    for (int bj = 0; bj < 2; ++bj)
 #pragma unroll
-        for (int bj = 0; bj < 2; ++bj)
-#pragma unroll
-            for (int r = 0; r < 16; ++r) {
-                const int row = row_of(bi, r, lane), col = 32 * bj + (lane & 31);
-                const double v = (double)acc[bi][bj][r];
-                const double val = __dadd_rn(__dmul_rn(__dmul_rn(v, v), a.n_inv_sq_r2u_a), a.r2u_b);
-                r2u[bi][bj][r] = pair_included(a, sh, I, J, row, col) ? val : 0.0;
-            }
-    band_contract(a, sh, tile, I, J, r2u);
+        for (int r = 0; r < 16; ++r) {
+            const int row = 32 * w + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+            const int col = 32 * bj + (lane & 31);
+            const double v = (double)acc[bj][r];
+            const double val = __dadd_rn(__dmul_rn(__dmul_rn(v, v), a.n_inv_sq_r2u_a), a.r2u_b);
+            const uint32_t jj = I * kTile + row, ii = J * kTile + col;
+            r2u[bj][r] = (ii < jj && jj < a.m && ii >= sh.lor[row]) ? val : 0.0;
+        }
+    band_contract2(a, sh, tile, I, J, w, r2u);
 }
 
-__global__ __launch_bounds__(64) void k_band_f32(BandArgs a) {
-    __shared__ EpiShared sh;
+__global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_band_f32_2w(BandArgs a) {
+    __shared__ EpiShared2 sh;
     const int tile = tile_of_block(a);
-    if (tile >= 0) band_f32_tile(a, sh, tile);
+    if (tile >= 0) band_f32_2w_tile(a, sh, tile);
 }
 
-__global__ __launch_bounds__(64) void k_band_f32_batch(BandBatch b) {
-    __shared__ EpiShared sh;
+__global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2, 2)))
+void k_band_f32_2w_batch(BandBatch b) {
+    __shared__ EpiShared2 sh;
     uint32_t r;
     int tile;
-    if (batch_tile(b, r, tile)) band_f32_tile(b.args[r], sh, tile);
+    if (batch_tile(b, r, tile)) band_f32_2w_tile(b.args[r], sh, tile);
 }
 
 // ------------------------------------------------------------------------
@@ -2085,7 +2137,7 @@ static hipError_t launch_band_form(int form, const BandArgs& ba, hipStream_t st)
     if (form == kBandSketch) return launch_band_sk(ba, st);
     if (form == kBandSketch8) return launch_band_sk8(ba, st);
     if (form == kBandF32)
-        hipLaunchKernelGGL(k_band_f32, dim3(8 * ba.per_xcd), dim3(64), 0, st, ba);
+        hipLaunchKernelGGL(k_band_f32_2w, dim3(8 * ba.per_xcd), dim3(128), 0, st, ba);
     else if (form == kBandFp4)
         hipLaunchKernelGGL(k_band_fp4, dim3(8 * ba.per_xcd), dim3(64), 0, st, ba);
     else
@@ -2257,7 +2309,7 @@ int band_batch_launch(L2Call* const* calls, uint32_t ncalls, BandBatchSlot* slot
                         bst));
     if (total > 0) {
         if (form == kBandF32)
-            hipLaunchKernelGGL(k_band_f32_batch, dim3(8 * b.per_xcd), dim3(64), 0, bst, b);
+            hipLaunchKernelGGL(k_band_f32_2w_batch, dim3(8 * b.per_xcd), dim3(128), 0, bst, b);
         else if (form == kBandFp4)
             hipLaunchKernelGGL(k_band_fp4_batch, dim3(8 * b.per_xcd), dim3(64), 0, bst, b);
         else if (form == kBandI8)
