@@ -236,47 +236,72 @@ def leg_panel(args, L, synth):
     if not args.skip_cpu:
         from oracle import oracle as O
 
+        # four slices of cpu_sample_snps / 4 SNPs spread over the panel (its
+        # start, quarter, middle and three quarters), each timed and checked
         ms = min(args.cpu_sample_snps, p.m)
-        bed_s = np.concatenate([p.buf[:3], p.body[:ms].reshape(-1)])
+        nsl = 4
+        per = max(ms // nsl, 1)
+        starts = [int(q * (p.m - per) / (nsl - 1)) for q in range(nsl)]
+        bed = p.bed_bytes()
 
-        def cpu_runs(use_f32):
+        def cpu_runs(sl, use_f32):
+            idx = np.arange(sl.start, sl.stop, dtype=np.uint64)
             kw = dict(mode=O.WINDOW_CM, window=1.0, chunk=200, masking=True, use_f32=use_f32,
                       threads=args.cpu_threads)
-            O.l2(bed_s, p.n, p.chr[:ms], p.cm[:ms], p.bp[:ms], **kw)  # warmup
+            O.l2(bed, p.n, p.chr[sl], p.cm[sl], p.bp[sl], idx, **kw)  # warmup
             walls, out = [], None
             for _ in range(3):
                 t = time.perf_counter()
-                out, _ = O.l2(bed_s, p.n, p.chr[:ms], p.cm[:ms], p.bp[:ms], **kw)
+                out, _ = O.l2(bed, p.n, p.chr[sl], p.cm[sl], p.bp[sl], idx, **kw)
                 walls.append(time.perf_counter() - t)
             return float(np.median(walls)), out
 
-        cpu_wall, exp = cpu_runs(False)
-        cpu_wall32, exp32 = cpu_runs(True)
-        with L.Panel(bed_s, p.n, ms) as ps:
-            got, _, perf_s = L.compute_ldscore(ps, snps[:ms], cfg8)
-            got32, _, _ = L.compute_ldscore(ps, snps[:ms], cfg32)
+        slices, acc = [], {"exact": [], "exact_rms": [], "f32_abs": [], "f32_rel": [],
+                           "f32_vs32": []}
+        panel = L.Panel(bed, p.n, p.m)
+        for st in starts:
+            sl = slice(st, st + per)
+            w64, exp = cpu_runs(sl, False)
+            w32, exp32 = cpu_runs(sl, True)
+            got, _, perf_s = L.compute_ldscore(panel, snps[sl], cfg8)
+            got32, _, _ = L.compute_ldscore(panel, snps[sl], cfg32)
+            slices.append({"first_snp": st, "snps": per, "pairs": perf_s.pairs_included,
+                           "f64_wall_s": w64, "f32_wall_s": w32,
+                           "f64_pairs_per_s": perf_s.pairs_included / w64,
+                           "f32_pairs_per_s": perf_s.pairs_included / w32})
+            acc["exact"].append(float(np.max(np.abs(got - exp) / np.abs(exp))))
+            acc["exact_rms"].append(float(np.sqrt(np.mean(((got - exp) / exp) ** 2))))
+            acc["f32_abs"].append(float(np.max(np.abs(got32 - exp))))
+            acc["f32_rel"].append(float(np.max(np.abs(got32 - exp) / np.abs(exp))))
+            acc["f32_vs32"].append(float(np.max(np.abs(got32 - exp32) / np.abs(exp32))))
+        panel.close()
         res["sample_accuracy"] = {
-            "exact_vs_fp64_max_rel": float(np.max(np.abs(got - exp) / np.abs(exp))),
-            "exact_vs_fp64_rms_rel": float(np.sqrt(np.mean(((got - exp) / exp) ** 2))),
-            "fp32_vs_fp64_max_abs": float(np.max(np.abs(got32 - exp))),
-            "fp32_vs_fp64_max_rel": float(np.max(np.abs(got32 - exp) / np.abs(exp))),
-            "fp32_vs_f32_restatement_max_rel": float(np.max(np.abs(got32 - exp32) /
-                                                            np.abs(exp32))),
+            "exact_vs_fp64_max_rel": max(acc["exact"]),
+            "exact_vs_fp64_rms_rel": max(acc["exact_rms"]),
+            "fp32_vs_fp64_max_abs": max(acc["f32_abs"]),
+            "fp32_vs_fp64_max_rel": max(acc["f32_rel"]),
+            "fp32_vs_f32_restatement_max_rel": max(acc["f32_vs32"]),
         }
         model = cpu_model()
-        P_s = perf_s.pairs_included
+        P_s = sum(x["pairs"] for x in slices)
+        cpu_wall = sum(x["f64_wall_s"] for x in slices)
+        cpu_wall32 = sum(x["f32_wall_s"] for x in slices)
+        rates = [x["f64_pairs_per_s"] for x in slices]
         res["cpu_baseline"] = {
             "value": P_s / cpu_wall, "unit": "SNP-pairs/s",
             "cores": args.cpu_threads, "kind": "port",
-            "sample": f"first {ms} SNPs of the same panel (N={p.n}, cM 1, masked), {P_s} pairs; "
-                      f"C restatement of the reference's f64 path, median of 3 runs after 1 "
-                      f"warmup: {cpu_wall:.2f}s",
+            "sample": f"{nsl} slices of {per} SNPs at SNPs {starts} of the same panel "
+                      f"(N={p.n}, cM 1, masked), {P_s} pairs in all; C restatement of the "
+                      f"reference's f64 path, per slice the median of 3 runs after 1 warmup, "
+                      f"{cpu_wall:.2f}s summed; slice rates {min(rates):.3e}-{max(rates):.3e}",
             "fast_f32_value": P_s / cpu_wall32,
-            "fast_f32_sample": f"same slice, f32 restatement (--fast-f32), median {cpu_wall32:.2f}s",
+            "fast_f32_sample": f"same slices, f32 restatement (--fast-f32), {cpu_wall32:.2f}s",
+            "slices": slices,
             "host": {"cpu_model": model, "nproc": os.cpu_count()},
         }
         log(f"cpu baseline {res['cpu_baseline']['value']:.3e} pairs/s (f32 "
             f"{res['cpu_baseline']['fast_f32_value']:.3e}); {res['sample_accuracy']}")
+        bed_s = np.concatenate([p.buf[:3], p.body[:ms].reshape(-1)])
         res["pair_accuracy"] = pair_accuracy(args, L, O, p, snps, ms, bed_s, cfg8, cfg32)
     return res
 
@@ -723,7 +748,7 @@ def main() -> int:
     ap.add_argument("--m", type=int, default=100_000)
     ap.add_argument("--fp32-steps", type=int, default=2)
     ap.add_argument("--i8-steps", type=int, default=3)
-    ap.add_argument("--cpu-sample-snps", type=int, default=4000)
+    ap.add_argument("--cpu-sample-snps", type=int, default=8000)
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-serving", action="store_true")

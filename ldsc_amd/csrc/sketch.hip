@@ -999,11 +999,39 @@ hipError_t launch_band_sk(const BandArgs& a, hipStream_t st) {
 // run.
 constexpr int kSk8Ahead = 3;
 
+// Square root of x > 0 in FP64 for arguments far from the subnormal and
+// overflow ranges (here the discriminant of the bias inversion, of the
+// order of d^2): v_rsq_f64 refined by the coupled Newton-Raphson iteration
+// for sqrt(x) and 1/(2 sqrt(x)) (Markstein's method: one step on both, then
+// two corrections of g from the residual x - g^2), so the result is within
+// an ulp of the rounded root.  The library sqrt adds range scaling and
+// special-case tests this argument does not need.
+__device__ __forceinline__ double sqrt_pos(double x) {
+    const double y = __builtin_amdgcn_rsq(x);
+    double g = x * y, h = 0.5 * y;
+    const double r = __fma_rn(-h, g, 0.5);
+    g = __fma_rn(g, r, g);
+    h = __fma_rn(h, r, h);
+    double dd = __fma_rn(-g, g, x);
+    g = __fma_rn(dd, h, g);
+    dd = __fma_rn(-g, g, x);
+    return __fma_rn(dd, h, g);
+}
+
 // Two waves per tile: wave w owns rows 32w .. 32w+31 (SNP group 2I + w)
 // against both column groups, 4 products x 2 blocks of int32 accumulators
 // (128 registers), so two waves fit per SIMD and one tile's FP64 epilogue
 // overlaps another's MFMAs (measured faster than one wave per tile with all
 // four blocks at d = 200, 1000 and 5000).
+//
+// Epilogue per pair, the inversion of quadratic_sketch_correction
+// (compute.rs:947-998) rearranged to fewer FP64 operations: with
+// u = D scale_r scale_c sqrt(8 d / N^2), the discriminant is
+// dm3^2 + 8 (d t - 1) = u^2 + (dm3^2 - 8), and
+// r2u = rs r2u_a + r2u_b = sqrt(disc) (r2u_a / 4) + (r2u_b - dm3 r2u_a / 4)
+// (r2u_b when disc < 0).  Without the MAF-aware kurtosis, dm3 = d - 3 and
+// both constants are per launch.
+template <bool MAF>
 __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_band_sk8_2w(BandArgs a) {
     __shared__ EpiShared2 sh;
     __shared__ double kr[64], kc[64], scr[64], scc[64], dlr[64], dlc[64], ar[64], ac[64];
@@ -1053,23 +1081,27 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
             }
         }
     }
-    // per-SNP constants: wave 0 the tile's rows, wave 1 its columns
+    // per-SNP constants: wave 0 the tile's rows (scale times sqrt(8 d / N^2)),
+    // wave 1 its columns
     if (w == 0) {
         const size_t ri = (size_t)I * kTile + lane;
         sh.lor[lane] = a.lo[ri];
-        kr[lane] = a.kappa ? a.kappa[ri] : 0.0;
-        scr[lane] = a.sk8[ri];
+        if (MAF) kr[lane] = a.kappa[ri];
+        scr[lane] = a.sk8[ri] * sqrt(8.0 * a.d_f * a.n_inv_sq);
         dlr[lane] = a.sk8[a.m_pad + ri];
         ar[lane] = a.sk8[2 * (size_t)a.m_pad + ri];
     } else {
         const size_t ci = (size_t)J * kTile + lane;
-        kc[lane] = a.kappa ? a.kappa[ci] : 0.0;
+        if (MAF) kc[lane] = a.kappa[ci];
         scc[lane] = a.sk8[ci];
         dlc[lane] = a.sk8[a.m_pad + ci];
         ac[lane] = a.sk8[2 * (size_t)a.m_pad + ci];
     }
     __syncthreads();
-    // quadratic_sketch_correction[_maf_aware] (compute.rs:947-998)
+    const double qa = 0.25 * a.r2u_a;
+    const double dm3_0 = a.d_f - 3.0;
+    const double c0_0 = __fma_rn(dm3_0, dm3_0, -8.0);
+    const double bb_0 = __fma_rn(-dm3_0, qa, a.r2u_b);
     double r2u[2][16];
 #pragma unroll
     for (int bj = 0; bj < 2; ++bj)
@@ -1081,20 +1113,29 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
             const double D = (double)acc[0][bj][r] + dc * (double)acc[1][bj][r] +
                              dr * (double)acc[2][bj][r] +
                              dr * dc * ((double)acc[3][bj][r] + a.sk_tt) - dc * ar[row] - dr * ac[col];
-            const double v = D * scr[row] * scc[col];
-            const double t = v * v * a.n_inv_sq;
-            const double dm3 = a.d_f - 3.0 - 2.0 * (kr[row] + kc[col]);
-            const double disc = dm3 * dm3 + 8.0 * (a.d_f * t - 1.0);
-            const double rs = disc >= 0.0 ? (-dm3 + sqrt(disc)) * 0.25 : 0.0;
-            const double val = rs * a.r2u_a + a.r2u_b;
+            const double u = D * scr[row] * scc[col];
+            double c0 = c0_0, bb = bb_0;
+            if (MAF) {
+                const double dm3 = a.d_f - 3.0 - 2.0 * (kr[row] + kc[col]);
+                c0 = __fma_rn(dm3, dm3, -8.0);
+                bb = __fma_rn(-dm3, qa, a.r2u_b);
+            }
+            const double disc = __fma_rn(u, u, c0);
+            const double sq = disc > 0.0 ? sqrt_pos(disc) : 0.0;
+            const double val = disc >= 0.0 ? __fma_rn(sq, qa, bb) : a.r2u_b;
             const uint32_t jj = I * kTile + row, ii = J * kTile + col;
             r2u[bj][r] = (ii < jj && jj < a.m && ii >= sh.lor[row]) ? val : 0.0;
+            // with the kurtosis, pair by pair: hoisted reads would spill
+            if (MAF) __builtin_amdgcn_sched_barrier(0);
         }
     band_contract2(a, sh, tile, I, J, w, r2u);
 }
 
 hipError_t launch_band_sk8(const BandArgs& a, hipStream_t st) {
-    hipLaunchKernelGGL(k_band_sk8_2w, dim3(8 * a.per_xcd), dim3(128), 0, st, a);
+    if (a.kappa)
+        hipLaunchKernelGGL(k_band_sk8_2w<true>, dim3(8 * a.per_xcd), dim3(128), 0, st, a);
+    else
+        hipLaunchKernelGGL(k_band_sk8_2w<false>, dim3(8 * a.per_xcd), dim3(128), 0, st, a);
     return hipGetLastError();
 }
 
