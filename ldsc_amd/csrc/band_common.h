@@ -13,6 +13,11 @@ typedef int v4i __attribute__((ext_vector_type(4)));
 typedef int v8i __attribute__((ext_vector_type(8)));
 typedef int v16i __attribute__((ext_vector_type(16)));
 typedef float v16f __attribute__((ext_vector_type(16)));
+
+// v_mfma_scale_f32_32x32x64_f8f6f4 with e2m1 (FP4) operands and scale bytes
+// 127 (E8M0 2^0) on both: a plain e2m1 product with f32 accumulation
+#define LDSC_MFMA_FP4(A, B, C) \
+    __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(A, B, C, 4, 4, 0, 0x7f7f7f7f, 0, 0x7f7f7f7f)
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 
 // ------------------------------------------------------------------------
@@ -428,13 +433,17 @@ inline uint32_t sketch_fused_ranges(uint64_t nblocks) {
     const uint64_t per = kSfWaves * sketch_fused_bpw(nblocks);
     return (uint32_t)((nblocks + per - 1) / per);
 }
+// The dense one-pass form (k_sketch_dense): natural individual order, used
+// up to this many bucket blocks (d <= 128).
+constexpr uint64_t kSdMaxBlocks = 4;
 // Per-SNP share slots of the norm buffer: enough for k_sketch (one per
 // kSkChunk = 4 blocks) and for k_sketch_fused (one per wave and range).
 inline uint32_t sketch_share_slots(uint64_t nblocks) {
     return (uint32_t)std::max<uint64_t>((nblocks + 3) / 4, kSfWaves * sketch_fused_ranges(nblocks));
 }
-// One-pass bucket sums of the chunk-local order (SketchPlan::d_local) in the
-// integer-operand form, straight from the resident .bed rows: the W / E
+// One-pass bucket sums in the integer-operand form, straight from the
+// resident .bed rows (natural order with SketchPlan::d_pcode, k_sketch_dense;
+// else the chunk-local order of SketchPlan::d_local, k_sketch_fused): the W / E
 // tiles, norm and A shares, range flag (as launch_sketch kSketchI8 before
 // k_sketch_scale) and each SNP's integer stats in stats_out (uint4 per SNP).
 // With sg_out / sm_out it writes the exact sums SG, SM instead (d x m

@@ -193,6 +193,11 @@ struct SketchPlan {
     // block B in chunk c (nblocks x nchunks)
     uint32_t* d_bo = nullptr;
     int32_t* d_bstart = nullptr;
+    // natural order with at most kSdMaxBlocks bucket blocks and no --keep:
+    // per individual, its entry in the P operand of its pair of K-steps
+    // (lane | word << 6 | K-step << 8 | block << 9 | nibble << 12 |
+    // negative << 15), for k_sketch_dense
+    uint16_t* d_pcode = nullptr;
     void release();
 };
 }  // namespace ldsc

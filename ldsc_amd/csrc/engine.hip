@@ -1068,9 +1068,6 @@ __device__ __forceinline__ void vcode_operands(uint32_t w0, uint32_t w1, v8i& X,
     Q = __builtin_shufflevector(q, q, 0, 1, 2, 3, -1, -1, -1, -1);
 }
 
-// scale bytes 127 (E8M0 2^0) on both operands: a plain e2m1 product
-#define LDSC_MFMA_FP4(A, B, C) \
-    __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(A, B, C, 4, 4, 0, 0x7f7f7f7f, 0, 0x7f7f7f7f)
 
 // Sub-blocks of a 64x64 tile: bit 2*bi + bj stands for rows 32bi.. x
 // columns 32bj.. .  VcodeBlock holds one K-block (two K-steps of 64
@@ -1856,9 +1853,10 @@ int l2_begin(const ldsc_panel* panel, const ldsc_snp* snps, uint64_t m, const ld
     uint32_t parts = 1;
     // chunk-local bucket order in the integer form with at most 8 bucket
     // blocks (d <= 256): one pass from the .bed rows to the bucket sums, no
-    // tiled copy (the range fallback of l2_band tiles the rows then).  With
-    // more blocks a wave owns several and the pass measured slower than the
-    // tiled gather plus k_sketch.
+    // tiled copy (the range fallback of l2_band tiles the rows then); up to
+    // 4 blocks in natural order (k_sketch_dense), else in bucket order
+    // (k_sketch_fused).  With more blocks a wave owns several and the pass
+    // measured slower than the tiled gather plus k_sketch.
     const bool fused = sk8 && sp->d_local && sp->nblocks <= kSfFusedMaxBlocks;
     c.sk_fused = fused;
     if (fused) {

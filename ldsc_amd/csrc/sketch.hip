@@ -17,7 +17,10 @@
 //    For d <= 256 (kSfFusedMaxBlocks blocks) with chunk-local order this is
 //    one pass over the .bed windows (k_sketch_fused: LDS-staged rows, gather
 //    by precomputed word codes, MFMA); otherwise the rows are first tiled in
-//    bucket order (k_gather_chunk) and k_sketch reads the tiles.
+//    bucket order (k_gather_chunk) and k_sketch reads the tiles.  For
+//    d <= 128 the one pass skips the bucket order altogether
+//    (k_sketch_dense: natural order, the +-1 bucket matrix built in LDS, the
+//    e2m1 MFMA over every block).
 //  * The bucket value y_b = inv_j (SG - mean_j SM) (the reference's f32 mean
 //    and inverse std: the same real number its f32 LUT scatter accumulates)
 //    is kept as two int8 integers, w = SG - k SM and e = T_b - SM with
@@ -47,6 +50,8 @@ void SketchPlan::release() {
     if (d_tsum) (void)hipFree(d_tsum);
     if (d_bo) (void)hipFree(d_bo);
     if (d_bstart) (void)hipFree(d_bstart);
+    if (d_pcode) (void)hipFree(d_pcode);
+    d_pcode = nullptr;
     d_tsum = nullptr;
     d_bo = nullptr;
     d_bstart = nullptr;
@@ -169,6 +174,24 @@ int sketch_plan(const ldsc_panel* panel_c, uint64_t d, SketchPlan** out, char* e
         if (e == hipSuccess) e = hipMalloc(&sp->d_bstart, bstart.size() * 4);
         if (e == hipSuccess)
             e = hipMemcpy(sp->d_bstart, bstart.data(), bstart.size() * 4, hipMemcpyHostToDevice);
+    }
+    if (panel->keep.empty() && sp->nblocks <= kSdMaxBlocks) {
+        // entry of individual i in the P operand of its pair of K-steps
+        // (k_sketch_dense): K-step 2(i / 128) + kk, lane half hh, lane
+        // bucket row b % 32; within the lane's four words the nibble order of
+        // the B operand (word q, nibble jj <-> individual o of the lane's 32)
+        std::vector<uint16_t> pc(n);
+        for (uint64_t i = 0; i < n; ++i) {
+            const uint32_t off = (uint32_t)(i % 128), hh = off / 64, kk = (off % 64) / 32;
+            const uint32_t o = off % 32, b = bucket[i], B = b / 32;
+            const uint32_t q = (o < 16 ? 0u : 2u) + (o & 1u), jj = (o % 16) / 2;
+            const uint32_t lane = b % 32 + 32 * hh;
+            pc[i] = (uint16_t)(lane | (q << 6) | (kk << 8) | (B << 9) | (jj << 12) |
+                               ((sign[i] < 0 ? 1u : 0u) << 15));
+        }
+        if (e == hipSuccess) e = hipMalloc(&sp->d_pcode, pc.size() * 2);
+        if (e == hipSuccess)
+            e = hipMemcpy(sp->d_pcode, pc.data(), pc.size() * 2, hipMemcpyHostToDevice);
     }
     if (e == hipSuccess) e = hipMalloc(&sp->d_tsum, tsum.size() * 4);
     if (e == hipSuccess)
@@ -782,6 +805,238 @@ __global__ __launch_bounds__(512) void k_sketch_fused(
     if (bad) flag[0] = 1;  // a plain store: every writer stores 1
 }
 
+// ------------------------------------------------------------------------
+// Dense one-pass bucket sums (at most kSdMaxBlocks blocks, no --keep)
+// ------------------------------------------------------------------------
+// The signed bucket sums of a 32-SNP group are the product P . G of the
+// group's .bed rows (K = individuals in FAM order) with the bucket matrix
+// (P[b][i] = s_i when individual i is in bucket b), on the e2m1 MFMA
+// v_mfma_scale_f32_32x32x64_f8f6f4: no gather and no bucket order.  Every
+// K-step of 64 individuals meets every bucket block, so the MFMA work is
+// two products per block and K-step; it pays while the blocks are few
+// (measured: faster than k_sketch_fused up to 4 blocks, slower beyond).
+//  * Workgroup q: kSdWaves waves, wave v owning SNP group kSdWaves q + v and
+//    all NB blocks (at most kSdMaxBlocks, so that two workgroups share a CU
+//    and hide each other's latencies).
+//  * P for a pair of K-steps (128 individuals) is built in LDS from the
+//    plan's per-individual code (lane, word, nibble, block and sign of the
+//    individual's entry in the A-operand layout), one LDS atomic OR per
+//    individual; three buffers rotate (read / being built / being cleared)
+//    with one barrier per pair.  The codes and the row bytes load kSdAhead
+//    pairs ahead of use.
+//  * The B operands come straight from the .bed rows: lane l holds SNP
+//    32g + l%32 and, per pair p, 16 bytes of its row (individuals
+//    128p + 64h .. +63, h = l/32: the first 8 bytes K-step 2p, the next 8
+//    K-step 2p+1).  Each 2-bit PLINK code c becomes the value code
+//    v = c ^ 2 ^ (c >> 1) (the dosage; 3 when missing), split into the nibble
+//    operands X = v/2 and Q = [v = 3]/2 as in the exact band kernel; the
+//    integer stats are counted from the same words.
+//  * The f32 sums are exact (multiples of 1/2 below 1.5 N), and
+//    SG = 2 PX - 6 PQ, SM = T_b - 2 PQ.  The epilogue is k_sketch_fused's, with
+//    one norm share slot.
+constexpr uint32_t kSdWaves = 4;
+constexpr int kSdAhead = 4;  // pairs of row bytes and codes in flight
+
+// X and Q nibble operands of 32 individuals from two .bed words (16 each):
+// word q of the operand holds, in its nibble jj, individual 2jj (q = 0),
+// 2jj + 1 (q = 1), 16 + 2jj (q = 2) or 17 + 2jj (q = 3) of the 32.
+__device__ __forceinline__ void plink_value_nibbles(uint32_t w0, uint32_t w1, v8i& X, v8i& Q) {
+    v4i x, q;
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+        const uint32_t w = t ? w1 : w0;
+        const uint32_t L = w & 0x55555555u, H = (w >> 1) & 0x55555555u;
+        const uint32_t v = (L ^ H) | ((H ^ 0x55555555u) << 1);  // value codes
+        const uint32_t mq = L & ~H;                             // code 01: missing
+        x[2 * t] = (int)(v & 0x33333333u);
+        x[2 * t + 1] = (int)((v >> 2) & 0x33333333u);
+        q[2 * t] = (int)(mq & 0x11111111u);
+        q[2 * t + 1] = (int)((mq >> 2) & 0x11111111u);
+    }
+    X = __builtin_shufflevector(x, x, 0, 1, 2, 3, -1, -1, -1, -1);
+    Q = __builtin_shufflevector(q, q, 0, 1, 2, 3, -1, -1, -1, -1);
+}
+
+template <int NB, bool RAW>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_sketch_dense(
+    const uint8_t* __restrict__ slabs, uint64_t pitch, uint32_t row_dw, uint32_t n,
+    const uint64_t* __restrict__ bed_idx, uint64_t base, uint32_t m, uint32_t m_pad,
+    const uint16_t* __restrict__ pcode,
+    const int32_t* __restrict__ tsum, uint32_t* __restrict__ stats_out,
+    double* __restrict__ norm, double* __restrict__ asum, void* __restrict__ cols,
+    int32_t* __restrict__ flag, uint32_t d, int32_t* __restrict__ sg_out,
+    int32_t* __restrict__ sm_out) {
+    extern __shared__ uint32_t lds[];  // three P buffers of kBuf words
+    constexpr uint32_t kBuf = 2 * NB * 64 * 4;
+    const uint32_t ngroups = m_pad / 32;
+    const int tid = threadIdx.x, wv = tid >> 6, ln = tid & 63, h = ln >> 5;
+    const uint32_t g = blockIdx.x * kSdWaves + wv;
+    const uint32_t snp = 32 * g + (ln & 31);
+    const bool real = g < ngroups && snp < m;  // padding SNPs read as all missing
+    const uint64_t row = real ? (bed_idx ? bed_idx[snp] : base + snp) : 0;
+    const uint32_t* rp = reinterpret_cast<const uint32_t*>(slabs + row * pitch);
+    const uint32_t npairs = (n + 127) / 128;
+    auto load_row = [&](uint32_t p) {
+        const uint32_t q0 = 8 * p + 4 * h;
+        return real && p < npairs && q0 + 4 <= row_dw
+                   ? *reinterpret_cast<const uint4*>(rp + q0)
+                   : make_uint4(0x55555555u, 0x55555555u, 0x55555555u, 0x55555555u);
+    };
+    auto clear = [&](uint32_t* buf) {
+        for (uint32_t x = 4 * tid; x < kBuf; x += 4 * 64 * kSdWaves)
+            *reinterpret_cast<uint4*>(buf + x) = make_uint4(0u, 0u, 0u, 0u);
+    };
+    // thread t < 128 places individual 128p + t (code: lane | word << 6 |
+    // K-step << 8 | block << 9 | nibble << 12 | negative << 15)
+    auto load_code = [&](uint32_t p) {
+        const uint32_t i = 128 * p + tid;
+        return tid < 128 && i < n ? (uint32_t)pcode[i] : 0xFFFFFFFFu;
+    };
+    auto build = [&](uint32_t c, uint32_t* buf) {
+        if (c != 0xFFFFFFFFu) {
+            const uint32_t word = ((((c >> 8) & 1u) * NB + ((c >> 9) & 7u)) * 64 + (c & 63u)) * 4 +
+                                  ((c >> 6) & 3u);
+            atomicOr(buf + word, ((c >> 15) & 1u ? 0xAu : 0x2u) << (4 * ((c >> 12) & 7u)));
+        }
+    };
+
+    v16f px[NB], pq[NB];
+#pragma unroll
+    for (int B = 0; B < NB; ++B)
+#pragma unroll
+        for (int x = 0; x < 16; ++x) px[B][x] = pq[B][x] = 0.0f;
+    uint32_t n2 = 0, n1 = 0, n0 = 0;
+    uint4 ring[kSdAhead];
+    uint32_t cring[kSdAhead];  // codes of pairs p + 1 .. p + kSdAhead
+#pragma unroll
+    for (int u = 0; u < kSdAhead; ++u) {
+        ring[u] = load_row(u);
+        cring[u] = load_code(u + 1);
+    }
+    clear(lds);
+    clear(lds + kBuf);
+    clear(lds + 2 * kBuf);
+    __syncthreads();
+    build(load_code(0), lds);
+    __syncthreads();
+    for (uint32_t p0 = 0; p0 < npairs; p0 += kSdAhead) {
+#pragma unroll
+        for (int u = 0; u < kSdAhead; ++u) {
+            const uint32_t p = p0 + u;
+            if (p < npairs) {  // uniform over the workgroup
+                const uint32_t* cur = lds + (p % 3) * kBuf;
+                build(cring[u], lds + ((p + 1) % 3) * kBuf);  // no-op past the last pair
+                cring[u] = load_code(p + 1 + kSdAhead);
+                clear(lds + ((p + 2) % 3) * kBuf);
+                const uint4 raw = ring[u];
+                ring[u] = load_row(p + kSdAhead);
+#pragma unroll
+                for (int kk = 0; kk < 2; ++kk) {
+                    const uint32_t w0 = kk ? raw.z : raw.x, w1 = kk ? raw.w : raw.y;
+                    {  // the integer stats
+                        const uint32_t i0 = 128 * p + 64 * h + 32 * kk;
+#pragma unroll
+                        for (int t = 0; t < 2; ++t) {
+                            const uint32_t w = t ? w1 : w0, ib = i0 + 16 * t;
+                            uint32_t vm = 0x55555555u;
+                            if (ib + 16 > n)
+                                vm = ib >= n ? 0u : ((1u << (2 * (n - ib))) - 1u) & 0x55555555u;
+                            const uint32_t L = w & 0x55555555u, H = (w >> 1) & 0x55555555u;
+                            n2 += __popc(~(L | H) & vm);  // 00: dosage 2
+                            n1 += __popc(~L & H & vm);    // 10: dosage 1
+                            n0 += __popc(L & H & vm);     // 11: dosage 0
+                        }
+                    }
+                    v8i X, Q;
+                    plink_value_nibbles(w0, w1, X, Q);
+#pragma unroll
+                    for (int B = 0; B < NB; ++B) {
+                        const v4i a4 = *reinterpret_cast<const v4i*>(cur + ((kk * NB + B) * 64 + ln) * 4);
+                        const v8i a8 = __builtin_shufflevector(a4, a4, 0, 1, 2, 3, -1, -1, -1, -1);
+                        px[B] = LDSC_MFMA_FP4(a8, X, px[B]);
+                        pq[B] = LDSC_MFMA_FP4(a8, Q, pq[B]);
+                    }
+                }
+                __syncthreads();
+            }
+        }
+    }
+    if (g >= ngroups) return;
+    n2 += __shfl_xor(n2, 32);
+    n1 += __shfl_xor(n1, 32);
+    n0 += __shfl_xor(n0, 32);
+    const uint32_t S = 2 * n2 + n1, C = n2 + n1 + n0, SS = 4 * n2 + n1;
+    // T_b of the lane's 16 buckets of block B (bucket 32B + 8(x/4) + 4h + x%4)
+    auto load_t = [&](uint32_t B, int (&T)[16]) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int4 t4 = *reinterpret_cast<const int4*>(tsum + 32 * B + 8 * q + 4 * h);
+            T[4 * q] = t4.x;
+            T[4 * q + 1] = t4.y;
+            T[4 * q + 2] = t4.z;
+            T[4 * q + 3] = t4.w;
+        }
+    };
+    if (RAW) {
+        if (snp >= m) return;
+#pragma unroll
+        for (int B = 0; B < NB; ++B) {
+            int T[16];
+            load_t(B, T);
+#pragma unroll
+            for (int x = 0; x < 16; ++x) {
+                const uint32_t b = 32 * B + 8 * (x >> 2) + 4 * h + (x & 3);
+                const int sq = (int)(2.0f * pq[B][x]);
+                if (b < d) {
+                    sg_out[(size_t)snp * d + b] = (int)(2.0f * px[B][x]) - 3 * sq;
+                    sm_out[(size_t)snp * d + b] = T[x] - sq;
+                }
+            }
+        }
+        return;
+    }
+    if (h == 0) reinterpret_cast<uint4*>(stats_out)[snp] = make_uint4(S, C, SS, n1);
+    float a32, i32;
+    (void)ref_mean_inv_f32(S, C, SS, n, a32, i32);
+    const float kf = rintf(a32);
+    const int kk = (int)kf;
+    const double dl = (double)a32 - (double)kf;
+    double nrm = 0.0, as = 0.0;
+    bool bad = false;
+    uint4* W = static_cast<uint4*>(cols);
+    uint4* E = W + (size_t)m_pad * NB * 2;  // m_pad x d_pad bytes later
+#pragma unroll
+    for (int B = 0; B < NB; ++B) {
+        int T[16];
+        load_t(B, T);
+        uint32_t wd[4] = {0u, 0u, 0u, 0u}, ed[4] = {0u, 0u, 0u, 0u};
+        uint32_t out = 0u;
+#pragma unroll
+        for (int x = 0; x < 16; ++x) {
+            const int sq = (int)(2.0f * pq[B][x]);
+            const int sg = (int)(2.0f * px[B][x]) - 3 * sq, sm = T[x] - sq;
+            const int w = sg - kk * sm, e = T[x] - sm;
+            out |= ((uint32_t)(w + 127) > 254u) | ((uint32_t)(e + 127) > 254u);
+            wd[x >> 2] |= ((uint32_t)w & 0xFFu) << (8 * (x & 3));
+            ed[x >> 2] |= ((uint32_t)e & 0xFFu) << (8 * (x & 3));
+            const float y = (float)((double)i32 * ((double)sg - (double)a32 * (double)sm));
+            nrm += (double)y * (double)y;
+            as += ((double)w + dl * (double)e) * (double)T[x];
+        }
+        bad = bad || (out && snp < m);
+        const size_t o = ((size_t)g * NB + B) * 64 + ln;
+        W[o] = make_uint4(wd[0], wd[1], wd[2], wd[3]);
+        E[o] = make_uint4(ed[0], ed[1], ed[2], ed[3]);
+    }
+    nrm += __shfl_xor(nrm, 32);
+    as += __shfl_xor(as, 32);
+    if (h == 0) {
+        norm[snp] = nrm;
+        asum[snp] = as;
+    }
+    if (bad) flag[0] = 1;  // a plain store: every writer stores 1
+}
+
 hipError_t launch_sketch_fused(const SketchPlan* sp, const ldsc_panel* panel,
                                const uint64_t* d_bed_idx, uint64_t base, uint32_t m, uint32_t m_pad,
                                uint32_t* stats_out, double* norm, void* cols, hipStream_t st,
@@ -796,6 +1051,27 @@ hipError_t launch_sketch_fused(const SketchPlan* sp, const ldsc_panel* panel,
     if (!raw) {
         hipLaunchKernelGGL(k_sketch_clear_flag, dim3(1), dim3(64), 0, st, flag);
         if ((e = hipGetLastError()) != hipSuccess) return e;
+    }
+    if (sp->d_pcode) {  // natural order, at most kSdMaxBlocks blocks
+        static const void* const dfn[2][kSdMaxBlocks] = {
+            {(const void*)k_sketch_dense<1, false>, (const void*)k_sketch_dense<2, false>,
+             (const void*)k_sketch_dense<3, false>, (const void*)k_sketch_dense<4, false>},
+            {(const void*)k_sketch_dense<1, true>, (const void*)k_sketch_dense<2, true>,
+             (const void*)k_sketch_dense<3, true>, (const void*)k_sketch_dense<4, true>}};
+        const void* kfn = dfn[raw ? 1 : 0][nb - 1];
+        const size_t lds = (size_t)3 * 2 * nb * 64 * 4 * 4;
+        const uint8_t* slabs = panel->d_slabs;
+        uint64_t pitch = panel->stride, a_base = base;
+        uint32_t row_dw = (uint32_t)(panel->stride / 4), n = (uint32_t)panel->n;
+        uint32_t a_m = m, a_mp = m_pad, a_d = (uint32_t)sp->d;
+        const uint16_t* pc = sp->d_pcode;
+        const int32_t* ts = sp->d_tsum;
+        void* kargs[] = {(void*)&slabs, (void*)&pitch, (void*)&row_dw, (void*)&n,
+                         (void*)&d_bed_idx, (void*)&a_base, (void*)&a_m, (void*)&a_mp,
+                         (void*)&pc, (void*)&ts, (void*)&stats_out, (void*)&norm, (void*)&asum,
+                         (void*)&cols, (void*)&flag, (void*)&a_d, (void*)&sg_out, (void*)&sm_out};
+        return hipLaunchKernel(kfn, dim3((ngroups + kSdWaves - 1) / kSdWaves), dim3(64 * kSdWaves),
+                               kargs, lds, st);
     }
     const size_t lds = (size_t)(8 * kSfStride + kSfWaves * kSfStage) * 4;
     static const void* const fns[2][3] = {
@@ -826,7 +1102,9 @@ hipError_t launch_sketch_fused(const SketchPlan* sp, const ldsc_panel* panel,
 
 hipError_t launch_sketch_fused_scale(const SketchPlan* sp, const float2* musc, uint32_t m_pad,
                                      double n, double* norm, hipStream_t st) {
-    const uint32_t slots = kSfWaves * sketch_fused_ranges(sp->nblocks);
+    // k_sketch_dense writes one share slot, k_sketch_fused one per wave and
+    // range
+    const uint32_t slots = sp->d_pcode ? 1u : kSfWaves * sketch_fused_ranges(sp->nblocks);
     double* asum = norm + (size_t)sketch_slots(sp) * m_pad;
     double* consts = const_cast<double*>(sketch_i8_consts(sp, norm, m_pad));
     hipLaunchKernelGGL(k_sketch_scale<kSketchI8>, dim3((m_pad + 255) / 256), dim3(256), 0, st,

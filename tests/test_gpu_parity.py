@@ -82,15 +82,16 @@ def test_sketch_buckets_bit_exact(oracle, n, d):
     assert np.array_equal(sg, esg) and np.array_equal(sm, esm)
 
 
-@pytest.mark.parametrize("n,d", [(16385, 7), (40000, 200), (40000, 3000), (16884, 2000),
-                                 (33000, 5000)])
+@pytest.mark.parametrize("n,d", [(16385, 7), (20001, 128), (40000, 200), (20001, 256),
+                                 (40000, 3000), (16884, 2000), (33000, 5000)])
 def test_sketch_buckets_chunked_order(oracle, n, d):
-    """More members than one 16,384-member chunk: each chunk is put in
-    bucket order on its own (the last one ragged, down to one member); the
-    integer bucket sums do not depend on that order.  These sums come from
-    the one-pass kernel, which gathers each chunk's bucket order from LDS;
-    a short last chunk (500 or 232 members) has K-steps that reach several
-    bucket blocks."""
+    """More members than one 16,384-member chunk.  Up to 4 bucket blocks
+    (d <= 128) the sums come from the dense one-pass kernel (natural order,
+    the bucket matrix built in LDS; n = 16,385 and 20,001 end in a partial
+    pair of K-steps).  Beyond, each chunk is put in bucket order on its own
+    (the last one ragged) and the one-pass gather kernel sums it; a short
+    last chunk (500 or 232 members) has K-steps that reach several bucket
+    blocks.  The integer sums do not depend on either order."""
     print(f"sketch buckets chunked n={n} d={d}", flush=True)
     p = _panel(n, 70, seed=n + d)
     idx = np.arange(70, dtype=np.uint64)[::-1].copy()
@@ -246,14 +247,18 @@ def test_l2_special_columns(oracle):
 @pytest.mark.parametrize("n,d,maf_aware", [(3000, 64, False), (3000, 300, False),
                                            (3000, 300, True), (3000, 1000, False),
                                            (20000, 500, True), (20000, 3, False),
-                                           (20000, 3, True), (16884, 2000, True)])
+                                           (20000, 3, True), (16884, 2000, True),
+                                           (20000, 200, True), (33001, 256, False),
+                                           (33001, 128, False), (20000, 100, True)])
 def test_l2_sketch_matches_oracle(oracle, n, d, maf_aware):
     """Sketch LD scores against the f32 sequential-scatter restatement.  The
     contraction runs on exact int8 operands (w = SG - k SM, e = T - SM)
     while they fit; at d = 3 over 20,000 individuals (buckets of ~6,700
     members) they do not, and the f32 form runs instead.  Over 16,884
     individuals the last member chunk has 500, so a K-step of its bucket
-    order reaches up to four blocks of d = 2000."""
+    order reaches up to four blocks of d = 2000.  Over 16,384 individuals
+    the bucket sums come from a one-pass kernel: the dense one up to d = 128
+    (d = 3, 100, 128 here), the gather one up to d = 256 (200, 256)."""
     m = 700
     p = _panel(n, m, seed=d)
     snps = L.make_snp_array(p.chr, p.cm, p.bp)
