@@ -173,8 +173,9 @@ def leg_panel(args, L, synth):
         k32 = float(np.mean([q.bb_dot_secs for q in pf32]))
         fp32 = {"value": P / float(np.mean(w32)), "unit": "SNP-pairs/s",
                 "ms_per_step": float(np.mean(w32)) * 1e3, "band_kernel_ms": k32 * 1e3,
-                "tflops_algorithmic": algo / k32 / 1e12,
+                "kernel": "k_band_f32_2w", "tflops_algorithmic": algo / k32 / 1e12,
                 "frac_of_fp32_peak": algo / k32 / 1e12 / FP32_PEAK_TFLOPS,
+                "profile_trace_avg_ms": profiled("trace_avg_ms", "k_band_f32_2w"),
                 "max_rel_vs_exact": float(np.max(np.abs(l2_32 - l2_8) /
                                                 np.maximum(1.0, np.abs(l2_8))))}
     exact_i8 = None
@@ -185,6 +186,8 @@ def leg_panel(args, L, synth):
                     "kernel": "k_band_i8", "achieved": algo / ki8 / 1e12,
                     "peak": INT8_PEAK_TOPS, "unit_peak": "TOP/s",
                     "frac": algo / ki8 / 1e12 / INT8_PEAK_TOPS,
+                    "traffic": profiled_traffic("k_band_i8"),
+                    "profile_trace_avg_ms": profiled("trace_avg_ms", "k_band_i8"),
                     "max_rel_vs_e2m1_form": float(np.max(np.abs(l2_i8 - l2_8) /
                                                          np.maximum(1.0, np.abs(l2_8))))}
     fp4 = p.n <= 1_800_000  # the e2m1 kernel's range (f32 sums stay exact)
@@ -606,11 +609,17 @@ def leg_partitioned(args, L, synth):
             t = float(np.mean(w))
             band = float(np.mean([q.bb_dot_secs for q in pf]))
             peak = FP32_PEAK_TFLOPS if f32 else INT8_PEAK_TOPS if kern == "i8" else FP4_PEAK_TFLOPS
+            kname = "k_band_f32_2w" if f32 else "k_band_i8" if kern == "i8" else "k_band_fp4"
             out[tag] = {"snp_pairs_per_s": P / t, "ms_per_step": t * 1e3,
                         "band_ms": band * 1e3, "pairs": P,
-                        "roofline": {"bound": "mfma", "achieved": 2.0 * n * P / band / 1e12,
+                        "roofline": {"bound": "mfma", "kernel": kname,
+                                     "achieved": 2.0 * n * P / band / 1e12,
                                      "peak": peak, "unit": "TFLOP/s",
                                      "frac": 2.0 * n * P / band / 1e12 / peak,
+                                     "traffic_per_call": profiled_traffic(
+                                         f"{kname}_per_call", PART_PROFILE),
+                                     "profile_trace_ms_per_call": profiled(
+                                         "trace_ms_per_call", kname, PART_PROFILE),
                                      "note": "2*N*P per call over the band's HIP-event time "
                                              "(its row-segment launches together); the FP64 "
                                              "annotation contraction, 4*P*K flops, runs in the "
