@@ -193,7 +193,7 @@ struct SketchPlan {
     // block B in chunk c (nblocks x nchunks)
     uint32_t* d_bo = nullptr;
     int32_t* d_bstart = nullptr;
-    // natural order with at most kSdMaxBlocks bucket blocks and no --keep:
+    // chunk-local plans with at most kSdMaxBlocks bucket blocks:
     // per individual, its entry in the P operand of its pair of K-steps
     // (lane | word << 6 | K-step << 8 | block << 9 | nibble << 12 |
     // negative << 15), for k_sketch_dense
@@ -274,6 +274,7 @@ struct L2Call {
     uint32_t m_pad = 0;
     int form = kBandFp4;
     bool use_f32 = false, early_maf = false, early_stats = false, stage_l2 = false;
+    bool stats_fetched = false;  // the integer stats come back on the copy stream (ev[7])
     bool any_full = false, done = true;
     size_t off_st = 0, off_l2 = 0;  // in ws->pinned_meta
     SketchPlan* sketch_plan = nullptr;

@@ -2024,9 +2024,18 @@ int l2_begin(const ldsc_panel* panel, const ldsc_snp* snps, uint64_t m, const ld
                         hipMemcpyHostToDevice, st));
 
     // 3. MAF (and pq-scaled annotations, kappa) on the host from the exact
-    //    stats; fetched now only when the device needs something derived
-    //    from them, else with the result at the end.
+    //    stats; fetched now and waited for only when the device needs
+    //    something derived from them.  Else, for the sketch form's host MAF,
+    //    they come back on the copy stream while the band runs and l2_end
+    //    forms the MAF before it waits for the result.
     uint32_t* h_stats = reinterpret_cast<uint32_t*>(hp + off_st);
+    if (maf_out && !early_maf && !early_stats) {
+        HIPC(hipStreamWaitEvent(ws->copy_stream, ws->ev[1], 0));
+        HIPC(hipMemcpyAsync(h_stats, ws->stats.p, (size_t)m_pad * 16, hipMemcpyDeviceToHost,
+                            ws->copy_stream));
+        HIPC(hipEventRecord(ws->ev[7], ws->copy_stream));
+        c.stats_fetched = true;
+    }
     double* h_maf = reinterpret_cast<double*>(hp + off_maf);
     std::vector<double>& maf = ws->host.maf;
     if (early_stats) {
@@ -2345,14 +2354,12 @@ int l2_end(L2Call* call, char* err, uint64_t err_len) {
     Workspace* ws = c.ws;
     hipStream_t st = ws->stream;
     const uint64_t m = c.m, K = c.K;
-    const uint32_t m_pad = c.m_pad;
-    const bool stage_l2 = c.stage_l2, early_maf = c.early_maf, early_stats = c.early_stats;
+    const bool stage_l2 = c.stage_l2, early_maf = c.early_maf;
     double* l2_out = c.l2_out;
     double* maf_out = c.maf_out;
     ldsc_l2_perf* perf = c.perf;
     BandPlan& plan = ws->host.plan;
     char* hp = ws->pinned_meta.as<char>();
-    uint32_t* h_stats = reinterpret_cast<uint32_t*>(hp + c.off_st);
     std::vector<double>& maf = ws->host.maf;
     // the band ran on this stream or on a batch stream that recorded ev[4]
     HIPC(hipStreamWaitEvent(st, ws->ev[4], 0));
@@ -2398,8 +2405,9 @@ int l2_end(L2Call* call, char* err, uint64_t err_len) {
     if (early_maf) {  // copy it out while the band kernel runs
         HIPC(hipEventSynchronize(ws->ev[7]));
         std::memcpy(maf_out, ws->pinned_maf.p, m * 8);
-    } else if (maf_out && !early_stats) {
-        HIPC(hipMemcpyAsync(h_stats, ws->stats.p, (size_t)m_pad * 16, hipMemcpyDeviceToHost, st));
+    } else if (c.stats_fetched) {  // the sketch form's MAF, while the band runs
+        HIPC(hipEventSynchronize(ws->ev[7]));
+        compute_maf_host(c);
     }
     double* h_l2 = reinterpret_cast<double*>(hp + c.off_l2);
     if (stage_l2) {
@@ -2428,10 +2436,7 @@ int l2_end(L2Call* call, char* err, uint64_t err_len) {
         HIPC(ring_download(ws, l2_out, ws->l2.p, (size_t)m * K * 8, ws->copy_stream, ready));
         HIPC(hipStreamSynchronize(st));
     }
-    if (maf_out && !early_maf) {
-        if (!early_stats) compute_maf_host(c);  // sketch form: from the integer stats
-        std::memcpy(maf_out, maf.data(), m * 8);
-    }
+    if (maf_out && !early_maf) std::memcpy(maf_out, maf.data(), m * 8);
 
     if (perf) {
         float t01 = 0, t12 = 0, t34 = 0, t45 = 0;

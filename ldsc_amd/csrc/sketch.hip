@@ -175,7 +175,7 @@ int sketch_plan(const ldsc_panel* panel_c, uint64_t d, SketchPlan** out, char* e
         if (e == hipSuccess)
             e = hipMemcpy(sp->d_bstart, bstart.data(), bstart.size() * 4, hipMemcpyHostToDevice);
     }
-    if (panel->keep.empty() && sp->nblocks <= kSdMaxBlocks) {
+    if (local && sp->nblocks <= kSdMaxBlocks) {  // the one-pass form's condition
         // entry of individual i in the P operand of its pair of K-steps
         // (k_sketch_dense): K-step 2(i / 128) + kk, lane half hh, lane
         // bucket row b % 32; within the lane's four words the nibble order of
