@@ -117,6 +117,8 @@ struct Workspace {
     hipStream_t stream = nullptr;
     hipStream_t copy_stream = nullptr;  // read-backs that overlap the band kernel
     hipStream_t up_stream = nullptr;    // annotation upload beside the tiling
+    hipStream_t plan_stream = nullptr;  // band plan upload beside the tiling
+    hipEvent_t plan_ev = nullptr;       // the band plan is on the device
     hipEvent_t ev[10] = {};
     DevBuf tiled, stats, bed_idx, lo, info, aeff, annot, pq, maf, mafd, tiles, j0, tstart,
         rowp, colp, l2, sk_cols, sk_norm, scratch;

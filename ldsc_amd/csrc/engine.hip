@@ -127,6 +127,10 @@ hipError_t Workspace::init(int dev) {
     if (e != hipSuccess) return e;
     e = hipStreamCreateWithFlags(&up_stream, hipStreamNonBlocking);
     if (e != hipSuccess) return e;
+    e = hipStreamCreateWithFlags(&plan_stream, hipStreamNonBlocking);
+    if (e != hipSuccess) return e;
+    e = hipEventCreateWithFlags(&plan_ev, hipEventDisableTiming);
+    if (e != hipSuccess) return e;
     for (auto& x : ev) {
         e = hipEventCreate(&x);
         if (e != hipSuccess) return e;
@@ -152,6 +156,7 @@ void Workspace::trim() {
     if (stream) (void)hipStreamSynchronize(stream);
     if (copy_stream) (void)hipStreamSynchronize(copy_stream);
     if (up_stream) (void)hipStreamSynchronize(up_stream);
+    if (plan_stream) (void)hipStreamSynchronize(plan_stream);
     for (auto* b : all) b->release();
 }
 
@@ -188,11 +193,15 @@ void Workspace::destroy() {
         if (x) (void)hipEventDestroy(x);
     for (auto& x : prog_ev)
         if (x) (void)hipEventDestroy(x);
+    if (plan_ev) (void)hipEventDestroy(plan_ev);
     if (stream) (void)hipStreamDestroy(stream);
     if (copy_stream) (void)hipStreamDestroy(copy_stream);
     if (up_stream) (void)hipStreamDestroy(up_stream);
+    if (plan_stream) (void)hipStreamDestroy(plan_stream);
     copy_stream = nullptr;
     up_stream = nullptr;
+    plan_stream = nullptr;
+    plan_ev = nullptr;
     stream = nullptr;
 }
 
@@ -2016,12 +2025,17 @@ int l2_begin(const ldsc_panel* panel, const ldsc_snp* snps, uint64_t m, const ld
     HIPC(ws->rowp.ensure((size_t)plan.ntiles * K * kTile * 8));
     HIPC(ws->colp.ensure((size_t)plan.ntiles * K * kTile * 8));
     HIPC(ws->l2.ensure((size_t)m * K * 8));
-    HIPC(hipMemcpyAsync(ws->lo.p, h_lo, (size_t)m_pad * 4, hipMemcpyHostToDevice, st));
+    // the plan goes up on a stream of its own, beside the tiling still
+    // running on st; st waits for it before anything that reads it
+    hipStream_t pst = ws->plan_stream;
+    HIPC(hipMemcpyAsync(ws->lo.p, h_lo, (size_t)m_pad * 4, hipMemcpyHostToDevice, pst));
     HIPC(hipMemcpyAsync(ws->tiles.p, hp + off_tiles, (size_t)plan.ntiles * 8,
-                        hipMemcpyHostToDevice, st));
-    HIPC(hipMemcpyAsync(ws->j0.p, hp + off_j0, (size_t)plan.nb * 4, hipMemcpyHostToDevice, st));
+                        hipMemcpyHostToDevice, pst));
+    HIPC(hipMemcpyAsync(ws->j0.p, hp + off_j0, (size_t)plan.nb * 4, hipMemcpyHostToDevice, pst));
     HIPC(hipMemcpyAsync(ws->tstart.p, hp + off_ts, (size_t)(plan.nb + 1) * 4,
-                        hipMemcpyHostToDevice, st));
+                        hipMemcpyHostToDevice, pst));
+    HIPC(hipEventRecord(ws->plan_ev, pst));
+    HIPC(hipStreamWaitEvent(st, ws->plan_ev, 0));
 
     // 3. MAF (and pq-scaled annotations, kappa) on the host from the exact
     //    stats; fetched now and waited for only when the device needs
