@@ -88,6 +88,15 @@ extern "C" int ldsc_l2_compute_noalloc(const ldsc_panel* panel, const ldsc_snp* 
         return fail(err, err_len, LDSC_ERR_DEVICE, "ldsc_l2_compute_noalloc set-up failed: %s",
                     hipGetErrorString(d.setup));
     std::lock_guard<std::mutex> lk(d.mu);
+    // A call that failed part-way can leave copies in flight on the side
+    // streams; the array is carved again from its start, in another layout
+    // when the shape differs, so they must end first.
+    for (hipStream_t s : {d.ws.stream, d.ws.copy_stream, d.ws.up_stream, d.ws.plan_stream}) {
+        const hipError_t e = hipStreamSynchronize(s);
+        if (e != hipSuccess)
+            return fail(err, err_len, LDSC_ERR_DEVICE, "ldsc_l2_compute_noalloc: %s",
+                        hipGetErrorString(e));
+    }
     d.ws.reset_arena();
     return run_l2(panel, snps, m, cfg, annot, K, l2_out, maf_out, perf, cb, cb_ctx, &d.ws, err,
                   err_len);
