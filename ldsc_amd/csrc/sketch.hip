@@ -1300,7 +1300,12 @@ __device__ __forceinline__ double sqrt_pos(double x) {
 // against both column groups, 4 products x 2 blocks of int32 accumulators
 // (128 registers), so two waves fit per SIMD and one tile's FP64 epilogue
 // overlaps another's MFMAs (measured faster than one wave per tile with all
-// four blocks at d = 200, 1000 and 5000).
+// four blocks at d = 200, 1000 and 5000).  Each wave loads W and E of its
+// rows and of one column group, 2J + w, and passes that group's operands to
+// the other wave through LDS (two buffers, one barrier per K-step), so the
+// tile's column operands cross the cache hierarchy once.  A wave's
+// accumulator block 0 is its own column group (static register indices),
+// block 1 the other wave's; the epilogue puts them back in column order.
 //
 // Epilogue per pair, the inversion of quadratic_sketch_correction
 // (compute.rs:947-998) rearranged to fewer FP64 operations: with
@@ -1319,10 +1324,12 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     const uint32_t I = (uint32_t)tl.x, J = (uint32_t)tl.y;
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const size_t gs = (size_t)a.nkb * 64;
-    // operand streams: W and E of the wave's row group and both column groups
-    const u32x4* src[6] = {a.skw + (2 * I + w) * gs + lane, a.skw + (2 * J) * gs + lane,
-                           a.skw + (2 * J + 1) * gs + lane, a.ske + (2 * I + w) * gs + lane,
-                           a.ske + (2 * J) * gs + lane, a.ske + (2 * J + 1) * gs + lane};
+    // operand streams: W and E of the wave's row group and of its column group
+    const u32x4* src[4] = {a.skw + (2 * I + w) * gs + lane, a.ske + (2 * I + w) * gs + lane,
+                           a.skw + (2 * J + w) * gs + lane, a.ske + (2 * J + w) * gs + lane};
+    // the column operands exchanged: xch[buffer][wave][W or E][lane], in the
+    // epilogue's LDS (unused until the K loop is done)
+    auto xch = reinterpret_cast<u32x4 (*)[2][2][64]>(&sh.u);
     v16i acc[4][2];
 #pragma unroll
     for (int p = 0; p < 4; ++p)
@@ -1330,32 +1337,43 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
         for (int bj = 0; bj < 2; ++bj)
 #pragma unroll
             for (int r = 0; r < 16; ++r) acc[p][bj][r] = 0;
-    u32x4 ring[kSk8Ahead][6];
+    u32x4 ring[kSk8Ahead][4];
     const uint32_t nk = a.nkb;
 #pragma unroll
     for (int u = 0; u < kSk8Ahead; ++u) {
         const size_t o = (size_t)(u < (int)nk ? u : nk - 1) * 64;
 #pragma unroll
-        for (int q = 0; q < 6; ++q) ring[u][q] = src[q][o];
+        for (int q = 0; q < 4; ++q) ring[u][q] = src[q][o];
     }
+    // every branch below is uniform over the workgroup (nk, kb, u), so both
+    // waves reach each barrier
     for (uint32_t kb = 0; kb < nk; kb += kSk8Ahead) {
 #pragma unroll
         for (int u = 0; u < kSk8Ahead; ++u) {
             if (kb + u < nk) {
-                v4i op[6];
+                const uint32_t step = kb + u, buf = step & 1u;
+                v4i op[4];
 #pragma unroll
-                for (int q = 0; q < 6; ++q) op[q] = __builtin_bit_cast(v4i, ring[u][q]);
-                const uint32_t nxt = kb + u + kSk8Ahead;
+                for (int q = 0; q < 4; ++q) op[q] = __builtin_bit_cast(v4i, ring[u][q]);
+                xch[buf][w][0][lane] = ring[u][2];
+                xch[buf][w][1][lane] = ring[u][3];
+                const uint32_t nxt = step + kSk8Ahead;
                 const size_t o = (size_t)(nxt < nk ? nxt : nk - 1) * 64;
 #pragma unroll
-                for (int q = 0; q < 6; ++q) ring[u][q] = src[q][o];
-#pragma unroll
-                for (int bj = 0; bj < 2; ++bj) {
-                    acc[0][bj] = __builtin_amdgcn_mfma_i32_32x32x32_i8(op[0], op[1 + bj], acc[0][bj], 0, 0, 0);
-                    acc[1][bj] = __builtin_amdgcn_mfma_i32_32x32x32_i8(op[0], op[4 + bj], acc[1][bj], 0, 0, 0);
-                    acc[2][bj] = __builtin_amdgcn_mfma_i32_32x32x32_i8(op[3], op[1 + bj], acc[2][bj], 0, 0, 0);
-                    acc[3][bj] = __builtin_amdgcn_mfma_i32_32x32x32_i8(op[3], op[4 + bj], acc[3][bj], 0, 0, 0);
-                }
+                for (int q = 0; q < 4; ++q) ring[u][q] = src[q][o];
+                // buffer `buf` was last read at step - 2, before the barrier
+                // of step - 1
+                __syncthreads();
+                const v4i ow = __builtin_bit_cast(v4i, xch[buf][w ^ 1][0][lane]);
+                const v4i oe = __builtin_bit_cast(v4i, xch[buf][w ^ 1][1][lane]);
+                acc[0][0] = __builtin_amdgcn_mfma_i32_32x32x32_i8(op[0], op[2], acc[0][0], 0, 0, 0);
+                acc[1][0] = __builtin_amdgcn_mfma_i32_32x32x32_i8(op[0], op[3], acc[1][0], 0, 0, 0);
+                acc[2][0] = __builtin_amdgcn_mfma_i32_32x32x32_i8(op[1], op[2], acc[2][0], 0, 0, 0);
+                acc[3][0] = __builtin_amdgcn_mfma_i32_32x32x32_i8(op[1], op[3], acc[3][0], 0, 0, 0);
+                acc[0][1] = __builtin_amdgcn_mfma_i32_32x32x32_i8(op[0], ow, acc[0][1], 0, 0, 0);
+                acc[1][1] = __builtin_amdgcn_mfma_i32_32x32x32_i8(op[0], oe, acc[1][1], 0, 0, 0);
+                acc[2][1] = __builtin_amdgcn_mfma_i32_32x32x32_i8(op[1], ow, acc[2][1], 0, 0, 0);
+                acc[3][1] = __builtin_amdgcn_mfma_i32_32x32x32_i8(op[1], oe, acc[3][1], 0, 0, 0);
             }
         }
     }
@@ -1380,13 +1398,13 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     const double dm3_0 = a.d_f - 3.0;
     const double c0_0 = __fma_rn(dm3_0, dm3_0, -8.0);
     const double bb_0 = __fma_rn(-dm3_0, qa, a.r2u_b);
-    double r2u[2][16];
+    double val2[2][16];  // in accumulator order: block bj is column block bj ^ w
 #pragma unroll
     for (int bj = 0; bj < 2; ++bj)
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
             const int row = 32 * w + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-            const int col = 32 * bj + (lane & 31);
+            const int col = 32 * (bj ^ w) + (lane & 31);
             const double dr = dlr[row], dc = dlc[col];
             const double D = (double)acc[0][bj][r] + dc * (double)acc[1][bj][r] +
                              dr * (double)acc[2][bj][r] +
@@ -1402,10 +1420,16 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
             const double sq = disc > 0.0 ? sqrt_pos(disc) : 0.0;
             const double val = disc >= 0.0 ? __fma_rn(sq, qa, bb) : a.r2u_b;
             const uint32_t jj = I * kTile + row, ii = J * kTile + col;
-            r2u[bj][r] = (ii < jj && jj < a.m && ii >= sh.lor[row]) ? val : 0.0;
+            val2[bj][r] = (ii < jj && jj < a.m && ii >= sh.lor[row]) ? val : 0.0;
             // with the kurtosis, pair by pair: hoisted reads would spill
             if (MAF) __builtin_amdgcn_sched_barrier(0);
         }
+    double r2u[2][16];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        r2u[0][r] = w ? val2[1][r] : val2[0][r];
+        r2u[1][r] = w ? val2[0][r] : val2[1][r];
+    }
     band_contract2(a, sh, tile, I, J, w, r2u);
 }
 
