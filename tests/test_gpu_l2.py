@@ -118,3 +118,43 @@ def test_progress_callbacks(per_chr, K):
     assert calls[-1].chunks_done == total and calls[-1].snps_done == m
     assert all(0 < c.snps_done < m for c in calls[:-1])
     assert all(c.chunk_wall_ms >= 0 for c in calls)
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 5, 63])
+@pytest.mark.parametrize("m", [1, 2, 40])
+@pytest.mark.parametrize("form", ["exact", "exact_i8", "f32"])
+def test_l2_tiny_panels(oracle, n, m, form):
+    """The degenerate sizes: N = 1 (every column monomorphic, so every pair
+    adds r2u = -1/D with D = N), N = 2 (D = N) and N = 3 (D = 1), a single
+    SNP and two SNPs, against the restatement (compute.rs:1067-1077)."""
+    p = synth.make_panel(n=n, m=m, seed=100 * n + m, miss_rate=0.05)
+    snps = L.make_snp_array(p.chr, p.cm, p.bp)
+    use_f32 = form == "f32"
+    print(f"tiny n={n} m={m} form={form}", flush=True)
+    with L.Panel(p.bed_bytes(), p.n, p.m) as panel:
+        for md, om, window, masking, chunk in ((L.WindowMode.Snp(3), oracle.WINDOW_SNP, 3, False, 2),
+                                               (L.WindowMode.Cm(0.05), oracle.WINDOW_CM, 0.05, True, 7)):
+            cfg = L.L2Config(mode=md, chunk_size=chunk, use_f32=use_f32, snp_level_masking=masking,
+                             exact_kernel="i8" if form == "exact_i8" else "auto", yes_really=True)
+            got, maf, _ = L.compute_ldscore(panel, snps, cfg)
+            exp, emaf = oracle.l2(p.bed_bytes(), p.n, p.chr, p.cm, p.bp, mode=om, window=window,
+                                  chunk=chunk, use_f32=use_f32, masking=masking)
+            assert _rel(got, exp) < (TOL_F32 if use_f32 else TOL_EXACT)
+            assert np.array_equal(maf, emaf)
+
+
+@pytest.mark.parametrize("n,m,d", [(4, 30, 3), (300, 60, 299), (2000, 120, 1999)])
+def test_l2_sketch_extreme_d(oracle, n, m, d):
+    """--sketch at its bounds 3 <= d < N (compute.rs:1083-1089): the smallest
+    panel that admits a sketch, and d = N - 1 (buckets of about one member),
+    against the f32 sequential-scatter restatement."""
+    p = synth.make_panel(n=n, m=m, seed=d, miss_rate=0.02)
+    snps = L.make_snp_array(p.chr, p.cm, p.bp)
+    cfg = L.L2Config(mode=L.WindowMode.Cm(0.05), chunk_size=7, sketch=d, snp_level_masking=True)
+    print(f"sketch n={n} d={d}", flush=True)
+    with L.Panel(p.bed_bytes(), p.n, p.m) as panel:
+        got, maf, _ = L.compute_ldscore(panel, snps, cfg)
+    exp, emaf = oracle.l2(p.bed_bytes(), p.n, p.chr, p.cm, p.bp, mode=oracle.WINDOW_CM,
+                          window=0.05, chunk=7, sketch=d, use_f32=True, masking=True)
+    assert _rel(got, exp) < 2e-5
+    assert np.array_equal(maf, emaf)
