@@ -1330,20 +1330,25 @@ __device__ __forceinline__ void f32_2w_kloop(const u32x4* pa, const u32x4* pb0, 
         const size_t nk = (size_t)(kb + 1 < nkb ? kb + 1 : kb) * 64;
         const u32x4 na = pa[nk], nb0 = kB0 ? pb0[nk] : z, nb1 = kB1 ? pb1[nk] : z;
         __builtin_amdgcn_sched_barrier(0);  // keep the prefetch a block ahead
+        // per code dword: all its operands first, then its MFMAs, so that
+        // the scheduler runs one dword's MFMAs beside the next one's selects
+        // (measured 3.6 % faster than selecting each operand just before its
+        // MFMA; the same MFMA sequence)
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
+            float fa[16], fb0[16], fb1[16];
 #pragma unroll
             for (int s = 0; s < 16; ++s) {
                 const uint32_t sh_bits = 8 * (s & 3) + 2 * (s >> 2);
-                const float fa = pick_code_bits(la, ca[t], sh_bits, zo);
-                if (kB0) {
-                    const float fb = pick_code_bits(lb0, cb0[t], sh_bits, zo);
-                    acc[0] = __builtin_amdgcn_mfma_f32_32x32x2f32(fa, fb, acc[0], 0, 0, 0);
-                }
-                if (kB1) {
-                    const float fb = pick_code_bits(lb1, cb1[t], sh_bits, zo);
-                    acc[1] = __builtin_amdgcn_mfma_f32_32x32x2f32(fa, fb, acc[1], 0, 0, 0);
-                }
+                fa[s] = pick_code_bits(la, ca[t], sh_bits, zo);
+                if (kB0) fb0[s] = pick_code_bits(lb0, cb0[t], sh_bits, zo);
+                if (kB1) fb1[s] = pick_code_bits(lb1, cb1[t], sh_bits, zo);
+            }
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int s = 0; s < 16; ++s) {
+                if (kB0) acc[0] = __builtin_amdgcn_mfma_f32_32x32x2f32(fa[s], fb0[s], acc[0], 0, 0, 0);
+                if (kB1) acc[1] = __builtin_amdgcn_mfma_f32_32x32x2f32(fa[s], fb1[s], acc[1], 0, 0, 0);
             }
         }
         ca = na;
