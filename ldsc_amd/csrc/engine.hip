@@ -1297,52 +1297,46 @@ void k_band_fp4_batch(BandBatch b) {
     if (batch_tile(b, r, tile)) band_fp4_tile(b.args[r], sh, tile);
 }
 
-// Standardized f32 value of 2-bit code field `sh` of w from the SNP's LUT
-// [(2-mu)inv, 0, (1-mu)inv, (0-mu)inv] as bit selects: m0, m1 are the
-// code's bits sign-extended to whole words (v_bfe_i32) and each select is
-// (m & x) | (~m & y) (v_bfi_b32): five VALU instructions, no compare.
-// `z` is zero, opaque to the compiler, so that the field offsets stay
-// operands of v_bfe_i32 rather than being folded into shifts whose
-// sign-splat the compiler would turn back into compares and selects.
-// Bit-identical to pick_code.
-__device__ __forceinline__ float pick_code_bits(const float4& L, uint32_t w, uint32_t sh,
-                                                uint32_t z) {
-    const uint32_t m0 = (uint32_t)__builtin_amdgcn_sbfe((int)w, sh + z, 1u);
-    const uint32_t m1 = (uint32_t)__builtin_amdgcn_sbfe((int)w, sh + 1u + z, 1u);
-    const uint32_t lo = (m0 & __float_as_uint(L.y)) | (~m0 & __float_as_uint(L.x));
-    const uint32_t hi = (m0 & __float_as_uint(L.w)) | (~m0 & __float_as_uint(L.z));
-    return __uint_as_float((m1 & hi) | (~m1 & lo));
+// Standardized f32 operands come from the tile's LUTs in LDS, stored code
+// major: entry c of slot x (rows 0-63, then columns 64-127) at word
+// 128 c + x.  A lane reads its own slot, so the address of code field `sh`
+// of w is (bfe(w, sh, 2) << 9) + slot * 4 (two VALU instructions and one
+// LDS read), and the 32 lanes of a half-wave read 32 consecutive words for
+// any mix of codes (no bank conflicts).
+constexpr uint32_t kLutSlots = 128;
+static_assert(sizeof(EpiShared2::u) >= 4 * kLutSlots * sizeof(float), "LUTs in the epilogue LDS");
+__device__ __forceinline__ float lut_pick(const float* slot, uint32_t w, uint32_t sh) {
+    return slot[__builtin_amdgcn_ubfe(w, sh, 2u) * kLutSlots];
 }
 
 // K loop of one wave of the two-wave FP32 kernel: its row group against the
 // column groups MASK names (bit bj: columns 32bj ..), one
-// v_mfma_f32_32x32x2f32 per column group and pair of individuals.
+// v_mfma_f32_32x32x2f32 per column group and pair of individuals; la, lb0,
+// lb1 are the lane's LUT slots (its row, its column in each group).
 template <int MASK>
 __device__ __forceinline__ void f32_2w_kloop(const u32x4* pa, const u32x4* pb0, const u32x4* pb1,
-                                             const float4& la, const float4& lb0,
-                                             const float4& lb1, uint32_t nkb, v16f (&acc)[2]) {
+                                             const float* la, const float* lb0, const float* lb1,
+                                             uint32_t nkb, v16f (&acc)[2]) {
     constexpr bool kB0 = (MASK & 1) != 0, kB1 = (MASK & 2) != 0;
     const u32x4 z = {0u, 0u, 0u, 0u};
-    uint32_t zo = 0;
-    asm("" : "+s"(zo));  // zero that the compiler cannot see through
     u32x4 ca = pa[0], cb0 = kB0 ? pb0[0] : z, cb1 = kB1 ? pb1[0] : z;
     for (uint32_t kb = 0; kb < nkb; ++kb) {
         const size_t nk = (size_t)(kb + 1 < nkb ? kb + 1 : kb) * 64;
         const u32x4 na = pa[nk], nb0 = kB0 ? pb0[nk] : z, nb1 = kB1 ? pb1[nk] : z;
         __builtin_amdgcn_sched_barrier(0);  // keep the prefetch a block ahead
         // per code dword: all its operands first, then its MFMAs, so that
-        // the scheduler runs one dword's MFMAs beside the next one's selects
-        // (measured 3.6 % faster than selecting each operand just before its
-        // MFMA; the same MFMA sequence)
+        // the scheduler runs one dword's MFMAs beside the next one's LUT
+        // reads (the same MFMA sequence as picking each operand just before
+        // its MFMA)
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
             float fa[16], fb0[16], fb1[16];
 #pragma unroll
             for (int s = 0; s < 16; ++s) {
                 const uint32_t sh_bits = 8 * (s & 3) + 2 * (s >> 2);
-                fa[s] = pick_code_bits(la, ca[t], sh_bits, zo);
-                if (kB0) fb0[s] = pick_code_bits(lb0, cb0[t], sh_bits, zo);
-                if (kB1) fb1[s] = pick_code_bits(lb1, cb1[t], sh_bits, zo);
+                fa[s] = lut_pick(la, ca[t], sh_bits);
+                if (kB0) fb0[s] = lut_pick(lb0, cb0[t], sh_bits);
+                if (kB1) fb1[s] = lut_pick(lb1, cb1[t], sh_bits);
             }
             __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
@@ -1359,7 +1353,7 @@ __device__ __forceinline__ void f32_2w_kloop(const u32x4* pa, const u32x4* pb0, 
 
 // FP32 form, two waves per 64x64 tile: wave w owns rows 32w .. 32w+31 (SNP
 // group 2I + w) against both column groups (2 blocks of 32x32 f32
-// accumulators), so two waves share a SIMD and one wave's code selects and
+// accumulators), so two waves share a SIMD and one wave's LUT reads and
 // FP64 epilogue run beside the other's MFMAs.  Each wave skips a 32x32
 // block of its rows that cannot hold an included pair.  Each dot product
 // is the same sequence of MFMAs over the individuals as one wave per tile
@@ -1373,9 +1367,22 @@ __device__ __forceinline__ void band_f32_2w_tile(const BandArgs& a, EpiShared2& 
     const u32x4* pb0 = a.tiled + (size_t)(2 * J) * gstride + lane;
     const u32x4* pb1 = pb0 + gstride;
     const uint32_t r0 = I * kTile + 32 * w, c0 = J * kTile;
-    const float4 la = a.lut[r0 + (lane & 31)];
-    const float4 lb0 = a.lut[c0 + (lane & 31)];
-    const float4 lb1 = a.lut[c0 + 32 + (lane & 31)];
+    // the tile's LUTs in LDS, in the epilogue's storage (unused until the K
+    // loop ends): wave 0 stores the row SNPs' (slots 0-63), wave 1 the
+    // column SNPs' (slots 64-127)
+    float* lut_s = reinterpret_cast<float*>(&sh.u);
+    {
+        const float4 v = a.lut[(w ? c0 : I * kTile) + lane];
+        float* d = lut_s + 64 * w + lane;
+        d[0] = v.x;
+        d[kLutSlots] = v.y;
+        d[2 * kLutSlots] = v.z;
+        d[3 * kLutSlots] = v.w;
+    }
+    __syncthreads();
+    const float* la = lut_s + 32 * w + (lane & 31);
+    const float* lb0 = lut_s + 64 + (lane & 31);
+    const float* lb1 = lut_s + 96 + (lane & 31);
 
     uint32_t lmin = a.lo[r0 + (lane & 31)];
 #pragma unroll
