@@ -395,12 +395,23 @@ def leg_serving(args, L, synth):
         reqs.append((which, snps, cfg, annot, pairs))
     gaps = rng.exponential(1.0 / args.qps, size=len(reqs))
     lats, tickets = [], []
+    def warm_workers(srv, inflight):
+        """Grow every worker's buffers to the largest request shape of the
+        stream (60,000 SNPs of the larger panel, 10 annotation columns, each
+        window mode): a worker whose buffers grow mid-stream frees its old
+        ones, and hipFree synchronizes the whole device, stalling every
+        request in flight."""
+        for md in modes:
+            cfg = L.L2Config(mode=md, chunk_size=200, snp_level_masking=True)
+            warm = [srv.submit(panels[0], snp_sets[0][:60_000], cfg,
+                               np.ascontiguousarray(annots[0][:60_000, :10]))
+                    for _ in range(inflight)]
+            for t in warm:
+                srv.wait(t)
+
     with Server(hbm_budget_bytes=96 << 30, max_inflight=args.serve_inflight,
                 queue_capacity=4096, batch_window_us=args.serve_window_us) as srv:
-        warm = [srv.submit(panels[0], snp_sets[0][:60_000], reqs[0][2])
-                for _ in range(args.serve_inflight)]
-        for t in warm:
-            srv.wait(t)
+        warm_workers(srv, args.serve_inflight)
         log("serving: workers warm; starting the Poisson stream")
         t_start = time.perf_counter()
         t_next = t_start
@@ -438,10 +449,7 @@ def leg_serving(args, L, synth):
     def run_burst(window_us):
         with Server(hbm_budget_bytes=96 << 30, max_inflight=args.serve_burst_inflight,
                     queue_capacity=4096, batch_window_us=window_us) as srv:
-            warm = [srv.submit(panels[0], snp_sets[0][:60_000], reqs[0][2])
-                    for _ in range(args.serve_burst_inflight)]
-            for t in warm:
-                srv.wait(t)
+            warm_workers(srv, args.serve_burst_inflight)
             s0 = srv.stats()
             tb = time.perf_counter()
             bt = [srv.submit(panels[w], snps, cfg, annot) for (w, snps, cfg, annot, _) in reqs]
