@@ -1297,26 +1297,28 @@ void k_band_fp4_batch(BandBatch b) {
     if (batch_tile(b, r, tile)) band_fp4_tile(b.args[r], sh, tile);
 }
 
-// Standardized f32 operands come from the tile's LUTs in LDS, stored code
-// major: entry c of slot x (rows 0-63, then columns 64-127) at word
-// 128 c + x.  A lane reads its own slot, so the address of code field `sh`
-// of w is (bfe(w, sh, 2) << 9) + slot * 4 (two VALU instructions and one
-// LDS read), and the 32 lanes of a half-wave read 32 consecutive words for
-// any mix of codes (no bank conflicts).
+// Standardized f32 operands come from the tile's LUTs in LDS as pairs: a
+// 4-bit field of a code dword holds the codes of two individuals (fields f
+// and f + 1 of one byte), and entry c = 0 .. 15 of a SNP's pair table is
+// (L[c & 3], L[c >> 2]) of its LUT L = [(2-mu)inv, 0, (1-mu)inv, (0-mu)inv].
+// Code major: entry c of slot x (rows 0-63, then columns 64-127) is float2
+// 128 c + x.  A lane reads its own slot, so two operands cost a bit-field
+// extract, a shift-or and one 8-byte LDS read, and the 32 lanes of a
+// half-wave read 32 consecutive float2 for any mix of codes.
 constexpr uint32_t kLutSlots = 128;
-static_assert(sizeof(EpiShared2::u) >= 4 * kLutSlots * sizeof(float), "LUTs in the epilogue LDS");
-__device__ __forceinline__ float lut_pick(const float* slot, uint32_t w, uint32_t sh) {
-    return slot[__builtin_amdgcn_ubfe(w, sh, 2u) * kLutSlots];
+static_assert(sizeof(EpiShared2::u) >= 16 * kLutSlots * sizeof(float2), "pair tables in the epilogue LDS");
+__device__ __forceinline__ float2 pair_pick(const float2* slot, uint32_t w, uint32_t sh) {
+    return slot[__builtin_amdgcn_ubfe(w, sh, 4u) * kLutSlots];
 }
 
 // K loop of one wave of the two-wave FP32 kernel: its row group against the
 // column groups MASK names (bit bj: columns 32bj ..), one
 // v_mfma_f32_32x32x2f32 per column group and pair of individuals; la, lb0,
-// lb1 are the lane's LUT slots (its row, its column in each group).
+// lb1 are the lane's pair-table slots (its row, its column in each group).
 template <int MASK>
 __device__ __forceinline__ void f32_2w_kloop(const u32x4* pa, const u32x4* pb0, const u32x4* pb1,
-                                             const float* la, const float* lb0, const float* lb1,
-                                             uint32_t nkb, v16f (&acc)[2]) {
+                                             const float2* la, const float2* lb0,
+                                             const float2* lb1, uint32_t nkb, v16f (&acc)[2]) {
     constexpr bool kB0 = (MASK & 1) != 0, kB1 = (MASK & 2) != 0;
     const u32x4 z = {0u, 0u, 0u, 0u};
     u32x4 ca = pa[0], cb0 = kB0 ? pb0[0] : z, cb1 = kB1 ? pb1[0] : z;
@@ -1325,18 +1327,30 @@ __device__ __forceinline__ void f32_2w_kloop(const u32x4* pa, const u32x4* pb0, 
         const u32x4 na = pa[nk], nb0 = kB0 ? pb0[nk] : z, nb1 = kB1 ? pb1[nk] : z;
         __builtin_amdgcn_sched_barrier(0);  // keep the prefetch a block ahead
         // per code dword: all its operands first, then its MFMAs, so that
-        // the scheduler runs one dword's MFMAs beside the next one's LUT
+        // the scheduler runs one dword's MFMAs beside the next one's table
         // reads (the same MFMA sequence as picking each operand just before
-        // its MFMA)
+        // its MFMA).  Individual s of the dword is field s >> 2 of byte
+        // s & 3, so nibble j of byte k holds individuals k + 8j and k + 8j + 4.
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
             float fa[16], fb0[16], fb1[16];
 #pragma unroll
-            for (int s = 0; s < 16; ++s) {
-                const uint32_t sh_bits = 8 * (s & 3) + 2 * (s >> 2);
-                fa[s] = lut_pick(la, ca[t], sh_bits);
-                if (kB0) fb0[s] = lut_pick(lb0, cb0[t], sh_bits);
-                if (kB1) fb1[s] = lut_pick(lb1, cb1[t], sh_bits);
+            for (int q = 0; q < 8; ++q) {
+                const int k = q & 3, j = q >> 2, s0 = k + 8 * j;
+                const uint32_t shn = 8 * k + 4 * j;
+                const float2 pa2 = pair_pick(la, ca[t], shn);
+                fa[s0] = pa2.x;
+                fa[s0 + 4] = pa2.y;
+                if (kB0) {
+                    const float2 pb = pair_pick(lb0, cb0[t], shn);
+                    fb0[s0] = pb.x;
+                    fb0[s0 + 4] = pb.y;
+                }
+                if (kB1) {
+                    const float2 pb = pair_pick(lb1, cb1[t], shn);
+                    fb1[s0] = pb.x;
+                    fb1[s0 + 4] = pb.y;
+                }
             }
             __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
@@ -1367,22 +1381,21 @@ __device__ __forceinline__ void band_f32_2w_tile(const BandArgs& a, EpiShared2& 
     const u32x4* pb0 = a.tiled + (size_t)(2 * J) * gstride + lane;
     const u32x4* pb1 = pb0 + gstride;
     const uint32_t r0 = I * kTile + 32 * w, c0 = J * kTile;
-    // the tile's LUTs in LDS, in the epilogue's storage (unused until the K
-    // loop ends): wave 0 stores the row SNPs' (slots 0-63), wave 1 the
+    // the tile's pair tables in LDS, in the epilogue's storage (unused until
+    // the K loop ends): wave 0 stores the row SNPs' (slots 0-63), wave 1 the
     // column SNPs' (slots 64-127)
-    float* lut_s = reinterpret_cast<float*>(&sh.u);
+    float2* pair_s = reinterpret_cast<float2*>(&sh.u);
     {
         const float4 v = a.lut[(w ? c0 : I * kTile) + lane];
-        float* d = lut_s + 64 * w + lane;
-        d[0] = v.x;
-        d[kLutSlots] = v.y;
-        d[2 * kLutSlots] = v.z;
-        d[3 * kLutSlots] = v.w;
+        const float L4[4] = {v.x, v.y, v.z, v.w};
+        float2* d = pair_s + 64 * w + lane;
+#pragma unroll
+        for (int c = 0; c < 16; ++c) d[c * kLutSlots] = make_float2(L4[c & 3], L4[c >> 2]);
     }
     __syncthreads();
-    const float* la = lut_s + 32 * w + (lane & 31);
-    const float* lb0 = lut_s + 64 + (lane & 31);
-    const float* lb1 = lut_s + 96 + (lane & 31);
+    const float2* la = pair_s + 32 * w + (lane & 31);
+    const float2* lb0 = pair_s + 64 + (lane & 31);
+    const float2* lb1 = pair_s + 96 + (lane & 31);
 
     uint32_t lmin = a.lo[r0 + (lane & 31)];
 #pragma unroll
