@@ -434,8 +434,8 @@ inline uint32_t sketch_fused_ranges(uint64_t nblocks) {
     return (uint32_t)((nblocks + per - 1) / per);
 }
 // The dense one-pass form (k_sketch_dense): natural individual order, used
-// up to this many bucket blocks (d <= 128).
-constexpr uint64_t kSdMaxBlocks = 4;
+// up to this many bucket blocks (d <= 224).
+constexpr uint64_t kSdMaxBlocks = 7;
 // Per-SNP share slots of the norm buffer: enough for k_sketch (one per
 // kSkChunk = 4 blocks) and for k_sketch_fused (one per wave and range).
 inline uint32_t sketch_share_slots(uint64_t nblocks) {

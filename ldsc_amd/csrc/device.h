@@ -200,6 +200,7 @@ struct SketchPlan {
     // (lane | word << 6 | K-step << 8 | block << 9 | nibble << 12 |
     // negative << 15), for k_sketch_dense
     uint16_t* d_pcode = nullptr;
+    uint8_t* d_miss = nullptr;  // a row of all-missing codes (0x55), 32 ceil(N / 128) bytes
     void release();
 };
 }  // namespace ldsc

@@ -18,7 +18,7 @@
 //    one pass over the .bed windows (k_sketch_fused: LDS-staged rows, gather
 //    by precomputed word codes, MFMA); otherwise the rows are first tiled in
 //    bucket order (k_gather_chunk) and k_sketch reads the tiles.  For
-//    d <= 128 the one pass skips the bucket order altogether
+//    d <= 224 the one pass skips the bucket order altogether
 //    (k_sketch_dense: natural order, the +-1 bucket matrix built in LDS, the
 //    e2m1 MFMA over every block).
 //  * The bucket value y_b = inv_j (SG - mean_j SM) (the reference's f32 mean
@@ -52,6 +52,8 @@ void SketchPlan::release() {
     if (d_bstart) (void)hipFree(d_bstart);
     if (d_pcode) (void)hipFree(d_pcode);
     d_pcode = nullptr;
+    if (d_miss) (void)hipFree(d_miss);
+    d_miss = nullptr;
     d_tsum = nullptr;
     d_bo = nullptr;
     d_bstart = nullptr;
@@ -192,6 +194,9 @@ int sketch_plan(const ldsc_panel* panel_c, uint64_t d, SketchPlan** out, char* e
         if (e == hipSuccess) e = hipMalloc(&sp->d_pcode, pc.size() * 2);
         if (e == hipSuccess)
             e = hipMemcpy(sp->d_pcode, pc.data(), pc.size() * 2, hipMemcpyHostToDevice);
+        const size_t miss = (n + 127) / 128 * 32;
+        if (e == hipSuccess) e = hipMalloc(&sp->d_miss, miss);
+        if (e == hipSuccess) e = hipMemset(sp->d_miss, 0x55, miss);
     }
     if (e == hipSuccess) e = hipMalloc(&sp->d_tsum, tsum.size() * 4);
     if (e == hipSuccess)
@@ -814,158 +819,235 @@ __global__ __launch_bounds__(512) void k_sketch_fused(
 // v_mfma_scale_f32_32x32x64_f8f6f4: no gather and no bucket order.  Every
 // K-step of 64 individuals meets every bucket block, so the MFMA work is
 // two products per block and K-step; it pays while the blocks are few
-// (measured: faster than k_sketch_fused up to 4 blocks, slower beyond).
-//  * Workgroup q: kSdWaves waves, wave v owning SNP group kSdWaves q + v and
-//    all NB blocks (at most kSdMaxBlocks, so that two workgroups share a CU
-//    and hide each other's latencies).
-//  * P for a pair of K-steps (128 individuals) is built in LDS from the
-//    plan's per-individual code (lane, word, nibble, block and sign of the
-//    individual's entry in the A-operand layout), one LDS atomic OR per
-//    individual; three buffers rotate (read / being built / being cleared)
-//    with one barrier per pair.  The codes and the row bytes load kSdAhead
-//    pairs ahead of use.
+// (measured: faster than k_sketch_fused up to 7 blocks, slower at 8).
+//  * Workgroup q: 8 waves, two on each SIMD.  Waves v and v + 4 own SNP
+//    group 4q + v; wave v the first ceil(NB/2) blocks, wave v + 4 the rest
+//    (so that no wave holds more than 4 x 2 x 16 accumulators).  The second
+//    half also builds P and counts the integer stats, which it passes to
+//    the first through LDS at the end.
+//  * P for a phase of two pairs of K-steps (256 individuals) is built in
+//    LDS from the plan's per-individual code (lane, word, nibble, block and
+//    sign of the individual's entry in the A-operand layout), one LDS atomic
+//    OR per individual; three buffers rotate (read / being built / being
+//    cleared) with one barrier per phase, and a buffer is cleared by XOR-ing
+//    out the entries its builders placed two phases before.
 //  * The B operands come straight from the .bed rows: lane l holds SNP
 //    32g + l%32 and, per pair p, 16 bytes of its row (individuals
 //    128p + 64h .. +63, h = l/32: the first 8 bytes K-step 2p, the next 8
 //    K-step 2p+1).  Each 2-bit PLINK code c becomes the value code
 //    v = c ^ 2 ^ (c >> 1) (the dosage; 3 when missing), split into the nibble
-//    operands X = v/2 and Q = [v = 3]/2 as in the exact band kernel; the
-//    integer stats are counted from the same words.
+//    operands X = v/2 and Q = [v = 3]/2 as in the exact band kernel.  Rows
+//    and codes load two phases ahead; padding SNPs read an all-missing row.
+//  * The K loop is software-pipelined by K-step: the next K-step's operands
+//    are expanded, and its P tiles read from LDS, while this K-step's MFMAs
+//    run.
 //  * The f32 sums are exact (multiples of 1/2 below 1.5 N), and
-//    SG = 2 PX - 6 PQ, SM = T_b - 2 PQ.  The epilogue is k_sketch_fused's, with
-//    one norm share slot.
+//    SG = 2 PX - 6 PQ, SM = T_b - 2 PQ.  The epilogue is k_sketch_fused's,
+//    with one norm share slot per half.
 constexpr uint32_t kSdWaves = 4;
-constexpr int kSdAhead = 4;  // pairs of row bytes and codes in flight
+constexpr uint32_t kSdPhasePairs = 2;  // pairs of K-steps per barrier interval (phase)
 
-// X and Q nibble operands of 32 individuals from two .bed words (16 each):
-// word q of the operand holds, in its nibble jj, individual 2jj (q = 0),
-// 2jj + 1 (q = 1), 16 + 2jj (q = 2) or 17 + 2jj (q = 3) of the 32.
-__device__ __forceinline__ void plink_value_nibbles(uint32_t w0, uint32_t w1, v8i& X, v8i& Q) {
-    v4i x, q;
-#pragma unroll
-    for (int t = 0; t < 2; ++t) {
-        const uint32_t w = t ? w1 : w0;
-        const uint32_t L = w & 0x55555555u, H = (w >> 1) & 0x55555555u;
-        const uint32_t v = (L ^ H) | ((H ^ 0x55555555u) << 1);  // value codes
-        const uint32_t mq = L & ~H;                             // code 01: missing
-        x[2 * t] = (int)(v & 0x33333333u);
-        x[2 * t + 1] = (int)((v >> 2) & 0x33333333u);
-        q[2 * t] = (int)(mq & 0x11111111u);
-        q[2 * t + 1] = (int)((mq >> 2) & 0x11111111u);
+// One .bed dword of the lane's row (16 cells of PLINK codes, L at the even
+// and H at the odd bit of each): the nibble operands X (value code v = the
+// dosage, 3 when missing; e2m1 v/2) and Q ([v = 3]; e2m1 1/2) of cells 2jj
+// (xe, qe) and 2jj + 1 (xo, qo) at nibble jj, and the counters a (codes 01
+// and 11), b (codes 10 and 11) and c (code 11).
+template <bool STATS>
+__device__ __forceinline__ void dense_word(uint32_t w, uint32_t& xe, uint32_t& xo, uint32_t& qe,
+                                           uint32_t& qo, uint32_t& a, uint32_t& b, uint32_t& c) {
+    const uint32_t t1 = w >> 1;
+    const uint32_t hb = t1 & 0x55555555u;         // H, at the cell's even bit
+    const uint32_t v = (w ^ hb) ^ 0xAAAAAAAAu;    // (L ^ H) | (~H << 1)
+    const uint32_t mq = (w & ~t1) & 0x55555555u;  // L & ~H: code 01
+    xe = v & 0x33333333u;
+    xo = (v >> 2) & 0x33333333u;
+    qe = mq & 0x11111111u;
+    qo = (mq >> 2) & 0x11111111u;
+    if (STATS) {
+        a += __popc(w & 0x55555555u);
+        b += __popc(hb);
+        c += __popc(w & hb);
     }
-    X = __builtin_shufflevector(x, x, 0, 1, 2, 3, -1, -1, -1, -1);
-    Q = __builtin_shufflevector(q, q, 0, 1, 2, 3, -1, -1, -1, -1);
 }
 
-template <int NB, bool RAW>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_sketch_dense(
-    const uint8_t* __restrict__ slabs, uint64_t pitch, uint32_t row_dw, uint32_t n,
+// The K loop and epilogue of one half of the workgroup: its four waves own
+// SNP groups 4 blockIdx.x + (wave & 3) and the NBW blocks B0 .. B0 + NBW - 1.
+// The BUILD half also keeps the P buffers (its 256 threads place the 256
+// individuals of a phase and clear the entries of two phases before).
+template <int NB, int NBW, bool BUILD, bool STATS, bool RAW>
+__device__ __forceinline__ void sketch_dense_half(
+    uint32_t* lds, const uint8_t* __restrict__ slabs, uint64_t pitch, uint32_t row_dw, uint32_t n,
     const uint64_t* __restrict__ bed_idx, uint64_t base, uint32_t m, uint32_t m_pad,
-    const uint16_t* __restrict__ pcode,
+    const uint16_t* __restrict__ pcode, const uint8_t* __restrict__ miss_row,
     const int32_t* __restrict__ tsum, uint32_t* __restrict__ stats_out,
     double* __restrict__ norm, double* __restrict__ asum, void* __restrict__ cols,
     int32_t* __restrict__ flag, uint32_t d, int32_t* __restrict__ sg_out,
-    int32_t* __restrict__ sm_out) {
-    extern __shared__ uint32_t lds[];  // three P buffers of kBuf words
-    constexpr uint32_t kBuf = 2 * NB * 64 * 4;
+    int32_t* __restrict__ sm_out, uint32_t B0, uint32_t slot) {
+    constexpr uint32_t kPair = 2 * NB * 64 * 4;  // P words of a pair of K-steps
+    constexpr uint32_t kBuf = kSdPhasePairs * kPair;
     const uint32_t ngroups = m_pad / 32;
-    const int tid = threadIdx.x, wv = tid >> 6, ln = tid & 63, h = ln >> 5;
+    const int tid = threadIdx.x & 255, wv = tid >> 6, ln = tid & 63, h = ln >> 5;
     const uint32_t g = blockIdx.x * kSdWaves + wv;
     const uint32_t snp = 32 * g + (ln & 31);
-    const bool real = g < ngroups && snp < m;  // padding SNPs read as all missing
+    const bool real = g < ngroups && snp < m;  // padding SNPs read an all-missing row
     const uint64_t row = real ? (bed_idx ? bed_idx[snp] : base + snp) : 0;
-    const uint32_t* rp = reinterpret_cast<const uint32_t*>(slabs + row * pitch);
+    const uint32_t* rp = reinterpret_cast<const uint32_t*>(real ? slabs + row * pitch : miss_row);
     const uint32_t npairs = (n + 127) / 128;
+    const uint32_t nph = (npairs + kSdPhasePairs - 1) / kSdPhasePairs;
+    const uint32_t niter = (nph + 1) / 2;  // two phases an iteration
+    // Row bytes and codes load unconditionally (from a clamped address) into
+    // registers that stay in flight for two phases; what is out of range is
+    // replaced where it is used.  The lane's 16 row bytes of pair p are
+    // individuals 128p + 64h .. +63.
+    auto row_ok = [&](uint32_t p) { return p < npairs && 8 * p + 4 * h + 4 <= row_dw; };
     auto load_row = [&](uint32_t p) {
-        const uint32_t q0 = 8 * p + 4 * h;
-        return real && p < npairs && q0 + 4 <= row_dw
-                   ? *reinterpret_cast<const uint4*>(rp + q0)
-                   : make_uint4(0x55555555u, 0x55555555u, 0x55555555u, 0x55555555u);
+        return *reinterpret_cast<const uint4*>(rp + (row_ok(p) ? 8 * p + 4 * h : 0));
     };
-    auto clear = [&](uint32_t* buf) {
-        for (uint32_t x = 4 * tid; x < kBuf; x += 4 * 64 * kSdWaves)
-            *reinterpret_cast<uint4*>(buf + x) = make_uint4(0u, 0u, 0u, 0u);
+    // thread t places individual 256 f + t of phase f (code: lane | word << 6 |
+    // K-step << 8 | block << 9 | nibble << 12 | negative << 15), in pair t / 128
+    auto load_code = [&](uint32_t f) {
+        const uint32_t i = 256 * f + tid;
+        return (uint32_t)pcode[i < n ? i : 0];
     };
-    // thread t < 128 places individual 128p + t (code: lane | word << 6 |
-    // K-step << 8 | block << 9 | nibble << 12 | negative << 15)
-    auto load_code = [&](uint32_t p) {
-        const uint32_t i = 128 * p + tid;
-        return tid < 128 && i < n ? (uint32_t)pcode[i] : 0xFFFFFFFFu;
+    auto code_ok = [&](uint32_t c, uint32_t f) { return 256 * f + tid < n ? c : 0xFFFFFFFFu; };
+    const uint32_t pair_off = (tid >> 7) * kPair;
+    auto entry = [&](uint32_t c, uint32_t& word, uint32_t& bits) {
+        // an absent code (c = ~0) ORs 0 into the pair's first word
+        const bool ok = c != 0xFFFFFFFFu;
+        word = pair_off + (ok ? ((((c >> 8) & 1u) * NB + ((c >> 9) & 7u)) * 64 + (c & 63u)) * 4 +
+                                    ((c >> 6) & 3u)
+                              : 0u);
+        bits = ok ? ((c >> 15) & 1u ? 0xAu : 0x2u) << (4 * ((c >> 12) & 7u)) : 0u;
     };
-    auto build = [&](uint32_t c, uint32_t* buf) {
-        if (c != 0xFFFFFFFFu) {
-            const uint32_t word = ((((c >> 8) & 1u) * NB + ((c >> 9) & 7u)) * 64 + (c & 63u)) * 4 +
-                                  ((c >> 6) & 3u);
-            atomicOr(buf + word, ((c >> 15) & 1u ? 0xAu : 0x2u) << (4 * ((c >> 12) & 7u)));
+    // Every pair before the last is whole and inside the row (the row
+    // stride holds ceil(N / 4) bytes); the last pair's cells past n and the
+    // pairs after it read as code 01 (missing).  The branch is uniform.
+    auto sanitize = [&](uint4 r, uint32_t p) {
+        if (p + 1 >= npairs) {
+            const uint32_t q0 = 8 * p + 4 * h;
+            if (!row_ok(p)) r = make_uint4(0x55555555u, 0x55555555u, 0x55555555u, 0x55555555u);
+            r.x = ts_mask(r.x, q0, n);
+            r.y = ts_mask(r.y, q0 + 1, n);
+            r.z = ts_mask(r.z, q0 + 2, n);
+            r.w = ts_mask(r.w, q0 + 3, n);
         }
+        return r;
     };
-
-    v16f px[NB], pq[NB];
+    constexpr int NBA = NBW > 0 ? NBW : 1;  // array extent
+    v16f px[NBA], pq[NBA];
 #pragma unroll
-    for (int B = 0; B < NB; ++B)
+    for (int B = 0; B < NBA; ++B)
 #pragma unroll
         for (int x = 0; x < 16; ++x) px[B][x] = pq[B][x] = 0.0f;
-    uint32_t n2 = 0, n1 = 0, n0 = 0;
-    uint4 ring[kSdAhead];
-    uint32_t cring[kSdAhead];  // codes of pairs p + 1 .. p + kSdAhead
+    uint32_t ca = 0, cb = 0, cc = 0;
+    auto expand = [&](uint32_t w0, uint32_t w1, v8i& X, v8i& Q) {
+        uint32_t e0, e1, e2, e3, g0, g1, g2, g3;
+        dense_word<STATS>(w0, e0, e1, g0, g1, ca, cb, cc);
+        dense_word<STATS>(w1, e2, e3, g2, g3, ca, cb, cc);
+        v4i x4, q4;
+        x4[0] = (int)e0; x4[1] = (int)e1; x4[2] = (int)e2; x4[3] = (int)e3;
+        q4[0] = (int)g0; q4[1] = (int)g1; q4[2] = (int)g2; q4[3] = (int)g3;
+        X = __builtin_shufflevector(x4, x4, 0, 1, 2, 3, -1, -1, -1, -1);
+        Q = __builtin_shufflevector(q4, q4, 0, 1, 2, 3, -1, -1, -1, -1);
+    };
+    // P tiles of the wave's blocks for K-step s of the phase in buffer `buf`
+    // (pair s / 2, K-step s % 2)
+    auto read_p = [&](uint32_t buf, int s, v4i (&a)[NBA]) {
+        const uint32_t* src =
+            lds + buf + (s >> 1) * kPair + ((s & 1) * NB + B0) * 256 + ln * 4;
 #pragma unroll
-    for (int u = 0; u < kSdAhead; ++u) {
-        ring[u] = load_row(u);
-        cring[u] = load_code(u + 1);
+        for (int B = 0; B < NBW; ++B) a[B] = *reinterpret_cast<const v4i*>(src + B * 256);
+    };
+    auto mma = [&](const v4i (&a)[NBA], const v8i& X, const v8i& Q) {
+#pragma unroll
+        for (int B = 0; B < NBW; ++B) {
+            const v8i a8 = __builtin_shufflevector(a[B], a[B], 0, 1, 2, 3, -1, -1, -1, -1);
+            px[B] = LDSC_MFMA_FP4(a8, X, px[B]);
+            pq[B] = LDSC_MFMA_FP4(a8, Q, pq[B]);
+        }
+    };
+    uint4 ring[2 * kSdPhasePairs];  // row bytes, pair 2f + pp in slot 2(f % 2) + pp
+    uint32_t cring[2] = {0u, 0u};   // code of phase f + 1, in slot f % 2
+    uint32_t cw[2] = {0u, 0u}, cbits[2] = {0u, 0u};  // entry built in phase f - 2 (slot f % 2)
+#pragma unroll
+    for (int k = 0; k < 2 * (int)kSdPhasePairs; ++k) ring[k] = load_row(k);
+    if (BUILD) {
+        cring[0] = load_code(1);
+        cring[1] = load_code(2);
     }
-    clear(lds);
-    clear(lds + kBuf);
-    clear(lds + 2 * kBuf);
+    for (uint32_t x = 4 * threadIdx.x; x < 3 * kBuf; x += 4 * 64 * 2 * kSdWaves)
+        *reinterpret_cast<uint4*>(lds + x) = make_uint4(0u, 0u, 0u, 0u);
     __syncthreads();
-    build(load_code(0), lds);
+    if (BUILD) {
+        uint32_t wd, bt;
+        entry(code_ok(load_code(0), 0), wd, bt);
+        atomicOr(lds + wd, bt);
+        cw[1] = wd;
+        cbits[1] = bt;
+    }
     __syncthreads();
-    for (uint32_t p0 = 0; p0 < npairs; p0 += kSdAhead) {
+    uint32_t rd = 0, bd = kBuf, cl = 2 * kBuf;  // read / being built / being cleared
+    // Software-pipelined by K-step (four a phase): the next K-step's
+    // operands are expanded, and its P tiles read, while this K-step's MFMAs
+    // run.  Past the last pair the rows read as all missing and no P entry
+    // exists.
+    v8i X0, Q0, X1, Q1;
+    ring[0] = sanitize(ring[0], 0);
+    expand(ring[0].x, ring[0].y, X0, Q0);
+    v4i pa[NBA], pb[NBA];
+    for (uint32_t it = 0; it < niter; ++it) {
 #pragma unroll
-        for (int u = 0; u < kSdAhead; ++u) {
-            const uint32_t p = p0 + u;
-            if (p < npairs) {  // uniform over the workgroup
-                const uint32_t* cur = lds + (p % 3) * kBuf;
-                build(cring[u], lds + ((p + 1) % 3) * kBuf);  // no-op past the last pair
-                cring[u] = load_code(p + 1 + kSdAhead);
-                clear(lds + ((p + 2) % 3) * kBuf);
-                const uint4 raw = ring[u];
-                ring[u] = load_row(p + kSdAhead);
-#pragma unroll
-                for (int kk = 0; kk < 2; ++kk) {
-                    const uint32_t w0 = kk ? raw.z : raw.x, w1 = kk ? raw.w : raw.y;
-                    {  // the integer stats
-                        const uint32_t i0 = 128 * p + 64 * h + 32 * kk;
-#pragma unroll
-                        for (int t = 0; t < 2; ++t) {
-                            const uint32_t w = t ? w1 : w0, ib = i0 + 16 * t;
-                            uint32_t vm = 0x55555555u;
-                            if (ib + 16 > n)
-                                vm = ib >= n ? 0u : ((1u << (2 * (n - ib))) - 1u) & 0x55555555u;
-                            const uint32_t L = w & 0x55555555u, H = (w >> 1) & 0x55555555u;
-                            n2 += __popc(~(L | H) & vm);  // 00: dosage 2
-                            n1 += __popc(~L & H & vm);    // 10: dosage 1
-                            n0 += __popc(L & H & vm);     // 11: dosage 0
-                        }
-                    }
-                    v8i X, Q;
-                    plink_value_nibbles(w0, w1, X, Q);
-#pragma unroll
-                    for (int B = 0; B < NB; ++B) {
-                        const v4i a4 = *reinterpret_cast<const v4i*>(cur + ((kk * NB + B) * 64 + ln) * 4);
-                        const v8i a8 = __builtin_shufflevector(a4, a4, 0, 1, 2, 3, -1, -1, -1, -1);
-                        px[B] = LDSC_MFMA_FP4(a8, X, px[B]);
-                        pq[B] = LDSC_MFMA_FP4(a8, Q, pq[B]);
-                    }
-                }
-                __syncthreads();
+        for (int u = 0; u < 2; ++u) {
+            const uint32_t f = 2 * it + u;
+            const uint32_t p0 = kSdPhasePairs * f;
+            read_p(rd, 0, pa);
+            if (BUILD) {
+                uint32_t wd, bt;
+                entry(code_ok(cring[u], f + 1), wd, bt);
+                atomicOr(lds + bd + wd, bt);
+                atomicXor(lds + cl + cw[u], cbits[u]);
+                cw[u] = wd;
+                cbits[u] = bt;
             }
+            const uint4 r1 = sanitize(ring[2 * u + 1], p0 + 1);
+            read_p(rd, 1, pb);
+            expand(ring[2 * u].z, ring[2 * u].w, X1, Q1);
+            mma(pa, X0, Q0);
+            ring[2 * u] = load_row(p0 + 2 * kSdPhasePairs);
+            read_p(rd, 2, pa);
+            expand(r1.x, r1.y, X0, Q0);
+            mma(pb, X1, Q1);
+            read_p(rd, 3, pb);
+            expand(r1.z, r1.w, X1, Q1);
+            mma(pa, X0, Q0);
+            ring[2 * u + 1] = load_row(p0 + 1 + 2 * kSdPhasePairs);
+            ring[2 * (1 - u)] = sanitize(ring[2 * (1 - u)], p0 + kSdPhasePairs);
+            expand(ring[2 * (1 - u)].x, ring[2 * (1 - u)].y, X0, Q0);
+            mma(pb, X1, Q1);
+            if (BUILD) cring[u] = load_code(f + 3);
+            __syncthreads();
+            const uint32_t t = rd;
+            rd = bd;
+            bd = cl;
+            cl = t;
         }
     }
+    // The STATS half counted the integer stats (every lane expanded
+    // 2 + 16 niter words of 16 cells; the words past the last pair read as
+    // missing) and passes them to the other half through LDS.
+    uint4 st = make_uint4(0u, 0u, 0u, 0u);
+    if (STATS) {
+        uint32_t n0 = cc, n1 = cb - cc, n2 = 16 * (2 + 16 * niter) - ca - cb + cc;
+        n2 += __shfl_xor(n2, 32);
+        n1 += __shfl_xor(n1, 32);
+        n0 += __shfl_xor(n0, 32);
+        st = make_uint4(2 * n2 + n1, n2 + n1 + n0, 4 * n2 + n1, n1);
+        if (h == 0) reinterpret_cast<uint4*>(lds)[wv * 32 + (ln & 31)] = st;
+    }
+    __syncthreads();
+    if (!STATS) st = reinterpret_cast<const uint4*>(lds)[wv * 32 + (ln & 31)];
     if (g >= ngroups) return;
-    n2 += __shfl_xor(n2, 32);
-    n1 += __shfl_xor(n1, 32);
-    n0 += __shfl_xor(n0, 32);
-    const uint32_t S = 2 * n2 + n1, C = n2 + n1 + n0, SS = 4 * n2 + n1;
+    const uint32_t S = st.x, C = st.y, SS = st.z;
     // T_b of the lane's 16 buckets of block B (bucket 32B + 8(x/4) + 4h + x%4)
     auto load_t = [&](uint32_t B, int (&T)[16]) {
 #pragma unroll
@@ -980,12 +1062,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     if (RAW) {
         if (snp >= m) return;
 #pragma unroll
-        for (int B = 0; B < NB; ++B) {
+        for (int B = 0; B < NBW; ++B) {
             int T[16];
-            load_t(B, T);
+            load_t(B0 + B, T);
 #pragma unroll
             for (int x = 0; x < 16; ++x) {
-                const uint32_t b = 32 * B + 8 * (x >> 2) + 4 * h + (x & 3);
+                const uint32_t b = 32 * (B0 + B) + 8 * (x >> 2) + 4 * h + (x & 3);
                 const int sq = (int)(2.0f * pq[B][x]);
                 if (b < d) {
                     sg_out[(size_t)snp * d + b] = (int)(2.0f * px[B][x]) - 3 * sq;
@@ -995,7 +1077,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
         }
         return;
     }
-    if (h == 0) reinterpret_cast<uint4*>(stats_out)[snp] = make_uint4(S, C, SS, n1);
+    if (STATS && h == 0) reinterpret_cast<uint4*>(stats_out)[snp] = st;
     float a32, i32;
     (void)ref_mean_inv_f32(S, C, SS, n, a32, i32);
     const float kf = rintf(a32);
@@ -1006,9 +1088,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     uint4* W = static_cast<uint4*>(cols);
     uint4* E = W + (size_t)m_pad * NB * 2;  // m_pad x d_pad bytes later
 #pragma unroll
-    for (int B = 0; B < NB; ++B) {
+    for (int B = 0; B < NBW; ++B) {
         int T[16];
-        load_t(B, T);
+        load_t(B0 + B, T);
         uint32_t wd[4] = {0u, 0u, 0u, 0u}, ed[4] = {0u, 0u, 0u, 0u};
         uint32_t out = 0u;
 #pragma unroll
@@ -1024,17 +1106,41 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
             as += ((double)w + dl * (double)e) * (double)T[x];
         }
         bad = bad || (out && snp < m);
-        const size_t o = ((size_t)g * NB + B) * 64 + ln;
+        const size_t o = ((size_t)g * NB + B0 + B) * 64 + ln;
         W[o] = make_uint4(wd[0], wd[1], wd[2], wd[3]);
         E[o] = make_uint4(ed[0], ed[1], ed[2], ed[3]);
     }
     nrm += __shfl_xor(nrm, 32);
     as += __shfl_xor(as, 32);
     if (h == 0) {
-        norm[snp] = nrm;
-        asum[snp] = as;
+        norm[(size_t)slot * m_pad + snp] = nrm;
+        asum[(size_t)slot * m_pad + snp] = as;
     }
     if (bad) flag[0] = 1;  // a plain store: every writer stores 1
+}
+
+// Eight waves: waves 0-3 own the first ceil(NB/2) blocks of SNP groups
+// 4 blockIdx.x .. +3, waves 4-7 the rest of the same groups (and build the
+// P buffers), so that two waves share each SIMD.
+template <int NB, bool RAW>
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_sketch_dense(
+    const uint8_t* __restrict__ slabs, uint64_t pitch, uint32_t row_dw, uint32_t n,
+    const uint64_t* __restrict__ bed_idx, uint64_t base, uint32_t m, uint32_t m_pad,
+    const uint16_t* __restrict__ pcode, const uint8_t* __restrict__ miss_row,
+    const int32_t* __restrict__ tsum, uint32_t* __restrict__ stats_out,
+    double* __restrict__ norm, double* __restrict__ asum, void* __restrict__ cols,
+    int32_t* __restrict__ flag, uint32_t d, int32_t* __restrict__ sg_out,
+    int32_t* __restrict__ sm_out) {
+    extern __shared__ uint32_t lds[];  // three phase buffers of P
+    constexpr int NB0 = (NB + 1) / 2;
+    if (threadIdx.x < 256)
+        sketch_dense_half<NB, NB0, false, false, RAW>(
+            lds, slabs, pitch, row_dw, n, bed_idx, base, m, m_pad, pcode, miss_row, tsum,
+            stats_out, norm, asum, cols, flag, d, sg_out, sm_out, 0, 0);
+    else
+        sketch_dense_half<NB, NB - NB0, true, true, RAW>(
+            lds, slabs, pitch, row_dw, n, bed_idx, base, m, m_pad, pcode, miss_row, tsum,
+            stats_out, norm, asum, cols, flag, d, sg_out, sm_out, NB0, 1);
 }
 
 hipError_t launch_sketch_fused(const SketchPlan* sp, const ldsc_panel* panel,
@@ -1055,23 +1161,30 @@ hipError_t launch_sketch_fused(const SketchPlan* sp, const ldsc_panel* panel,
     if (sp->d_pcode) {  // natural order, at most kSdMaxBlocks blocks
         static const void* const dfn[2][kSdMaxBlocks] = {
             {(const void*)k_sketch_dense<1, false>, (const void*)k_sketch_dense<2, false>,
-             (const void*)k_sketch_dense<3, false>, (const void*)k_sketch_dense<4, false>},
+             (const void*)k_sketch_dense<3, false>, (const void*)k_sketch_dense<4, false>,
+             (const void*)k_sketch_dense<5, false>, (const void*)k_sketch_dense<6, false>,
+             (const void*)k_sketch_dense<7, false>},
             {(const void*)k_sketch_dense<1, true>, (const void*)k_sketch_dense<2, true>,
-             (const void*)k_sketch_dense<3, true>, (const void*)k_sketch_dense<4, true>}};
+             (const void*)k_sketch_dense<3, true>, (const void*)k_sketch_dense<4, true>,
+             (const void*)k_sketch_dense<5, true>, (const void*)k_sketch_dense<6, true>,
+             (const void*)k_sketch_dense<7, true>}};
         const void* kfn = dfn[raw ? 1 : 0][nb - 1];
-        const size_t lds = (size_t)3 * 2 * nb * 64 * 4 * 4;
+        const size_t lds = (size_t)3 * kSdPhasePairs * 2 * nb * 64 * 4 * 4;
         const uint8_t* slabs = panel->d_slabs;
         uint64_t pitch = panel->stride, a_base = base;
         uint32_t row_dw = (uint32_t)(panel->stride / 4), n = (uint32_t)panel->n;
         uint32_t a_m = m, a_mp = m_pad, a_d = (uint32_t)sp->d;
         const uint16_t* pc = sp->d_pcode;
+        const uint8_t* miss = sp->d_miss;
         const int32_t* ts = sp->d_tsum;
         void* kargs[] = {(void*)&slabs, (void*)&pitch, (void*)&row_dw, (void*)&n,
                          (void*)&d_bed_idx, (void*)&a_base, (void*)&a_m, (void*)&a_mp,
-                         (void*)&pc, (void*)&ts, (void*)&stats_out, (void*)&norm, (void*)&asum,
+                         (void*)&pc, (void*)&miss, (void*)&ts, (void*)&stats_out, (void*)&norm, (void*)&asum,
                          (void*)&cols, (void*)&flag, (void*)&a_d, (void*)&sg_out, (void*)&sm_out};
-        return hipLaunchKernel(kfn, dim3((ngroups + kSdWaves - 1) / kSdWaves), dim3(64 * kSdWaves),
-                               kargs, lds, st);
+        e = hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+        return hipLaunchKernel(kfn, dim3((ngroups + kSdWaves - 1) / kSdWaves),
+                               dim3(2 * 64 * kSdWaves), kargs, lds, st);
     }
     const size_t lds = (size_t)(8 * kSfStride + kSfWaves * kSfStage) * 4;
     static const void* const fns[2][3] = {
@@ -1104,7 +1217,7 @@ hipError_t launch_sketch_fused_scale(const SketchPlan* sp, const float2* musc, u
                                      double n, double* norm, hipStream_t st) {
     // k_sketch_dense writes one share slot, k_sketch_fused one per wave and
     // range
-    const uint32_t slots = sp->d_pcode ? 1u : kSfWaves * sketch_fused_ranges(sp->nblocks);
+    const uint32_t slots = sp->d_pcode ? 2u : kSfWaves * sketch_fused_ranges(sp->nblocks);
     double* asum = norm + (size_t)sketch_slots(sp) * m_pad;
     double* consts = const_cast<double*>(sketch_i8_consts(sp, norm, m_pad));
     hipLaunchKernelGGL(k_sketch_scale<kSketchI8>, dim3((m_pad + 255) / 256), dim3(256), 0, st,

@@ -82,16 +82,19 @@ def test_sketch_buckets_bit_exact(oracle, n, d):
     assert np.array_equal(sg, esg) and np.array_equal(sm, esm)
 
 
-@pytest.mark.parametrize("n,d", [(16385, 7), (20001, 128), (40000, 200), (20001, 256),
+@pytest.mark.parametrize("n,d", [(16385, 7), (20001, 128), (16385, 150), (20001, 190),
+                                 (40000, 200), (16500, 224), (20001, 256),
                                  (40000, 3000), (16884, 2000), (33000, 5000)])
 def test_sketch_buckets_chunked_order(oracle, n, d):
-    """More members than one 16,384-member chunk.  Up to 4 bucket blocks
-    (d <= 128) the sums come from the dense one-pass kernel (natural order,
-    the bucket matrix built in LDS; n = 16,385 and 20,001 end in a partial
-    pair of K-steps).  Beyond, each chunk is put in bucket order on its own
-    (the last one ragged) and the one-pass gather kernel sums it; a short
-    last chunk (500 or 232 members) has K-steps that reach several bucket
-    blocks.  The integer sums do not depend on either order."""
+    """More members than one 16,384-member chunk.  Up to 7 bucket blocks
+    (d <= 224) the sums come from the dense one-pass kernel (natural order,
+    the bucket matrix built in LDS, the blocks of a SNP group split over two
+    waves; n = 16,385, 16,500 and 20,001 end in a partial pair of K-steps,
+    and in a phase of two pairs that is partly past the last individual).
+    Beyond, each chunk is put in bucket order on its own (the last one
+    ragged) and the one-pass gather kernel sums it; a short last chunk (500
+    or 232 members) has K-steps that reach several bucket blocks.  The
+    integer sums do not depend on either order."""
     print(f"sketch buckets chunked n={n} d={d}", flush=True)
     p = _panel(n, 70, seed=n + d)
     idx = np.arange(70, dtype=np.uint64)[::-1].copy()
