@@ -944,9 +944,8 @@ __device__ __forceinline__ void sketch_dense_half(
         uint32_t e0, e1, e2, e3, g0, g1, g2, g3;
         dense_word<STATS>(w0, e0, e1, g0, g1, ca, cb, cc);
         dense_word<STATS>(w1, e2, e3, g2, g3, ca, cb, cc);
-        v4i x4, q4;
-        x4[0] = (int)e0; x4[1] = (int)e1; x4[2] = (int)e2; x4[3] = (int)e3;
-        q4[0] = (int)g0; q4[1] = (int)g1; q4[2] = (int)g2; q4[3] = (int)g3;
+        const v4i x4 = {(int)e0, (int)e1, (int)e2, (int)e3};
+        const v4i q4 = {(int)g0, (int)g1, (int)g2, (int)g3};
         X = __builtin_shufflevector(x4, x4, 0, 1, 2, 3, -1, -1, -1, -1);
         Q = __builtin_shufflevector(q4, q4, 0, 1, 2, 3, -1, -1, -1, -1);
     };
