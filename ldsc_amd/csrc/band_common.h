@@ -75,6 +75,27 @@ __device__ __forceinline__ void expand_codes(uint32_t w, v4i& G, v4i& M) {
     }
 }
 
+// e2m1 operands of one K-step from the lane's two value-code dwords w0, w1
+// (32 individuals): X = v/2 (nibble = v in its low bits: 0, 0.5, 1, 1.5) and
+// Q = q/2 (q = 1 iff v = 3: missing or padding).  Fields 2n and 2n+1 of a
+// dword go to nibble n of two operand dwords; the same placement on both
+// operands keeps every individual paired with itself.  The MFMA reads 4 of
+// the 8 operand dwords.
+__device__ __forceinline__ void vcode_operands(uint32_t w0, uint32_t w1, v8i& X, v8i& Q) {
+    // shifts of the 64-bit pair (w1:w0): the bits w1 pushes into the top of
+    // w0 land outside every mask below
+    const uint64_t w = ((uint64_t)w1 << 32) | w0;
+    const uint64_t s1 = w >> 1, s2 = w >> 2, s3 = w >> 3;
+    const uint32_t a2 = (uint32_t)s2, b2 = (uint32_t)(s2 >> 32);
+    const v4i x = {(int)(w0 & 0x33333333u), (int)(a2 & 0x33333333u),
+                   (int)(w1 & 0x33333333u), (int)(b2 & 0x33333333u)};
+    const v4i q = {(int)(w0 & (uint32_t)s1 & 0x11111111u), (int)(a2 & (uint32_t)s3 & 0x11111111u),
+                   (int)(w1 & (uint32_t)(s1 >> 32) & 0x11111111u),
+                   (int)(b2 & (uint32_t)(s3 >> 32) & 0x11111111u)};
+    X = __builtin_shufflevector(x, x, 0, 1, 2, 3, -1, -1, -1, -1);
+    Q = __builtin_shufflevector(q, q, 0, 1, 2, 3, -1, -1, -1, -1);
+}
+
 // Cells of individuals >= nv (the row's real count; 0 for a padding row)
 // of dword q read as code 01 (missing).
 __device__ __forceinline__ uint32_t ts_mask(uint32_t raw, uint64_t q, uint32_t nv) {

@@ -185,9 +185,13 @@ struct SketchPlan {
     uint16_t* d_local = nullptr;  // chunks: gc_code of the member at each position (its
                                   // LDS word and field in the staged chunk window;
                                   // kGcPastCode past the last member), nkb * 128 entries
-    void* d_ptiles = nullptr;     // ntiles x 1 KiB signed int8 MFMA A-operand tiles
-    int4* d_blocks = nullptr;     // per bucket block: first tile, number of tiles
-    int32_t* d_tile_ks = nullptr; // K-step (32 positions) of each tile
+    void* d_ptiles = nullptr;     // ntiles x 1 KiB signed int8 MFMA A-operand tiles (k_sketch_fused)
+    // k_sketch: e2m1 A-operand tiles (1 KiB each) of 64-position K-steps,
+    // per bucket block its first tile and number of tiles, and each tile's
+    // K-step
+    void* d_ptiles4 = nullptr;
+    int4* d_blocks = nullptr;
+    int32_t* d_tile_ks = nullptr;
     int32_t* d_tsum = nullptr;    // per bucket (d_pad, zero padded): T_b = sum of its signs
     double tt = 0.0;              // sum over buckets of T_b^2
     // chunk-local order only: per chunk c, the chunk-local position where

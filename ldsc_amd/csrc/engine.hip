@@ -714,7 +714,7 @@ __global__ __launch_bounds__(512) void k_gather4(const uint8_t* __restrict__ sla
 }
 
 // ------------------------------------------------------------------------
-// Kernel 1d: chunk-local gather into PLINK-code tiles (CountSketch order)
+// Kernel 1d: chunk-local gather into dosage-value tiles (CountSketch order)
 // ------------------------------------------------------------------------
 // Workgroup (h, c) owns SNP rows 16h .. 16h+15 and member chunk c of
 // MEMBERS members, whose members come from FAM rows c*MEMBERS .. : one
@@ -724,8 +724,9 @@ __global__ __launch_bounds__(512) void k_gather4(const uint8_t* __restrict__ sla
 // quad w = byte k of its four rows, in 16-byte slots placed by gc_slot).
 // Then thread t builds output dwords t + 256u of the chunk (16 positions
 // each, local[] gives each position's member within the chunk) for the
-// four quads, and stores them into the tiles; positions past the last
-// member and padding SNPs read as code 01 (missing).
+// four quads, re-codes them to dosage values and stores them into the
+// dosage-value tiles of k_tile_stats<true>; positions past the last member
+// and padding SNPs read as missing (value 3).
 constexpr int kGcRows = 16;
 
 template <uint32_t MEMBERS>
@@ -813,8 +814,8 @@ __global__ __launch_bounds__(256) void k_gather_chunk(const uint8_t* __restrict_
         }
     }
     __syncthreads();
-    // gather: output dwords ql = tid + 256u of the chunk.  Member e = 4f + k
-    // of an output dword goes to bits 8k + 2f (the tiles' cell order) by
+    // gather: output dwords ql = tid + 256u of the chunk.  Member e = 4k + f
+    // of an output dword goes to bits 2e (the rows' own cell order) by
     // placing its four-row code word in acc[k] at bits 2f before the byte
     // transpose.
     uint32_t* tbase = tiled + ((size_t)(j0 / kGroup) * nkb * 64 + (j0 % kGroup)) * 4;
@@ -832,21 +833,23 @@ __global__ __launch_bounds__(256) void k_gather_chunk(const uint8_t* __restrict_
             widx[e] = code >> 2;
             shv[e] = 2 * (code & 3);
         }
-        // row dword 2t + h of K-block kb -> lane s + 32h, dword t
+        // dosage-value tiles: row dword 2t + h of K-block kb -> lane
+        // s + 32(t%2), dword 2(t/2) + h
         const uint32_t kb = q / 8, dq = q % 8;
-        uint32_t* ob = tbase + ((size_t)kb * 64 + 32 * (dq & 1)) * 4 + (dq >> 1);
+        uint32_t* ob =
+            tbase + ((size_t)kb * 64 + 32 * ((dq >> 1) & 1)) * 4 + 2 * (dq >> 2) + (dq & 1);
 #pragma unroll
         for (int Q = 0; Q < 4; ++Q) {
             const uint32_t* quad = lds + Q * kGcStride;
             uint32_t acc[4] = {0u, 0u, 0u, 0u};
 #pragma unroll
             for (int e = 0; e < 16; ++e)
-                acc[e & 3] |= ((quad[widx[e]] >> shv[e]) & 0x03030303u) << (2 * (e >> 2));
+                acc[e >> 2] |= ((quad[widx[e]] >> shv[e]) & 0x03030303u) << (2 * (e & 3));
             uint32_t o[4];
             transpose_bytes4(acc[0], acc[1], acc[2], acc[3], o);  // o[r] = row 4Q + r
 #pragma unroll
             for (int r = 0; r < 4; ++r)
-                ob[(4 * Q + r) * 4] = j0 + 4 * Q + r < m ? o[r] : 0x55555555u;
+                ob[(4 * Q + r) * 4] = j0 + 4 * Q + r < m ? value_codes(o[r]) : 0xFFFFFFFFu;
         }
     }
 }
@@ -1056,26 +1059,6 @@ __device__ __forceinline__ void finish_2w(const BandArgs& a, EpiShared2& sh, int
     band_contract2(a, sh, tile, I, J, w, r2u);
 }
 
-// e2m1 operands of one K-step from the lane's two value-code dwords w0, w1
-// (32 individuals): X = v/2 (nibble = v in its low bits: 0, 0.5, 1, 1.5) and
-// Q = q/2 (q = 1 iff v = 3: missing or padding).  Fields 2n and 2n+1 of a
-// dword go to nibble n of two operand dwords; the same placement on both
-// operands keeps every individual paired with itself.  The MFMA reads 4 of
-// the 8 operand dwords.
-__device__ __forceinline__ void vcode_operands(uint32_t w0, uint32_t w1, v8i& X, v8i& Q) {
-    // shifts of the 64-bit pair (w1:w0): the bits w1 pushes into the top of
-    // w0 land outside every mask below
-    const uint64_t w = ((uint64_t)w1 << 32) | w0;
-    const uint64_t s1 = w >> 1, s2 = w >> 2, s3 = w >> 3;
-    const uint32_t a2 = (uint32_t)s2, b2 = (uint32_t)(s2 >> 32);
-    const v4i x = {(int)(w0 & 0x33333333u), (int)(a2 & 0x33333333u),
-                   (int)(w1 & 0x33333333u), (int)(b2 & 0x33333333u)};
-    const v4i q = {(int)(w0 & (uint32_t)s1 & 0x11111111u), (int)(a2 & (uint32_t)s3 & 0x11111111u),
-                   (int)(w1 & (uint32_t)(s1 >> 32) & 0x11111111u),
-                   (int)(b2 & (uint32_t)(s3 >> 32) & 0x11111111u)};
-    X = __builtin_shufflevector(x, x, 0, 1, 2, 3, -1, -1, -1, -1);
-    Q = __builtin_shufflevector(q, q, 0, 1, 2, 3, -1, -1, -1, -1);
-}
 
 
 // Sub-blocks of a 64x64 tile: bit 2*bi + bj stands for rows 32bi.. x
@@ -1909,7 +1892,7 @@ int l2_begin(const ldsc_panel* panel, const ldsc_snp* snps, uint64_t m, const ld
     else
         rc = gather_and_tile(panel, contiguous ? nullptr : ws->bed_idx.as<uint64_t>(),
                              contiguous ? base : 0, m, d ? sp->d_fam : panel->d_keep32, m_pad,
-                             use_fp4, /*tile=*/true, ws, &parts, err, err_len);
+                             use_fp4 || d, /*tile=*/true, ws, &parts, err, err_len);
     if (rc) return rc;
     // MAF on the device for the exact and FP32 forms; the sketch form's
     // rounding depends on chunk contiguity and is finished on the host

@@ -11,16 +11,19 @@
 //    contiguous run of each chunk's positions.
 //  * The signed bucket sums are a product with a sparse +-1 matrix P.  For
 //    each block of 32 buckets only the K-steps covering its members are
-//    visited; P's non-zero 32x32 tiles are precomputed as int8 MFMA
-//    A-operands.  v_mfma_i32_32x32x32_i8 then gives exact integers
+//    visited; P's non-zero 32-bucket tiles are precomputed as MFMA
+//    A-operands, and the MFMA gives exact integers
 //      SG[b, j] = sum_{i in b} s_i g_ij,  SM[b, j] = sum_{i in b} s_i [present].
 //    For d <= 256 (kSfFusedMaxBlocks blocks) with chunk-local order this is
 //    one pass over the .bed windows (k_sketch_fused: LDS-staged rows, gather
-//    by precomputed word codes, MFMA); otherwise the rows are first tiled in
-//    bucket order (k_gather_chunk) and k_sketch reads the tiles.  For
-//    d <= 224 the one pass skips the bucket order altogether
-//    (k_sketch_dense: natural order, the +-1 bucket matrix built in LDS, the
-//    e2m1 MFMA over every block).
+//    by precomputed word codes, int8 MFMA on int8 tiles of 32 positions);
+//    otherwise the rows are first tiled in bucket order as dosage values
+//    (k_gather_chunk, or k_gather4 with --keep) and k_sketch reads the tiles
+//    on the e2m1 MFMA (e2m1 tiles of 64 positions; SG = 2 PX - 6 PQ and
+//    SM = T_b - 2 PQ from the X = v/2 and Q = [v = 3]/2 operands of the exact
+//    band kernel).  For d <= 224 the one pass skips the bucket order
+//    altogether (k_sketch_dense: natural order, the +-1 bucket matrix built
+//    in LDS, the e2m1 MFMA over every block).
 //  * The bucket value y_b = inv_j (SG - mean_j SM) (the reference's f32 mean
 //    and inverse std: the same real number its f32 LUT scatter accumulates)
 //    is kept as two int8 integers, w = SG - k SM and e = T_b - SM with
@@ -45,6 +48,8 @@ void SketchPlan::release() {
     if (d_fam) (void)hipFree(d_fam);
     if (d_local) (void)hipFree(d_local);
     if (d_ptiles) (void)hipFree(d_ptiles);
+    if (d_ptiles4) (void)hipFree(d_ptiles4);
+    d_ptiles4 = nullptr;
     if (d_blocks) (void)hipFree(d_blocks);
     if (d_tile_ks) (void)hipFree(d_tile_ks);
     if (d_tsum) (void)hipFree(d_tsum);
@@ -64,8 +69,9 @@ void SketchPlan::release() {
     d_tile_ks = nullptr;
 }
 
-// Builds (once per panel and d) the bucket order of individuals and the
-// int8 P tiles.  Host work is O(N + chunks * d).
+// Builds (once per panel and d) the bucket order of individuals and the P
+// tiles (int8 for k_sketch_fused, e2m1 for k_sketch).  Host work is
+// O(N + chunks * d) plus 2 KiB of tile entries per tile.
 int sketch_plan(const ldsc_panel* panel_c, uint64_t d, SketchPlan** out, char* err,
                 uint64_t err_len) {
     ldsc_panel* panel = const_cast<ldsc_panel*>(panel_c);
@@ -140,7 +146,38 @@ int sketch_plan(const ldsc_panel* panel_c, uint64_t d, SketchPlan** out, char* e
                 }
             }
         }
-    if (tile_ks.empty()) tile_ks.push_back(0);
+    // e2m1 A-operand tiles of k_sketch, in (bucket block, chunk, K-step of
+    // 64 positions) order: lane l is bucket row 32B + l%32, and nibble jj of
+    // its dword q is position ks*64 + (l/32)*32 + 16(q/2) + 2jj + q%2 (the
+    // operand placement of the dosage-value tiles), s_i as e2m1 (0x2 = +1,
+    // 0xA = -1).
+    std::vector<int4> blocks4(sp->nblocks);
+    std::vector<int32_t> tile_ks4;
+    for (uint64_t B = 0; B < sp->nblocks; ++B) {
+        blocks4[B] = make_int4((int)tile_ks4.size(), 0, 0, 0);
+        for (uint64_t c = 0; c < sp->nchunks; ++c) {
+            const uint64_t lo = bo[c][B], hi = bo[c][B + 1];
+            if (hi > lo)
+                for (uint64_t ks = lo / 64; ks < (hi + 63) / 64; ++ks) tile_ks4.push_back((int32_t)ks);
+        }
+        blocks4[B].y = (int)tile_ks4.size() - blocks4[B].x;
+    }
+    std::vector<uint32_t> tiles4(std::max<size_t>(tile_ks4.size(), 1) * 256, 0u);
+    for (uint64_t B = 0; B < sp->nblocks; ++B)
+        for (int t = blocks4[B].x; t < blocks4[B].x + blocks4[B].y; ++t) {
+            uint32_t* tp = tiles4.data() + (size_t)t * 256;
+            const uint64_t ks = (uint64_t)tile_ks4[t];
+            for (int lane = 0; lane < 64; ++lane) {
+                const uint64_t b = 32 * B + (lane & 31);
+                for (int q = 0; q < 4; ++q)
+                    for (int jj = 0; jj < 8; ++jj) {
+                        const uint64_t k = ks * 64 + (lane >> 5) * 32 + 16 * (q >> 1) + 2 * jj + (q & 1);
+                        if (k < n && bucket[perm[k]] == b)
+                            tp[lane * 4 + q] |= (sign[perm[k]] < 0 ? 0xAu : 0x2u) << (4 * jj);
+                    }
+            }
+        }
+    if (tile_ks4.empty()) tile_ks4.push_back(0);
     // gather maps
     std::vector<int32_t> fam;
     std::vector<uint16_t> loc;
@@ -164,9 +201,12 @@ int sketch_plan(const ldsc_panel* panel_c, uint64_t d, SketchPlan** out, char* e
         if (e == hipSuccess)
             e = hipMemcpy(sp->d_fam, fam.data(), fam.size() * 4, hipMemcpyHostToDevice);
     }
-    if (e == hipSuccess) e = hipMalloc(&sp->d_tile_ks, tile_ks.size() * 4);
+    if (e == hipSuccess) e = hipMalloc(&sp->d_tile_ks, tile_ks4.size() * 4);
     if (e == hipSuccess)
-        e = hipMemcpy(sp->d_tile_ks, tile_ks.data(), tile_ks.size() * 4, hipMemcpyHostToDevice);
+        e = hipMemcpy(sp->d_tile_ks, tile_ks4.data(), tile_ks4.size() * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMalloc(&sp->d_ptiles4, tiles4.size() * 4);
+    if (e == hipSuccess)
+        e = hipMemcpy(sp->d_ptiles4, tiles4.data(), tiles4.size() * 4, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMalloc(&sp->d_ptiles, tiles.size());
     if (e == hipSuccess) e = hipMemcpy(sp->d_ptiles, tiles.data(), tiles.size(), hipMemcpyHostToDevice);
     if (local) {
@@ -201,9 +241,10 @@ int sketch_plan(const ldsc_panel* panel_c, uint64_t d, SketchPlan** out, char* e
     if (e == hipSuccess) e = hipMalloc(&sp->d_tsum, tsum.size() * 4);
     if (e == hipSuccess)
         e = hipMemcpy(sp->d_tsum, tsum.data(), tsum.size() * 4, hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMalloc(&sp->d_blocks, blocks.size() * sizeof(int4));
+    if (e == hipSuccess) e = hipMalloc(&sp->d_blocks, blocks4.size() * sizeof(int4));
     if (e == hipSuccess)
-        e = hipMemcpy(sp->d_blocks, blocks.data(), blocks.size() * sizeof(int4), hipMemcpyHostToDevice);
+        e = hipMemcpy(sp->d_blocks, blocks4.data(), blocks4.size() * sizeof(int4),
+                      hipMemcpyHostToDevice);
     if (e != hipSuccess) {
         sp->release();
         delete sp;
@@ -242,8 +283,10 @@ size_t sketch_norm_bytes(const SketchPlan* sp, uint64_t m_pad) {
 
 // Wave (gp, c) handles the SNP groups 2gp, 2gp+1 (64 SNPs) and bucket
 // blocks [c * kSkChunk, (c+1) * kSkChunk).  Within a block it walks the
-// block's tiles (each with its K-step) with the next kSkAhead tiles' loads
-// in flight.  The exact integer sums SG, SM of a block are then written as:
+// block's e2m1 P tiles (each with its K-step of 64 bucket-order positions)
+// with the next kSkAhead tiles' loads in flight, and multiplies each with
+// the X and Q operands of the two groups' dosage-value tiles.  The exact
+// integer sums SG, SM of a block are then written as:
 //  * kSketchRaw: the integer bucket sums (d x m column-major);
 //  * kSketchF32: unscaled f32 bucket values y = inv (SG - mean SM) into the
 //    sketch tiles sk[((g * nblocks + B) * 64 + lane) * 16 + r] (bucket
@@ -275,8 +318,9 @@ __global__ __launch_bounds__(64) void k_sketch(const uint32_t* __restrict__ tile
     const uint32_t gp = blockIdx.x, c = blockIdx.y;
     const int lane = threadIdx.x;
     const int h = lane >> 5;
-    const uint32_t* w0p = tiled + ((size_t)(2 * gp) * nkb * 64 + lane) * 4;
-    const uint32_t* w1p = w0p + (size_t)nkb * 256;
+    // the lane's dosage-value dwords of K-block 0 of groups 2gp, 2gp + 1
+    const uint32_t* v0p = tiled + ((size_t)(2 * gp) * nkb * 64 + lane) * 4;
+    const uint32_t* v1p = v0p + (size_t)nkb * 256;
     const uint32_t snp0 = 64 * gp + (lane & 31), snp1 = snp0 + 32;
     const float2 ms0 = musc[snp0], ms1 = musc[snp1];
     double nrm0 = 0.0, nrm1 = 0.0, as0 = 0.0, as1 = 0.0;
@@ -288,31 +332,32 @@ __global__ __launch_bounds__(64) void k_sketch(const uint32_t* __restrict__ tile
     const uint32_t B1 = min(nblocks, (c + 1) * kSkChunk);
     for (uint32_t B = c * kSkChunk; B < B1; ++B) {
         const int4 blk = blocks[B];
-        v16i sg0, sm0, sg1, sm1;
+        v16f px0, pq0, px1, pq1;
 #pragma unroll
-        for (int r = 0; r < 16; ++r) sg0[r] = sm0[r] = sg1[r] = sm1[r] = 0;
-        // K-steps in groups of kSkAhead, the next group's operands in flight;
-        // steps past the block's last get P = 0 (they add nothing)
-        uint32_t w0[kSkAhead], w1[kSkAhead];
+        for (int r = 0; r < 16; ++r) px0[r] = pq0[r] = px1[r] = pq1[r] = 0.0f;
+        // K-steps (64 positions) in groups of kSkAhead, the next group's
+        // operands in flight; steps past the block's last get P = 0 (they add
+        // nothing)
+        uint2 w0[kSkAhead], w1[kSkAhead];
         v4i P[kSkAhead];
         auto fetch = [&](int s0) {
 #pragma unroll
             for (int u = 0; u < kSkAhead; ++u) {
                 if (s0 + u < blk.y) {
                     const uint32_t ks = (uint32_t)tile_ks[blk.x + s0 + u];
-                    const size_t off = (size_t)(ks >> 2) * 256 + (ks & 3);
-                    w0[u] = w0p[off];
-                    w1[u] = w1p[off];
+                    const size_t off = (size_t)(ks >> 1) * 256 + 2 * (ks & 1);
+                    w0[u] = *reinterpret_cast<const uint2*>(v0p + off);
+                    w1[u] = *reinterpret_cast<const uint2*>(v1p + off);
                     P[u] = ptiles[(size_t)(blk.x + s0 + u) * 64 + lane];
                 } else {
-                    w0[u] = w1[u] = 0u;
+                    w0[u] = w1[u] = make_uint2(0u, 0u);
                     P[u] = v4i{0, 0, 0, 0};
                 }
             }
         };
         fetch(0);
         for (int s0 = 0; s0 < blk.y; s0 += kSkAhead) {
-            uint32_t cw0[kSkAhead], cw1[kSkAhead];
+            uint2 cw0[kSkAhead], cw1[kSkAhead];
             v4i cP[kSkAhead];
 #pragma unroll
             for (int u = 0; u < kSkAhead; ++u) {
@@ -323,14 +368,36 @@ __global__ __launch_bounds__(64) void k_sketch(const uint32_t* __restrict__ tile
             if (s0 + kSkAhead < blk.y) fetch(s0 + kSkAhead);
 #pragma unroll
             for (int u = 0; u < kSkAhead; ++u) {
-                v4i G0, M0, G1, M1;
-                expand_codes(cw0[u], G0, M0);
-                expand_codes(cw1[u], G1, M1);
-                sg0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(cP[u], G0, sg0, 0, 0, 0);
-                sm0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(cP[u], M0, sm0, 0, 0, 0);
-                sg1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(cP[u], G1, sg1, 0, 0, 0);
-                sm1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(cP[u], M1, sm1, 0, 0, 0);
+                v8i X0, Q0, X1, Q1;
+                vcode_operands(cw0[u].x, cw0[u].y, X0, Q0);
+                vcode_operands(cw1[u].x, cw1[u].y, X1, Q1);
+                const v8i a8 = __builtin_shufflevector(cP[u], cP[u], 0, 1, 2, 3, -1, -1, -1, -1);
+                px0 = LDSC_MFMA_FP4(a8, X0, px0);
+                pq0 = LDSC_MFMA_FP4(a8, Q0, pq0);
+                px1 = LDSC_MFMA_FP4(a8, X1, px1);
+                pq1 = LDSC_MFMA_FP4(a8, Q1, pq1);
             }
+        }
+        // T_b of the lane's buckets 32B + 8q + 4h + t (t = 0..3), and the
+        // exact sums: with PX = sum s v/2 and PQ = sum s [missing]/2 (f32,
+        // multiples of 1/2 below 1.5 N), SG = 2 PX - 6 PQ, SM = T_b - 2 PQ
+        int T[16];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int4 t4 = *reinterpret_cast<const int4*>(tsum + 32 * B + 8 * q + 4 * h);
+            T[4 * q] = t4.x;
+            T[4 * q + 1] = t4.y;
+            T[4 * q + 2] = t4.z;
+            T[4 * q + 3] = t4.w;
+        }
+        v16i sg0, sm0, sg1, sm1;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const int q0 = (int)(2.0f * pq0[r]), q1 = (int)(2.0f * pq1[r]);
+            sg0[r] = (int)(2.0f * px0[r]) - 3 * q0;
+            sm0[r] = T[r] - q0;
+            sg1[r] = (int)(2.0f * px1[r]) - 3 * q1;
+            sm1[r] = T[r] - q1;
         }
         if (MODE == kSketchRaw) {
 #pragma unroll
@@ -361,16 +428,6 @@ __global__ __launch_bounds__(64) void k_sketch(const uint32_t* __restrict__ tile
                 t1[r] = y1;
             }
         } else {
-            // T_b of the lane's buckets 32B + 8q + 4h + t (t = 0..3)
-            int T[16];
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const int4 t4 = *reinterpret_cast<const int4*>(tsum + 32 * B + 8 * q + 4 * h);
-                T[4 * q] = t4.x;
-                T[4 * q + 1] = t4.y;
-                T[4 * q + 2] = t4.z;
-                T[4 * q + 3] = t4.w;
-            }
             uint32_t wd0[4] = {0u, 0u, 0u, 0u}, ed0[4] = {0u, 0u, 0u, 0u};
             uint32_t wd1[4] = {0u, 0u, 0u, 0u}, ed1[4] = {0u, 0u, 0u, 0u};
             uint32_t out0 = 0u, out1 = 0u;
@@ -475,15 +532,15 @@ hipError_t launch_sketch(const SketchPlan* sp, int mode, const uint32_t* tiled, 
     const dim3 grid(m_pad / 64, nch), block(64);
     if (mode == kSketchRaw) {
         hipLaunchKernelGGL(k_sketch<kSketchRaw>, grid, block, 0, st, tiled, nkb,
-                           (const v4i*)sp->d_ptiles, sp->d_blocks, sp->d_tile_ks, nb, musc, m, m_pad,
-                           (uint32_t)sp->d, nullptr, nullptr, sg, sm, nullptr, nullptr, nullptr);
+                           (const v4i*)sp->d_ptiles4, sp->d_blocks, sp->d_tile_ks, nb, musc, m, m_pad,
+                           (uint32_t)sp->d, nullptr, nullptr, sg, sm, sp->d_tsum, nullptr, nullptr);
         return hipGetLastError();
     }
     hipError_t e;
     if (mode == kSketchF32) {
         hipLaunchKernelGGL(k_sketch<kSketchF32>, grid, block, 0, st, tiled, nkb,
-                           (const v4i*)sp->d_ptiles, sp->d_blocks, sp->d_tile_ks, nb, musc, m, m_pad,
-                           (uint32_t)sp->d, norm, cols, nullptr, nullptr, nullptr, nullptr, nullptr);
+                           (const v4i*)sp->d_ptiles4, sp->d_blocks, sp->d_tile_ks, nb, musc, m, m_pad,
+                           (uint32_t)sp->d, norm, cols, nullptr, nullptr, sp->d_tsum, nullptr, nullptr);
         if ((e = hipGetLastError()) != hipSuccess) return e;
         hipLaunchKernelGGL(k_sketch_scale<kSketchF32>, dim3((m_pad + 255) / 256), dim3(256), 0, st,
                            norm, nch, m_pad, n, const_cast<float*>(sketch_scales(sp, norm, m_pad)),
@@ -496,7 +553,7 @@ hipError_t launch_sketch(const SketchPlan* sp, int mode, const uint32_t* tiled, 
     hipLaunchKernelGGL(k_sketch_clear_flag, dim3(1), dim3(64), 0, st, flag);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     hipLaunchKernelGGL(k_sketch<kSketchI8>, grid, block, 0, st, tiled, nkb,
-                       (const v4i*)sp->d_ptiles, sp->d_blocks, sp->d_tile_ks, nb, musc, m, m_pad,
+                       (const v4i*)sp->d_ptiles4, sp->d_blocks, sp->d_tile_ks, nb, musc, m, m_pad,
                        (uint32_t)sp->d, norm, cols, nullptr, nullptr, sp->d_tsum, asum, flag);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     hipLaunchKernelGGL(k_sketch_scale<kSketchI8>, dim3((m_pad + 255) / 256), dim3(256), 0, st,
@@ -1607,7 +1664,7 @@ extern "C" int ldsc_l2_sketch_buckets(const ldsc_panel* panel, const uint64_t* b
                                               sp->d_local, (uint32_t)sp->chunk, m_pad, ws,
                                               &parts, err, err_len)
                          : gather_and_tile(panel, ws->bed_idx.as<uint64_t>(), 0, m, sp->d_fam,
-                                           m_pad, false, /*tile=*/true, ws, &parts, err, err_len);
+                                           m_pad, true, /*tile=*/true, ws, &parts, err, err_len);
         if (rc) break;
         // musc is unused by the integer form; pass any valid per-SNP array
         SCHK(hipMemsetAsync(ws->info.p, 0, m_pad * 48, st));
