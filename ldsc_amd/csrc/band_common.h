@@ -458,7 +458,8 @@ inline uint32_t sketch_fused_ranges(uint64_t nblocks) {
 // up to this many bucket blocks (d <= 224).
 constexpr uint64_t kSdMaxBlocks = 7;
 // Per-SNP share slots of the norm buffer: enough for k_sketch (one per
-// kSkChunk = 4 blocks) and for k_sketch_fused (one per wave and range).
+// kSkChunk = 4 blocks), k_sketch_fused (one per wave and range) and
+// k_sketch_dense (two).
 inline uint32_t sketch_share_slots(uint64_t nblocks) {
     return (uint32_t)std::max<uint64_t>((nblocks + 3) / 4, kSfWaves * sketch_fused_ranges(nblocks));
 }

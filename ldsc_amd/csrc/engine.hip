@@ -1871,9 +1871,10 @@ int l2_begin(const ldsc_panel* panel, const ldsc_snp* snps, uint64_t m, const ld
     // chunk-local bucket order in the integer form with at most 8 bucket
     // blocks (d <= 256): one pass from the .bed rows to the bucket sums, no
     // tiled copy (the range fallback of l2_band tiles the rows then); up to
-    // 4 blocks in natural order (k_sketch_dense), else in bucket order
-    // (k_sketch_fused).  With more blocks a wave owns several and the pass
-    // measured slower than the tiled gather plus k_sketch.
+    // 7 blocks in natural order (k_sketch_dense, when the plan built its
+    // codes), else in bucket order (k_sketch_fused).  With more blocks a wave
+    // owns several and the pass measured slower than the tiled gather plus
+    // k_sketch.
     const bool fused = sk8 && sp->d_local && sp->nblocks <= kSfFusedMaxBlocks;
     c.sk_fused = fused;
     if (fused) {
