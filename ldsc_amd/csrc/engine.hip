@@ -717,19 +717,20 @@ __global__ __launch_bounds__(512) void k_gather4(const uint8_t* __restrict__ sla
 // ------------------------------------------------------------------------
 // Kernel 1d: chunk-local gather into dosage-value tiles (CountSketch order)
 // ------------------------------------------------------------------------
-// Workgroup (h, c) owns SNP rows 16h .. 16h+15 and member chunk c of
-// MEMBERS members, whose members come from FAM rows c*MEMBERS .. : one
-// MEMBERS/4-byte window of each .bed row.  Eight waves: waves w and w + 4
-// load rows 16h + 4w .. +3 (row quad w), each half of the window (1 KiB of
-// one row per load), re-code them to dosage values, count their integer
-// stats (the chunk's slot) and stage them byte-interleaved in LDS (word k
-// of quad w = byte k of its four rows, in 16-byte slots placed by gc_slot).
-// Then thread t builds output dwords t + 512u of the chunk (16 positions
-// each, local[] gives each position's member within the chunk) for the
-// four quads and stores them into the dosage-value tiles of
-// k_tile_stats<true>; positions past the last member and padding SNPs read
-// as missing (value 3).  Only a pass that reaches past the last individual,
-// or a quad with padding SNPs, masks its cells (a uniform branch).
+// Workgroup h owns SNP rows 16h .. 16h+15 and walks the member chunks c of
+// MEMBERS members (FAM rows c*MEMBERS ..): one MEMBERS/4-byte window of
+// each .bed row per chunk.  Eight waves: waves w and w + 4 load rows
+// 16h + 4w .. +3 (row quad w), each half of the window (1 KiB of one row
+// per load), re-code them to dosage values, count their integer stats (the
+// chunk's slot) and stage them byte-interleaved in LDS (word k of quad w =
+// byte k of its four rows, in 16-byte slots placed by gc_slot).  Then
+// thread t builds output dwords t + 512u of the chunk (16 positions each,
+// local[] gives each position's member within the chunk) for the four
+// quads and stores them into the dosage-value tiles of k_tile_stats<true>;
+// positions past the last member and padding SNPs read as missing (value
+// 3).  The next chunk's rows load into registers while this chunk's
+// gather runs.  Only a pass that reaches past the last individual, or a
+// quad with padding SNPs, masks its cells (a uniform branch).
 constexpr int kGcRows = 16;
 constexpr int kGcThreads = 512;
 
@@ -737,152 +738,157 @@ template <uint32_t MEMBERS>
 __global__ __launch_bounds__(kGcThreads) void k_gather_chunk(
     const uint8_t* __restrict__ slabs, uint64_t pitch, uint32_t src_dw, uint32_t n,
     const uint64_t* __restrict__ bed_idx, uint64_t base, uint32_t m, uint32_t m_pad,
-    const uint16_t* __restrict__ local, uint32_t nkb, uint32_t* __restrict__ tiled,
-    uint32_t* __restrict__ part) {
+    const uint16_t* __restrict__ local, uint32_t nkb, uint32_t nchunks,
+    uint32_t* __restrict__ tiled, uint32_t* __restrict__ part) {
     constexpr uint32_t kGcWords = MEMBERS / 4;    // LDS words per quad
     constexpr uint32_t kGcStride = kGcWords + 4;  // quad stride; word kGcWords (its slot
                                                   // maps to itself) reads as value 3
                                                   // (missing) in every field
     constexpr uint32_t cw = MEMBERS / 16;         // window dwords per row
     constexpr int kPasses = (int)(cw / 256);      // 256 window dwords per pass
+    constexpr int kHalf = kPasses / 2;            // passes of one wave
     static_assert(kPasses % 2 == 0, "the two waves of a quad take half the passes each");
     extern __shared__ uint32_t lds[];  // 4 quads x kGcStride
-    __shared__ uint32_t red[8][12];    // per wave: (a, b, c) of its quad's four rows
+    __shared__ uint32_t red[8][8];     // per wave: packed (a, b, c) of its quad's four rows
     const int tid = threadIdx.x, ln = tid & 63;
     const int wv = __builtin_amdgcn_readfirstlane(tid >> 6), qd = wv & 3, hf = wv >> 2;
-    const uint32_t j0 = blockIdx.x * kGcRows, c = blockIdx.y;
+    const uint32_t j0 = blockIdx.x * kGcRows;
     if (wv < 4 && ln == 0) lds[wv * kGcStride + kGcWords] = 0xFFFFFFFFu;
-    // load: wave wv, rows j0 + 4qd + r, passes hf * kPasses/2 ..
-    {
-        const uint32_t* rp[4];
-        uint32_t nv[4];
-        bool pad = false;
+    // rows j0 + 4qd + r of the wave (padding SNPs read row 0, masked)
+    const uint32_t* rp[4];
+    uint32_t nv[4];
+    bool pad = false;
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const uint32_t j = j0 + 4 * qd + r;
-            const uint64_t row = j < m ? (bed_idx ? bed_idx[j] : base + j) : 0;
-            rp[r] = reinterpret_cast<const uint32_t*>(slabs + row * pitch);
-            nv[r] = j < m ? n : 0;
-            pad |= j >= m;
+    for (int r = 0; r < 4; ++r) {
+        const uint32_t j = j0 + 4 * qd + r;
+        const uint64_t row = j < m ? (bed_idx ? bed_idx[j] : base + j) : 0;
+        rp[r] = reinterpret_cast<const uint32_t*>(slabs + row * pitch);
+        nv[r] = j < m ? n : 0;
+        pad |= j >= m;
+    }
+    // the lane's 16 bytes of row r in pass hf * kHalf + ii of chunk c, from
+    // a clamped address (rows are padded to 16 bytes: a 16-byte load starting
+    // below src_dw stays inside the row; what lies past n is masked)
+    uint4 v[kHalf][4];
+    auto issue = [&](uint32_t c) {
+#pragma unroll
+        for (int ii = 0; ii < kHalf; ++ii) {
+            const uint32_t q0 = c * cw + (hf * kHalf + ii) * 256 + 4 * ln;
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+                v[ii][r] = *reinterpret_cast<const uint4*>(rp[r] + (q0 < src_dw ? q0 : 0u));
         }
-        // per row: a = codes 01 and 11, b = codes 10 and 11, cc = code 11
-        uint32_t ca[4] = {}, cb[4] = {}, cc[4] = {};
-        auto pass = [&](int i, auto mask) {
-            const uint32_t k0 = i * 256 + 4 * ln;  // window dword of the lane's first word
-            const uint64_t q0 = (uint64_t)c * cw + k0;
-            uint4 v[4];
+    };
+    // per row: a = codes 01 and 11, b = codes 10 and 11, cc = code 11
+    uint32_t ca[4], cb[4], cc[4];
+    auto stage = [&](uint32_t c, int ii, auto mask) {
+        const uint32_t k0 = (hf * kHalf + ii) * 256 + 4 * ln;  // window dword
+        const uint32_t q0 = c * cw + k0;
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            uint32_t w[4];
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                // rows are padded to 16 bytes: a 16-byte load starting below
-                // src_dw stays inside the row
-                if (decltype(mask)::value)
-                    v[r] = q0 < src_dw ? *reinterpret_cast<const uint4*>(rp[r] + q0)
-                                       : make_uint4(0u, 0u, 0u, 0u);
-                else
-                    v[r] = *reinterpret_cast<const uint4*>(rp[r] + q0);
-            }
-#pragma unroll
-            for (int t = 0; t < 4; ++t) {
-                uint32_t w[4];
-#pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    const uint32_t raw =
-                        t == 0 ? v[r].x : t == 1 ? v[r].y : t == 2 ? v[r].z : v[r].w;
-                    const uint32_t x = decltype(mask)::value ? ts_mask(raw, q0 + t, nv[r]) : raw;
-                    const uint32_t hb = (x >> 1) & 0x55555555u;  // H, at the cell's even bit
-                    ca[r] += __popc(x & 0x55555555u);
-                    cb[r] += __popc(hb);
-                    cc[r] += __popc(x & hb);
-                    w[r] = x ^ hb ^ 0xAAAAAAAAu;  // value code: (L ^ H) | (~H << 1)
-                }
-                uint32_t o[4];
-                transpose_bytes4(w[0], w[1], w[2], w[3], o);
-                reinterpret_cast<uint4*>(lds + qd * kGcStride)[gc_slot(k0 + t)] =
-                    make_uint4(o[0], o[1], o[2], o[3]);
-            }
-        };
-#pragma unroll
-        for (int ii = 0; ii < kPasses / 2; ++ii) {
-            const int i = hf * (kPasses / 2) + ii;
-            // every cell of the pass belongs to a real individual of a real SNP
-            if (!pad && ((uint64_t)c * cw + (uint64_t)(i + 1) * 256) * 16 <= n)
-                pass(i, std::false_type{});
-            else
-                pass(i, std::true_type{});
-        }
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-#pragma unroll
-            for (int o = 32; o >= 1; o >>= 1) {
-                ca[r] += __shfl_xor(ca[r], o);
-                cb[r] += __shfl_xor(cb[r], o);
-                cc[r] += __shfl_xor(cc[r], o);
-            }
-        }
-        if (ln < 4) {
-#pragma unroll
-            for (int r = 0; r < 4; ++r)
-                if (r == ln) {
-                    red[wv][3 * r + 0] = ca[r];
-                    red[wv][3 * r + 1] = cb[r];
-                    red[wv][3 * r + 2] = cc[r];
-                }
-        }
-    }
-    __syncthreads();
-    if (tid < kGcRows) {
-        // the window's MEMBERS cells of row j: codes 00 (dosage 2), 10
-        // (dosage 1), 11 (dosage 0) and 01 (missing, cells past n included)
-        const int q4 = tid >> 2, r = tid & 3;
-        const uint32_t j = j0 + tid;
-        const uint32_t a = red[q4][3 * r] + red[q4 + 4][3 * r];
-        const uint32_t b = red[q4][3 * r + 1] + red[q4 + 4][3 * r + 1];
-        const uint32_t c3 = red[q4][3 * r + 2] + red[q4 + 4][3 * r + 2];
-        const uint32_t n0 = c3, n1 = b - c3, n2 = MEMBERS - a - b + c3;
-        if (j < m_pad)
-            reinterpret_cast<uint4*>(part)[(size_t)c * m_pad + j] =
-                make_uint4(2 * n2 + n1, n2 + n1 + n0, 4 * n2 + n1, n1);
-    }
-    // gather: output dwords ql = tid + 512u of the chunk.  Member e = 4k + f
-    // of an output dword goes to bits 2e (the rows' own cell order): its
-    // four-row word, rotated so that its field lands at bits 2f of each
-    // byte, is inserted into acc[k] before the byte transpose.
-    uint32_t* tbase = tiled + ((size_t)(j0 / kGroup) * nkb * 64 + (j0 % kGroup)) * 4;
-#pragma unroll 1
-    for (int u = 0; u < kPasses / 2; ++u) {
-        const uint32_t ql = tid + kGcThreads * u, q = c * cw + ql;
-        if (q >= nkb * 8) break;
-        const uint4* lp = reinterpret_cast<const uint4*>(local + (size_t)q * 16);
-        const uint4 e0 = lp[0], e1 = lp[1];
-        const uint32_t ev[8] = {e0.x, e0.y, e0.z, e0.w, e1.x, e1.y, e1.z, e1.w};
-        uint32_t widx[16], rot[16];
-#pragma unroll
-        for (int e = 0; e < 16; ++e) {  // gc_code: LDS word * 4 + field
-            const uint32_t code = (ev[e >> 1] >> (16 * (e & 1))) & 0xFFFFu;
-            widx[e] = code >> 2;
-            rot[e] = (2 * (code & 3) - 2 * (e & 3)) & 31u;
-        }
-        // dosage-value tiles: row dword 2t + h of K-block kb -> lane
-        // s + 32(t%2), dword 2(t/2) + h
-        const uint32_t kb = q / 8, dq = q % 8;
-        uint32_t* ob =
-            tbase + ((size_t)kb * 64 + 32 * ((dq >> 1) & 1)) * 4 + 2 * (dq >> 2) + (dq & 1);
-#pragma unroll
-        for (int Q = 0; Q < 4; ++Q) {
-            const uint32_t* quad = lds + Q * kGcStride;
-            uint32_t acc[4] = {0u, 0u, 0u, 0u};
-#pragma unroll
-            for (int e = 0; e < 16; ++e) {
-                const uint32_t y = __builtin_amdgcn_alignbit(quad[widx[e]], quad[widx[e]], rot[e]);
-                const uint32_t mk = 0x03030303u << (2 * (e & 3));
-                acc[e >> 2] = (acc[e >> 2] & ~mk) | (y & mk);
+                const uint4 vr = v[ii][r];
+                const uint32_t raw = t == 0 ? vr.x : t == 1 ? vr.y : t == 2 ? vr.z : vr.w;
+                const uint32_t x = decltype(mask)::value ? ts_mask(raw, q0 + t, nv[r]) : raw;
+                const uint32_t hb = (x >> 1) & 0x55555555u;  // H, at the cell's even bit
+                ca[r] += __popc(x & 0x55555555u);
+                cb[r] += __popc(hb);
+                cc[r] += __popc(x & hb);
+                w[r] = x ^ hb ^ 0xAAAAAAAAu;  // value code: (L ^ H) | (~H << 1)
             }
             uint32_t o[4];
-            transpose_bytes4(acc[0], acc[1], acc[2], acc[3], o);  // o[r] = row 4Q + r
-#pragma unroll
-            for (int r = 0; r < 4; ++r)
-                ob[(4 * Q + r) * 4] = j0 + 4 * Q + r < m ? o[r] : 0xFFFFFFFFu;
+            transpose_bytes4(w[0], w[1], w[2], w[3], o);
+            reinterpret_cast<uint4*>(lds + qd * kGcStride)[gc_slot(k0 + t)] =
+                make_uint4(o[0], o[1], o[2], o[3]);
         }
+    };
+    uint32_t* tbase = tiled + ((size_t)(j0 / kGroup) * nkb * 64 + (j0 % kGroup)) * 4;
+    issue(0);
+#pragma unroll 1
+    for (uint32_t c = 0; c < nchunks; ++c) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) ca[r] = cb[r] = cc[r] = 0u;
+#pragma unroll
+        for (int ii = 0; ii < kHalf; ++ii) {
+            const uint32_t i = hf * kHalf + ii;
+            // every cell of the pass belongs to a real individual of a real SNP
+            if (!pad && ((uint64_t)c * cw + (uint64_t)(i + 1) * 256) * 16 <= n)
+                stage(c, ii, std::false_type{});
+            else
+                stage(c, ii, std::true_type{});
+        }
+        // wave sums, two counts a word (each below 2^14)
+        uint32_t pk[6] = {ca[0] | cb[0] << 16, ca[1] | cb[1] << 16, ca[2] | cb[2] << 16,
+                          ca[3] | cb[3] << 16, cc[0] | cc[1] << 16, cc[2] | cc[3] << 16};
+#pragma unroll
+        for (int x = 0; x < 6; ++x)
+#pragma unroll
+            for (int o = 32; o >= 1; o >>= 1) pk[x] += __shfl_xor(pk[x], o);
+        if (ln < 6) {
+#pragma unroll
+            for (int x = 0; x < 6; ++x)
+                if (x == ln) red[wv][x] = pk[x];
+        }
+        __syncthreads();
+        if (c + 1 < nchunks) issue(c + 1);  // in flight during the gather
+        if (tid < kGcRows) {
+            // the window's MEMBERS cells of row j: codes 00 (dosage 2), 10
+            // (dosage 1), 11 (dosage 0) and 01 (missing, cells past n included)
+            const int q4 = tid >> 2, r = tid & 3;
+            const uint32_t j = j0 + tid;
+            const uint32_t ab0 = red[q4][r], ab1 = red[q4 + 4][r];
+            const uint32_t cc0 = red[q4][4 + (r >> 1)], cc1 = red[q4 + 4][4 + (r >> 1)];
+            const uint32_t a = (ab0 & 0xFFFFu) + (ab1 & 0xFFFFu), b = (ab0 >> 16) + (ab1 >> 16);
+            const uint32_t c3 = ((cc0 >> (16 * (r & 1))) & 0xFFFFu) + ((cc1 >> (16 * (r & 1))) & 0xFFFFu);
+            const uint32_t n0 = c3, n1 = b - c3, n2 = MEMBERS - a - b + c3;
+            if (j < m_pad)
+                reinterpret_cast<uint4*>(part)[(size_t)c * m_pad + j] =
+                    make_uint4(2 * n2 + n1, n2 + n1 + n0, 4 * n2 + n1, n1);
+        }
+        // gather: output dwords ql = tid + 512u of the chunk.  Member e = 4k + f
+        // of an output dword goes to bits 2e (the rows' own cell order): its
+        // four-row word, rotated so that its field lands at bits 2f of each
+        // byte, is inserted into acc[k] before the byte transpose.
+#pragma unroll 1
+        for (int u = 0; u < kHalf; ++u) {
+            const uint32_t ql = tid + kGcThreads * u, q = c * cw + ql;
+            if (q >= nkb * 8) break;
+            const uint4* lp = reinterpret_cast<const uint4*>(local + (size_t)q * 16);
+            const uint4 e0 = lp[0], e1 = lp[1];
+            const uint32_t ev[8] = {e0.x, e0.y, e0.z, e0.w, e1.x, e1.y, e1.z, e1.w};
+            uint32_t widx[16], rot[16];
+#pragma unroll
+            for (int e = 0; e < 16; ++e) {  // gc_code: LDS word * 4 + field
+                const uint32_t code = (ev[e >> 1] >> (16 * (e & 1))) & 0xFFFFu;
+                widx[e] = code >> 2;
+                rot[e] = (2 * (code & 3) - 2 * (e & 3)) & 31u;
+            }
+            // dosage-value tiles: row dword 2t + h of K-block kb -> lane
+            // s + 32(t%2), dword 2(t/2) + h
+            const uint32_t kb = q / 8, dq = q % 8;
+            uint32_t* ob =
+                tbase + ((size_t)kb * 64 + 32 * ((dq >> 1) & 1)) * 4 + 2 * (dq >> 2) + (dq & 1);
+#pragma unroll
+            for (int Q = 0; Q < 4; ++Q) {
+                const uint32_t* quad = lds + Q * kGcStride;
+                uint32_t acc[4] = {0u, 0u, 0u, 0u};
+#pragma unroll
+                for (int e = 0; e < 16; ++e) {
+                    const uint32_t y =
+                        __builtin_amdgcn_alignbit(quad[widx[e]], quad[widx[e]], rot[e]);
+                    const uint32_t mk = 0x03030303u << (2 * (e & 3));
+                    acc[e >> 2] = (acc[e >> 2] & ~mk) | (y & mk);
+                }
+                uint32_t o[4];
+                transpose_bytes4(acc[0], acc[1], acc[2], acc[3], o);  // o[r] = row 4Q + r
+#pragma unroll
+                for (int r = 0; r < 4; ++r)
+                    ob[(4 * Q + r) * 4] = j0 + 4 * Q + r < m ? o[r] : 0xFFFFFFFFu;
+            }
+        }
+        __syncthreads();  // the quads are restaged for the next chunk
     }
 }
 
@@ -1731,11 +1737,12 @@ int gather_tile_chunks(const ldsc_panel* panel, const uint64_t* d_bed_idx, uint6
     const size_t lds = (size_t)4 * (chunk / 4 + 4) * 4;  // four row quads
     HIPC(hipFuncSetAttribute((const void*)k_gather_chunk<kSkMembers>,
                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    hipLaunchKernelGGL(k_gather_chunk<kSkMembers>, dim3((unsigned)(m_pad / kGcRows), nchunks),
+    hipLaunchKernelGGL(k_gather_chunk<kSkMembers>, dim3((unsigned)(m_pad / kGcRows)),
                        dim3(kGcThreads), lds,
                        st, panel->d_slabs, panel->stride, (uint32_t)((panel->bps + 3) / 4),
                        (uint32_t)n, d_bed_idx, base, (uint32_t)m, (uint32_t)m_pad, d_local, nkb,
-                       ws->tiled.as<uint32_t>(), ws->stats.as<uint32_t>() + (size_t)m_pad * 4);
+                       nchunks, ws->tiled.as<uint32_t>(),
+                       ws->stats.as<uint32_t>() + (size_t)m_pad * 4);
     HIPC(hipGetLastError());
     return LDSC_OK;
 }
