@@ -847,13 +847,20 @@ __global__ __launch_bounds__(kGcThreads) void k_gather_chunk(
                 reinterpret_cast<uint4*>(part)[(size_t)c * m_pad + j] =
                     make_uint4(2 * n2 + n1, n2 + n1 + n0, 4 * n2 + n1, n1);
         }
-        // gather: output dwords ql = tid + 512u of the chunk.  Member e = 4k + f
-        // of an output dword goes to bits 2e (the rows' own cell order): its
-        // four-row word, rotated so that its field lands at bits 2f of each
-        // byte, is inserted into acc[k] before the byte transpose.
+        // gather: output dwords ql = tid + 512u of the chunk, each lane octet
+        // one K-block, lane x + 4h of the octet its dword 2(x/2) + (x%2) of
+        // slot half h (position dword dq = 4(x/2) + 2h + x%2).  Member e = 4k
+        // + f of an output dword goes to bits 2e (the rows' own cell order):
+        // its four-row word, rotated so that its field lands at bits 2f of
+        // each byte, is inserted into acc[k] before the byte transpose.  A
+        // 4 x 4 transpose across the lane quad then gives lane x the four
+        // dwords of row 4Q + x in its slot, stored as one 16-byte piece (the
+        // quad's four rows: 64 contiguous bytes).
+        const uint32_t t8 = tid & 7u, xl = t8 & 3u, hh = t8 >> 2;
 #pragma unroll 1
         for (int u = 0; u < kHalf; ++u) {
-            const uint32_t ql = tid + kGcThreads * u, q = c * cw + ql;
+            const uint32_t ql = tid + kGcThreads * u;
+            const uint32_t q = c * cw + (ql & ~7u) + 4 * (xl >> 1) + 2 * hh + (xl & 1);
             if (q >= nkb * 8) break;
             const uint4* lp = reinterpret_cast<const uint4*>(local + (size_t)q * 16);
             const uint4 e0 = lp[0], e1 = lp[1];
@@ -867,9 +874,7 @@ __global__ __launch_bounds__(kGcThreads) void k_gather_chunk(
             }
             // dosage-value tiles: row dword 2t + h of K-block kb -> lane
             // s + 32(t%2), dword 2(t/2) + h
-            const uint32_t kb = q / 8, dq = q % 8;
-            uint32_t* ob =
-                tbase + ((size_t)kb * 64 + 32 * ((dq >> 1) & 1)) * 4 + 2 * (dq >> 2) + (dq & 1);
+            uint4* ob = reinterpret_cast<uint4*>(tbase) + (size_t)(q / 8) * 64 + 32 * hh + xl;
 #pragma unroll
             for (int Q = 0; Q < 4; ++Q) {
                 const uint32_t* quad = lds + Q * kGcStride;
@@ -883,9 +888,21 @@ __global__ __launch_bounds__(kGcThreads) void k_gather_chunk(
                 }
                 uint32_t o[4];
                 transpose_bytes4(acc[0], acc[1], acc[2], acc[3], o);  // o[r] = row 4Q + r
+                // lane x keeps o[r] where r % 2 = x % 2, else takes the partner's
+                // o[r ^ 1] (lanes x ^ 1), then the same at distance 2
+                uint32_t b[4];
 #pragma unroll
-                for (int r = 0; r < 4; ++r)
-                    ob[(4 * Q + r) * 4] = j0 + 4 * Q + r < m ? o[r] : 0xFFFFFFFFu;
+                for (int r = 0; r < 4; ++r) {
+                    const uint32_t pr = (uint32_t)__builtin_amdgcn_mov_dpp((int)o[r ^ 1], 0xB1, 0xF, 0xF, true);
+                    b[r] = (uint32_t)(r & 1) == (xl & 1) ? o[r] : pr;
+                }
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const uint32_t pr = (uint32_t)__builtin_amdgcn_mov_dpp((int)b[r ^ 2], 0x4E, 0xF, 0xF, true);
+                    o[r] = (uint32_t)(r >> 1) == (xl >> 1) ? b[r] : pr;
+                }
+                ob[4 * Q] = j0 + 4 * Q + xl < m ? make_uint4(o[0], o[1], o[2], o[3])
+                                                 : make_uint4(~0u, ~0u, ~0u, ~0u);
             }
         }
         __syncthreads();  // the quads are restaged for the next chunk
