@@ -734,6 +734,17 @@ __global__ __launch_bounds__(512) void k_gather4(const uint8_t* __restrict__ sla
 constexpr int kGcRows = 16;
 constexpr int kGcThreads = 512;
 
+// Sum of v over the wave's 64 lanes (DPP within rows of 16, then the four
+// rows' first lanes).
+__device__ __forceinline__ uint32_t wave_sum_u32(uint32_t v) {
+    v += (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0xB1, 0xF, 0xF, true);   // quad_perm 1,0,3,2
+    v += (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x4E, 0xF, 0xF, true);   // quad_perm 2,3,0,1
+    v += (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x141, 0xF, 0xF, true);  // row_half_mirror
+    v += (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x140, 0xF, 0xF, true);  // row_mirror
+    return __builtin_amdgcn_readlane(v, 0) + __builtin_amdgcn_readlane(v, 16) +
+           __builtin_amdgcn_readlane(v, 32) + __builtin_amdgcn_readlane(v, 48);
+}
+
 template <uint32_t MEMBERS>
 __global__ __launch_bounds__(kGcThreads) void k_gather_chunk(
     const uint8_t* __restrict__ slabs, uint64_t pitch, uint32_t src_dw, uint32_t n,
@@ -820,16 +831,14 @@ __global__ __launch_bounds__(kGcThreads) void k_gather_chunk(
                 stage(c, ii, std::true_type{});
         }
         // wave sums, two counts a word (each below 2^14)
-        uint32_t pk[6] = {ca[0] | cb[0] << 16, ca[1] | cb[1] << 16, ca[2] | cb[2] << 16,
-                          ca[3] | cb[3] << 16, cc[0] | cc[1] << 16, cc[2] | cc[3] << 16};
+        const uint32_t pk[6] = {ca[0] | cb[0] << 16, ca[1] | cb[1] << 16, ca[2] | cb[2] << 16,
+                                ca[3] | cb[3] << 16, cc[0] | cc[1] << 16, cc[2] | cc[3] << 16};
+        uint32_t sum[6];
 #pragma unroll
-        for (int x = 0; x < 6; ++x)
+        for (int x = 0; x < 6; ++x) sum[x] = wave_sum_u32(pk[x]);
+        if (ln == 0) {
 #pragma unroll
-            for (int o = 32; o >= 1; o >>= 1) pk[x] += __shfl_xor(pk[x], o);
-        if (ln < 6) {
-#pragma unroll
-            for (int x = 0; x < 6; ++x)
-                if (x == ln) red[wv][x] = pk[x];
+            for (int x = 0; x < 6; ++x) red[wv][x] = sum[x];
         }
         __syncthreads();
         if (c + 1 < nchunks) issue(c + 1);  // in flight during the gather
