@@ -260,8 +260,9 @@ def test_l2_sketch_matches_oracle(oracle, n, d, maf_aware):
     members) they do not, and the f32 form runs instead.  Over 16,884
     individuals the last member chunk has 500, so a K-step of its bucket
     order reaches up to four blocks of d = 2000.  Over 16,384 individuals
-    the bucket sums come from a one-pass kernel: the dense one up to d = 128
-    (d = 3, 100, 128 here), the gather one up to d = 256 (200, 256)."""
+    the bucket sums come from a one-pass kernel: the dense one up to d = 224
+    (d = 3, 100, 128, 200 here), the gather one up to d = 256 (256); above,
+    the chunk gather into bucket-order tiles and the tile sums (500, 2000)."""
     m = 700
     p = _panel(n, m, seed=d)
     snps = L.make_snp_array(p.chr, p.cm, p.bp)
