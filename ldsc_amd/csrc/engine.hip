@@ -1987,9 +1987,14 @@ int l2_begin(const ldsc_panel* panel, const ldsc_snp* snps, uint64_t m, const ld
         nseg = bytes >= (128ull << 20) ? 4u : bytes >= (32ull << 20) ? 2u : 1u;
         if (m < (uint64_t)nseg * 1024) nseg = 1;
     }
+    // The first segment's rows must be on the device before the band starts
+    // and the last segment's results come back after it ends, so those two
+    // are half the size of the others (row weights 1, 2, ..., 2, 1).
     uint32_t seg_r[L2Call::kMaxSeg + 1];
     for (uint32_t sg = 0; sg <= nseg; ++sg)
-        seg_r[sg] = sg == nseg ? m_pad : (uint32_t)round_up(m * sg / nseg, kTile);
+        seg_r[sg] = sg == nseg ? m_pad
+                    : sg == 0  ? 0u
+                               : (uint32_t)round_up(m * (2 * sg - 1) / (2 * nseg - 2), kTile);
     if (annot) {
         HIPC(ws->annot.ensure((size_t)m * K * 8));
         const int dev = panel->device;
