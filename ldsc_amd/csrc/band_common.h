@@ -457,11 +457,16 @@ inline uint32_t sketch_fused_ranges(uint64_t nblocks) {
 // The dense one-pass form (k_sketch_dense): natural individual order, used
 // up to this many bucket blocks (d <= 224).
 constexpr uint64_t kSdMaxBlocks = 7;
+// Bucket blocks per wave of k_sketch: the waves of a SNP pair-group split the
+// d/32 bucket blocks so that the whole chip works at small d too.  Each
+// wave writes one norm share slot per SNP.
+constexpr uint32_t kSkChunk = 4;
 // Per-SNP share slots of the norm buffer: enough for k_sketch (one per
-// kSkChunk = 4 blocks), k_sketch_fused (one per wave and range) and
+// kSkChunk blocks), k_sketch_fused (one per wave and range) and
 // k_sketch_dense (two).
 inline uint32_t sketch_share_slots(uint64_t nblocks) {
-    return (uint32_t)std::max<uint64_t>((nblocks + 3) / 4, kSfWaves * sketch_fused_ranges(nblocks));
+    return (uint32_t)std::max<uint64_t>((nblocks + kSkChunk - 1) / kSkChunk,
+                                        kSfWaves * sketch_fused_ranges(nblocks));
 }
 // One-pass bucket sums in the integer-operand form, straight from the
 // resident .bed rows (natural order with SketchPlan::d_pcode, k_sketch_dense;

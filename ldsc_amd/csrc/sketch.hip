@@ -256,9 +256,6 @@ int sketch_plan(const ldsc_panel* panel_c, uint64_t d, SketchPlan** out, char* e
     return LDSC_OK;
 }
 
-// Bucket blocks per wave of k_sketch: the waves of a SNP pair-group split the
-// d/32 bucket blocks so that the whole chip works at small d too.
-constexpr uint32_t kSkChunk = 4;
 // K-steps whose operands one k_sketch wave has in flight together.
 constexpr int kSkAhead = 8;
 
